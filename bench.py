@@ -1,0 +1,179 @@
+#!/usr/bin/env python3
+"""Render-path benchmark: Msamples/s of the MI355X wavefront path tracer.
+
+Workload (default): 8 displaced icospheres (subdivision 7, 327,680 triangles
+each) on a 2-triangle ground plane + a 2-triangle area light = 2,621,444
+triangles; 1024x1024 pixels, 64 spp, depth 16, hero wavelengths (360..830 nm),
+fp32.  One step = one full render of that image (67,108,864 camera samples)
+with the scene and its BVH already resident in HBM.
+
+Besides value/ms_per_step the JSON line carries:
+  roofline     : the extension-ray traversal kernel (the dominant kernel):
+                 algorithmic bytes (32 B ray + 16 B hit + 128 B per node
+                 visited + 48 B per triangle tested, counted by the kernel's
+                 own counters on the identical rays) / its HIP-event time.
+  cpu_baseline : the CPU oracle (the reference integrator restated, 60-bin
+                 spectra) on a bounded slice of the same image, host threads.
+  cornell      : the small Cornell workload (256^2 x 16 spp, depth 8): GPU
+                 Msamples/s, the oracle's Msamples/s and their ratio.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "filianore-core_amd"))
+
+METRIC = "Msamples/s + relMSE at equal time (guided vs plain), 1 MI355X vs host CPU"
+HBM_PEAK_GBS = 8000.0
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def host_threads():
+    n = os.cpu_count() or 1
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        pass
+    return max(1, min(16, n))
+
+
+def cpu_baseline(gen, spp, depth, budget_s, threads):
+    """Oracle (CPU restatement) on horizontal slices of the same image until
+    about budget_s seconds of CPU work; returns Msamples/s."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+
+    sc = oracle.Scene(gen)
+    H = gen.camera.height
+    rows = 8
+    done_samples, secs, y = 0, 0.0, H // 2 - 64
+    while secs < budget_s and y < H:
+        _, _, st = sc.render(spp, depth, seed=17 + y, threads=threads, rows=(y, min(H, y + rows)))
+        done_samples += st.samples
+        secs += st.seconds
+        y += rows
+        rows = min(64, rows * 2)
+    sc.close()
+    return done_samples / secs / 1e6, done_samples, secs
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default="spheres", choices=["spheres", "cornell", "guided"])
+    ap.add_argument("--spp", type=int, default=0)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py: this build runs on exactly one GPU (renders shard by pixels/samples; see DESIGN.md)",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import flgpu
+
+    cfg = {"spheres": flgpu.FL_GEN_SPHERES, "cornell": flgpu.FL_GEN_CORNELL, "guided": flgpu.FL_GEN_GUIDED}[args.workload]
+    t0 = time.time()
+    gen = flgpu.GenSceneData(cfg)
+    spp = args.spp or gen.spp
+    W, H = gen.camera.width, gen.camera.height
+    log("scene: %d triangles, %dx%d, %d spp, depth %d (gen %.1fs)" % (gen.n_triangles, W, H, spp, gen.depth,
+                                                                      time.time() - t0))
+    out = {}
+    with flgpu.Context(0) as ctx:
+        with ctx.scene(gen) as sc:
+            info = sc.info()
+            log("BVH: %d nodes, depth %d, build %.1f ms" % (info.n_nodes, info.depth, info.build_ms))
+            params = flgpu.default_params(spp=spp, depth=gen.depth, mode=flgpu.FL_SPECTRAL_HERO, seed=1,
+                                          flags=flgpu.FL_RENDER_VNDF_PDF)
+            for i in range(args.warmup):
+                _, _, st = sc.render(params, want_film=False)
+                log("warmup %d: %.1f ms" % (i, st.ms_total))
+            times = []
+            t_start = time.perf_counter()
+            for i in range(args.steps):
+                ts = time.perf_counter()
+                _, _, st = sc.render(params, want_film=False)
+                times.append(time.perf_counter() - ts)
+                log("step %d: %.1f ms (device %.1f ms), %d ext + %d shadow rays" % (
+                    i, times[-1] * 1e3, st.ms_total, st.rays_extension, st.rays_shadow))
+            wall = time.perf_counter() - t_start
+            samples = W * H * spp
+            ms_per_step = wall / args.steps * 1e3
+            value = samples * args.steps / wall / 1e6
+            rays = st.rays_extension + st.rays_shadow
+            # timing + counter passes over the identical rays (same seed)
+            tp = flgpu.default_params(spp=spp, depth=gen.depth, mode=flgpu.FL_SPECTRAL_HERO, seed=1,
+                                      flags=flgpu.FL_RENDER_VNDF_PDF | flgpu.FL_RENDER_TIMING)
+            _, _, tst = sc.render(tp, want_film=False)
+            cp = flgpu.default_params(spp=spp, depth=gen.depth, mode=flgpu.FL_SPECTRAL_HERO, seed=1,
+                                      flags=flgpu.FL_RENDER_VNDF_PDF | flgpu.FL_RENDER_COUNTERS)
+            _, _, cst = sc.render(cp, want_film=False)
+            ext_bytes = cst.rays_extension * (32 + 16) + cst.nodes_visited * 128 + cst.tris_tested * 48
+            launches = max(1, tst.launches_extend)
+            achieved = ext_bytes / (tst.ms_extend * 1e-3) / 1e9
+            roofline = {
+                "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "kernel": "k_extend (closest-hit traversal)",
+                "bytes_per_launch": int(ext_bytes / launches),
+                "avg_launch_ms": round(tst.ms_extend / launches, 4),
+                "nodes_per_ray": round(cst.nodes_visited / max(1, cst.rays_extension), 2),
+                "tris_per_ray": round(cst.tris_tested / max(1, cst.rays_extension), 2),
+                "ms_extend": round(tst.ms_extend, 2), "ms_shade": round(tst.ms_shade, 2),
+                "ms_shadow": round(tst.ms_shadow, 2),
+            }
+            out["mrays_per_s"] = round(rays * args.steps / wall / 1e6, 1)
+            out["rays_per_sample"] = round(rays / samples, 3)
+            out["bvh_build_ms"] = round(info.build_ms, 2)
+            out["nan_samples"] = int(st.nan_samples)
+
+        # the small Cornell workload (the CPU-runnable one): GPU rate
+        cgen = flgpu.GenSceneData(flgpu.FL_GEN_CORNELL)
+        with ctx.scene(cgen) as csc:
+            cparams = flgpu.default_params(spp=cgen.spp, depth=cgen.depth, mode=flgpu.FL_SPECTRAL_HERO, seed=1)
+            csc.render(cparams, want_film=False)
+            ct = []
+            for _ in range(5):
+                ts = time.perf_counter()
+                csc.render(cparams, want_film=False)
+                ct.append(time.perf_counter() - ts)
+            c_samples = cgen.camera.width * cgen.camera.height * cgen.spp
+            cornell_gpu = c_samples * len(ct) / sum(ct) / 1e6
+
+    threads = host_threads()
+    cpu = None
+    cornell = {"gpu_msamples_s": round(cornell_gpu, 2)}
+    if not args.no_cpu:
+        v, ns, secs = cpu_baseline(gen, 1, gen.depth, args.cpu_seconds, threads)
+        cpu = {"value": round(v, 4), "unit": "Msamples/s", "cores": threads, "kind": "port",
+               "sample": "%d samples (1 spp, rows of the same %dx%d image, depth %d), %.1f s" % (ns, W, H, gen.depth,
+                                                                                             secs)}
+        cv, cns, csecs = cpu_baseline(cgen, cgen.spp, cgen.depth, args.cpu_seconds / 3, threads)
+        cornell.update({"cpu_msamples_s": round(cv, 4), "cpu_cores": threads,
+                        "speedup": round(cornell_gpu / cv, 1),
+                        "cpu_sample": "%d samples of the 256x256x16 image" % cns})
+        out["speedup_vs_cpu"] = round(value / v, 1)
+
+    res = {
+        "metric": METRIC, "value": round(value, 2), "unit": "Msamples/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(ms_per_step, 2), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": "%s: %d tris, %dx%d, %d spp, depth %d, hero 360-830nm" % (
+            args.workload, gen.n_triangles, W, H, spp, gen.depth), "samples_per_step": samples},
+        "roofline": roofline, "cpu_baseline": cpu, "cornell": cornell,
+    }
+    res.update(out)
+    print(json.dumps(res), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,306 @@
+// fl_api.hip -- the C ABI of include/fl_gpu.h.
+#include <cmath>
+#include <cstring>
+#include <new>
+
+#include "fl_internal.h"
+
+namespace flx {
+
+int set_error(fl_ctx_t ctx, int code, const std::string &msg) {
+    if (ctx) ctx->last_error = msg;
+    return code;
+}
+
+int render(fl_scene_s *s, const fl_camera *camera, const fl_render_params *p, fl_film *film, fl_stats *stats);
+void release_render_buffers(fl_ctx_t ctx);
+
+} // namespace flx
+
+using flx::set_error;
+
+namespace {
+
+template <typename T>
+int upload(fl_ctx_t ctx, T **d, const T *h, size_t n) {
+    *d = nullptr;
+    if (n == 0) n = 1;
+    if (hipMalloc((void **)d, sizeof(T) * n) != hipSuccess) return set_error(ctx, FL_E_OOM, "hipMalloc");
+    if (h) {
+        hipError_t e = hipMemcpy(*d, h, sizeof(T) * n, hipMemcpyHostToDevice);
+        if (e != hipSuccess) return set_error(ctx, FL_E_HIP, std::string("hipMemcpy: ") + hipGetErrorString(e));
+    }
+    return FL_OK;
+}
+
+void free_scene(fl_scene_s *s) {
+    if (!s) return;
+    (void)hipFree(s->d_nodes);
+    (void)hipFree(s->d_tris);
+    (void)hipFree(s->d_mats);
+    (void)hipFree(s->d_lights);
+    for (auto &t : s->spec) (void)hipFree(t.d_tab);
+    delete s;
+}
+
+} // namespace
+
+extern "C" {
+
+int fl_abi_version(void) { return FL_ABI_VERSION; }
+
+int fl_device_count(void) {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+    return n;
+}
+
+int fl_ctx_create(int device, fl_ctx_t *out) {
+    if (!out) return FL_E_INVALID;
+    *out = nullptr;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n == 0) return FL_E_UNSUPPORTED;
+    if (device < 0 || device >= n) return FL_E_INVALID;
+    if (hipSetDevice(device) != hipSuccess) return FL_E_HIP;
+    fl_ctx_s *c = new (std::nothrow) fl_ctx_s();
+    if (!c) return FL_E_OOM;
+    c->device = device;
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete c;
+        return FL_E_HIP;
+    }
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) == hipSuccess) c->n_cu = prop.multiProcessorCount;
+    if (c->n_cu <= 0) c->n_cu = 256;
+    *out = c;
+    return FL_OK;
+}
+
+void fl_ctx_destroy(fl_ctx_t ctx) {
+    if (!ctx) return;
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
+    flx::release_render_buffers(ctx);
+    if (ctx->scratch) (void)hipFree(ctx->scratch);
+    (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+}
+
+const char *fl_last_error(fl_ctx_t ctx) { return ctx ? ctx->last_error.c_str() : "null context"; }
+
+int fl_scene_create(fl_ctx_t ctx, const fl_scene_desc *desc, fl_scene_t *out) {
+    if (!ctx || !desc || !out) return set_error(ctx, FL_E_INVALID, "fl_scene_create: null argument");
+    *out = nullptr;
+    if (desc->n_triangles && (!desc->vertices || !desc->indices))
+        return set_error(ctx, FL_E_INVALID, "fl_scene_create: missing geometry");
+    if (desc->n_materials == 0 || !desc->materials) return set_error(ctx, FL_E_INVALID, "fl_scene_create: no materials");
+    if (desc->n_lights && (!desc->lights || !desc->emission_spd || desc->n_emission == 0))
+        return set_error(ctx, FL_E_INVALID, "fl_scene_create: lights need emission spectra");
+    for (uint32_t t = 0; t < 3ull * desc->n_triangles; ++t)
+        if (desc->indices[t] >= desc->n_vertices) return set_error(ctx, FL_E_INVALID, "fl_scene_create: index out of range");
+    if (desc->tri_material)
+        for (uint32_t t = 0; t < desc->n_triangles; ++t)
+            if (desc->tri_material[t] >= desc->n_materials && desc->tri_material[t] != fl::kNoMaterial)
+                return set_error(ctx, FL_E_INVALID, "fl_scene_create: material index out of range");
+    (void)hipSetDevice(ctx->device);
+    fl_scene_s *s = new (std::nothrow) fl_scene_s();
+    if (!s) return set_error(ctx, FL_E_OOM, "scene");
+    s->ctx = ctx;
+    s->n_tris = desc->n_triangles;
+    s->n_verts = desc->n_vertices;
+    s->n_mats = desc->n_materials;
+    s->n_lights = desc->n_lights;
+
+    // per-triangle light index + 1, lights' geometry (Triangle::sample / area)
+    std::vector<uint32_t> tri_light(desc->n_triangles, 0u);
+    s->h_lights.resize(desc->n_lights);
+    for (uint32_t l = 0; l < desc->n_lights; ++l) {
+        const fl_area_light &al = desc->lights[l];
+        if (al.tri >= desc->n_triangles || al.emission >= desc->n_emission) {
+            free_scene(s);
+            return set_error(ctx, FL_E_INVALID, "fl_scene_create: bad light");
+        }
+        tri_light[al.tri] = l + 1;
+        DevLight &d = s->h_lights[l];
+        const float *P[3];
+        for (int k = 0; k < 3; ++k) P[k] = desc->vertices + 3ull * desc->indices[3ull * al.tri + k];
+        for (int k = 0; k < 3; ++k) { d.p0[k] = P[0][k]; d.p1[k] = P[1][k]; d.p2[k] = P[2][k]; }
+        // cross evaluated in double then rounded, as maths/geometry.h:213-219 does
+        double e1[3], e2[3];
+        for (int k = 0; k < 3; ++k) { e1[k] = (double)(float)(P[1][k] - P[0][k]); e2[k] = (double)(float)(P[2][k] - P[0][k]); }
+        float c[3] = {(float)(e1[1] * e2[2] - e1[2] * e2[1]), (float)(e1[2] * e2[0] - e1[0] * e2[2]),
+                      (float)(e1[0] * e2[1] - e1[1] * e2[0])};
+        float len = std::sqrt(c[0] * c[0] + c[1] * c[1] + c[2] * c[2]);
+        for (int k = 0; k < 3; ++k) d.n[k] = c[k] / len;
+        d.area = 0.5f * len;
+        d.intensity = al.intensity;
+        d.tri = al.tri;
+        d.em_slot = 0; // set after tables are sized
+    }
+    std::vector<DevMaterial> mats;
+    flx::build_spectral_tables(s, desc, FL_SPECTRAL_PARITY, mats);
+    flx::build_spectral_tables(s, desc, FL_SPECTRAL_HERO, mats);
+    s->h_mats = mats;
+    for (uint32_t l = 0; l < desc->n_lights; ++l)
+        s->h_lights[l].em_slot = 3 + 2 * desc->n_materials + desc->lights[l].emission;
+
+    int rc;
+    float *d_verts = nullptr;
+    uint32_t *d_idx = nullptr, *d_mat = nullptr, *d_light = nullptr;
+    std::vector<uint32_t> mat_h;
+    if ((rc = upload(ctx, &d_verts, desc->vertices, 3ull * desc->n_vertices)) ||
+        (rc = upload(ctx, &d_idx, desc->indices, 3ull * desc->n_triangles)) ||
+        (rc = upload(ctx, &d_light, tri_light.data(), desc->n_triangles))) {
+        (void)hipFree(d_verts); (void)hipFree(d_idx); (void)hipFree(d_light);
+        free_scene(s);
+        return rc;
+    }
+    if (desc->tri_material) {
+        // a material without BSDF (FL_MAT_NONE) is a pass-through primitive
+        mat_h.assign(desc->tri_material, desc->tri_material + desc->n_triangles);
+        for (auto &m : mat_h)
+            if (m != fl::kNoMaterial && desc->materials[m].kind == FL_MAT_NONE) m = fl::kNoMaterial;
+        if ((rc = upload(ctx, &d_mat, mat_h.data(), desc->n_triangles))) {
+            (void)hipFree(d_verts); (void)hipFree(d_idx); (void)hipFree(d_light);
+            free_scene(s);
+            return rc;
+        }
+    }
+    rc = flx::build_bvh(ctx, s, d_verts, d_idx, d_mat, d_light);
+    (void)hipFree(d_verts); (void)hipFree(d_idx); (void)hipFree(d_mat); (void)hipFree(d_light);
+    if (rc) { free_scene(s); return rc; }
+    if ((rc = upload(ctx, &s->d_mats, s->h_mats.data(), s->h_mats.size())) ||
+        (rc = upload(ctx, &s->d_lights, s->h_lights.data(), s->h_lights.size())) ||
+        (rc = upload(ctx, &s->spec[0].d_tab, s->spec[0].host.data(), s->spec[0].host.size())) ||
+        (rc = upload(ctx, &s->spec[1].d_tab, s->spec[1].host.data(), s->spec[1].host.size()))) {
+        free_scene(s);
+        return rc;
+    }
+    for (int k = 0; k < 3; ++k) { s->bmin[k] = fl::kInf; s->bmax[k] = -fl::kInf; }
+    for (uint32_t v = 0; v < desc->n_vertices; ++v)
+        for (int k = 0; k < 3; ++k) {
+            s->bmin[k] = std::fmin(s->bmin[k], desc->vertices[3ull * v + k]);
+            s->bmax[k] = std::fmax(s->bmax[k], desc->vertices[3ull * v + k]);
+        }
+    *out = s;
+    return FL_OK;
+}
+
+void fl_scene_destroy(fl_scene_t s) {
+    if (!s) return;
+    (void)hipSetDevice(s->ctx->device);
+    (void)hipStreamSynchronize(s->ctx->stream);
+    free_scene(s);
+}
+
+int fl_scene_get_info(fl_scene_t s, fl_scene_info *out) {
+    if (!s || !out) return FL_E_INVALID;
+    memset(out, 0, sizeof(*out));
+    out->n_triangles = s->n_tris;
+    out->n_nodes = s->n_nodes;
+    out->n_leaf_tris = s->n_tris;
+    out->depth = s->depth;
+    out->node_bytes = (uint64_t)s->n_nodes * sizeof(fl::Node8);
+    out->tri_bytes = (uint64_t)s->n_tris * sizeof(fl::TriRec);
+    out->build_ms = s->build_ms;
+    for (int k = 0; k < 3; ++k) { out->bounds_min[k] = s->bmin[k]; out->bounds_max[k] = s->bmax[k]; }
+    return FL_OK;
+}
+
+int fl_scene_download_bvh(fl_scene_t s, void *nodes, size_t node_bytes, void *tris, size_t tri_bytes) {
+    if (!s) return FL_E_INVALID;
+    fl_ctx_t ctx = s->ctx;
+    size_t nb = (size_t)s->n_nodes * sizeof(fl::Node8), tb = (size_t)s->n_tris * sizeof(fl::TriRec);
+    if ((nodes && node_bytes < nb) || (tris && tri_bytes < tb)) return set_error(ctx, FL_E_INVALID, "buffer too small");
+    (void)hipSetDevice(ctx->device);
+    if (nodes) FL_HIP_CHECK(ctx, hipMemcpy(nodes, s->d_nodes, nb, hipMemcpyDeviceToHost));
+    if (tris && tb) FL_HIP_CHECK(ctx, hipMemcpy(tris, s->d_tris, tb, hipMemcpyDeviceToHost));
+    return FL_OK;
+}
+
+int fl_intersect_device(fl_scene_t s, const fl_ray *d_rays, fl_hit *d_hits, uint32_t n, uint32_t flags) {
+    if (!s || (n && (!d_rays || !d_hits))) return FL_E_INVALID;
+    (void)hipSetDevice(s->ctx->device);
+    return flx::trace_rays(s->ctx, s, d_rays, d_hits, n, flags & ~FL_COUNTERS, nullptr);
+}
+
+int fl_intersect(fl_scene_t s, const fl_ray *rays, fl_hit *hits, uint32_t n, uint32_t flags, uint64_t *counters) {
+    if (!s || (n && (!rays || !hits))) return FL_E_INVALID;
+    fl_ctx_t ctx = s->ctx;
+    if (n == 0) return FL_OK;
+    (void)hipSetDevice(ctx->device);
+    fl_ray *dr = nullptr;
+    fl_hit *dh = nullptr;
+    unsigned long long *dc = nullptr;
+    int rc = FL_OK;
+    if (hipMalloc((void **)&dr, sizeof(fl_ray) * (size_t)n) != hipSuccess ||
+        hipMalloc((void **)&dh, sizeof(fl_hit) * (size_t)n) != hipSuccess ||
+        hipMalloc((void **)&dc, 64) != hipSuccess) {
+        rc = set_error(ctx, FL_E_OOM, "fl_intersect buffers");
+    }
+    if (!rc && hipMemcpy(dr, rays, sizeof(fl_ray) * (size_t)n, hipMemcpyHostToDevice) != hipSuccess)
+        rc = set_error(ctx, FL_E_HIP, "upload rays");
+    if (!rc && hipMemset(dc, 0, 64) != hipSuccess) rc = set_error(ctx, FL_E_HIP, "counters");
+    if (!rc) rc = flx::trace_rays(ctx, s, dr, dh, n, flags, (flags & FL_COUNTERS) ? dc : nullptr);
+    if (!rc && hipStreamSynchronize(ctx->stream) != hipSuccess) rc = set_error(ctx, FL_E_HIP, "trace");
+    if (!rc && hipMemcpy(hits, dh, sizeof(fl_hit) * (size_t)n, hipMemcpyDeviceToHost) != hipSuccess)
+        rc = set_error(ctx, FL_E_HIP, "download hits");
+    if (!rc && counters) {
+        unsigned long long c[3];
+        if (hipMemcpy(c, dc, sizeof(c), hipMemcpyDeviceToHost) == hipSuccess) {
+            counters[0] = c[0];
+            counters[1] = c[1];
+        }
+    }
+    (void)hipFree(dr);
+    (void)hipFree(dh);
+    (void)hipFree(dc);
+    return rc;
+}
+
+int fl_render(fl_scene_t s, const fl_camera *camera, const fl_render_params *params, fl_film *film, fl_stats *stats) {
+    if (!s) return FL_E_INVALID;
+    (void)hipSetDevice(s->ctx->device);
+    return flx::render(s, camera, params, film, stats);
+}
+
+int fl_bsdf_query_batch(fl_ctx_t ctx, const fl_bsdf_query *q, fl_bsdf_result *r, uint32_t n) {
+    if (!ctx || (n && (!q || !r))) return FL_E_INVALID;
+    if (n == 0) return FL_OK;
+    (void)hipSetDevice(ctx->device);
+    fl_bsdf_query *dq = nullptr;
+    fl_bsdf_result *dr = nullptr;
+    int rc = FL_OK;
+    if (hipMalloc((void **)&dq, sizeof(*q) * (size_t)n) != hipSuccess ||
+        hipMalloc((void **)&dr, sizeof(*r) * (size_t)n) != hipSuccess)
+        rc = set_error(ctx, FL_E_OOM, "bsdf buffers");
+    if (!rc && hipMemcpy(dq, q, sizeof(*q) * (size_t)n, hipMemcpyHostToDevice) != hipSuccess)
+        rc = set_error(ctx, FL_E_HIP, "upload");
+    if (!rc) rc = flx::bsdf_queries(ctx, dq, dr, n);
+    if (!rc && hipStreamSynchronize(ctx->stream) != hipSuccess) rc = set_error(ctx, FL_E_HIP, "bsdf kernel");
+    if (!rc && hipMemcpy(r, dr, sizeof(*r) * (size_t)n, hipMemcpyDeviceToHost) != hipSuccess)
+        rc = set_error(ctx, FL_E_HIP, "download");
+    (void)hipFree(dq);
+    (void)hipFree(dr);
+    return rc;
+}
+
+// Not yet available entry points fail loudly.
+int fl_render_batch(fl_ctx_t ctx, const fl_batch_item *, uint32_t, const fl_render_params *, fl_stats *) {
+    return set_error(ctx, FL_E_UNSUPPORTED, "fl_render_batch: not built");
+}
+int fl_autotune(fl_scene_t s, const fl_camera *, const fl_render_params *, fl_tune_result *) {
+    return set_error(s ? s->ctx : nullptr, FL_E_UNSUPPORTED, "fl_autotune: not built");
+}
+int fl_guide_create(fl_ctx_t ctx, const fl_guide_config *, fl_guide_t *) {
+    return set_error(ctx, FL_E_UNSUPPORTED, "guide: not built");
+}
+void fl_guide_destroy(fl_guide_t) {}
+int fl_guide_train_step(fl_guide_t, const float *, uint32_t, int, float *) { return FL_E_UNSUPPORTED; }
+int fl_guide_forward(fl_guide_t, const float *, float *, uint32_t) { return FL_E_UNSUPPORTED; }
+int fl_guide_get_params(fl_guide_t, float *, uint32_t, uint32_t *) { return FL_E_UNSUPPORTED; }
+int fl_guide_set_params(fl_guide_t, const float *, uint32_t) { return FL_E_UNSUPPORTED; }
+int fl_guide_gradient(fl_guide_t, const float *, uint32_t, float *, uint32_t, float *) { return FL_E_UNSUPPORTED; }
+int fl_guide_stats(fl_guide_t, uint32_t *, float *, int *) { return FL_E_UNSUPPORTED; }
+
+} // extern "C"

@@ -1,0 +1,542 @@
+// fl_build.hip -- GPU LBVH build collapsed to 8-wide, 16-bit quantized nodes.
+//
+// Replaces the Embree scene build (bvh/bvh.cpp:7-47: one RTCGeometry per
+// triangle, rtcCommitScene).  Steps, all on the device:
+//   1. triangle boxes + centroid bounds
+//   2. 63-bit Morton codes of centroids, radix sort (hipCUB)
+//   3. binary radix tree (Karras, HPG 2012), parent links
+//   4. bottom-up box refit (one thread per leaf, second arrival continues)
+//   5. top-down greedy collapse into 8-wide nodes: repeatedly open the child
+//      with the largest surface area; children with <= 3 triangles become
+//      leaf slots; slots are ordered by octant so that the traversal visits
+//      near children first (bit position = slot ^ ray octant)
+//   6. 16-bit conservative quantization against a power-of-two grid whose
+//      origin is a multiple of the step (planes dequantize exactly in fp32)
+#include <hipcub/hipcub.hpp>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+
+#include "fl_internal.h"
+
+namespace {
+
+constexpr uint32_t kLeafBit = 0x80000000u;
+constexpr int kMaxLeafTris = 3;
+
+__device__ __forceinline__ uint32_t ord(float f) {
+    uint32_t u = __float_as_uint(f);
+    return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+__device__ __forceinline__ float unord(uint32_t u) {
+    return __uint_as_float((u & 0x80000000u) ? (u & 0x7fffffffu) : ~u);
+}
+
+__global__ void k_fill_u32(uint32_t *p, uint32_t v, size_t n) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+// Triangle boxes (lo.xyz, hi.xyz) and centroid bounds (ordered-uint atomics).
+__global__ void k_tri_boxes(const float *__restrict__ verts, const uint32_t *__restrict__ idx,
+                            uint32_t n, float *__restrict__ box, uint32_t *cb) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    float c[3] = {0.f, 0.f, 0.f};
+    bool valid = i < n;
+    if (valid) {
+        const float *a = verts + 3ull * idx[3ull * i + 0];
+        const float *b = verts + 3ull * idx[3ull * i + 1];
+        const float *d = verts + 3ull * idx[3ull * i + 2];
+        for (int k = 0; k < 3; ++k) {
+            float lo = fminf(a[k], fminf(b[k], d[k]));
+            float hi = fmaxf(a[k], fmaxf(b[k], d[k]));
+            box[6ull * i + k] = lo;
+            box[6ull * i + 3 + k] = hi;
+            c[k] = 0.5f * lo + 0.5f * hi;
+        }
+    }
+    float mn[3], mx[3];
+    for (int k = 0; k < 3; ++k) {
+        mn[k] = valid ? c[k] : fl::kInf;
+        mx[k] = valid ? c[k] : -fl::kInf;
+    }
+    for (int off = 32; off >= 1; off >>= 1) {
+        for (int k = 0; k < 3; ++k) {
+            mn[k] = fminf(mn[k], __shfl_xor(mn[k], off));
+            mx[k] = fmaxf(mx[k], __shfl_xor(mx[k], off));
+        }
+    }
+    if ((threadIdx.x & 63) == 0) {
+        for (int k = 0; k < 3; ++k) {
+            atomicMin(&cb[k], ord(mn[k]));
+            atomicMax(&cb[3 + k], ord(mx[k]));
+        }
+    }
+}
+
+__device__ __forceinline__ uint64_t expand21(uint32_t v) {
+    uint64_t x = v & 0x1fffffu;
+    x = (x | (x << 32)) & 0x1f00000000ffffull;
+    x = (x | (x << 16)) & 0x1f0000ff0000ffull;
+    x = (x | (x << 8)) & 0x100f00f00f00f00full;
+    x = (x | (x << 4)) & 0x10c30c30c30c30c3ull;
+    x = (x | (x << 2)) & 0x1249249249249249ull;
+    return x;
+}
+
+__global__ void k_morton(const float *__restrict__ box, uint32_t n, const uint32_t *cb,
+                         uint64_t *keys, uint32_t *vals) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    uint32_t q[3];
+    for (int k = 0; k < 3; ++k) {
+        float lo = unord(cb[k]), hi = unord(cb[3 + k]);
+        float c = 0.5f * box[6ull * i + k] + 0.5f * box[6ull * i + 3 + k];
+        float ext = hi - lo;
+        float x = ext > 0.f ? (c - lo) / ext : 0.f;
+        x = fminf(fmaxf(x, 0.f), 1.f);
+        q[k] = (uint32_t)(x * 2097151.0f);
+    }
+    keys[i] = (expand21(q[0]) << 2) | (expand21(q[1]) << 1) | expand21(q[2]);
+    vals[i] = i;
+}
+
+__device__ __forceinline__ int delta(const uint64_t *keys, int n, int i, int j) {
+    if (j < 0 || j >= n) return -1;
+    uint64_t a = keys[i], b = keys[j];
+    if (a == b) return 64 + __clz((uint32_t)(i ^ j));
+    return __clzll(a ^ b);
+}
+
+// Binary radix tree over n sorted keys: internal nodes 0..n-2 (root 0).
+__global__ void k_karras(const uint64_t *__restrict__ keys, int n, uint32_t *left, uint32_t *right,
+                         int *parent_int, int *parent_leaf) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n - 1) return;
+    int d = (delta(keys, n, i, i + 1) - delta(keys, n, i, i - 1)) > 0 ? 1 : -1;
+    int dmin = delta(keys, n, i, i - d);
+    int lmax = 2;
+    while (delta(keys, n, i, i + lmax * d) > dmin) lmax <<= 1;
+    int l = 0;
+    for (int t = lmax >> 1; t >= 1; t >>= 1)
+        if (delta(keys, n, i, i + (l + t) * d) > dmin) l += t;
+    int j = i + l * d;
+    int dnode = delta(keys, n, i, j);
+    int s = 0, t = l;
+    do {
+        t = (t + 1) >> 1;
+        if (delta(keys, n, i, i + (s + t) * d) > dnode) s += t;
+    } while (t > 1);
+    int gamma = i + s * d + (d < 0 ? d : 0);
+    int lo = i < j ? i : j, hi = i < j ? j : i;
+    uint32_t L = (lo == gamma) ? ((uint32_t)gamma | kLeafBit) : (uint32_t)gamma;
+    uint32_t R = (hi == gamma + 1) ? ((uint32_t)(gamma + 1) | kLeafBit) : (uint32_t)(gamma + 1);
+    left[i] = L;
+    right[i] = R;
+    if (L & kLeafBit) parent_leaf[L & ~kLeafBit] = i; else parent_int[L] = i;
+    if (R & kLeafBit) parent_leaf[R & ~kLeafBit] = i; else parent_int[R] = i;
+    if (i == 0) parent_int[0] = -1;
+}
+
+__device__ __forceinline__ float ld_relaxed(const float *p) {
+    uint32_t u = __hip_atomic_load((const uint32_t *)p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    return __uint_as_float(u);
+}
+__device__ __forceinline__ void st_relaxed(float *p, float v) {
+    __hip_atomic_store((uint32_t *)p, __float_as_uint(v), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Bottom-up refit: the second thread to reach a node computes its box.
+__global__ void k_refit(int n, const uint32_t *__restrict__ vals, const float *__restrict__ tbox,
+                        const uint32_t *__restrict__ left, const uint32_t *__restrict__ right,
+                        const int *__restrict__ parent_int, const int *__restrict__ parent_leaf,
+                        float *nbox, uint32_t *ncount, uint32_t *flags) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    int node = parent_leaf[i];
+    while (node >= 0) {
+        uint32_t old = __hip_atomic_fetch_add(&flags[node], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+        if (old == 0) return;
+        float lo[3] = {fl::kInf, fl::kInf, fl::kInf}, hi[3] = {-fl::kInf, -fl::kInf, -fl::kInf};
+        uint32_t cnt = 0;
+        uint32_t ch[2] = {left[node], right[node]};
+        for (int c = 0; c < 2; ++c) {
+            if (ch[c] & kLeafBit) {
+                uint32_t t = vals[ch[c] & ~kLeafBit];
+                for (int k = 0; k < 3; ++k) {
+                    lo[k] = fminf(lo[k], tbox[6ull * t + k]);
+                    hi[k] = fmaxf(hi[k], tbox[6ull * t + 3 + k]);
+                }
+                cnt += 1;
+            } else {
+                for (int k = 0; k < 3; ++k) {
+                    lo[k] = fminf(lo[k], ld_relaxed(&nbox[6ull * ch[c] + k]));
+                    hi[k] = fmaxf(hi[k], ld_relaxed(&nbox[6ull * ch[c] + 3 + k]));
+                }
+                cnt += __hip_atomic_load(&ncount[ch[c]], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            }
+        }
+        for (int k = 0; k < 3; ++k) {
+            st_relaxed(&nbox[6ull * node + k], lo[k]);
+            st_relaxed(&nbox[6ull * node + 3 + k], hi[k]);
+        }
+        __hip_atomic_store(&ncount[node], cnt, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        node = parent_int[node];
+    }
+}
+
+struct BuildView {
+    const float *verts;
+    const uint32_t *idx, *mat, *light;
+    const uint32_t *vals;
+    const float *tbox;
+    const uint32_t *left, *right;
+    const float *nbox;
+    const uint32_t *ncount;
+    fl::Node8 *nodes;
+    fl::TriRec *tris;
+    uint32_t *counters;     // [0] nodes, [1] tris, [2] next work count
+};
+
+__device__ void child_box(const BuildView &v, uint32_t c, float lo[3], float hi[3]) {
+    const float *b;
+    if (c & kLeafBit) b = v.tbox + 6ull * v.vals[c & ~kLeafBit];
+    else b = v.nbox + 6ull * c;
+    for (int k = 0; k < 3; ++k) { lo[k] = b[k]; hi[k] = b[3 + k]; }
+}
+__device__ uint32_t child_count(const BuildView &v, uint32_t c) {
+    return (c & kLeafBit) ? 1u : v.ncount[c];
+}
+
+// exponent e with step 2^e: >= ext/65532, >= 2*ulp(maxabs), >= 2^-126
+__device__ int quant_exponent(float lo, float hi) {
+    double ext = (double)hi - (double)lo;
+    int e = -126;
+    if (ext > 0.0) {
+        int E;
+        double m = frexp(ext / 65532.0, &E); // x = m*2^E, m in [0.5,1)
+        int ce = (m == 0.5) ? (E - 1) : E;  // ceil(log2 x)
+        e = ce > e ? ce : e;
+    }
+    float maxabs = fmaxf(fabsf(lo), fabsf(hi));
+    if (maxabs > 0.f) {
+        int E;
+        frexpf(maxabs, &E);                  // maxabs in [2^(E-1), 2^E)
+        int eu = E - 23;                     // 2 * ulp(maxabs)
+        e = eu > e ? eu : e;
+    }
+    return e > 127 ? 127 : e;
+}
+
+// One thread per work item: (binary node, wide node index).
+__global__ void k_collapse(BuildView v, const uint32_t *__restrict__ work_in, uint32_t n_work,
+                           uint32_t *work_out) {
+    uint32_t w = blockIdx.x * blockDim.x + threadIdx.x;
+    if (w >= n_work) return;
+    uint32_t bnode = work_in[2 * w];
+    uint32_t widx = work_in[2 * w + 1];
+
+    uint32_t ch[8];
+    int cnt = 0;
+    if (bnode & kLeafBit) {
+        ch[cnt++] = bnode;
+    } else {
+        ch[cnt++] = v.left[bnode];
+        ch[cnt++] = v.right[bnode];
+        while (cnt < 8) {
+            int best = -1;
+            float bestA = -1.f;
+            for (int c = 0; c < cnt; ++c) {
+                if (ch[c] & kLeafBit) continue;
+                float lo[3], hi[3];
+                child_box(v, ch[c], lo, hi);
+                float dx = hi[0] - lo[0], dy = hi[1] - lo[1], dz = hi[2] - lo[2];
+                float A = dx * dy + dy * dz + dz * dx;
+                if (A > bestA) { bestA = A; best = c; }
+            }
+            if (best < 0) break;
+            uint32_t b = ch[best];
+            ch[best] = v.left[b];
+            ch[cnt++] = v.right[b];
+        }
+    }
+
+    // union box, centroids, slot assignment by octant
+    float plo[3] = {fl::kInf, fl::kInf, fl::kInf}, phi[3] = {-fl::kInf, -fl::kInf, -fl::kInf};
+    float clo[8][3], chi[8][3];
+    for (int c = 0; c < cnt; ++c) {
+        child_box(v, ch[c], clo[c], chi[c]);
+        for (int k = 0; k < 3; ++k) {
+            plo[k] = fminf(plo[k], clo[c][k]);
+            phi[k] = fmaxf(phi[k], chi[c][k]);
+        }
+    }
+    int slot_of[8];
+    uint32_t used = 0;
+    for (int c = 0; c < cnt; ++c) {
+        int ideal = 0;
+        for (int k = 0; k < 3; ++k) {
+            float cc = 0.5f * clo[c][k] + 0.5f * chi[c][k];
+            float pc = 0.5f * plo[k] + 0.5f * phi[k];
+            if (cc > pc) ideal |= 1 << k;
+        }
+        int best = -1, bestd = 99;
+        for (int s = 0; s < 8; ++s) {
+            if (used & (1u << s)) continue;
+            int dd = __popc((uint32_t)(s ^ ideal));
+            if (dd < bestd) { bestd = dd; best = s; }
+        }
+        slot_of[c] = best;
+        used |= 1u << best;
+    }
+
+    // classify, allocate
+    uint32_t n_int = 0, n_tri = 0;
+    uint32_t leafcnt[8];
+    for (int c = 0; c < cnt; ++c) {
+        uint32_t k = child_count(v, ch[c]);
+        leafcnt[c] = k;
+        if (k <= (uint32_t)kMaxLeafTris) n_tri += k; else n_int += 1;
+    }
+    uint32_t child_base = n_int ? atomicAdd(&v.counters[0], n_int) : 0u;
+    uint32_t tri_base = n_tri ? atomicAdd(&v.counters[1], n_tri) : 0u;
+    uint32_t work_base = n_int ? atomicAdd(&v.counters[2], n_int) : 0u;
+
+    fl::Node8 node;
+    int ex[3];
+    for (int k = 0; k < 3; ++k) {
+        ex[k] = quant_exponent(plo[k], phi[k]);
+        double step = ldexp(1.0, ex[k]);
+        double org = floor((double)plo[k] / step) * step;
+        node.org[k] = (float)org;
+    }
+    node.ex = (uint8_t)(ex[0] + 127);
+    node.ey = (uint8_t)(ex[1] + 127);
+    node.ez = (uint8_t)(ex[2] + 127);
+    node.imask = 0;
+    node.child_base = child_base;
+    node.tri_base = tri_base;
+    for (int s = 0; s < 8; ++s) {
+        node.meta[s] = 0;
+        for (int k = 0; k < 3; ++k) { node.qlo[k][s] = 0xffff; node.qhi[k][s] = 0; }
+    }
+    // slot order for ranks / triangle offsets
+    int child_in_slot[8];
+    for (int s = 0; s < 8; ++s) child_in_slot[s] = -1;
+    for (int c = 0; c < cnt; ++c) child_in_slot[slot_of[c]] = c;
+
+    uint32_t rank = 0, toff = 0;
+    for (int s = 0; s < 8; ++s) {
+        int c = child_in_slot[s];
+        if (c < 0) continue;
+        for (int k = 0; k < 3; ++k) {
+            double step = ldexp(1.0, ex[k]);
+            double org = (double)node.org[k];
+            double ql = floor(((double)clo[c][k] - org) / step) - 1.0;
+            double qh = ceil(((double)chi[c][k] - org) / step) + 1.0;
+            ql = ql < 0.0 ? 0.0 : (ql > 65535.0 ? 65535.0 : ql);
+            qh = qh < 0.0 ? 0.0 : (qh > 65535.0 ? 65535.0 : qh);
+            node.qlo[k][s] = (uint16_t)ql;
+            node.qhi[k][s] = (uint16_t)qh;
+        }
+        if (leafcnt[c] > (uint32_t)kMaxLeafTris) {
+            node.imask |= (uint8_t)(1u << s);
+            node.meta[s] = (uint8_t)rank;
+            work_out[2 * (work_base + rank)] = ch[c];
+            work_out[2 * (work_base + rank) + 1] = child_base + rank;
+            ++rank;
+        } else {
+            node.meta[s] = (uint8_t)((toff & 31u) | (leafcnt[c] << 5));
+            // gather the subtree's triangles (<= 3 leaves)
+            uint32_t stk[8];
+            int sp = 0;
+            stk[sp++] = ch[c];
+            while (sp) {
+                uint32_t b = stk[--sp];
+                if (b & kLeafBit) {
+                    uint32_t t = v.vals[b & ~kLeafBit];
+                    fl::TriRec r;
+                    const uint32_t *ix = v.idx + 3ull * t;
+                    for (int k = 0; k < 3; ++k) {
+                        r.v0[k] = v.verts[3ull * ix[0] + k];
+                        r.v1[k] = v.verts[3ull * ix[1] + k];
+                        r.v2[k] = v.verts[3ull * ix[2] + k];
+                    }
+                    r.id = t;
+                    r.material = v.mat ? v.mat[t] : 0u;
+                    r.light = v.light[t];
+                    v.tris[tri_base + toff] = r;
+                    ++toff;
+                } else {
+                    stk[sp++] = v.right[b];
+                    stk[sp++] = v.left[b];
+                }
+            }
+        }
+    }
+    v.nodes[widx] = node;
+}
+
+__global__ void k_set_root_work(uint32_t *work, uint32_t root, uint32_t *counters) {
+    work[0] = root;
+    work[1] = 0;
+    counters[0] = 1;  // root node allocated
+    counters[1] = 0;
+    counters[2] = 0;
+}
+
+__global__ void k_reset_u32(uint32_t *p) { *p = 0; }
+
+template <typename T>
+int dalloc(fl_ctx_t ctx, T **p, size_t count) {
+    hipError_t e = hipMalloc((void **)p, sizeof(T) * (count ? count : 1));
+    if (e != hipSuccess)
+        return flx::set_error(ctx, FL_E_OOM, std::string("hipMalloc: ") + hipGetErrorString(e));
+    return FL_OK;
+}
+
+struct Scratch {
+    std::vector<void *> ptrs;
+    ~Scratch() {
+        for (void *p : ptrs) (void)hipFree(p);
+    }
+};
+
+} // namespace
+
+namespace flx {
+
+int build_bvh(fl_ctx_t ctx, fl_scene_s *s, const float *d_verts, const uint32_t *d_idx,
+              const uint32_t *d_mat, const uint32_t *d_light) {
+    const uint32_t n = s->n_tris;
+    hipStream_t st = ctx->stream;
+    if (n == 0) {   // empty scene: one root node without children, never hit
+        fl::Node8 root;
+        memset(&root, 0, sizeof(root));
+        for (int k = 0; k < 3; ++k)
+            for (int q = 0; q < 8; ++q) { root.qlo[k][q] = 0xffff; root.qhi[k][q] = 0; }
+        root.ex = root.ey = root.ez = 127;
+        if (hipMalloc((void **)&s->d_nodes, sizeof(fl::Node8)) != hipSuccess ||
+            hipMalloc((void **)&s->d_tris, sizeof(fl::TriRec)) != hipSuccess)
+            return set_error(ctx, FL_E_OOM, "hipMalloc empty scene");
+        FL_HIP_CHECK(ctx, hipMemcpy(s->d_nodes, &root, sizeof(root), hipMemcpyHostToDevice));
+        s->n_nodes = 1;
+        s->depth = 1;
+        s->build_ms = 0.0;
+        return FL_OK;
+    }
+    Scratch sc;
+#define ALLOC(p, cnt)                                                   \
+    do {                                                                \
+        int _r = dalloc(ctx, &p, (cnt));                                \
+        if (_r) return _r;                                              \
+        sc.ptrs.push_back((void *)p);                                   \
+    } while (0)
+
+    float *tbox; uint32_t *cb;
+    uint64_t *keys_a, *keys_b; uint32_t *vals_a, *vals_b;
+    uint32_t *left, *right; int *pint, *pleaf;
+    float *nbox; uint32_t *ncount, *flags, *work_a, *work_b, *counters;
+    ALLOC(tbox, 6ull * n);
+    ALLOC(cb, 6);
+    ALLOC(keys_a, n); ALLOC(keys_b, n);
+    ALLOC(vals_a, n); ALLOC(vals_b, n);
+    uint32_t ni = n > 1 ? n - 1 : 1;
+    ALLOC(left, ni); ALLOC(right, ni); ALLOC(pint, ni); ALLOC(pleaf, n);
+    ALLOC(nbox, 6ull * ni); ALLOC(ncount, ni); ALLOC(flags, ni);
+    ALLOC(work_a, 2ull * n + 2); ALLOC(work_b, 2ull * n + 2);
+    ALLOC(counters, 4);
+
+    hipEvent_t e0, e1;
+    FL_HIP_CHECK(ctx, hipEventCreate(&e0));
+    FL_HIP_CHECK(ctx, hipEventCreate(&e1));
+    FL_HIP_CHECK(ctx, hipEventRecord(e0, st));
+
+    const int B = 256;
+    auto grid = [&](size_t m) { return (unsigned)((m + B - 1) / B); };
+    uint32_t init_cb[6] = {0xffffffffu, 0xffffffffu, 0xffffffffu, 0u, 0u, 0u};
+    FL_HIP_CHECK(ctx, hipMemcpyAsync(cb, init_cb, sizeof(init_cb), hipMemcpyHostToDevice, st));
+    hipLaunchKernelGGL(k_tri_boxes, dim3(grid(n)), dim3(B), 0, st, d_verts, d_idx, n, tbox, cb);
+    hipLaunchKernelGGL(k_morton, dim3(grid(n)), dim3(B), 0, st, tbox, n, cb, keys_a, vals_a);
+
+    size_t tmp_bytes = 0;
+    FL_HIP_CHECK(ctx, hipcub::DeviceRadixSort::SortPairs(nullptr, tmp_bytes, keys_a, keys_b, vals_a,
+                                                         vals_b, (int)n, 0, 63, st));
+    void *tmp = nullptr;
+    {
+        uint8_t *t8;
+        ALLOC(t8, tmp_bytes);
+        tmp = t8;
+    }
+    FL_HIP_CHECK(ctx, hipcub::DeviceRadixSort::SortPairs(tmp, tmp_bytes, keys_a, keys_b, vals_a, vals_b,
+                                                         (int)n, 0, 63, st));
+    if (n >= 2) {
+        hipLaunchKernelGGL(k_karras, dim3(grid(n - 1)), dim3(B), 0, st, keys_b, (int)n, left, right,
+                           pint, pleaf);
+        hipLaunchKernelGGL(k_fill_u32, dim3(grid(n - 1)), dim3(B), 0, st, flags, 0u, (size_t)(n - 1));
+        hipLaunchKernelGGL(k_refit, dim3(grid(n)), dim3(B), 0, st, (int)n, vals_b, tbox, left, right,
+                           pint, pleaf, nbox, ncount, flags);
+    }
+
+    // device outputs (node capacity: one wide node per binary internal node + root)
+    fl::Node8 *nodes;
+    fl::TriRec *tris;
+    {
+        hipError_t e = hipMalloc((void **)&nodes, sizeof(fl::Node8) * (size_t)(ni + 1));
+        if (e != hipSuccess) return set_error(ctx, FL_E_OOM, "hipMalloc nodes");
+        e = hipMalloc((void **)&tris, sizeof(fl::TriRec) * (size_t)(n ? n : 1));
+        if (e != hipSuccess) { (void)hipFree(nodes); return set_error(ctx, FL_E_OOM, "hipMalloc tris"); }
+    }
+    BuildView v;
+    v.verts = d_verts; v.idx = d_idx; v.mat = d_mat; v.light = d_light;
+    v.vals = vals_b; v.tbox = tbox; v.left = left; v.right = right;
+    v.nbox = nbox; v.ncount = ncount; v.nodes = nodes; v.tris = tris; v.counters = counters;
+
+    uint32_t root = (n >= 2) ? 0u : (0u | kLeafBit);
+    hipLaunchKernelGGL(k_set_root_work, dim3(1), dim3(1), 0, st, work_a, root, counters);
+    uint32_t n_work = 1, depth = 0;
+    uint32_t *win = work_a, *wout = work_b;
+    while (n_work > 0) {
+        if (++depth > 256) {
+            (void)hipFree(nodes); (void)hipFree(tris);
+            return set_error(ctx, FL_E_INVALID, "BVH collapse exceeded 256 levels");
+        }
+        hipLaunchKernelGGL(k_reset_u32, dim3(1), dim3(1), 0, st, counters + 2);
+        hipLaunchKernelGGL(k_collapse, dim3(grid(n_work)), dim3(B), 0, st, v, win, n_work, wout);
+        FL_HIP_CHECK(ctx, hipMemcpyAsync(&n_work, counters + 2, 4, hipMemcpyDeviceToHost, st));
+        FL_HIP_CHECK(ctx, hipStreamSynchronize(st));
+        uint32_t *t = win; win = wout; wout = t;
+    }
+    uint32_t cnts[2];
+    FL_HIP_CHECK(ctx, hipMemcpyAsync(cnts, counters, 8, hipMemcpyDeviceToHost, st));
+    FL_HIP_CHECK(ctx, hipEventRecord(e1, st));
+    FL_HIP_CHECK(ctx, hipStreamSynchronize(st));
+    float ms = 0.f;
+    (void)hipEventElapsedTime(&ms, e0, e1);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+
+    if (cnts[1] != n) {
+        (void)hipFree(nodes); (void)hipFree(tris);
+        return set_error(ctx, FL_E_INVALID, "BVH build lost triangles");
+    }
+    // shrink nodes to the exact count
+    fl::Node8 *exact;
+    if (hipMalloc((void **)&exact, sizeof(fl::Node8) * cnts[0]) == hipSuccess) {
+        FL_HIP_CHECK(ctx, hipMemcpyAsync(exact, nodes, sizeof(fl::Node8) * cnts[0], hipMemcpyDeviceToDevice, st));
+        FL_HIP_CHECK(ctx, hipStreamSynchronize(st));
+        (void)hipFree(nodes);
+        nodes = exact;
+    }
+    uint32_t h_cb[6];
+    FL_HIP_CHECK(ctx, hipMemcpy(h_cb, cb, sizeof(h_cb), hipMemcpyDeviceToHost));
+    s->d_nodes = nodes;
+    s->d_tris = tris;
+    s->n_nodes = cnts[0];
+    s->depth = depth;
+    s->build_ms = ms;
+#undef ALLOC
+    return FL_OK;
+}
+
+} // namespace flx

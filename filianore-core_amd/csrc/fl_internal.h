@@ -1,0 +1,91 @@
+// fl_internal.h -- host-side objects behind the C ABI handles.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <string>
+#include <vector>
+
+#include "../../include/fl_gpu.h"
+#include "fl_bvh.h"
+#include "fl_shade.h"
+
+struct DevMaterial {        // 32 bytes
+    int32_t kind;
+    float ax, ay;
+    float weight;
+    float eta_bsdf;         // BSDF::eta (ksIOR when ks>0 && kt>0, else 1)
+    uint32_t spec_a;        // table slot: Lambert R / conductor eta / dielectric R
+    uint32_t spec_b;        // table slot: conductor k / dielectric T
+    float cauchy_b;         // dielectric Cauchy B (um^2), 0 = scalar eta
+};
+
+struct DevLight {           // 64 bytes: one DiffuseAreaIlluminant over one triangle
+    float p0[3], p1[3], p2[3];
+    float n[3];             // normalize(cross(p1-p0, p2-p0)), triangle.cpp:146
+    float area;             // 0.5*|cross(p1-p0, p2-p0)|, triangle.cpp:130-132
+    float intensity;
+    uint32_t em_slot;       // emission table slot
+    uint32_t tri;
+};
+
+// Spectral tables of one scene in one grid: [cmf x, y, z, materials (2 each),
+// emissions], each `stride` floats.
+struct SpecTables {
+    fl::SpecGrid grid;
+    float *d_tab = nullptr;
+    std::vector<float> host;
+};
+
+struct fl_ctx_s {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string last_error;
+    int n_cu = 0;
+    // scratch reused across calls (grown on demand)
+    void *scratch = nullptr;
+    size_t scratch_bytes = 0;
+    hipEvent_t ev[16] = {};
+};
+
+struct fl_scene_s {
+    fl_ctx_t ctx = nullptr;
+    uint32_t n_tris = 0, n_verts = 0, n_nodes = 0, depth = 0, n_mats = 0, n_lights = 0;
+    fl::Node8 *d_nodes = nullptr;
+    fl::TriRec *d_tris = nullptr;
+    DevMaterial *d_mats = nullptr;
+    DevLight *d_lights = nullptr;
+    SpecTables spec[2];     // [FL_SPECTRAL_HERO, FL_SPECTRAL_PARITY]
+    double build_ms = 0.0;
+    float bmin[3], bmax[3];
+    std::vector<DevMaterial> h_mats;
+    std::vector<DevLight> h_lights;
+};
+
+namespace flx {
+
+int set_error(fl_ctx_t ctx, int code, const std::string &msg);
+
+#define FL_HIP_CHECK(ctx, call)                                                              \
+    do {                                                                                     \
+        hipError_t _e = (call);                                                              \
+        if (_e != hipSuccess)                                                                \
+            return flx::set_error((ctx), FL_E_HIP,                                           \
+                                  std::string(#call) + ": " + hipGetErrorString(_e));        \
+    } while (0)
+
+// BVH build (fl_build.hip)
+int build_bvh(fl_ctx_t ctx, fl_scene_s *s, const float *d_verts, const uint32_t *d_idx,
+              const uint32_t *d_mat, const uint32_t *d_light);
+
+// spectral preprocessing (fl_spectra.cpp)
+void build_spectral_tables(fl_scene_s *s, const fl_scene_desc *desc, int mode,
+                           std::vector<DevMaterial> &mats);
+
+// traversal (fl_trace.hip)
+int trace_rays(fl_ctx_t ctx, fl_scene_s *s, const fl_ray *d_rays, fl_hit *d_hits, uint32_t n,
+               uint32_t flags, unsigned long long *d_counters);
+
+// BSDF queries (fl_render.hip)
+int bsdf_queries(fl_ctx_t ctx, const fl_bsdf_query *d_q, fl_bsdf_result *d_r, uint32_t n);
+
+} // namespace flx

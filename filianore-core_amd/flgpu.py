@@ -1,0 +1,366 @@
+"""ctypes binding of libfl_gpu.so (include/fl_gpu.h, include/fl_scenegen.h).
+
+This is the Python view of the C ABI used by the tests and the benchmark; the
+product itself is the shared library.  Loading fails loudly when the library
+is missing: there is no CPU fallback for the render path.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libfl_gpu.so")
+
+FL_OK = 0
+FL_E_INVALID, FL_E_HIP, FL_E_OOM, FL_E_UNSUPPORTED, FL_E_OVERFLOW, FL_E_NUMERIC = -1, -2, -3, -4, -5, -6
+FL_MAT_NONE, FL_MAT_LAMBERT, FL_MAT_CONDUCTOR, FL_MAT_DIELECTRIC = 0, 1, 2, 3
+FL_CLOSEST_HIT, FL_ANY_HIT, FL_SKIP_LIGHTS, FL_COUNTERS = 0, 1, 2, 4
+FL_SPECTRAL_HERO, FL_SPECTRAL_PARITY = 0, 1
+FL_RENDER_GUIDED, FL_RENDER_COUNTERS, FL_RENDER_VNDF_PDF, FL_RENDER_GRAPH = 1, 2, 4, 8
+FL_RENDER_TRAIN_GUIDE, FL_RENDER_NO_SORT, FL_RENDER_TIMING = 16, 32, 64
+FL_NO_HIT = 0xFFFFFFFF
+FL_SPD_N = 471
+FL_GEN_CORNELL, FL_GEN_SPHERES, FL_GEN_BATCH, FL_GEN_GUIDED, FL_GEN_TERRAIN = 0, 1, 2, 3, 4
+BSDF_REFLECTION, BSDF_TRANSMISSION, BSDF_DIFFUSE, BSDF_GLOSSY, BSDF_SPECULAR = 1, 2, 4, 8, 16
+
+f3 = C.c_float * 3
+f4 = C.c_float * 4
+
+
+class Material(C.Structure):
+    _fields_ = [("kind", C.c_int32), ("rgb", f3), ("rgb_t", f3), ("roughness", C.c_float),
+                ("anisotropic", C.c_float), ("weight", C.c_float), ("metal", C.c_int32), ("ior", C.c_float),
+                ("cauchy_b", C.c_float), ("pad", C.c_uint32 * 3)]
+
+
+class AreaLight(C.Structure):
+    _fields_ = [("tri", C.c_uint32), ("emission", C.c_uint32), ("intensity", C.c_float), ("pad", C.c_uint32)]
+
+
+class SceneDesc(C.Structure):
+    _fields_ = [("vertices", C.POINTER(C.c_float)), ("n_vertices", C.c_uint32),
+                ("indices", C.POINTER(C.c_uint32)), ("n_triangles", C.c_uint32),
+                ("tri_material", C.POINTER(C.c_uint32)),
+                ("materials", C.POINTER(Material)), ("n_materials", C.c_uint32),
+                ("lights", C.POINTER(AreaLight)), ("n_lights", C.c_uint32),
+                ("emission_spd", C.POINTER(C.c_float)), ("n_emission", C.c_uint32)]
+
+
+class SceneInfo(C.Structure):
+    _fields_ = [("n_triangles", C.c_uint32), ("n_nodes", C.c_uint32), ("n_leaf_tris", C.c_uint32),
+                ("depth", C.c_uint32), ("node_bytes", C.c_uint64), ("tri_bytes", C.c_uint64),
+                ("build_ms", C.c_double), ("bounds_min", f3), ("bounds_max", f3)]
+
+
+class Camera(C.Structure):
+    _fields_ = [("position", f3), ("look_at", f3), ("up", f3), ("vfov_deg", C.c_float),
+                ("width", C.c_uint32), ("height", C.c_uint32)]
+
+
+class RenderParams(C.Structure):
+    _fields_ = [("spp", C.c_uint32), ("diffuse_depth", C.c_int32), ("reflection_depth", C.c_int32),
+                ("transmission_depth", C.c_int32), ("rr_threshold", C.c_float), ("rr_start", C.c_uint32),
+                ("spectral_mode", C.c_uint32), ("flags", C.c_uint32), ("seed", C.c_uint64),
+                ("sample_offset", C.c_uint32), ("max_paths", C.c_uint32), ("stack_depth", C.c_uint32),
+                ("block_size", C.c_uint32), ("guide", C.c_void_p), ("guide_mix", C.c_float),
+                ("guide_train_fraction", C.c_float)]
+
+
+class Film(C.Structure):
+    _fields_ = [("xyz_sum", C.POINTER(C.c_float)), ("xyz_sqsum", C.POINTER(C.c_float)),
+                ("width", C.c_uint32), ("height", C.c_uint32)]
+
+
+class Stats(C.Structure):
+    _fields_ = [("ms_total", C.c_double), ("ms_generate", C.c_double), ("ms_extend", C.c_double),
+                ("ms_shade", C.c_double), ("ms_shadow", C.c_double), ("ms_resolve", C.c_double),
+                ("ms_guide", C.c_double),
+                ("samples", C.c_uint64), ("rays_extension", C.c_uint64), ("rays_shadow", C.c_uint64),
+                ("nodes_visited", C.c_uint64), ("tris_tested", C.c_uint64),
+                ("shadow_nodes_visited", C.c_uint64), ("shadow_tris_tested", C.c_uint64),
+                ("launches_extend", C.c_uint32), ("launches_shade", C.c_uint32), ("launches_shadow", C.c_uint32),
+                ("pad0", C.c_uint32),
+                ("nan_samples", C.c_uint64), ("stack_spills", C.c_uint64),
+                ("passes", C.c_uint32), ("max_bounce", C.c_uint32),
+                ("paths_per_pass", C.c_uint32), ("stack_depth", C.c_uint32)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class BatchItem(C.Structure):
+    _fields_ = [("scene", C.c_void_p), ("camera", Camera), ("film", C.POINTER(Film)), ("spp", C.c_uint32),
+                ("pad", C.c_uint32)]
+
+
+class TuneResult(C.Structure):
+    _fields_ = [("max_paths", C.c_uint32), ("stack_depth", C.c_uint32), ("block_size", C.c_uint32),
+                ("sort_shading", C.c_uint32), ("ms_best", C.c_double), ("ms_default", C.c_double),
+                ("n_tried", C.c_uint32)]
+
+
+class GuideConfig(C.Structure):
+    _fields_ = [("hidden", C.c_uint32), ("layers", C.c_uint32), ("lobes", C.c_uint32), ("bands", C.c_uint32),
+                ("lr", C.c_float), ("beta1", C.c_float), ("beta2", C.c_float), ("eps", C.c_float),
+                ("batch", C.c_uint32), ("seed", C.c_uint64), ("scene_min", f3), ("scene_max", f3)]
+
+
+class BsdfQuery(C.Structure):
+    _fields_ = [("kind", C.c_int32), ("alpha_x", C.c_float), ("alpha_y", C.c_float), ("eta", f4), ("k", f4),
+                ("r", f4), ("t", f4), ("wo", f3), ("wi", f3), ("u", C.c_float * 2), ("flags", C.c_uint32)]
+
+
+class BsdfResult(C.Structure):
+    _fields_ = [("f_eval", f4), ("pdf_eval", C.c_float), ("f_sample", f4), ("pdf_sample", C.c_float),
+                ("wi_sample", f3), ("sampled_type", C.c_uint32)]
+
+
+class GenOptions(C.Structure):
+    _fields_ = [("subdiv", C.c_int32), ("n_spheres", C.c_int32), ("grid_x", C.c_int32), ("grid_y", C.c_int32),
+                ("width", C.c_uint32), ("height", C.c_uint32), ("scene_index", C.c_uint32),
+                ("camera_index", C.c_uint32), ("seed", C.c_uint64)]
+
+
+class GenScene(C.Structure):
+    _fields_ = [("vertices", C.POINTER(C.c_float)), ("n_vertices", C.c_uint32),
+                ("indices", C.POINTER(C.c_uint32)), ("n_triangles", C.c_uint32),
+                ("tri_material", C.POINTER(C.c_uint32)),
+                ("materials", C.POINTER(Material)), ("n_materials", C.c_uint32),
+                ("lights", C.POINTER(AreaLight)), ("n_lights", C.c_uint32),
+                ("emission_spd", C.POINTER(C.c_float)), ("n_emission", C.c_uint32),
+                ("camera", Camera), ("spp", C.c_uint32), ("depth", C.c_int32)]
+
+
+# exported symbols of include/fl_gpu.h and include/fl_scenegen.h
+EXPORTS = [
+    "fl_abi_version", "fl_device_count", "fl_ctx_create", "fl_ctx_destroy", "fl_last_error",
+    "fl_scene_create", "fl_scene_destroy", "fl_scene_get_info", "fl_scene_download_bvh",
+    "fl_intersect", "fl_intersect_device", "fl_render", "fl_render_batch", "fl_autotune",
+    "fl_guide_create", "fl_guide_destroy", "fl_guide_train_step", "fl_guide_forward",
+    "fl_guide_get_params", "fl_guide_set_params", "fl_guide_gradient", "fl_guide_stats",
+    "fl_bsdf_query_batch",
+    "fl_gen_default_options", "fl_gen_config", "fl_gen_free", "fl_gen_describe", "fl_gen_blackbody",
+]
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError("libfl_gpu.so is not built (%s); run __graft_entry__.build()" % LIB_PATH)
+        L = C.CDLL(LIB_PATH)
+        vp = C.c_void_p
+        L.fl_ctx_create.argtypes = [C.c_int, C.POINTER(vp)]
+        L.fl_ctx_destroy.argtypes = [vp]
+        L.fl_last_error.argtypes = [vp]
+        L.fl_last_error.restype = C.c_char_p
+        L.fl_scene_create.argtypes = [vp, C.POINTER(SceneDesc), C.POINTER(vp)]
+        L.fl_scene_destroy.argtypes = [vp]
+        L.fl_scene_get_info.argtypes = [vp, C.POINTER(SceneInfo)]
+        L.fl_scene_download_bvh.argtypes = [vp, vp, C.c_size_t, vp, C.c_size_t]
+        L.fl_intersect.argtypes = [vp, vp, vp, C.c_uint32, C.c_uint32, C.POINTER(C.c_uint64)]
+        L.fl_intersect_device.argtypes = [vp, vp, vp, C.c_uint32, C.c_uint32]
+        L.fl_render.argtypes = [vp, C.POINTER(Camera), C.POINTER(RenderParams), C.POINTER(Film), C.POINTER(Stats)]
+        L.fl_render_batch.argtypes = [vp, C.POINTER(BatchItem), C.c_uint32, C.POINTER(RenderParams),
+                                      C.POINTER(Stats)]
+        L.fl_autotune.argtypes = [vp, C.POINTER(Camera), C.POINTER(RenderParams), C.POINTER(TuneResult)]
+        L.fl_guide_create.argtypes = [vp, C.POINTER(GuideConfig), C.POINTER(vp)]
+        L.fl_guide_destroy.argtypes = [vp]
+        L.fl_guide_train_step.argtypes = [vp, vp, C.c_uint32, C.c_int, C.POINTER(C.c_float)]
+        L.fl_guide_forward.argtypes = [vp, vp, vp, C.c_uint32]
+        L.fl_guide_get_params.argtypes = [vp, vp, C.c_uint32, C.POINTER(C.c_uint32)]
+        L.fl_guide_set_params.argtypes = [vp, vp, C.c_uint32]
+        L.fl_guide_gradient.argtypes = [vp, vp, C.c_uint32, vp, C.c_uint32, C.POINTER(C.c_float)]
+        L.fl_guide_stats.argtypes = [vp, C.POINTER(C.c_uint32), C.POINTER(C.c_float), C.POINTER(C.c_int)]
+        L.fl_bsdf_query_batch.argtypes = [vp, C.POINTER(BsdfQuery), C.POINTER(BsdfResult), C.c_uint32]
+        L.fl_gen_default_options.argtypes = [C.POINTER(GenOptions)]
+        L.fl_gen_config.argtypes = [C.c_int, C.POINTER(GenOptions), C.POINTER(GenScene)]
+        L.fl_gen_free.argtypes = [C.POINTER(GenScene)]
+        L.fl_gen_describe.argtypes = [C.POINTER(GenScene), C.POINTER(SceneDesc)]
+        L.fl_gen_blackbody.argtypes = [C.c_float, C.POINTER(C.c_float)]
+        _lib = L
+    return _lib
+
+
+class FlError(RuntimeError):
+    pass
+
+
+def _check(rc, ctx=None, what=""):
+    if rc != FL_OK:
+        msg = lib().fl_last_error(ctx).decode() if ctx else ""
+        raise FlError("%s failed (%d): %s" % (what, rc, msg))
+
+
+# ------------------------------------------------------------------ scenes --
+class GenSceneData:
+    """A generated scene as numpy arrays (owned copies)."""
+
+    def __init__(self, config, **opts):
+        L = lib()
+        o = GenOptions()
+        L.fl_gen_default_options(C.byref(o))
+        for k, v in opts.items():
+            setattr(o, k, v)
+        g = GenScene()
+        _check(L.fl_gen_config(config, C.byref(o), C.byref(g)), what="fl_gen_config")
+        try:
+            nv, nt = g.n_vertices, g.n_triangles
+            self.vertices = np.ctypeslib.as_array(g.vertices, (nv * 3,)).copy().reshape(nv, 3)
+            self.indices = np.ctypeslib.as_array(g.indices, (nt * 3,)).copy().reshape(nt, 3)
+            self.tri_material = np.ctypeslib.as_array(g.tri_material, (nt,)).copy()
+            self.materials = (Material * g.n_materials)(*[g.materials[i] for i in range(g.n_materials)])
+            self.lights = (AreaLight * max(1, g.n_lights))(*[g.lights[i] for i in range(g.n_lights)])
+            self.n_lights = g.n_lights
+            self.emission = np.ctypeslib.as_array(g.emission_spd, (g.n_emission * FL_SPD_N,)).copy() \
+                if g.n_emission else np.zeros(FL_SPD_N, np.float32)
+            self.n_emission = g.n_emission
+            self.camera = Camera()
+            C.pointer(self.camera)[0] = g.camera
+            self.spp = g.spp
+            self.depth = g.depth
+        finally:
+            L.fl_gen_free(C.byref(g))
+
+    @property
+    def n_triangles(self):
+        return self.indices.shape[0]
+
+    def desc(self):
+        d = SceneDesc()
+        self._keep = [np.ascontiguousarray(self.vertices, np.float32), np.ascontiguousarray(self.indices, np.uint32),
+                      np.ascontiguousarray(self.tri_material, np.uint32), np.ascontiguousarray(self.emission, np.float32)]
+        v, i, m, e = self._keep
+        d.vertices = v.ctypes.data_as(C.POINTER(C.c_float))
+        d.n_vertices = v.shape[0]
+        d.indices = i.ctypes.data_as(C.POINTER(C.c_uint32))
+        d.n_triangles = i.shape[0]
+        d.tri_material = m.ctypes.data_as(C.POINTER(C.c_uint32))
+        d.materials = C.cast(self.materials, C.POINTER(Material))
+        d.n_materials = len(self.materials)
+        d.lights = C.cast(self.lights, C.POINTER(AreaLight))
+        d.n_lights = self.n_lights
+        d.emission_spd = e.ctypes.data_as(C.POINTER(C.c_float))
+        d.n_emission = self.n_emission
+        return d
+
+    def triangles(self):
+        """(n, 3, 3) float32 vertex positions per triangle."""
+        return self.vertices[self.indices]
+
+
+def default_params(spp=16, depth=8, mode=FL_SPECTRAL_HERO, seed=1, flags=0):
+    p = RenderParams()
+    p.spp = spp
+    p.diffuse_depth = p.reflection_depth = p.transmission_depth = depth
+    p.rr_threshold = 1.0
+    p.rr_start = 3
+    p.spectral_mode = mode
+    p.flags = flags
+    p.seed = seed
+    p.guide_mix = 0.5
+    p.guide_train_fraction = 0.25
+    return p
+
+
+class Context:
+    def __init__(self, device=0):
+        self.L = lib()
+        h = C.c_void_p()
+        rc = self.L.fl_ctx_create(device, C.byref(h))
+        if rc != FL_OK:
+            raise FlError("fl_ctx_create failed (%d): no usable GPU" % rc)
+        self.h = h
+
+    def close(self):
+        if self.h:
+            self.L.fl_ctx_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def err(self):
+        return self.L.fl_last_error(self.h).decode()
+
+    def scene(self, gen):
+        return Scene(self, gen)
+
+    def bsdf_queries(self, queries):
+        n = len(queries)
+        q = (BsdfQuery * n)(*queries)
+        r = (BsdfResult * n)()
+        _check(self.L.fl_bsdf_query_batch(self.h, q, r, n), self.h, "fl_bsdf_query_batch")
+        return r
+
+
+class Scene:
+    def __init__(self, ctx, gen):
+        self.ctx = ctx
+        self.gen = gen
+        d = gen.desc()
+        h = C.c_void_p()
+        _check(ctx.L.fl_scene_create(ctx.h, C.byref(d), C.byref(h)), ctx.h, "fl_scene_create")
+        self.h = h
+
+    def close(self):
+        if self.h:
+            self.ctx.L.fl_scene_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def info(self):
+        i = SceneInfo()
+        _check(self.ctx.L.fl_scene_get_info(self.h, C.byref(i)), self.ctx.h, "fl_scene_get_info")
+        return i
+
+    def bvh(self):
+        i = self.info()
+        nodes = np.zeros(i.node_bytes, np.uint8)
+        tris = np.zeros(max(1, i.tri_bytes), np.uint8)
+        _check(self.ctx.L.fl_scene_download_bvh(self.h, nodes.ctypes.data, nodes.nbytes, tris.ctypes.data,
+                                                tris.nbytes), self.ctx.h, "fl_scene_download_bvh")
+        return nodes.reshape(-1, 128), tris[: i.tri_bytes].reshape(-1, 48)
+
+    def intersect(self, rays, flags=FL_CLOSEST_HIT, counters=False):
+        """rays: (n, 8) float32 [ox, oy, oz, tmin, dx, dy, dz, tmax] -> hits (n, 4) [t, tri(u32), u, v]."""
+        rays = np.ascontiguousarray(rays, np.float32)
+        n = rays.shape[0]
+        hits = np.zeros((n, 4), np.float32)
+        c = (C.c_uint64 * 2)()
+        f = flags | (FL_COUNTERS if counters else 0)
+        _check(self.ctx.L.fl_intersect(self.h, rays.ctypes.data, hits.ctypes.data, n, f, c), self.ctx.h,
+               "fl_intersect")
+        t = hits[:, 0].copy()
+        tri = hits[:, 1].view(np.uint32).copy()
+        uv = hits[:, 2:4].copy()
+        if counters:
+            return t, tri, uv, (int(c[0]), int(c[1]))
+        return t, tri, uv
+
+    def render(self, params, camera=None, want_film=True):
+        cam = camera if camera is not None else self.gen.camera
+        W, H = cam.width, cam.height
+        st = Stats()
+        film = None
+        if want_film:
+            s = np.zeros((H, W, 3), np.float32)
+            q = np.zeros((H, W, 3), np.float32)
+            film = Film(s.ctypes.data_as(C.POINTER(C.c_float)), q.ctypes.data_as(C.POINTER(C.c_float)), W, H)
+        rc = self.ctx.L.fl_render(self.h, C.byref(cam), C.byref(params), C.byref(film) if film else None,
+                                  C.byref(st))
+        _check(rc, self.ctx.h, "fl_render")
+        if want_film:
+            return s, q, st
+        return None, None, st

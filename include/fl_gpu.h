@@ -1,0 +1,287 @@
+/*
+ * fl_gpu.h -- C ABI of the MI355X render path (libfl_gpu.so).
+ *
+ * Drop-in boundary for filianore-core's render path.  The reference has no C
+ * ABI: its "operator API" is the C++ interface set below, and each entry point
+ * here replaces one of them (reference files cited per entry point):
+ *
+ *   Scene(SceneGeometry, illuminants)        core/scene.h:14, core/scene.cpp:7-19
+ *   BVH::initialize_scene_geometry/finalise  bvh/bvh.cpp:7-47           -> fl_scene_create
+ *   Scene::intersect / BVH::intersect        core/scene.h:20, bvh/bvh.cpp:49-84
+ *   Scene::intersect_p / BVH::intersect_p    core/scene.h:22, bvh/bvh.cpp:86-88 -> fl_intersect*
+ *   Integrator::Li (per camera sample)       integrators/integrator.h:17,
+ *                                            integrators/pathintegrator.cpp:12-87 -> fl_render
+ *   BxDF::evaluate/sample/pdf                shading/bxdfs/bxdf.h:29-33  -> fl_bsdf_query
+ *
+ * Conventions (kept from the reference where it has one):
+ *   - status: 0 = FL_OK, negative = FL_E_*; no C++ exception crosses the ABI;
+ *     fl_last_error() gives a message for the last failing call on a context.
+ *   - host arrays are caller-owned and copied; handles own all device memory.
+ *   - a context owns one HIP stream on one device and is externally
+ *     synchronised (one host thread at a time), like the reference's
+ *     per-thread Sampler; scenes are immutable after creation and may be
+ *     shared by any number of render calls, like Scene (const intersect).
+ *   - triangle IDs are the flattened (mesh, triangle) order the reference
+ *     gives Embree geometries (bvh/bvh.cpp:12-16,41).
+ *   - calls block until results are visible to the host unless an *_device
+ *     variant is used (then ordering is on the context stream).
+ */
+#ifndef FL_GPU_H
+#define FL_GPU_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FL_ABI_VERSION 1
+
+/* ---- status codes ------------------------------------------------------ */
+enum {
+    FL_OK = 0,
+    FL_E_INVALID = -1,     /* bad argument / handle */
+    FL_E_HIP = -2,         /* HIP runtime error */
+    FL_E_OOM = -3,         /* device allocation failed */
+    FL_E_UNSUPPORTED = -4, /* feature not available (e.g. no GPU) */
+    FL_E_OVERFLOW = -5,    /* a bounded device structure overflowed */
+    FL_E_NUMERIC = -6      /* divergence / NaN detected where it is fatal */
+};
+
+typedef struct fl_ctx_s *fl_ctx_t;
+typedef struct fl_scene_s *fl_scene_t;
+typedef struct fl_guide_s *fl_guide_t;
+
+/* ---- materials --------------------------------------------------------- */
+/* Material kinds: the StandardSurface branches and LambertMaterial that the
+ * render path implements.  FL_MAT_NONE is a primitive without material: the
+ * reference lets such hits pass through (pathintegrator.cpp:33-38). */
+enum {
+    FL_MAT_NONE = 0,
+    FL_MAT_LAMBERT = 1,    /* LambertMaterial, shading/materials/lambert.cpp:9-17 */
+    FL_MAT_CONDUCTOR = 2,  /* StandardSurface WavelengthDependentMetallic branch,
+                              standardsurface.cpp:101-125 (kdweight = 0) */
+    FL_MAT_DIELECTRIC = 3  /* StandardSurface ks>0, kt>0 branch: FresnelSpecularBXDF,
+                              standardsurface.cpp:77-81,113-117 */
+};
+
+typedef struct {
+    int32_t kind;        /* FL_MAT_* */
+    float rgb[3];        /* Lambert kd, or dielectric ks (reflectance RGB) */
+    float rgb_t[3];      /* dielectric kt (reflectance RGB) */
+    float roughness;     /* conductor ksroughness: alpha = roughness^2 (aspect 1) */
+    float anisotropic;   /* conductor ksanisotropic */
+    float weight;        /* Lambert: unused (1); conductor: ksweight */
+    int32_t metal;       /* forge metal 0..8 (metallurgy.cpp:42-75) */
+    float ior;           /* dielectric ksIOR; with cauchy_b != 0: Cauchy A */
+    float cauchy_b;      /* Cauchy B [um^2]; 0 = scalar eta as in the reference */
+    uint32_t pad[3];
+} fl_material;
+
+/* One DiffuseAreaIlluminant over one triangle (illuminants/diffusearea.cpp). */
+typedef struct {
+    uint32_t tri;        /* emitting triangle ID */
+    uint32_t emission;   /* index into fl_scene_desc.emission_spd */
+    float intensity;     /* DiffuseAreaIlluminant::intensity */
+    uint32_t pad;
+} fl_area_light;
+
+#define FL_SPD_N 471     /* emission SPDs are sampled on 360..830 nm at 1 nm */
+
+typedef struct {
+    const float *vertices;       /* n_vertices * 3 floats */
+    uint32_t n_vertices;
+    const uint32_t *indices;     /* n_triangles * 3 */
+    uint32_t n_triangles;
+    const uint32_t *tri_material;/* n_triangles material indices (NULL: all 0) */
+    const fl_material *materials;
+    uint32_t n_materials;
+    const fl_area_light *lights;
+    uint32_t n_lights;
+    const float *emission_spd;   /* n_emission * FL_SPD_N */
+    uint32_t n_emission;
+} fl_scene_desc;
+
+typedef struct {
+    uint32_t n_triangles, n_nodes, n_leaf_tris, depth;
+    uint64_t node_bytes, tri_bytes;
+    double build_ms;             /* device time of the LBVH build */
+    float bounds_min[3], bounds_max[3];
+} fl_scene_info;
+
+/* ---- rays and hits (Scene::intersect) ----------------------------------- */
+typedef struct { float ox, oy, oz, tmin, dx, dy, dz, tmax; } fl_ray; /* 32 B */
+typedef struct { float t; uint32_t tri; float u, v; } fl_hit;        /* 16 B */
+#define FL_NO_HIT 0xffffffffu
+
+enum {
+    FL_CLOSEST_HIT = 0,
+    FL_ANY_HIT = 1,          /* shadow query: any hit in (tmin, tmax) */
+    FL_SKIP_LIGHTS = 2,      /* any-hit ignores emitting triangles, like
+                                VisibilityEvaluator::unoccluded (illuminant.cpp:40-50) */
+    FL_COUNTERS = 4          /* count nodes/triangles visited (slower) */
+};
+
+/* ---- camera / render ---------------------------------------------------- */
+typedef struct {
+    float position[3], look_at[3], up[3];
+    float vfov_deg;
+    uint32_t width, height;
+} fl_camera;
+
+enum {
+    FL_SPECTRAL_HERO = 0,    /* 4 hero wavelengths, 360..830 nm, continuous spectra */
+    FL_SPECTRAL_PARITY = 1   /* 4 hero wavelengths, 400..700 nm, the reference's 60 bins */
+};
+
+enum {
+    FL_RENDER_GUIDED = 1,        /* use the learned vMF proposal (needs params.guide) */
+    FL_RENDER_COUNTERS = 2,      /* per-kernel counters */
+    FL_RENDER_VNDF_PDF = 4,      /* GGX: matched VNDF pdf instead of the reference's D*cos */
+    FL_RENDER_GRAPH = 8,         /* capture the bounce loop in a hipGraph */
+    FL_RENDER_TRAIN_GUIDE = 16,  /* train the guide online on this render's samples */
+    FL_RENDER_NO_SORT = 32,      /* disable shading sort */
+    FL_RENDER_TIMING = 64        /* per-kernel HIP-event timing (ms_* fields of fl_stats) */
+};
+
+typedef struct {
+    uint32_t spp;
+    int32_t diffuse_depth, reflection_depth, transmission_depth; /* RenderParams */
+    float rr_threshold;          /* PathIntegrator::rrThreshold (default 1) */
+    uint32_t rr_start;           /* RR when bounces > rr_start (reference: 3) */
+    uint32_t spectral_mode;      /* FL_SPECTRAL_* */
+    uint32_t flags;              /* FL_RENDER_* */
+    uint64_t seed;
+    uint32_t sample_offset;      /* progressive rendering: index of first sample */
+    uint32_t max_paths;          /* paths in flight per pass (0: auto) */
+    uint32_t stack_depth;        /* LDS short-stack entries (0: auto) */
+    uint32_t block_size;         /* traversal workgroup size (0: auto) */
+    fl_guide_t guide;            /* guiding network when FL_RENDER_GUIDED */
+    float guide_mix;             /* proposal probability of the guide (0.5) */
+    float guide_train_fraction;  /* with TRAIN_GUIDE: fraction of spp used to train */
+} fl_render_params;
+
+/* Film: per-pixel sums of linear CIE XYZ (3 floats per pixel) and of its
+ * square, over `spp` samples.  The pixel estimate is sum/spp. */
+typedef struct {
+    float *xyz_sum;              /* width*height*3 host floats (NULL: keep on device) */
+    float *xyz_sqsum;            /* width*height*3 or NULL */
+    uint32_t width, height;
+} fl_film;
+
+typedef struct {
+    double ms_total, ms_generate, ms_extend, ms_shade, ms_shadow, ms_resolve, ms_guide;
+    uint64_t samples, rays_extension, rays_shadow;
+    uint64_t nodes_visited, tris_tested;   /* with FL_RENDER_COUNTERS: extension rays */
+    uint64_t shadow_nodes_visited, shadow_tris_tested;
+    uint32_t launches_extend, launches_shade, launches_shadow, pad0;
+    uint64_t nan_samples, stack_spills;
+    uint32_t passes, max_bounce;
+    uint32_t paths_per_pass, stack_depth;
+} fl_stats;
+
+/* ---- context / scene ---------------------------------------------------- */
+int fl_ctx_create(int device, fl_ctx_t *out);
+void fl_ctx_destroy(fl_ctx_t ctx);
+const char *fl_last_error(fl_ctx_t ctx);
+int fl_abi_version(void);
+int fl_device_count(void);
+
+/* Copies the geometry, builds the 8-wide quantized LBVH on the device. */
+int fl_scene_create(fl_ctx_t ctx, const fl_scene_desc *desc, fl_scene_t *out);
+void fl_scene_destroy(fl_scene_t scene);
+int fl_scene_get_info(fl_scene_t scene, fl_scene_info *out);
+/* Copies BVH nodes (128 B each) and triangle records (48 B each) to host
+ * buffers sized from fl_scene_get_info (for structural checks). */
+int fl_scene_download_bvh(fl_scene_t scene, void *nodes, size_t node_bytes,
+                          void *tris, size_t tri_bytes);
+
+/* Closest/any hit for n host rays (replaces Scene::intersect/intersect_p). */
+int fl_intersect(fl_scene_t scene, const fl_ray *rays, fl_hit *hits, uint32_t n,
+                 uint32_t flags, uint64_t *counters /* [nodes, tris] or NULL */);
+/* Same on device-resident arrays, ordered on the context stream. */
+int fl_intersect_device(fl_scene_t scene, const fl_ray *d_rays, fl_hit *d_hits,
+                        uint32_t n, uint32_t flags);
+
+/* Renders `params->spp` samples per pixel (PathIntegrator::Li for every
+ * camera sample).  film may be NULL (result stays on the device). */
+int fl_render(fl_scene_t scene, const fl_camera *camera, const fl_render_params *params,
+              fl_film *film, fl_stats *stats);
+
+/* ---- batched rendering -------------------------------------------------- */
+typedef struct {
+    fl_scene_t scene;
+    fl_camera camera;
+    fl_film *film;
+    uint32_t spp;
+    uint32_t pad;
+} fl_batch_item;
+
+/* Renders many (scene, camera) pairs of uneven size in one persistent launch
+ * per kernel stage, with the bounce loop captured in a hipGraph. */
+int fl_render_batch(fl_ctx_t ctx, const fl_batch_item *items, uint32_t n,
+                    const fl_render_params *params, fl_stats *stats);
+
+/* ---- autotuner -------------------------------------------------------------- */
+typedef struct {
+    uint32_t max_paths, stack_depth, block_size, sort_shading;
+    double ms_best, ms_default;
+    uint32_t n_tried;
+} fl_tune_result;
+
+int fl_autotune(fl_scene_t scene, const fl_camera *camera, const fl_render_params *params,
+                fl_tune_result *out);
+
+/* ---- learned proposal (vMF mixture MLP) ----------------------------------- */
+typedef struct {
+    uint32_t hidden;        /* 64 */
+    uint32_t layers;        /* 4 hidden layers */
+    uint32_t lobes;         /* 8 vMF lobes */
+    uint32_t bands;         /* 8 frequency bands per position axis */
+    float lr;               /* Adam learning rate */
+    float beta1, beta2, eps;
+    uint32_t batch;         /* training batch (samples per step) */
+    uint64_t seed;
+    float scene_min[3], scene_max[3]; /* position normalisation box */
+} fl_guide_config;
+
+int fl_guide_create(fl_ctx_t ctx, const fl_guide_config *cfg, fl_guide_t *out);
+void fl_guide_destroy(fl_guide_t guide);
+/* One Adam step on n training records (device or host), returns the loss. */
+int fl_guide_train_step(fl_guide_t guide, const float *records, uint32_t n, int records_on_device,
+                        float *loss);
+/* Forward pass: inputs n x 64 floats (encoded), outputs n x 40 floats (raw head). */
+int fl_guide_forward(fl_guide_t guide, const float *inputs, float *outputs, uint32_t n);
+/* Parameter access (fp32 master weights, then Adam m, v), for checks/checkpoints. */
+int fl_guide_get_params(fl_guide_t guide, float *params, uint32_t n_params, uint32_t *n_out);
+int fl_guide_set_params(fl_guide_t guide, const float *params, uint32_t n_params);
+/* Gradient of the loss on n records w.r.t. all parameters (no update). */
+int fl_guide_gradient(fl_guide_t guide, const float *records, uint32_t n, float *grad,
+                      uint32_t n_params, float *loss);
+int fl_guide_stats(fl_guide_t guide, uint32_t *steps, float *last_loss, int *diverged);
+
+/* ---- BSDF queries (BxDF::evaluate/sample/pdf on the device) ---------------- */
+typedef struct {
+    int32_t kind;                 /* FL_MAT_* */
+    float alpha_x, alpha_y;       /* GGX */
+    float eta[4], k[4];           /* conductor eta/k per lane; dielectric: eta[0] = ksIOR */
+    float r[4], t[4];             /* reflectance / transmittance per lane */
+    float wo[3], wi[3];           /* local shading frame */
+    float u[2];
+    uint32_t flags;               /* FL_RENDER_VNDF_PDF */
+} fl_bsdf_query;
+
+typedef struct {
+    float f_eval[4], pdf_eval;    /* evaluate(wo, wi), pdf(wo, wi) */
+    float f_sample[4], pdf_sample;/* sample(wo, u) */
+    float wi_sample[3];
+    uint32_t sampled_type;        /* BxDFType bits */
+} fl_bsdf_result;
+
+int fl_bsdf_query_batch(fl_ctx_t ctx, const fl_bsdf_query *q, fl_bsdf_result *r, uint32_t n);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* FL_GPU_H */

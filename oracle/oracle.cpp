@@ -1,0 +1,1296 @@
+// oracle.cpp -- CPU oracle of the render path.  TEST INFRASTRUCTURE ONLY:
+// loaded by tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg,
+// never by the product library.
+//
+// Each function restates the reference routine cited beside it.  Repairs of
+// reference defects (so that a lit image exists at all) are marked "repair".
+#include "oracle.h"
+
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <limits>
+#include <memory>
+#include <thread>
+#include <vector>
+
+#include "../filianore-core_amd/data/fl_tables.inc"
+
+namespace {
+
+constexpr float INFINITI = std::numeric_limits<float>::infinity();
+constexpr float SHADOW_EPSILON = 0.0001f;               // mathutils.h:20
+constexpr float PI = 3.14159265358979323846f;
+constexpr float INV_PI = 0.31830988618379067154f;
+constexpr float PI_OVER_2 = 1.57079632679489661923f;
+constexpr float PI_OVER_4 = 0.78539816339744830961f;
+constexpr int NS = 60;                                   // principalspectrum.h:7-9
+constexpr int WL_START = 400, WL_END = 700;
+
+// ------------------------------------------------------------- vectors ----
+template <class T>
+struct V3 {
+    T x, y, z;
+    V3() : x(0), y(0), z(0) {}
+    V3(T a, T b, T c) : x(a), y(b), z(c) {}
+    T operator[](int i) const { return i == 0 ? x : (i == 1 ? y : z); }
+    T &operator[](int i) { return i == 0 ? x : (i == 1 ? y : z); }
+    V3 operator+(const V3 &o) const { return V3(x + o.x, y + o.y, z + o.z); }
+    V3 operator-(const V3 &o) const { return V3(x - o.x, y - o.y, z - o.z); }
+    V3 operator-() const { return V3(-x, -y, -z); }
+    V3 operator*(T s) const { return V3(x * s, y * s, z * s); }
+    V3 operator/(T s) const { return V3(x / s, y / s, z / s); }
+    T length_squared() const { return x * x + y * y + z * z; }
+    T length() const { return std::sqrt(length_squared()); }
+};
+using V3f = V3<float>;
+
+template <class T> T dot(const V3<T> &a, const V3<T> &b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+template <class T> T abs_dot(const V3<T> &a, const V3<T> &b) { return std::abs(dot(a, b)); }
+template <class T> V3<T> vabs(const V3<T> &a) { return V3<T>(std::abs(a.x), std::abs(a.y), std::abs(a.z)); }
+// cross evaluated in double, geometry.h:213-219
+template <class T> V3<T> cross(const V3<T> &a, const V3<T> &b) {
+    double ax = a.x, ay = a.y, az = a.z, bx = b.x, by = b.y, bz = b.z;
+    return V3<T>((T)((ay * bz) - (az * by)), (T)((az * bx) - (ax * bz)), (T)((ax * by) - (ay * bx)));
+}
+template <class T> V3<T> normalize(const V3<T> &v) { return v / v.length(); }
+template <class T> V3<T> face_forward(const V3<T> &v, const V3<T> &w) { return (dot(v, w) < T(0)) ? -v : v; }
+template <class T> V3<T> reflect(const V3<T> &a, const V3<T> &b) { return b * T(2) * dot(a, b) - a; }
+template <class T> bool refract(const V3<T> &wi, const V3<T> &n, T eta, V3<T> *wt) {
+    T cosI = dot(wi, n);
+    T sinI2 = std::max(T(0), T(1) - (cosI * cosI));
+    T sinT2 = eta * eta * sinI2;
+    if (sinT2 >= 1) return false;
+    T cosT = std::sqrt(1 - sinT2);
+    *wt = wi * -eta + n * (eta * cosI - cosT);
+    return true;
+}
+template <class T> void coordinate_system(const V3<T> &v1, V3<T> *v2, V3<T> *v3) {
+    if (std::abs(v1.x) > std::abs(v1.y)) {
+        T invLen = T(1) / std::sqrt(v1.x * v1.x + v1.z * v1.z);
+        *v2 = V3<T>(v1.z * invLen, 0, -v1.x * invLen);
+    } else {
+        T invLen = T(1) / std::sqrt(v1.y * v1.y + v1.z * v1.z);
+        *v2 = V3<T>(0, v1.z * invLen, -v1.y * invLen);
+    }
+    *v3 = cross(v1, *v2);
+}
+template <class T> T clampv(T v, T lo, T hi) { return v < lo ? lo : (v > hi ? hi : v); }
+
+inline uint32_t f2u(float f) { uint32_t u; memcpy(&u, &f, 4); return u; }
+inline float u2f(uint32_t u) { float f; memcpy(&f, &u, 4); return f; }
+float next_float_up(float v) {                          // mathutils.h:97-110
+    if (std::isinf(v) && v > 0.) return v;
+    if (v == -0.f) v = 0.f;
+    uint32_t ui = f2u(v);
+    if (v >= 0) ++ui; else --ui;
+    return u2f(ui);
+}
+float next_float_down(float v) {                        // mathutils.h:112-124
+    if (std::isinf(v) && v < 0.) return v;
+    if (v == 0.f) v = -0.f;
+    uint32_t ui = f2u(v);
+    if (v > 0) --ui; else ++ui;
+    return u2f(ui);
+}
+float gamma5() {                                         // gamma<float>(5), mathutils.h:40-48
+    float eps = std::numeric_limits<float>::epsilon() * 0.5f;
+    return (5 * eps) / (1.f - 5 * eps);
+}
+V3f offset_point_origin(const V3f &p, const V3f &pError, const V3f &n, const V3f &w) {   // interaction.h:11-31
+    float d = dot(vabs(n), pError);
+    V3f offset = n * d;
+    if (dot(w, n) < 0) offset = -offset;
+    V3f po = p + offset;
+    for (int i = 0; i < 3; ++i) {
+        if (offset[i] > 0) po[i] = next_float_up(po[i]);
+        else if (offset[i] < 0) po[i] = next_float_down(po[i]);
+    }
+    return po;
+}
+
+// ------------------------------------------------------------ spectra -----
+template <class T, int N>
+struct Spec {
+    T c[N];
+    Spec() { for (int i = 0; i < N; ++i) c[i] = T(0); }
+    explicit Spec(T v) { for (int i = 0; i < N; ++i) c[i] = v; }
+    Spec &operator+=(const Spec &o) { for (int i = 0; i < N; ++i) c[i] += o.c[i]; return *this; }
+    Spec operator+(const Spec &o) const { Spec r = *this; for (int i = 0; i < N; ++i) r.c[i] += o.c[i]; return r; }
+    Spec operator-(const Spec &o) const { Spec r = *this; for (int i = 0; i < N; ++i) r.c[i] -= o.c[i]; return r; }
+    Spec operator*(const Spec &o) const { Spec r = *this; for (int i = 0; i < N; ++i) r.c[i] *= o.c[i]; return r; }
+    Spec operator/(const Spec &o) const { Spec r = *this; for (int i = 0; i < N; ++i) r.c[i] /= o.c[i]; return r; }
+    Spec &operator*=(const Spec &o) { for (int i = 0; i < N; ++i) c[i] *= o.c[i]; return *this; }
+    Spec operator*(T a) const { Spec r = *this; for (int i = 0; i < N; ++i) r.c[i] *= a; return r; }
+    Spec &operator*=(T a) { for (int i = 0; i < N; ++i) c[i] *= a; return *this; }
+    Spec operator/(T a) const { Spec r = *this; for (int i = 0; i < N; ++i) r.c[i] /= a; return r; }
+    Spec &operator/=(T a) { for (int i = 0; i < N; ++i) c[i] /= a; return *this; }
+    friend Spec operator*(T a, const Spec &s) { return s * a; }
+    Spec sqrt_() const { Spec r; for (int i = 0; i < N; ++i) r.c[i] = std::sqrt(c[i]); return r; }
+    bool is_black() const { for (int i = 0; i < N; ++i) if (c[i] != 0.) return false; return true; }
+    Spec clamp0() const { Spec r; for (int i = 0; i < N; ++i) r.c[i] = clampv<T>(c[i], T(0), std::numeric_limits<T>::infinity()); return r; }
+    T max_component() const { T m = c[0]; for (int i = 1; i < N; ++i) m = std::max(m, c[i]); return m; }
+};
+using S60 = Spec<float, NS>;
+
+float lerpf(float t, float v1, float v2) { return (1.f - t) * v1 + t * v2; }   // mathutils.h:30-33
+
+// spectrumoperations.cpp:28-56
+float average_spectrum_samples(const float *wavelength, const float *vals, int n, float l0, float l1) {
+    if (l1 <= wavelength[0]) return vals[0];
+    if (l0 >= wavelength[n - 1]) return vals[n - 1];
+    if (n == 1) return vals[0];
+    float sum = 0;
+    if (l0 < wavelength[0]) sum += vals[0] * (wavelength[0] - l0);
+    if (l1 > wavelength[n - 1]) sum += vals[n - 1] * (l1 - wavelength[n - 1]);
+    int i = 0;
+    while (l0 > wavelength[i + 1]) ++i;
+    auto interp = [wavelength, vals](float w, int j) {
+        return lerpf((w - wavelength[j]) / (wavelength[j + 1] - wavelength[j]), vals[j], vals[j + 1]);
+    };
+    for (; i + 1 < n && l1 >= wavelength[i]; ++i) {
+        float a = std::max(l0, wavelength[i]);
+        float b = std::min(l1, wavelength[i + 1]);
+        sum += 0.5 * (interp(a, i) + interp(b, i)) * (b - a);
+    }
+    return sum / (l1 - l0);
+}
+
+void bin_range(int i, float *a, float *b) {
+    *a = lerpf(float(i) / float(NS), WL_START, WL_END);
+    *b = lerpf(float(i + 1) / float(NS), WL_START, WL_END);
+}
+
+S60 from_spd(const float *lam, const float *val, int n) {   // spectrumoperations.cpp:88-103 (sorted input)
+    S60 r;
+    for (int i = 0; i < NS; ++i) {
+        float a, b;
+        bin_range(i, &a, &b);
+        r.c[i] = average_spectrum_samples(lam, val, n, a, b);
+    }
+    return r;
+}
+
+struct Tables {
+    S60 X, Y, Z;
+    S60 refl[7];                // white cyan magenta yellow red green blue
+    S60 eta[FL_N_METALS], k[FL_N_METALS];
+    Tables() {                  // InitializeSpectrals / InitializeForge
+        float lam[FL_CIE_N];
+        for (int i = 0; i < FL_CIE_N; ++i) lam[i] = 360.f + (float)i;
+        X = from_spd(lam, FL_CIE_X, FL_CIE_N);
+        Y = from_spd(lam, FL_CIE_Y, FL_CIE_N);
+        Z = from_spd(lam, FL_CIE_Z, FL_CIE_N);
+        const float *b[7] = {FL_RGBREFL_WHITE, FL_RGBREFL_CYAN, FL_RGBREFL_MAGENTA, FL_RGBREFL_YELLOW,
+                             FL_RGBREFL_RED, FL_RGBREFL_GREEN, FL_RGBREFL_BLUE};
+        for (int j = 0; j < 7; ++j) refl[j] = from_spd(FL_RGB2SPEC_LAMBDA, b[j], FL_RGB2SPEC_N);
+        for (int m = 0; m < FL_N_METALS; ++m) {
+            eta[m] = from_spd(FL_METAL_LAMBDA[m], FL_METAL_ETA[m], FL_METAL_COUNT[m]);
+            k[m] = from_spd(FL_METAL_LAMBDA[m], FL_METAL_K[m], FL_METAL_COUNT[m]);
+        }
+    }
+};
+const Tables &tables() {
+    static Tables t;
+    return t;
+}
+
+S60 from_reflectance_rgb(const float rgb[3]) {          // spectrumoperations.cpp:113-147
+    const Tables &T = tables();
+    const S60 &W = T.refl[0], &Cy = T.refl[1], &M = T.refl[2], &Ye = T.refl[3], &R = T.refl[4], &G = T.refl[5],
+              &B = T.refl[6];
+    S60 r(0.f);
+    if (rgb[0] <= rgb[1] && rgb[0] <= rgb[2]) {
+        r += rgb[0] * W;
+        if (rgb[1] <= rgb[2]) { r += (rgb[1] - rgb[0]) * Cy; r += (rgb[2] - rgb[1]) * B; }
+        else { r += (rgb[2] - rgb[0]) * Cy; r += (rgb[1] - rgb[2]) * G; }
+    } else if (rgb[1] <= rgb[0] && rgb[1] <= rgb[2]) {
+        r += rgb[1] * W;
+        if (rgb[0] <= rgb[2]) { r += (rgb[0] - rgb[1]) * M; r += (rgb[2] - rgb[0]) * B; }
+        else { r += (rgb[2] - rgb[1]) * M; r += (rgb[0] - rgb[2]) * R; }
+    } else {
+        r += rgb[2] * W;
+        if (rgb[0] <= rgb[1]) { r += (rgb[0] - rgb[2]) * Ye; r += (rgb[1] - rgb[0]) * G; }
+        else { r += (rgb[1] - rgb[2]) * Ye; r += (rgb[0] - rgb[1]) * R; }
+    }
+    r *= (float).94;
+    return r.clamp0();
+}
+
+void to_xyz(const S60 &r, float xyz[3]) {                 // spectrumoperations.cpp:193-207
+    const Tables &T = tables();
+    float x = 0, y = 0, z = 0;
+    for (int i = 0; i < NS; ++i) {
+        x += T.X.c[i] * r.c[i];
+        y += T.Y.c[i] * r.c[i];
+        z += T.Z.c[i] * r.c[i];
+    }
+    float scale = float(WL_END - WL_START) / float(FL_CIE_Y_INTEGRAL * NS);
+    xyz[0] = x * scale; xyz[1] = y * scale; xyz[2] = z * scale;
+}
+
+// ----------------------------------------------------------- BxDFs -------
+enum : uint32_t { BSDF_REFLECTION = 1, BSDF_TRANSMISSION = 2, BSDF_DIFFUSE = 4, BSDF_GLOSSY = 8, BSDF_SPECULAR = 16,
+                  BSDF_ALL = 31 };
+
+template <class T> T cos2t(const V3<T> &w) { return w.z * w.z; }
+template <class T> T sin2t(const V3<T> &w) { return std::max(T(0), T(1) - cos2t(w)); }
+template <class T> T sint(const V3<T> &w) { return std::sqrt(sin2t(w)); }
+template <class T> T tant(const V3<T> &w) { return sint(w) / w.z; }
+template <class T> T tan2t(const V3<T> &w) { return sin2t(w) / cos2t(w); }
+template <class T> T cosphi(const V3<T> &w) { T s = sint(w); return (s == 0) ? T(1) : clampv<T>(w.x / s, -1, 1); }
+template <class T> T sinphi(const V3<T> &w) { T s = sint(w); return (s == 0) ? T(0) : clampv<T>(w.y / s, -1, 1); }
+template <class T> T cos2phi(const V3<T> &w) { return cosphi(w) * cosphi(w); }
+template <class T> T sin2phi(const V3<T> &w) { return sinphi(w) * sinphi(w); }
+template <class T> bool same_hemisphere(const V3<T> &a, const V3<T> &b) { return (a.z * b.z) > 0; }
+
+template <class T> V3<T> cosine_hemisphere_sample(T u0, T u1) {   // shadingcore.h:65-89
+    T ox = u0 * T(2) - T(1), oy = u1 * T(2) - T(1);
+    T dx = 0, dy = 0;
+    if (!(ox == 0 && oy == 0)) {
+        T theta, r;
+        if (std::abs(ox) > std::abs(oy)) { r = ox; theta = T(PI_OVER_4) * (oy / ox); }
+        else { r = oy; theta = T(PI_OVER_2) - T(PI_OVER_4) * (ox / oy); }
+        dx = std::cos(theta) * r;
+        dy = std::sin(theta) * r;
+    }
+    T z = std::sqrt(std::max(T(0), T(1) - dx * dx - dy * dy));
+    return V3<T>(dx, dy, z);
+}
+
+template <class T>
+struct GGXd {                                            // ggx.cpp
+    T ax, ay;
+    T D(const V3<T> &wh) const {
+        T t2 = tan2t(wh);
+        if (std::isinf(t2)) return 0.;
+        const T c4 = cos2t(wh) * cos2t(wh);
+        T e = (cos2phi(wh) / (ax * ax) + sin2phi(wh) / (ay * ay)) * t2;
+        return T(1) / (T(PI) * ax * ay * c4 * (1 + e) * (1 + e));
+    }
+    T lambda(const V3<T> &w) const {
+        T at = std::abs(tant(w));
+        if (std::isinf(at)) return 0.;
+        T alpha = std::sqrt(cos2phi(w) * ax * ax + sin2phi(w) * ay * ay);
+        T a2 = (alpha * at) * (alpha * at);
+        return (-1 + std::sqrt(T(1) + a2)) / 2;
+    }
+    T G(const V3<T> &wo, const V3<T> &wi) const { return 1 / (1 + lambda(wo) + lambda(wi)); }
+    T G1(const V3<T> &w) const { return 1 / (1 + lambda(w)); }
+    T pdf(const V3<T> &wo, const V3<T> &wh, bool vndf) const {   // microfacet.cpp:14-16
+        if (!vndf) return D(wh) * std::abs(wh.z);
+        return D(wh) * G1(wo) * std::abs(dot(wo, wh)) / std::abs(wo.z);
+    }
+    V3<T> sample_wh(const V3<T> &wo, T u0, T u1) const {      // ggx.cpp:32-57
+        V3<T> vh = normalize(V3<T>(wo.x * ax, wo.y * ay, wo.z));
+        if (vh.z < 0) vh = -vh;
+        T lensq = vh.x * vh.x + vh.y * vh.y;
+        V3<T> T1 = lensq > 0 ? V3<T>(-vh.y, vh.x, T(0)) / std::sqrt(lensq) : V3<T>(1, 0, 0);
+        V3<T> T2 = cross(vh, T1);
+        T r = std::sqrt(u0);
+        T phi = T(2) * T(PI) * u1;
+        T t1 = r * std::cos(phi);
+        T t2 = r * std::sin(phi);
+        T s = (T(1) + vh.z) * T(0.5);
+        t2 = (T(1) - s) * std::sqrt(1 - (t1 * t1)) + (s * t2);
+        V3<T> Nh = T1 * t1 + T2 * t2 + vh * (std::sqrt(std::max(T(0), T(1) - t1 * t1 - t2 * t2)));
+        return normalize(V3<T>(ax * Nh.x, ay * Nh.y, std::max<T>(T(1e-6f), Nh.z)));
+    }
+};
+
+template <class T, int N>
+Spec<T, N> fresnel_conductor(T cosI, const Spec<T, N> &etaT, const Spec<T, N> &kk) {   // fresnelconductor.cpp:9-33
+    cosI = std::abs(cosI);
+    cosI = clampv<T>(cosI, -1, 1);
+    Spec<T, N> etaI(T(1));
+    Spec<T, N> eta = etaT / etaI;
+    Spec<T, N> etak = kk / etaI;
+    T c2 = cosI * cosI;
+    T s2 = 1. - c2;
+    Spec<T, N> eta2 = eta * eta, etak2 = etak * etak;
+    Spec<T, N> t0 = eta2 - etak2 - Spec<T, N>(s2);
+    Spec<T, N> a2b2 = (t0 * t0 + T(4) * eta2 * etak2).sqrt_();
+    Spec<T, N> t1 = a2b2 + Spec<T, N>(c2);
+    Spec<T, N> a = (T(0.5) * (a2b2 + t0)).sqrt_();
+    Spec<T, N> t2 = (T)2 * cosI * a;
+    Spec<T, N> Rs = (t1 - t2) / (t1 + t2);
+    Spec<T, N> t3 = c2 * a2b2 + Spec<T, N>(s2 * s2);
+    Spec<T, N> t4 = t2 * s2;
+    Spec<T, N> Rp = Rs * (t3 - t4) / (t3 + t4);
+    return T(0.5) * (Rp + Rs);
+}
+
+template <class T>
+T fresnel_dielectric(T cosThetaI, T etaI, T etaT) {     // fresneldielectric.cpp:8-38 (swap kept local: repair R8)
+    cosThetaI = clampv<T>(cosThetaI, -1, 1);
+    bool enter = cosThetaI > 0.0;
+    if (!enter) { std::swap(etaI, etaT); cosThetaI = std::abs(cosThetaI); }
+    T sinThetaI = std::sqrt(std::max(T(0), T(1) - cosThetaI * cosThetaI));
+    T sinThetaT = etaI / etaT * sinThetaI;
+    if (sinThetaT >= 1) return T(1);
+    T cosThetaT = std::sqrt(std::max(T(0), 1 - sinThetaT * sinThetaT));
+    T Rparl = (etaT * cosThetaI - etaI * cosThetaT) / (etaT * cosThetaI + etaI * cosThetaT);
+    T Rperp = (etaI * cosThetaI - etaT * cosThetaT) / (etaI * cosThetaI + etaT * cosThetaT);
+    return (Rparl * Rparl + Rperp * Rperp) * T(0.5);
+}
+
+// One-lobe BSDF (every supported material has one lobe).
+template <class T, int N>
+struct Bxdf {
+    int kind = 0;               // FL_MAT_*
+    GGXd<T> ggx;
+    T weight = 1;
+    Spec<T, N> R, Tt, eta, k;
+    T ior = 1;
+    bool vndf = false;
+    uint32_t type() const {
+        switch (kind) {
+        case FL_MAT_LAMBERT: return BSDF_REFLECTION | BSDF_DIFFUSE;
+        case FL_MAT_CONDUCTOR: return BSDF_REFLECTION | BSDF_GLOSSY;
+        case FL_MAT_DIELECTRIC: return BSDF_REFLECTION | BSDF_TRANSMISSION | BSDF_SPECULAR;
+        }
+        return 0;
+    }
+    bool matches(uint32_t flags) const { return (type() & flags) == type(); }
+    Spec<T, N> f(const V3<T> &wo, const V3<T> &wi) const {
+        if (kind == FL_MAT_LAMBERT) return R * T(INV_PI) * T(1);             // lambert.cpp:6-8
+        if (kind == FL_MAT_CONDUCTOR) {                                      // microfacetreflection.cpp:13-32
+            T cO = std::abs(wo.z), cI = std::abs(wi.z);
+            V3<T> wh = wi + wo;
+            if (cI == 0 || cO == 0) return Spec<T, N>(0);
+            if (wh.x == 0 && wh.y == 0 && wh.z == 0) return Spec<T, N>(0);
+            wh = normalize(wh);
+            Spec<T, N> F = fresnel_conductor<T, N>(dot(wi, face_forward(wh, V3<T>(0, 0, 1))), eta, k);
+            return Spec<T, N>(1) * weight * ggx.D(wh) * ggx.G(wo, wi) * F / (T(4) * cO * cI);
+        }
+        return Spec<T, N>(0);                                               // fresnelspecular.cpp:13-15
+    }
+    T pdf(const V3<T> &wo, const V3<T> &wi) const {
+        if (kind == FL_MAT_LAMBERT) return same_hemisphere(wo, wi) ? std::abs(wi.z) * T(INV_PI) : T(0);
+        if (kind == FL_MAT_CONDUCTOR) {                                      // microfacetreflection.cpp:55-61
+            if (!same_hemisphere(wo, wi)) return 0.f;
+            V3<T> wh = normalize(wo + wi);
+            return ggx.pdf(wo, wh, vndf) / (T(4) * dot(wo, wh));
+        }
+        return 0.f;
+    }
+    Spec<T, N> sample(const V3<T> &wo, V3<T> *wi, T u0, T u1, T *pdf_, uint32_t *st) const {
+        if (kind == FL_MAT_LAMBERT) {                                        // lambert.cpp:10-20
+            *wi = cosine_hemisphere_sample(u0, u1);
+            if (wo.z < 0) wi->z *= -1.f;
+            *pdf_ = pdf(wo, *wi);
+            *st = type();
+            return f(wo, *wi);
+        }
+        if (kind == FL_MAT_CONDUCTOR) {                                      // microfacetreflection.cpp:34-53
+            if (wo.z == 0) return Spec<T, N>(0);
+            V3<T> wh = ggx.sample_wh(wo, u0, u1);
+            if (dot(wo, wh) < 0) return Spec<T, N>(0);
+            *wi = reflect(wo, wh);
+            if (!same_hemisphere(wo, *wi)) return Spec<T, N>(0);
+            *pdf_ = ggx.pdf(wo, wh, vndf) / (T(4) * dot(wo, wh));
+            return f(wo, *wi);
+        }
+        if (kind == FL_MAT_DIELECTRIC) {                                     // fresnelspecular.cpp:21-48
+            T F = fresnel_dielectric<T>(wo.z, T(1), ior);
+            if (u0 < F) {
+                *wi = V3<T>(-wo.x, -wo.y, wo.z);
+                *st = BSDF_SPECULAR | BSDF_REFLECTION;
+                *pdf_ = F;
+                return R * F / std::abs(wi->z);
+            }
+            bool entering = wo.z > 0;
+            T etaI = entering ? T(1) : ior;
+            T etaT = entering ? ior : T(1);
+            if (!refract(wo, face_forward(V3<T>(0, 0, 1), wo), etaI / etaT, wi)) return Spec<T, N>(0);
+            Spec<T, N> ft = Tt * (T(1) - F);
+            ft *= (etaI * etaI) / (etaT * etaT);
+            *st = BSDF_SPECULAR | BSDF_TRANSMISSION;
+            *pdf_ = T(1) - F;
+            return ft / std::abs(wi->z);
+        }
+        return Spec<T, N>(0);
+    }
+};
+
+// BSDF (bsdf.cpp) over one lobe with frame (ng, ns, s, t)
+template <class T, int N>
+struct Bsdf {
+    Bxdf<T, N> lobe;
+    bool has = false;
+    T eta = 1;
+    V3<T> ng, ns, s, t;
+    void frame_identity() {                             // local frame == world frame (BxDF queries)
+        ng = ns = V3<T>(0, 0, 1);
+        s = V3<T>(1, 0, 0);
+        t = V3<T>(0, 1, 0);
+    }
+    void frame(const V3<T> &ng_, const V3<T> &ns_) {   // bsdf.cpp:6-10
+        ng = ng_; ns = ns_;
+        coordinate_system(ns, &s, &t);
+        s = normalize(s);
+    }
+    V3<T> to_local(const V3<T> &v) const { return V3<T>(dot(v, s), dot(v, t), dot(v, ns)); }
+    V3<T> to_world(const V3<T> &v) const { return (s * v.x + t * v.y + ns * v.z); }
+    int num_components(uint32_t flags) const { return (has && lobe.matches(flags)) ? 1 : 0; }
+    Spec<T, N> evaluate(const V3<T> &woW, const V3<T> &wiW, uint32_t flags) const {   // bsdf.cpp:30-50
+        V3<T> wi = to_local(wiW), wo = to_local(woW);
+        if (wo.z == 0) return Spec<T, N>(0);
+        bool refl = dot(wiW, ng) * dot(woW, ng) > 0;
+        Spec<T, N> total(0);
+        if (has && lobe.matches(flags) && ((refl && (lobe.type() & BSDF_REFLECTION)) ||
+                                           (!refl && (lobe.type() & BSDF_TRANSMISSION))))
+            total += lobe.f(wo, wi);
+        return total;
+    }
+    Spec<T, N> sample(const V3<T> &woW, V3<T> *wiW, T u0, T u1, T *pdf, uint32_t flags, uint32_t *st) const {
+        int match = num_components(flags);                                   // bsdf.cpp:52-125
+        if (match == 0) { *pdf = 0; *st = 0; return Spec<T, N>(0); }
+        int comp = std::min((int)std::floor(u0 * match), match - 1);
+        T ur = std::min(u0 * match - comp, T(1) - T(SHADOW_EPSILON));
+        V3<T> wi, wo = to_local(woW);
+        if (wo.z == 0) { *pdf = 0; *st = 0; return Spec<T, N>(0); }
+        *pdf = 0;
+        *st = lobe.type();
+        Spec<T, N> f = lobe.sample(wo, &wi, ur, u1, pdf, st);
+        if (*pdf == 0) { *st = 0; return Spec<T, N>(0); }
+        *wiW = to_world(wi);
+        if (!(lobe.type() & BSDF_SPECULAR)) {
+            bool refl = dot(*wiW, ng) * dot(woW, ng) > 0;
+            f = Spec<T, N>(0);
+            if ((refl && (lobe.type() & BSDF_REFLECTION)) || (!refl && (lobe.type() & BSDF_TRANSMISSION)))
+                f += lobe.f(wo, wi);
+        }
+        return f;
+    }
+    T pdf(const V3<T> &woW, const V3<T> &wiW, uint32_t flags) const {       // bsdf.cpp:127-151
+        if (!has) return 0.f;
+        V3<T> wi = to_local(wiW), wo = to_local(woW);
+        if (wo.z == 0) return 0.f;
+        T p = 0;
+        int m = 0;
+        if (lobe.matches(flags)) { ++m; p += lobe.pdf(wo, wi); }
+        return m > 0 ? p / m : T(0);
+    }
+};
+
+// ---------------------------------------------- watertight triangle test --
+struct RayPre { int kx, ky, kz; float Sx, Sy, Sz; };
+RayPre ray_pre(const V3f &d) {
+    RayPre r;
+    float ax = std::fabs(d.x), ay = std::fabs(d.y), az = std::fabs(d.z);
+    int kz = (ax > ay) ? ((ax > az) ? 0 : 2) : ((ay > az) ? 1 : 2);
+    int kx = kz + 1; if (kx == 3) kx = 0;
+    int ky = kx + 1; if (ky == 3) ky = 0;
+    float dkz = d[kz];
+    if (dkz < 0.f) std::swap(kx, ky);
+    r.kx = kx; r.ky = ky; r.kz = kz;
+    r.Sx = d[kx] / dkz;
+    r.Sy = d[ky] / dkz;
+    r.Sz = 1.0f / dkz;
+    return r;
+}
+// Woop, Benthin, Wald 2013; hit interval (tmin, tmax]; u,v weights of v1,v2
+bool tri_watertight(const RayPre &rp, const V3f &o, const V3f &p0, const V3f &p1, const V3f &p2, float tmin,
+                    float tmax, float *to, float *uo, float *vo) {
+    V3f A = p0 - o, B = p1 - o, Cc = p2 - o;
+    float Ax = A[rp.kx] - rp.Sx * A[rp.kz], Ay = A[rp.ky] - rp.Sy * A[rp.kz];
+    float Bx = B[rp.kx] - rp.Sx * B[rp.kz], By = B[rp.ky] - rp.Sy * B[rp.kz];
+    float Cx = Cc[rp.kx] - rp.Sx * Cc[rp.kz], Cy = Cc[rp.ky] - rp.Sy * Cc[rp.kz];
+    float U = Cx * By - Cy * Bx;
+    float V = Ax * Cy - Ay * Cx;
+    float W = Bx * Ay - By * Ax;
+    if (U == 0.f || V == 0.f || W == 0.f) {
+        U = (float)((double)Cx * (double)By - (double)Cy * (double)Bx);
+        V = (float)((double)Ax * (double)Cy - (double)Ay * (double)Cx);
+        W = (float)((double)Bx * (double)Ay - (double)By * (double)Ax);
+    }
+    if ((U < 0.f || V < 0.f || W < 0.f) && (U > 0.f || V > 0.f || W > 0.f)) return false;
+    float det = U + V + W;
+    if (det == 0.f) return false;
+    float Az = rp.Sz * A[rp.kz], Bz = rp.Sz * B[rp.kz], Cz = rp.Sz * Cc[rp.kz];
+    float Tn = U * Az + V * Bz + W * Cz;
+    float inv = 1.0f / det;
+    float t = Tn * inv;
+    if (!(t > tmin && t <= tmax)) return false;
+    *to = t;
+    *uo = V * inv;
+    *vo = W * inv;
+    return true;
+}
+
+// Triangle::intersect of the reference (used only by Shape::pdf), triangle.cpp:13-124
+bool tri_reference(const V3f &ro, const V3f &rd, float tMin, float tMax, const V3f &v1, const V3f &v2,
+                   const V3f &v3, float *t_out, V3f *p_out, V3f *n_out) {
+    V3f e1 = v2 - v1, e2 = v3 - v1;
+    V3f n = cross(e1, e2);
+    float det = -dot(rd, n);
+    V3f ao = ro - v1;
+    V3f dao = cross(ao, rd);
+    float invDet = 1.f / det;
+    float alpha = dot(e2, dao) * invDet;
+    float beta = -dot(e1, dao) * invDet;
+    float t = dot(ao, n) * invDet;
+    if ((t < tMax && t > tMin) && alpha > 0 && beta > 0 && (alpha + beta) < 1.f) {
+        if (n.length_squared() == 0) return false;    // degenerate uv branch (uv all zero)
+        *t_out = t;
+        *p_out = ro + rd * t;
+        V3f dp02 = v1 - v3, dp12 = v2 - v3;
+        *n_out = normalize(cross(dp02, dp12));
+        return true;
+    }
+    return false;
+}
+
+// ------------------------------------------------------------- scene -----
+struct BNode {
+    float lo[3], hi[3];
+    uint32_t left_or_first;     // internal: left child index (right = left+1)? no: explicit
+    uint32_t right;
+    uint32_t count;             // >0: leaf with `count` prims from left_or_first
+};
+
+struct Light {
+    V3f p0, p1, p2;
+    float area;
+    S60 color;
+    float intensity;
+    uint32_t tri;
+};
+
+struct Material {
+    int kind;
+    S60 R, T;
+    S60 eta, k;
+    float ax, ay, weight, ior;
+};
+
+} // namespace
+
+struct or_scene_s {
+    std::vector<V3f> v0, v1, v2;      // per triangle (flattened ID order)
+    std::vector<uint32_t> mat, light; // per triangle: material, light+1
+    std::vector<Material> mats;
+    std::vector<Light> lights;
+    std::vector<BNode> nodes;
+    std::vector<uint32_t> prims;
+};
+
+namespace {
+
+// binned SAH build (12 bins), leaves <= 4 triangles
+void build_bvh(or_scene_s *s) {
+    const uint32_t n = (uint32_t)s->v0.size();
+    s->prims.resize(n);
+    for (uint32_t i = 0; i < n; ++i) s->prims[i] = i;
+    std::vector<float> blo(3 * (size_t)n), bhi(3 * (size_t)n), cen(3 * (size_t)n);
+    for (uint32_t i = 0; i < n; ++i)
+        for (int k = 0; k < 3; ++k) {
+            float a = s->v0[i][k], b = s->v1[i][k], c = s->v2[i][k];
+            blo[3 * i + k] = std::min(a, std::min(b, c));
+            bhi[3 * i + k] = std::max(a, std::max(b, c));
+            cen[3 * i + k] = 0.5f * blo[3 * i + k] + 0.5f * bhi[3 * i + k];
+        }
+    struct Task { uint32_t node, begin, end; };
+    s->nodes.clear();
+    s->nodes.reserve(2 * (size_t)n + 1);
+    s->nodes.push_back(BNode{});
+    std::vector<Task> stack;
+    stack.push_back({0, 0, n});
+    while (!stack.empty()) {
+        Task tk = stack.back();
+        stack.pop_back();
+        BNode nd;
+        float clo[3] = {INFINITI, INFINITI, INFINITI}, chi[3] = {-INFINITI, -INFINITI, -INFINITI};
+        for (int k = 0; k < 3; ++k) { nd.lo[k] = INFINITI; nd.hi[k] = -INFINITI; }
+        for (uint32_t i = tk.begin; i < tk.end; ++i) {
+            uint32_t p = s->prims[i];
+            for (int k = 0; k < 3; ++k) {
+                nd.lo[k] = std::min(nd.lo[k], blo[3 * p + k]);
+                nd.hi[k] = std::max(nd.hi[k], bhi[3 * p + k]);
+                clo[k] = std::min(clo[k], cen[3 * p + k]);
+                chi[k] = std::max(chi[k], cen[3 * p + k]);
+            }
+        }
+        uint32_t cnt = tk.end - tk.begin;
+        int axis = -1;
+        uint32_t mid = 0;
+        if (cnt > 4) {
+            const int NB = 12;
+            float best = INFINITI;
+            int best_axis = -1, best_bin = -1;
+            for (int k = 0; k < 3; ++k) {
+                float ext = chi[k] - clo[k];
+                if (!(ext > 0.f)) continue;
+                uint32_t bc[NB] = {0};
+                float bl[NB][3], bh[NB][3];
+                for (int b = 0; b < NB; ++b)
+                    for (int q = 0; q < 3; ++q) { bl[b][q] = INFINITI; bh[b][q] = -INFINITI; }
+                for (uint32_t i = tk.begin; i < tk.end; ++i) {
+                    uint32_t p = s->prims[i];
+                    int b = (int)((cen[3 * p + k] - clo[k]) / ext * NB);
+                    b = std::min(std::max(b, 0), NB - 1);
+                    bc[b]++;
+                    for (int q = 0; q < 3; ++q) {
+                        bl[b][q] = std::min(bl[b][q], blo[3 * p + q]);
+                        bh[b][q] = std::max(bh[b][q], bhi[3 * p + q]);
+                    }
+                }
+                auto area = [](const float *l, const float *h) {
+                    float dx = h[0] - l[0], dy = h[1] - l[1], dz = h[2] - l[2];
+                    return (dx < 0) ? 0.f : 2.f * (dx * dy + dy * dz + dz * dx);
+                };
+                for (int split = 1; split < NB; ++split) {
+                    float l0[3] = {INFINITI, INFINITI, INFINITI}, h0[3] = {-INFINITI, -INFINITI, -INFINITI};
+                    float l1[3] = {INFINITI, INFINITI, INFINITI}, h1[3] = {-INFINITI, -INFINITI, -INFINITI};
+                    uint32_t c0 = 0, c1 = 0;
+                    for (int b = 0; b < NB; ++b) {
+                        float *L = b < split ? l0 : l1, *H = b < split ? h0 : h1;
+                        if (b < split) c0 += bc[b]; else c1 += bc[b];
+                        for (int q = 0; q < 3; ++q) { L[q] = std::min(L[q], bl[b][q]); H[q] = std::max(H[q], bh[b][q]); }
+                    }
+                    if (!c0 || !c1) continue;
+                    float cost = area(l0, h0) * c0 + area(l1, h1) * c1;
+                    if (cost < best) { best = cost; best_axis = k; best_bin = split; }
+                }
+            }
+            if (best_axis >= 0) {
+                axis = best_axis;
+                float ext = chi[axis] - clo[axis];
+                auto it = std::partition(s->prims.begin() + tk.begin, s->prims.begin() + tk.end, [&](uint32_t p) {
+                    int b = (int)((cen[3 * p + axis] - clo[axis]) / ext * 12);
+                    b = std::min(std::max(b, 0), 11);
+                    return b < best_bin;
+                });
+                mid = (uint32_t)(it - s->prims.begin());
+                if (mid == tk.begin || mid == tk.end) axis = -1;
+            }
+            if (axis < 0) {   // median split on the widest centroid axis
+                int k = 0;
+                for (int q = 1; q < 3; ++q) if (chi[q] - clo[q] > chi[k] - clo[k]) k = q;
+                mid = tk.begin + cnt / 2;
+                std::nth_element(s->prims.begin() + tk.begin, s->prims.begin() + mid, s->prims.begin() + tk.end,
+                                 [&](uint32_t a, uint32_t b) { return cen[3 * a + k] < cen[3 * b + k]; });
+                axis = k;
+            }
+        }
+        if (axis < 0) {
+            nd.left_or_first = tk.begin;
+            nd.right = 0;
+            nd.count = cnt;
+            s->nodes[tk.node] = nd;
+        } else {
+            uint32_t l = (uint32_t)s->nodes.size();
+            s->nodes.push_back(BNode{});
+            s->nodes.push_back(BNode{});
+            nd.left_or_first = l;
+            nd.right = l + 1;
+            nd.count = 0;
+            s->nodes[tk.node] = nd;
+            stack.push_back({l + 1, mid, tk.end});
+            stack.push_back({l, tk.begin, mid});
+        }
+    }
+}
+
+inline float safe_inv(float d) {
+    const float eps = 1e-20f;
+    float dd = std::fabs(d) > eps ? d : std::copysign(eps, d);
+    return 1.0f / dd;
+}
+
+// robust slab test: tFar enlarged by 1 + 2*gamma(3) (pbrt-v3 Bounds3::IntersectP)
+inline bool box_hit(const BNode &b, const V3f &o, const float inv[3], float tmin, float tmax, float *tnear) {
+    const float g3 = (3 * 5.9604644775390625e-08f) / (1 - 3 * 5.9604644775390625e-08f);
+    float t0 = tmin, t1 = tmax;
+    for (int k = 0; k < 3; ++k) {
+        float tn = (b.lo[k] - o[k]) * inv[k];
+        float tf = (b.hi[k] - o[k]) * inv[k];
+        if (tn > tf) std::swap(tn, tf);
+        tf *= 1 + 2 * g3;
+        t0 = tn > t0 ? tn : t0;
+        t1 = tf < t1 ? tf : t1;
+        if (t0 > t1) return false;
+    }
+    *tnear = t0;
+    return true;
+}
+
+struct OHit { float t, u, v; uint32_t id; };
+
+// closest hit (tie: smaller ID) or any hit; skip_lights ignores emitters
+template <bool ANY>
+bool bvh_trace(const or_scene_s *s, const V3f &o, const V3f &d, float tmin, float tmax, bool skip_lights, OHit *h) {
+    const float inv[3] = {safe_inv(d.x), safe_inv(d.y), safe_inv(d.z)};
+    RayPre rp = ray_pre(d);
+    float best = tmax;
+    uint32_t best_id = FL_NO_HIT;
+    float bu = 0, bv = 0;
+    uint32_t stack[128];
+    int sp = 0;
+    stack[sp++] = 0;
+    while (sp) {
+        const BNode &nd = s->nodes[stack[--sp]];
+        float tn;
+        if (!box_hit(nd, o, inv, tmin, best, &tn)) continue;
+        if (nd.count) {
+            for (uint32_t i = 0; i < nd.count; ++i) {
+                uint32_t p = s->prims[nd.left_or_first + i];
+                if (skip_lights && s->light[p]) continue;
+                float t, u, v;
+                if (tri_watertight(rp, o, s->v0[p], s->v1[p], s->v2[p], tmin, best, &t, &u, &v)) {
+                    if (ANY) { h->t = t; h->u = u; h->v = v; h->id = p; return true; }
+                    if (t < best || (t == best && p < best_id)) { best = t; best_id = p; bu = u; bv = v; }
+                }
+            }
+        } else {
+            const BNode &L = s->nodes[nd.left_or_first], &R = s->nodes[nd.right];
+            float tl = 0, tr = 0;
+            bool hl = box_hit(L, o, inv, tmin, best, &tl), hr = box_hit(R, o, inv, tmin, best, &tr);
+            if (hl && hr) {
+                if (tl <= tr) { stack[sp++] = nd.right; stack[sp++] = nd.left_or_first; }
+                else { stack[sp++] = nd.left_or_first; stack[sp++] = nd.right; }
+            } else if (hl) stack[sp++] = nd.left_or_first;
+            else if (hr) stack[sp++] = nd.right;
+        }
+    }
+    h->t = best; h->u = bu; h->v = bv; h->id = best_id;
+    return best_id != FL_NO_HIT;
+}
+
+bool brute_trace(const or_scene_s *s, const V3f &o, const V3f &d, float tmin, float tmax, bool any, bool skip,
+                 OHit *h) {
+    RayPre rp = ray_pre(d);
+    float best = tmax;
+    uint32_t best_id = FL_NO_HIT;
+    float bu = 0, bv = 0;
+    for (uint32_t p = 0; p < s->v0.size(); ++p) {
+        if (skip && s->light[p]) continue;
+        float t, u, v;
+        if (tri_watertight(rp, o, s->v0[p], s->v1[p], s->v2[p], tmin, best, &t, &u, &v)) {
+            if (any) { h->t = t; h->u = u; h->v = v; h->id = p; return true; }
+            if (t < best || (t == best && p < best_id)) { best = t; best_id = p; bu = u; bv = v; }
+        }
+    }
+    h->t = best; h->u = bu; h->v = bv; h->id = best_id;
+    return best_id != FL_NO_HIT;
+}
+
+// best two hits on distinct triangles (brute order independent)
+void bvh_trace2(const or_scene_s *s, const V3f &o, const V3f &d, float tmin, float tmax, float *t2, uint32_t *id2) {
+    const float inv[3] = {safe_inv(d.x), safe_inv(d.y), safe_inv(d.z)};
+    RayPre rp = ray_pre(d);
+    float b1 = tmax, b2 = tmax;
+    uint32_t i1 = FL_NO_HIT, i2 = FL_NO_HIT;
+    uint32_t stack[128];
+    int sp = 0;
+    stack[sp++] = 0;
+    while (sp) {
+        const BNode &nd = s->nodes[stack[--sp]];
+        float tn;
+        if (!box_hit(nd, o, inv, tmin, b2, &tn)) continue;
+        if (nd.count) {
+            for (uint32_t i = 0; i < nd.count; ++i) {
+                uint32_t p = s->prims[nd.left_or_first + i];
+                float t, u, v;
+                if (!tri_watertight(rp, o, s->v0[p], s->v1[p], s->v2[p], tmin, b2, &t, &u, &v)) continue;
+                if (t < b1 || (t == b1 && p < i1)) { b2 = b1; i2 = i1; b1 = t; i1 = p; }
+                else if (t < b2 || (t == b2 && p < i2)) { b2 = t; i2 = p; }
+            }
+        } else {
+            stack[sp++] = nd.left_or_first;
+            stack[sp++] = nd.right;
+        }
+    }
+    t2[0] = b1; t2[1] = b2;
+    id2[0] = i1; id2[1] = i2;
+}
+
+// ------------------------------------------------------ surface interaction
+struct Isect {
+    float t;
+    V3f p, pError, n, wo;
+    uint32_t prim;
+};
+
+// BVH::intersect with repairs R1-R4 (bvh/bvh.cpp:49-84)
+bool scene_intersect(const or_scene_s *s, const V3f &o, const V3f &d, float tmin, float tmax, Isect *it) {
+    OHit h;
+    if (!bvh_trace<false>(s, o, d, tmin, tmax, false, &h)) return false;
+    it->t = h.t;
+    it->p = o + d * h.t;                                           // ray.point_at_t(t)
+    it->pError = vabs(it->p) * gamma5();                           // repair R4
+    const uint32_t p = h.id;
+    it->n = normalize(cross(s->v1[p] - s->v0[p], s->v2[p] - s->v0[p]));  // repair R3: Ng = (v1-v0)x(v2-v0)
+    it->wo = normalize(-d);
+    it->prim = p;                                                  // repair R2
+    return true;
+}
+
+// PCG32 sampler (published algorithm; the reference vendors it as src/ext/pcg32.h)
+struct Sampler {
+    uint64_t state, inc;
+    Sampler(uint64_t seed, uint64_t seq) {
+        state = 0u; inc = (seq << 1u) | 1u; next(); state += seed; next();
+    }
+    uint32_t next() {
+        uint64_t old = state;
+        state = old * 6364136223846793005ull + inc;
+        uint32_t xs = (uint32_t)(((old >> 18u) ^ old) >> 27u);
+        uint32_t rot = (uint32_t)(old >> 59u);
+        return (xs >> rot) | (xs << ((-rot) & 31u));
+    }
+    float get_1d() { return (float)(next() >> 8) * 0x1p-24f; }
+    void get_2d(float *a, float *b) { *a = get_1d(); *b = get_1d(); }
+};
+
+struct Ctx {
+    const or_scene_s *s;
+    int dd, rd, td;
+    float rr_thr;
+    uint32_t rr_start;
+    uint64_t rays = 0;
+};
+
+Bsdf<float, NS> make_bsdf(const or_scene_s *s, const Isect &it) {
+    Bsdf<float, NS> b;
+    uint32_t m = s->mat[it.prim];
+    if (m == 0xffffu || s->mats[m].kind == FL_MAT_NONE) return b;
+    const Material &M = s->mats[m];
+    b.has = true;
+    b.frame(it.n, it.n);                 // ns = Shading.n = n (repair R3)
+    b.lobe.kind = M.kind;
+    b.lobe.ggx.ax = M.ax;
+    b.lobe.ggx.ay = M.ay;
+    b.lobe.weight = M.weight;
+    b.lobe.R = M.R;
+    b.lobe.Tt = M.T;
+    b.lobe.eta = M.eta;
+    b.lobe.k = M.k;
+    b.lobe.ior = M.ior;
+    b.eta = (M.kind == FL_MAT_DIELECTRIC) ? M.ior : 1.f;
+    return b;
+}
+
+// DiffuseAreaIlluminant::L (diffusearea.cpp:37-39)
+S60 light_L(const Light &l, const V3f &n, const V3f &w) { return dot(n, w) > 0.f ? l.color * l.intensity : S60(0.f); }
+
+// VisibilityEvaluator::unoccluded (illuminant.cpp:40-50): closest hit on the segment
+bool unoccluded(Ctx &c, const Isect &p0, const V3f &p1) {
+    V3f d = p1 - p0.p;
+    V3f o = offset_point_origin(p0.p, p0.pError, p0.n, d);       // kindle_ray_to
+    c.rays++;
+    OHit h;
+    if (bvh_trace<false>(c.s, o, d, SHADOW_EPSILON, 1.f - SHADOW_EPSILON, false, &h))
+        return c.s->light[h.id] != 0;
+    return true;
+}
+
+// estimate_direct (integrator.cpp:25-110) for one area light
+S60 estimate_direct(Ctx &c, const Isect &it, const Bsdf<float, NS> &bsdf, float us0, float us1, uint32_t li,
+                    float ul0, float ul1) {
+    const or_scene_s *s = c.s;
+    const Light &L = s->lights[li];
+    const uint32_t flags = BSDF_ALL & ~BSDF_SPECULAR;
+    S60 Ld(0.f);
+    // --- light sample: Triangle::sample + Shape::sample(ref) (triangle.cpp:134-154, shape.cpp:28-41)
+    float uu = std::sqrt(ul0);
+    float u1 = 1.f - uu, u2 = ul1 * uu;
+    V3f ps = L.p0 * u1 + L.p1 * u2 + L.p2 * (1.f - u1 - u2);
+    V3f nl = normalize(cross(L.p1 - L.p0, L.p2 - L.p0));
+    float illumPdf = 1.f / L.area;
+    V3f wi = ps - it.p;
+    if (wi.length_squared() == 0) illumPdf = 0.f;
+    else {
+        wi = normalize(wi);
+        illumPdf *= (it.p - ps).length_squared() / abs_dot(nl, -wi);
+        if (std::isinf(illumPdf)) illumPdf = 0.f;
+    }
+    S60 Li(0.f);
+    if (!(illumPdf == 0 || (ps - it.p).length_squared() == 0)) {   // DiffuseAreaIlluminant::sample_li
+        wi = normalize(ps - it.p);
+        Li = light_L(L, nl, V3f(-wi.x, -wi.y, -wi.z));
+    } else illumPdf = 0.f;
+    float scatteringPdf = 0.f;
+    if (illumPdf > 0 && !Li.is_black()) {
+        S60 f = bsdf.evaluate(it.wo, wi, flags) * abs_dot(wi, it.n);
+        scatteringPdf = bsdf.pdf(it.wo, wi, flags);
+        if (!f.is_black()) {
+            if (!unoccluded(c, it, ps)) Li = S60(0.f);              // shadowColor = 0
+            if (!Li.is_black()) {
+                float w = (illumPdf * illumPdf) / (illumPdf * illumPdf + scatteringPdf * scatteringPdf);
+                Ld += f * Li * w / illumPdf;
+            }
+        }
+    }
+    // --- BSDF sample with MIS
+    {
+        uint32_t st = 0;
+        S60 f = bsdf.sample(it.wo, &wi, us0, us1, &scatteringPdf, flags, &st);
+        f *= abs_dot(wi, it.n);
+        bool sampledSpecular = (st & BSDF_SPECULAR) != 0;
+        if (!f.is_black() && scatteringPdf > 0) {
+            float weight = 1.f;
+            if (!sampledSpecular) {
+                // Shape::pdf(ref, wi): the reference's own triangle test (shape.cpp:13-26)
+                V3f ro = offset_point_origin(it.p, it.pError, it.n, wi);
+                float tt;
+                V3f pp, nn;
+                float lp = 0.f;
+                if (tri_reference(ro, wi, SHADOW_EPSILON, INFINITI, L.p0, L.p1, L.p2, &tt, &pp, &nn)) {
+                    lp = (it.p - pp).length_squared() / (abs_dot(nn, -wi) * L.area);
+                    if (std::isinf(lp)) lp = 0.f;
+                }
+                if (lp == 0) return Ld;
+                weight = (scatteringPdf * scatteringPdf) / (scatteringPdf * scatteringPdf + lp * lp);
+            }
+            V3f ro = offset_point_origin(it.p, it.pError, it.n, wi);     // kindle_ray
+            Isect li_it;
+            c.rays++;
+            S60 Le(0.f);
+            if (scene_intersect(s, ro, wi, SHADOW_EPSILON, INFINITI, &li_it)) {
+                if (s->light[li_it.prim] == li + 1) Le = light_L(L, li_it.n, -wi);
+            }
+            if (!Le.is_black()) Ld += f * Le * weight / scatteringPdf;
+        }
+    }
+    return Ld;
+}
+
+// PathIntegrator::Li (pathintegrator.cpp:12-87)
+S60 Li(Ctx &c, V3f ro, V3f rd, float tmin, Sampler &sampler) {
+    const or_scene_s *s = c.s;
+    S60 L(0.f), beta(1.f);
+    bool specularBounce = false;
+    float etaScale = 1.f;
+    for (int bounces = 0;; ++bounces) {
+        Isect it;
+        c.rays++;
+        bool hit = scene_intersect(s, ro, rd, tmin, INFINITI, &it);
+        if (bounces == 0 || specularBounce) {
+            if (hit && s->light[it.prim]) L += beta * light_L(s->lights[s->light[it.prim] - 1], it.n, -rd);
+        }
+        if (!hit) break;
+        Bsdf<float, NS> bsdf = make_bsdf(s, it);
+        if (!bsdf.has) {                                   // pass-through
+            ro = offset_point_origin(it.p, it.pError, it.n, rd);
+            tmin = SHADOW_EPSILON;
+            bounces--;
+            continue;
+        }
+        if (bsdf.num_components(BSDF_ALL & ~BSDF_SPECULAR) > 0) {
+            S60 Ld(0.f);                                   // uniform_sample_all_lights
+            for (uint32_t li = 0; li < s->lights.size(); ++li) {
+                float ul0, ul1, us0, us1;
+                sampler.get_2d(&ul0, &ul1);
+                sampler.get_2d(&us0, &us1);
+                Ld += estimate_direct(c, it, bsdf, us0, us1, li, ul0, ul1);
+            }
+            L += beta * Ld;
+        }
+        V3f wo = -rd, wi;
+        float pdf = 0.f;
+        uint32_t flags = 0;
+        float u0, u1;
+        sampler.get_2d(&u0, &u1);
+        S60 f = bsdf.sample(wo, &wi, u0, u1, &pdf, BSDF_ALL, &flags);
+        if (f.is_black() || pdf == 0) break;
+        if (c.dd == bounces && (flags & BSDF_DIFFUSE)) break;
+        if ((c.rd == bounces && (flags & BSDF_SPECULAR)) || (c.rd == bounces && (flags & BSDF_GLOSSY))) break;
+        if (c.td == bounces && (flags & BSDF_TRANSMISSION)) break;
+        beta *= f * abs_dot(wi, it.n) / pdf;
+        specularBounce = (flags & BSDF_SPECULAR) != 0;
+        if ((flags & BSDF_SPECULAR) && (flags & BSDF_TRANSMISSION)) {
+            float eta = bsdf.eta;
+            etaScale *= (dot(wo, it.n) > 0) ? (eta * eta) : 1.f / (eta * eta);
+        }
+        ro = offset_point_origin(it.p, it.pError, it.n, wi);
+        rd = wi;
+        tmin = SHADOW_EPSILON;
+        S60 rrBeta = beta * etaScale;
+        if (rrBeta.max_component() < c.rr_thr && bounces > (int)c.rr_start) {
+            float q = std::max(0.05f, 1.f - rrBeta.max_component());
+            if (sampler.get_1d() < q) break;
+            beta /= 1.f - q;
+        }
+    }
+    return L;
+}
+
+struct Cam { V3f o, llc, hor, ver; uint32_t W, H; };
+
+// PhysicalCamera, flipCameraDirection = true, pinhole (physical.h:21-43, physical.cpp:12-22; repair R6)
+Cam make_cam(const fl_camera *c) {
+    Cam k;
+    V3f pos(c->position[0], c->position[1], c->position[2]), la(c->look_at[0], c->look_at[1], c->look_at[2]);
+    V3f up(c->up[0], c->up[1], c->up[2]);
+    float fd = (pos - la).length();
+    float theta = c->vfov_deg * PI / 180.f;
+    float hh = tanf(theta / 2.f);
+    float aspect = (float)c->width / (float)c->height;
+    float hw = aspect * hh;
+    V3f w = normalize(pos - la);
+    w = w * -1.f;
+    V3f u = normalize(cross(up, w));
+    V3f v = cross(w, u);
+    w = w * -1.f;
+    k.o = pos;
+    k.llc = pos - (u * hw * fd) - (v * fd * hh) - (w * fd);
+    k.hor = u * 2.f * hw * fd;
+    k.ver = v * 2.f * hh * fd;
+    k.W = c->width;
+    k.H = c->height;
+    return k;
+}
+
+} // namespace
+
+namespace {
+
+template <typename F>
+static void parallel_for(uint32_t n, int threads, F fn) {
+    if (threads <= 1 || n < 1024) { for (uint32_t i = 0; i < n; ++i) fn(i); return; }
+    std::atomic<uint32_t> next(0);
+    std::vector<std::thread> ts;
+    for (int t = 0; t < threads; ++t)
+        ts.emplace_back([&]() {
+            while (true) {
+                uint32_t b = next.fetch_add(1024);
+                if (b >= n) break;
+                uint32_t e = std::min(n, b + 1024);
+                for (uint32_t i = b; i < e; ++i) fn(i);
+            }
+        });
+    for (auto &t : ts) t.join();
+}
+
+template <class T>
+static void bsdf_query(const fl_bsdf_query &Q, fl_bsdf_result &R) {
+    Bsdf<T, 4> b;
+    b.has = true;
+    b.frame_identity();
+    b.lobe.kind = Q.kind;
+    b.lobe.ggx.ax = Q.alpha_x;
+    b.lobe.ggx.ay = Q.alpha_y;
+    b.lobe.weight = 1;
+    b.lobe.vndf = (Q.flags & FL_RENDER_VNDF_PDF) != 0;
+    for (int k = 0; k < 4; ++k) {
+        b.lobe.R.c[k] = Q.r[k]; b.lobe.Tt.c[k] = Q.t[k]; b.lobe.eta.c[k] = Q.eta[k]; b.lobe.k.c[k] = Q.k[k];
+    }
+    b.lobe.ior = Q.eta[0];
+    V3<T> wo(Q.wo[0], Q.wo[1], Q.wo[2]), wi(Q.wi[0], Q.wi[1], Q.wi[2]);
+    const uint32_t ns = BSDF_ALL & ~BSDF_SPECULAR;
+    Spec<T, 4> fe = b.evaluate(wo, wi, ns);
+    for (int k = 0; k < 4; ++k) R.f_eval[k] = (float)fe.c[k];
+    R.pdf_eval = (float)b.pdf(wo, wi, ns);
+    V3<T> ws;
+    T pdf = 0;
+    uint32_t st = 0;
+    Spec<T, 4> fs = b.sample(wo, &ws, (T)Q.u[0], (T)Q.u[1], &pdf, BSDF_ALL, &st);
+    for (int k = 0; k < 4; ++k) R.f_sample[k] = (float)fs.c[k];
+    R.pdf_sample = (float)pdf;
+    R.wi_sample[0] = (float)ws.x; R.wi_sample[1] = (float)ws.y; R.wi_sample[2] = (float)ws.z;
+    R.sampled_type = st;
+}
+
+} // namespace
+
+extern "C" {
+
+int or_spectral_bins(void) { return NS; }
+
+void or_cie_bins(float *x, float *y, float *z) {
+    const Tables &T = tables();
+    for (int i = 0; i < NS; ++i) { x[i] = T.X.c[i]; y[i] = T.Y.c[i]; z[i] = T.Z.c[i]; }
+}
+void or_refl_basis_bins(int which, float *out) {
+    const Tables &T = tables();
+    for (int i = 0; i < NS; ++i) out[i] = T.refl[which].c[i];
+}
+void or_from_reflectance_rgb(const float rgb[3], float *out) {
+    S60 r = from_reflectance_rgb(rgb);
+    memcpy(out, r.c, sizeof(r.c));
+}
+void or_from_spd(const float *lam, const float *val, int n, float *out) {
+    S60 r = from_spd(lam, val, n);
+    memcpy(out, r.c, sizeof(r.c));
+}
+float or_average_spectrum_samples(const float *lam, const float *val, int n, float l0, float l1) {
+    return average_spectrum_samples(lam, val, n, l0, l1);
+}
+void or_to_xyz(const float *spec, float xyz[3]) {
+    S60 r;
+    memcpy(r.c, spec, sizeof(r.c));
+    to_xyz(r, xyz);
+}
+void or_metal_bins(int m, float *eta, float *k) {
+    const Tables &T = tables();
+    for (int i = 0; i < NS; ++i) { eta[i] = T.eta[m].c[i]; k[i] = T.k[m].c[i]; }
+}
+
+int or_tri_intersect(const float ray[8], const float a[3], const float b[3], const float c[3], float out[3]) {
+    V3f o(ray[0], ray[1], ray[2]), d(ray[4], ray[5], ray[6]);
+    RayPre rp = ray_pre(d);
+    float t, u, v;
+    if (!tri_watertight(rp, o, V3f(a[0], a[1], a[2]), V3f(b[0], b[1], b[2]), V3f(c[0], c[1], c[2]), ray[3], ray[7],
+                        &t, &u, &v))
+        return 0;
+    out[0] = t; out[1] = u; out[2] = v;
+    return 1;
+}
+
+int or_scene_create(const fl_scene_desc *d, or_scene_t *out) {
+    if (!d || !out) return FL_E_INVALID;
+    or_scene_s *s = new or_scene_s();
+    const uint32_t n = d->n_triangles;
+    s->v0.resize(n); s->v1.resize(n); s->v2.resize(n);
+    s->mat.resize(n); s->light.assign(n, 0u);
+    for (uint32_t t = 0; t < n; ++t) {
+        const uint32_t *ix = d->indices + 3ull * t;
+        auto V = [&](uint32_t i) { return V3f(d->vertices[3ull * i], d->vertices[3ull * i + 1], d->vertices[3ull * i + 2]); };
+        s->v0[t] = V(ix[0]); s->v1[t] = V(ix[1]); s->v2[t] = V(ix[2]);
+        s->mat[t] = d->tri_material ? d->tri_material[t] : 0u;
+    }
+    const Tables &T = tables();
+    s->mats.resize(d->n_materials);
+    for (uint32_t m = 0; m < d->n_materials; ++m) {
+        const fl_material &fm = d->materials[m];
+        Material &M = s->mats[m];
+        M.kind = fm.kind;
+        M.ax = M.ay = 1.f; M.weight = 1.f; M.ior = 1.f;
+        if (fm.kind == FL_MAT_LAMBERT) {
+            M.R = from_reflectance_rgb(fm.rgb).clamp0();                  // LambertMaterial: kd.spectrum_clamp()
+        } else if (fm.kind == FL_MAT_CONDUCTOR) {                           // standardsurface.cpp:101-125
+            float aspect = std::sqrt(1.f - fm.anisotropic * 0.9f);
+            float r2 = fm.roughness * fm.roughness;
+            M.ax = std::max(0.001f, std::max(.001f, r2 / aspect));
+            M.ay = std::max(0.001f, std::max(.001f, r2 * aspect));
+            M.weight = fm.weight;
+            int mt = (fm.metal >= 0 && fm.metal < FL_N_METALS) ? fm.metal : 0;
+            M.eta = T.eta[mt];
+            M.k = T.k[mt];
+        } else if (fm.kind == FL_MAT_DIELECTRIC) {                          // standardsurface.cpp:113-117
+            M.R = from_reflectance_rgb(fm.rgb).clamp0();
+            M.T = from_reflectance_rgb(fm.rgb_t).clamp0();
+            M.ior = fm.ior;
+        }
+    }
+    float lam[FL_SPD_N];
+    for (int i = 0; i < FL_SPD_N; ++i) lam[i] = 360.f + (float)i;
+    s->lights.resize(d->n_lights);
+    for (uint32_t l = 0; l < d->n_lights; ++l) {
+        const fl_area_light &a = d->lights[l];
+        Light &L = s->lights[l];
+        L.p0 = s->v0[a.tri]; L.p1 = s->v1[a.tri]; L.p2 = s->v2[a.tri];
+        L.area = 0.5f * cross(L.p1 - L.p0, L.p2 - L.p0).length();       // Triangle::area
+        L.color = from_spd(lam, d->emission_spd + (size_t)a.emission * FL_SPD_N, FL_SPD_N);
+        L.intensity = a.intensity;
+        L.tri = a.tri;
+        s->light[a.tri] = l + 1;
+    }
+    build_bvh(s);
+    *out = s;
+    return FL_OK;
+}
+
+void or_scene_destroy(or_scene_t s) { delete s; }
+
+int or_intersect(or_scene_t s, const fl_ray *rays, fl_hit *hits, uint32_t n, uint32_t flags, int mode, int threads) {
+    if (!s) return FL_E_INVALID;
+    bool any = flags & FL_ANY_HIT, skip = flags & FL_SKIP_LIGHTS;
+    parallel_for(n, threads, [&](uint32_t i) {
+        const fl_ray &r = rays[i];
+        V3f o(r.ox, r.oy, r.oz), d(r.dx, r.dy, r.dz);
+        OHit h;
+        bool hit;
+        if (mode == 1) hit = brute_trace(s, o, d, r.tmin, r.tmax, any, skip, &h);
+        else if (any) hit = bvh_trace<true>(s, o, d, r.tmin, r.tmax, skip, &h);
+        else hit = bvh_trace<false>(s, o, d, r.tmin, r.tmax, false, &h);
+        hits[i].t = hit ? h.t : INFINITI;
+        hits[i].tri = hit ? h.id : FL_NO_HIT;
+        hits[i].u = hit ? h.u : 0.f;
+        hits[i].v = hit ? h.v : 0.f;
+    });
+    return FL_OK;
+}
+
+int or_intersect2(or_scene_t s, const fl_ray *rays, float *t2, uint32_t *id2, uint32_t n, int threads) {
+    if (!s) return FL_E_INVALID;
+    parallel_for(n, threads, [&](uint32_t i) {
+        const fl_ray &r = rays[i];
+        bvh_trace2(s, V3f(r.ox, r.oy, r.oz), V3f(r.dx, r.dy, r.dz), r.tmin, r.tmax, t2 + 2 * (size_t)i, id2 + 2 * (size_t)i);
+    });
+    return FL_OK;
+}
+
+int or_bsdf_batch(const fl_bsdf_query *q, fl_bsdf_result *r, uint32_t n, int precision) {
+    for (uint32_t i = 0; i < n; ++i) {
+        memset(&r[i], 0, sizeof(r[i]));
+        if (precision == 64) bsdf_query<double>(q[i], r[i]);
+        else bsdf_query<float>(q[i], r[i]);
+    }
+    return FL_OK;
+}
+
+int or_render(or_scene_t s, const fl_camera *cam, const or_render_params *p, float *film_sum, float *film_sq,
+              or_render_stats *st) {
+    if (!s || !cam || !p) return FL_E_INVALID;
+    Cam k = make_cam(cam);
+    const uint32_t W = cam->width, H = cam->height;
+    uint32_t y0 = p->y1 > p->y0 ? p->y0 : 0, y1 = p->y1 > p->y0 ? std::min(p->y1, H) : H;
+    uint32_t x0 = p->x1 > p->x0 ? p->x0 : 0, x1 = p->x1 > p->x0 ? std::min(p->x1, W) : W;
+    int threads = p->threads > 0 ? p->threads : (int)std::max(1u, std::thread::hardware_concurrency());
+    const uint32_t TS = 16;
+    uint32_t tx = (x1 - x0 + TS - 1) / TS, ty = (y1 - y0 + TS - 1) / TS;
+    std::atomic<uint32_t> next(0);
+    std::atomic<uint64_t> rays(0);
+    auto t0 = std::chrono::steady_clock::now();
+    auto worker = [&]() {
+        Ctx c;
+        c.s = s; c.dd = p->diffuse_depth; c.rd = p->reflection_depth; c.td = p->transmission_depth;
+        c.rr_thr = p->rr_threshold; c.rr_start = p->rr_start;
+        while (true) {
+            uint32_t tile = next.fetch_add(1);
+            if (tile >= tx * ty) break;
+            Sampler smp(p->seed * 0x9E3779B97F4A7C15ull + tile, tile + 1);
+            uint32_t bx = x0 + (tile % tx) * TS, by = y0 + (tile / tx) * TS;
+            for (uint32_t y = by; y < std::min(by + TS, y1); ++y)
+                for (uint32_t x = bx; x < std::min(bx + TS, x1); ++x) {
+                    float sx = 0, sy = 0, sz = 0, qx = 0, qy = 0, qz = 0;
+                    for (uint32_t sp = 0; sp < p->spp; ++sp) {
+                        float jx, jy;
+                        smp.get_2d(&jx, &jy);
+                        float su = 1.f - ((float)x + jx) / (float)W;
+                        float tv = 1.f - ((float)y + jy) / (float)H;
+                        V3f dir = k.llc + k.hor * su + k.ver * tv - k.o - V3f(0, 0, 0);
+                        S60 L = Li(c, k.o, normalize(dir), 0.f, smp);   // tMin = time = 0
+                        float xyz[3];
+                        to_xyz(L, xyz);
+                        if (!std::isfinite(xyz[0]) || !std::isfinite(xyz[1]) || !std::isfinite(xyz[2]))
+                            xyz[0] = xyz[1] = xyz[2] = 0.f;
+                        sx += xyz[0]; sy += xyz[1]; sz += xyz[2];
+                        qx += xyz[0] * xyz[0]; qy += xyz[1] * xyz[1]; qz += xyz[2] * xyz[2];
+                    }
+                    size_t pix = (size_t)y * W + x;
+                    film_sum[3 * pix] = sx; film_sum[3 * pix + 1] = sy; film_sum[3 * pix + 2] = sz;
+                    if (film_sq) { film_sq[3 * pix] = qx; film_sq[3 * pix + 1] = qy; film_sq[3 * pix + 2] = qz; }
+                }
+        }
+        rays += c.rays;
+    };
+    std::vector<std::thread> ts;
+    for (int t = 0; t < threads; ++t) ts.emplace_back(worker);
+    for (auto &t : ts) t.join();
+    auto t1 = std::chrono::steady_clock::now();
+    if (st) {
+        st->seconds = std::chrono::duration<double>(t1 - t0).count();
+        st->samples = (uint64_t)(x1 - x0) * (y1 - y0) * p->spp;
+        st->rays = rays.load();
+        st->threads = threads;
+    }
+    return FL_OK;
+}
+
+} // extern "C"

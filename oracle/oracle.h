@@ -1,0 +1,77 @@
+/*
+ * oracle.h -- CPU oracle of the render path.  TEST INFRASTRUCTURE ONLY.
+ *
+ * This library is the checker: tests, the smoke check and the CPU-baseline
+ * leg of bench.py load it; the product (libfl_gpu.so) never does.  It
+ * restates, in plain C++ on the CPU, what the reference computes:
+ *   - 60-bin spectra and their tables            color/ (spectrum tables)
+ *   - the BxDFs, in fp32 and in fp64              shading/ (BxDFs)
+ *   - PathIntegrator::Li with estimate_direct     integrators/
+ *   - Embree's role (closest hit) by a binned-SAH BVH over the same
+ *     watertight triangle test the GPU uses, plus a brute-force search.
+ * The reference's defects are repaired as listed in oracle/README.md.
+ */
+#ifndef FL_ORACLE_H
+#define FL_ORACLE_H
+
+#include <stdint.h>
+
+#include "../include/fl_gpu.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct or_scene_s *or_scene_t;
+
+/* spectral tables (60 bins, 400..700 nm) */
+int or_spectral_bins(void);
+void or_cie_bins(float *x, float *y, float *z);                       /* InitializeSpectrals */
+void or_refl_basis_bins(int which /* 0..6 WCMYRGB */, float *out);
+void or_from_reflectance_rgb(const float rgb[3], float *out60);
+void or_from_spd(const float *lam, const float *val, int n, float *out60);
+float or_average_spectrum_samples(const float *lam, const float *val, int n, float l0, float l1);
+void or_to_xyz(const float *spec60, float xyz[3]);
+void or_metal_bins(int metal, float *eta60, float *k60);
+
+/* watertight ray/triangle test (the arithmetic the GPU uses) */
+int or_tri_intersect(const float ray[8], const float v0[3], const float v1[3], const float v2[3], float out[3]);
+
+/* scene: triangles + materials + lights (same descriptor as fl_scene_create) */
+int or_scene_create(const fl_scene_desc *desc, or_scene_t *out);
+void or_scene_destroy(or_scene_t s);
+/* rays: n x fl_ray; hits: n x fl_hit.  mode 0 = BVH, 1 = brute force */
+int or_intersect(or_scene_t s, const fl_ray *rays, fl_hit *hits, uint32_t n, uint32_t flags, int mode, int threads);
+/* best two distinct-triangle hits per ray (for near-tie accounting) */
+int or_intersect2(or_scene_t s, const fl_ray *rays, float *t2, uint32_t *id2, uint32_t n, int threads);
+
+/* BxDF queries: precision 32 or 64 */
+int or_bsdf_batch(const fl_bsdf_query *q, fl_bsdf_result *r, uint32_t n, int precision);
+
+/* Render with the restated PathIntegrator (60-bin spectra, fp32).
+ * film_sum/film_sq: W*H*3 floats (CIE XYZ per sample summed); pixels
+ * restricted to rows [y0, y1) when y1 > y0. */
+typedef struct {
+    uint32_t spp;
+    int32_t diffuse_depth, reflection_depth, transmission_depth;
+    float rr_threshold;
+    uint32_t rr_start;
+    uint64_t seed;
+    int32_t threads;
+    uint32_t y0, y1;
+    uint32_t x0, x1;
+} or_render_params;
+
+typedef struct {
+    double seconds;
+    uint64_t samples, rays;
+    int32_t threads;
+} or_render_stats;
+
+int or_render(or_scene_t s, const fl_camera *cam, const or_render_params *p, float *film_sum, float *film_sq,
+              or_render_stats *st);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

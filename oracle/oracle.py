@@ -1,0 +1,179 @@
+"""ctypes binding of the CPU oracle (liboracle.so).  TEST INFRASTRUCTURE ONLY.
+
+Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg use this
+module, as the checker / CPU baseline; the product never imports it.
+"""
+import ctypes as C
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(os.path.dirname(HERE), "filianore-core_amd"))
+import flgpu  # noqa: E402  (ABI struct layouts only)
+
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+
+
+class RenderParams(C.Structure):
+    _fields_ = [("spp", C.c_uint32), ("diffuse_depth", C.c_int32), ("reflection_depth", C.c_int32),
+                ("transmission_depth", C.c_int32), ("rr_threshold", C.c_float), ("rr_start", C.c_uint32),
+                ("seed", C.c_uint64), ("threads", C.c_int32), ("y0", C.c_uint32), ("y1", C.c_uint32),
+                ("x0", C.c_uint32), ("x1", C.c_uint32)]
+
+
+class RenderStats(C.Structure):
+    _fields_ = [("seconds", C.c_double), ("samples", C.c_uint64), ("rays", C.c_uint64), ("threads", C.c_int32)]
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError("oracle not built: %s" % LIB_PATH)
+        L = C.CDLL(LIB_PATH)
+        vp = C.c_void_p
+        fp = C.POINTER(C.c_float)
+        L.or_cie_bins.argtypes = [fp, fp, fp]
+        L.or_refl_basis_bins.argtypes = [C.c_int, fp]
+        L.or_from_reflectance_rgb.argtypes = [fp, fp]
+        L.or_from_spd.argtypes = [fp, fp, C.c_int, fp]
+        L.or_average_spectrum_samples.argtypes = [fp, fp, C.c_int, C.c_float, C.c_float]
+        L.or_average_spectrum_samples.restype = C.c_float
+        L.or_to_xyz.argtypes = [fp, fp]
+        L.or_metal_bins.argtypes = [C.c_int, fp, fp]
+        L.or_tri_intersect.argtypes = [fp, fp, fp, fp, fp]
+        L.or_scene_create.argtypes = [C.POINTER(flgpu.SceneDesc), C.POINTER(vp)]
+        L.or_scene_destroy.argtypes = [vp]
+        L.or_intersect.argtypes = [vp, vp, vp, C.c_uint32, C.c_uint32, C.c_int, C.c_int]
+        L.or_intersect2.argtypes = [vp, vp, vp, vp, C.c_uint32, C.c_int]
+        L.or_bsdf_batch.argtypes = [C.POINTER(flgpu.BsdfQuery), C.POINTER(flgpu.BsdfResult), C.c_uint32, C.c_int]
+        L.or_render.argtypes = [vp, C.POINTER(flgpu.Camera), C.POINTER(RenderParams), fp, fp, C.POINTER(RenderStats)]
+        _lib = L
+    return _lib
+
+
+def _f(a):
+    return a.ctypes.data_as(C.POINTER(C.c_float))
+
+
+def cie_bins():
+    x, y, z = (np.zeros(60, np.float32) for _ in range(3))
+    lib().or_cie_bins(_f(x), _f(y), _f(z))
+    return x, y, z
+
+
+def refl_basis(which):
+    o = np.zeros(60, np.float32)
+    lib().or_refl_basis_bins(which, _f(o))
+    return o
+
+
+def from_reflectance_rgb(rgb):
+    r = np.asarray(rgb, np.float32)
+    o = np.zeros(60, np.float32)
+    lib().or_from_reflectance_rgb(_f(r), _f(o))
+    return o
+
+
+def from_spd(lam, val):
+    lam = np.ascontiguousarray(lam, np.float32)
+    val = np.ascontiguousarray(val, np.float32)
+    o = np.zeros(60, np.float32)
+    lib().or_from_spd(_f(lam), _f(val), len(lam), _f(o))
+    return o
+
+
+def average_spectrum_samples(lam, val, l0, l1):
+    lam = np.ascontiguousarray(lam, np.float32)
+    val = np.ascontiguousarray(val, np.float32)
+    return lib().or_average_spectrum_samples(_f(lam), _f(val), len(lam), l0, l1)
+
+
+def to_xyz(spec):
+    s = np.ascontiguousarray(spec, np.float32)
+    o = np.zeros(3, np.float32)
+    lib().or_to_xyz(_f(s), _f(o))
+    return o
+
+
+def metal_bins(m):
+    e, k = np.zeros(60, np.float32), np.zeros(60, np.float32)
+    lib().or_metal_bins(m, _f(e), _f(k))
+    return e, k
+
+
+def tri_intersect(ray, v0, v1, v2):
+    r = np.ascontiguousarray(ray, np.float32)
+    a, b, c = (np.ascontiguousarray(x, np.float32) for x in (v0, v1, v2))
+    o = np.zeros(3, np.float32)
+    hit = lib().or_tri_intersect(_f(r), _f(a), _f(b), _f(c), _f(o))
+    return bool(hit), o
+
+
+def bsdf_batch(queries, precision=64):
+    n = len(queries)
+    q = (flgpu.BsdfQuery * n)(*queries)
+    r = (flgpu.BsdfResult * n)()
+    lib().or_bsdf_batch(q, r, n, precision)
+    return r
+
+
+class Scene:
+    def __init__(self, gen):
+        self.gen = gen
+        d = gen.desc()
+        h = C.c_void_p()
+        rc = lib().or_scene_create(C.byref(d), C.byref(h))
+        if rc != 0:
+            raise RuntimeError("or_scene_create failed %d" % rc)
+        self.h = h
+
+    def close(self):
+        if self.h:
+            lib().or_scene_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def intersect(self, rays, flags=0, brute=False, threads=8):
+        rays = np.ascontiguousarray(rays, np.float32)
+        n = rays.shape[0]
+        hits = np.zeros((n, 4), np.float32)
+        lib().or_intersect(self.h, rays.ctypes.data, hits.ctypes.data, n, flags, 1 if brute else 0, threads)
+        return hits[:, 0].copy(), hits[:, 1].view(np.uint32).copy(), hits[:, 2:4].copy()
+
+    def intersect2(self, rays, threads=8):
+        rays = np.ascontiguousarray(rays, np.float32)
+        n = rays.shape[0]
+        t2 = np.zeros((n, 2), np.float32)
+        id2 = np.zeros((n, 2), np.uint32)
+        lib().or_intersect2(self.h, rays.ctypes.data, t2.ctypes.data, id2.ctypes.data, n, threads)
+        return t2, id2
+
+    def render(self, spp, depth, seed=1, threads=0, rows=None, cols=None, camera=None, rr_start=3):
+        cam = camera if camera is not None else self.gen.camera
+        W, H = cam.width, cam.height
+        p = RenderParams()
+        p.spp = spp
+        p.diffuse_depth = p.reflection_depth = p.transmission_depth = depth
+        p.rr_threshold = 1.0
+        p.rr_start = rr_start
+        p.seed = seed
+        p.threads = threads
+        if rows is not None:
+            p.y0, p.y1 = rows
+        if cols is not None:
+            p.x0, p.x1 = cols
+        s = np.zeros((H, W, 3), np.float32)
+        q = np.zeros((H, W, 3), np.float32)
+        st = RenderStats()
+        rc = lib().or_render(self.h, C.byref(cam), C.byref(p), _f(s), _f(q), C.byref(st))
+        if rc != 0:
+            raise RuntimeError("or_render failed %d" % rc)
+        return s, q, st
