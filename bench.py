@@ -52,9 +52,11 @@ def cpu_baseline(gen, spp, depth, budget_s, threads):
     sc = oracle.Scene(gen)
     H = gen.camera.height
     rows = 8
-    done_samples, secs, y = 0, 0.0, H // 2 - 64
-    while secs < budget_s and y < H:
-        _, _, st = sc.render(spp, depth, seed=17 + y, threads=threads, rows=(y, min(H, y + rows)))
+    done_samples, secs, y, sweep = 0, 0.0, H // 2 - 64, 0
+    while secs < budget_s:
+        if y >= H:
+            y, sweep = 0, sweep + 1
+        _, _, st = sc.render(spp, depth, seed=17 + y + 7919 * sweep, threads=threads, rows=(y, min(H, y + rows)))
         done_samples += st.samples
         secs += st.seconds
         y += rows

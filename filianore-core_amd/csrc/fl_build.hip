@@ -410,6 +410,7 @@ int build_bvh(fl_ctx_t ctx, fl_scene_s *s, const float *d_verts, const uint32_t 
               const uint32_t *d_mat, const uint32_t *d_light) {
     const uint32_t n = s->n_tris;
     hipStream_t st = ctx->stream;
+    (void)hipGetLastError();  // do not report a stale error of an earlier call
     if (n == 0) {   // empty scene: one root node without children, never hit
         fl::Node8 root;
         memset(&root, 0, sizeof(root));

@@ -339,7 +339,7 @@ __global__ void __launch_bounds__(kBlock) k_shade(RenderArgs a, int cur) {
                             if (!refl) {
                                 for (int k = 0; k < 4; ++k) bs.f[k] = 0.f;
                             } else if (b.kind == MAT_CONDUCTOR) {
-                                microfacet_f(b, wo, bs.wi, bs.f);
+                                microfacet_f(b, wo, bs.wi, bs.f, &bs.wh);
                             }
                         }
                         bool black = true;
@@ -482,7 +482,7 @@ __global__ void k_bsdf_query(const fl_bsdf_query *__restrict__ q, fl_bsdf_result
         } else if (!(s.type & BSDF_SPECULAR)) {
             bool rf = s.wi.z * wo.z > 0.f;
             if (!rf) for (int k = 0; k < 4; ++k) s.f[k] = 0.f;
-            else if (b.kind == MAT_CONDUCTOR) microfacet_f(b, wo, s.wi, s.f);
+            else if (b.kind == MAT_CONDUCTOR) microfacet_f(b, wo, s.wi, s.f, &s.wh);
         }
     }
     for (int k = 0; k < 4; ++k) R.f_sample[k] = s.f[k];
@@ -578,6 +578,7 @@ int render(fl_scene_s *s, const fl_camera *camera, const fl_render_params *p, fl
     if (!camera || !p || camera->width == 0 || camera->height == 0 || p->spp == 0)
         return set_error(ctx, FL_E_INVALID, "fl_render: bad camera/params");
     if (p->flags & FL_RENDER_GUIDED) return set_error(ctx, FL_E_UNSUPPORTED, "guided rendering not built");
+    (void)hipGetLastError();  // do not report a stale error of an earlier call
     const int mode = p->spectral_mode == FL_SPECTRAL_PARITY ? 1 : 0;
     const SpecTables &tab = s->spec[mode];
 
@@ -630,7 +631,7 @@ int render(fl_scene_s *s, const fl_camera *camera, const fl_render_params *p, fl
     a.sq_d = grow<float4>(B.p[13], B.cap[13], 2 * cap * 16, err);
     a.sq_tgt = grow<uint32_t>(B.p[14], B.cap[14], 2 * cap * 4, err);
     a.counts = grow<uint32_t>(B.p[15], B.cap[15], 64, err);
-    a.stats = grow<unsigned long long>(B.p[16], B.cap[16], 64, err);
+    a.stats = grow<unsigned long long>(B.p[16], B.cap[16], 256, err);
     a.film_sum = grow<float>(B.p[17], B.cap[17], (size_t)W * H * 12, err);
     a.film_sq = grow<float>(B.p[18], B.cap[18], (size_t)W * H * 12, err);
     a.spill = grow<uint2>(B.p[19], B.cap[19], (size_t)a.spill_stride * spill_per * 8, err);

@@ -81,23 +81,28 @@ FL_HD float cos2phi(v3 w) { float c = cosphi(w); return c * c; }
 FL_HD float sin2phi(v3 w) { float s = sinphi(w); return s * s; }
 FL_HD bool same_hemi(v3 a, v3 b) { return (a.z * b.z) > 0.f; }
 
-// concentric_sample_disk + cosine_hemisphere_sample, shadingcore.h:65-89
+// concentric_sample_disk + cosine_hemisphere_sample, shadingcore.h:65-89.
+// z = sqrt(1 - dx^2 - dy^2) with dx^2 + dy^2 = r^2 and r = 2u - 1 (u the
+// coordinate that set r) equals sqrt(4u(1-u)): that form keeps full relative
+// precision for grazing samples, where 1 - dx^2 - dy^2 cancels.
 FL_HD v3 cosine_hemisphere(float u0, float u1) {
     float ox = u0 * 2.f - 1.f, oy = u1 * 2.f - 1.f;
-    float dx = 0.f, dy = 0.f;
+    float dx = 0.f, dy = 0.f, ur = 0.5f;
     if (!(ox == 0.f && oy == 0.f)) {
         float theta, r;
         if (fabsf(ox) > fabsf(oy)) {
             r = ox;
+            ur = u0;
             theta = kPiOver4 * (oy / ox);
         } else {
             r = oy;
+            ur = u1;
             theta = kPiOver2 - kPiOver4 * (ox / oy);
         }
         dx = cosf(theta) * r;
         dy = sinf(theta) * r;
     }
-    float z = sqrtf(fmaxf(0.f, 1.f - dx * dx - dy * dy));
+    float z = 2.f * sqrtf(fmaxf(0.f, ur * (1.f - ur)));
     return v3(dx, dy, z);
 }
 
@@ -106,22 +111,25 @@ struct GGX {
     float ax, ay;
 };
 
-// GGXDistribution::evaluate_distribution, ggx.cpp:13-21
+// GGXDistribution::evaluate_distribution, ggx.cpp:13-21.  For a unit vector
+// (cos^2 phi/ax^2 + sin^2 phi/ay^2) tan^2 theta = (x^2/ax^2 + y^2/ay^2)/z^2;
+// that form avoids 1 - cos^2 theta, which cancels near the pole where small
+// alphas amplify the error by 1/alpha^2.  z = 0 (tan = inf) gives 0 as in the
+// reference.
 FL_HD float ggx_D(const GGX &g, v3 wh) {
-    float t2 = tan2t(wh);
-    if (isinf(t2)) return 0.f;
-    float c4 = cos2t(wh) * cos2t(wh);
-    float e = (cos2phi(wh) / (g.ax * g.ax) + sin2phi(wh) / (g.ay * g.ay)) * t2;
-    return 1.f / (kPi * g.ax * g.ay * c4 * (1.f + e) * (1.f + e));
+    float z2 = wh.z * wh.z;
+    if (z2 == 0.f) return 0.f;
+    float e = (wh.x * wh.x / (g.ax * g.ax) + wh.y * wh.y / (g.ay * g.ay)) / z2;
+    return 1.f / (kPi * g.ax * g.ay * (z2 * z2) * (1.f + e) * (1.f + e));
 }
 
-// GGXDistribution::lambda, ggx.cpp:23-30
+// GGXDistribution::lambda, ggx.cpp:23-30, with alpha^2 tan^2 theta =
+// (x^2 ax^2 + y^2 ay^2)/z^2 and (-1 + sqrt(1+a))/2 = a / (2 (1 + sqrt(1+a))).
 FL_HD float ggx_lambda(const GGX &g, v3 w) {
-    float at = fabsf(tant(w));
-    if (isinf(at)) return 0.f;
-    float alpha = sqrtf(cos2phi(w) * g.ax * g.ax + sin2phi(w) * g.ay * g.ay);
-    float a2t2 = (alpha * at) * (alpha * at);
-    return (-1.f + sqrtf(1.f + a2t2)) / 2.f;
+    float z2 = w.z * w.z;
+    if (z2 == 0.f) return 0.f;
+    float a2t2 = (w.x * w.x * g.ax * g.ax + w.y * w.y * g.ay * g.ay) / z2;
+    return a2t2 / (2.f * (1.f + sqrtf(1.f + a2t2)));
 }
 
 FL_HD float ggx_G(const GGX &g, v3 wo, v3 wi) { return 1.f / (1.f + ggx_lambda(g, wo) + ggx_lambda(g, wi)); }
@@ -134,21 +142,34 @@ FL_HD float ggx_pdf_wh(const GGX &g, v3 wo, v3 wh, bool vndf) {
     return ggx_D(g, wh) * ggx_G1(g, wo) * fabsf(dot(wo, wh)) / fabsf(wo.z);
 }
 
-// GGXDistribution::sample_wh, ggx.cpp:32-57 (visible-normal sampling)
+// GGXDistribution::sample_wh, ggx.cpp:32-57 (visible-normal sampling),
+// evaluated in fp64: 1 - t1^2 - t2^2 cancels for samples near the rim of the
+// projected disk, and the small-alpha stretch amplifies that error, so fp32
+// cannot stay within 1e-5 of the exact direction there.
 FL_HD v3 ggx_sample_wh(const GGX &g, v3 wo, float u0, float u1) {
-    v3 vh = normalize(v3(wo.x * g.ax, wo.y * g.ay, wo.z));
-    if (vh.z < 0.f) vh = -vh;
-    float lensq = vh.x * vh.x + vh.y * vh.y;
-    v3 T1 = lensq > 0.f ? v3(-vh.y, vh.x, 0.f) / sqrtf(lensq) : v3(1.f, 0.f, 0.f);
-    v3 T2 = cross(vh, T1);
-    float r = sqrtf(u0);
-    float phi = 2.f * kPi * u1;
-    float t1 = r * cosf(phi);
-    float t2 = r * sinf(phi);
-    float s = (1.f + vh.z) * 0.5f;
-    t2 = (1.f - s) * sqrtf(1.f - t1 * t1) + s * t2;
-    v3 Nh = T1 * t1 + T2 * t2 + vh * sqrtf(fmaxf(0.f, 1.f - t1 * t1 - t2 * t2));
-    return normalize(v3(g.ax * Nh.x, g.ay * Nh.y, fmaxf(1e-6f, Nh.z)));
+    const double ax = g.ax, ay = g.ay;
+    double vx = wo.x * ax, vy = wo.y * ay, vz = wo.z;
+    double vl = sqrt(vx * vx + vy * vy + vz * vz);
+    vx /= vl; vy /= vl; vz /= vl;
+    if (vz < 0.0) { vx = -vx; vy = -vy; vz = -vz; }
+    double lensq = vx * vx + vy * vy;
+    double T1x = 1.0, T1y = 0.0;
+    if (lensq > 0.0) { double il = sqrt(lensq); T1x = -vy / il; T1y = vx / il; }
+    // T2 = cross(vh, T1), T1.z = 0
+    double T2x = vy * 0.0 - vz * T1y, T2y = vz * T1x - vx * 0.0, T2z = vx * T1y - vy * T1x;
+    double r = sqrt((double)u0);
+    double phi = 2.0 * 3.14159265358979323846 * (double)u1;
+    double t1 = r * cos(phi);
+    double t2 = r * sin(phi);
+    double s = (1.0 + vz) * 0.5;
+    t2 = (1.0 - s) * sqrt(1.0 - t1 * t1) + s * t2;
+    double nz = sqrt(fmax(0.0, 1.0 - t1 * t1 - t2 * t2));
+    double Nx = T1x * t1 + T2x * t2 + vx * nz;
+    double Ny = T1y * t1 + T2y * t2 + vy * nz;
+    double Nz = 0.0 * t1 + T2z * t2 + vz * nz;
+    double hx = ax * Nx, hy = ay * Ny, hz = fmax((double)1e-6f, Nz);
+    double hl = sqrt(hx * hx + hy * hy + hz * hz);
+    return v3((float)(hx / hl), (float)(hy / hl), (float)(hz / hl));
 }
 
 // FresnelConductor::evaluate with etaI = 1, fresnelconductor.cpp:9-33
@@ -186,6 +207,22 @@ FL_HD float fresnel_dielectric(float cosI, float etaI, float etaT) {
     return (Rparl * Rparl + Rperp * Rperp) * 0.5f;
 }
 
+// fresnel_dielectric in fp64 (same steps)
+FL_HD double fresnel_dielectric_d(double cosI, double etaI, double etaT) {
+    cosI = cosI < -1.0 ? -1.0 : (cosI > 1.0 ? 1.0 : cosI);
+    if (!(cosI > 0.0)) {
+        double t = etaI; etaI = etaT; etaT = t;
+        cosI = fabs(cosI);
+    }
+    double sinI = sqrt(fmax(0.0, 1.0 - cosI * cosI));
+    double sinT = etaI / etaT * sinI;
+    if (sinT >= 1.0) return 1.0;
+    double cosT = sqrt(fmax(0.0, 1.0 - sinT * sinT));
+    double Rparl = (etaT * cosI - etaI * cosT) / (etaT * cosI + etaI * cosT);
+    double Rperp = (etaI * cosI - etaT * cosT) / (etaI * cosI + etaT * cosT);
+    return (Rparl * Rparl + Rperp * Rperp) * 0.5;
+}
+
 // ----------------------------------------------------------------- BSDF ----
 // One lobe per material in every supported material (bsdf.h:41 allows 8).
 struct Bsdf {
@@ -208,14 +245,16 @@ FL_HD uint32_t lobe_type(int kind) {
 
 FL_HD bool is_specular_kind(int kind) { return kind == MAT_DIELECTRIC; }
 
-// MicrofacetReflectionBRDF::evaluate, microfacetreflection.cpp:13-32 (R = 1)
-FL_HD void microfacet_f(const Bsdf &b, v3 wo, v3 wi, float f[4]) {
+// MicrofacetReflectionBRDF::evaluate, microfacetreflection.cpp:13-32 (R = 1).
+// `whp` (optional) is the half vector when it is already known exactly (a
+// sampled microfacet normal): normalize(wo + wi) cancels for grazing pairs.
+FL_HD void microfacet_f(const Bsdf &b, v3 wo, v3 wi, float f[4], const v3 *whp = nullptr) {
     float cO = fabsf(wo.z), cI = fabsf(wi.z);
     v3 wh = wi + wo;
     for (int k = 0; k < 4; ++k) f[k] = 0.f;
     if (cI == 0.f || cO == 0.f) return;
     if (wh.x == 0.f && wh.y == 0.f && wh.z == 0.f) return;
-    wh = normalize(wh);
+    wh = whp ? *whp : normalize(wh);
     float cosF = dot(wi, face_forward(wh, v3(0.f, 0.f, 1.f)));
     float D = ggx_D(b.ggx, wh);
     float G = ggx_G(b.ggx, wo, wi);
@@ -262,6 +301,7 @@ FL_HD float bsdf_pdf_nonspec(const Bsdf &b, v3 wo, v3 wi) {
 // the caller passes ng = (0,0,1) semantics, see bsdf_sample_local.
 struct BsdfSample {
     v3 wi;
+    v3 wh;               // sampled microfacet normal (conductor)
     float f[4];
     float pdf;
     uint32_t type;
@@ -290,14 +330,18 @@ FL_HD void lobe_sample(const Bsdf &b, v3 wo, float u0, float u1, BsdfSample &s) 
         v3 wi = reflect(wo, wh);
         if (!same_hemi(wo, wi)) return;
         s.wi = wi;
+        s.wh = wh;
         s.pdf = ggx_pdf_wh(b.ggx, wo, wh, b.vndf_pdf) / (4.f * dot(wo, wh));
-        microfacet_f(b, wo, wi, s.f);
+        microfacet_f(b, wo, wi, s.f, &wh);
     } else if (b.kind == MAT_DIELECTRIC) {
         // FresnelSpecularBXDF::sample, fresnelspecular.cpp:21-48; the hero lane
         // (lane 0) decides; a wavelength-dependent eta terminates lanes 1..3.
+        // F, 1-F and the refracted direction are formed in fp64: 1-F and cos(theta_t)
+        // cancel near grazing incidence / the critical angle.
         float ior = b.eta[0];
         bool dispersive = (b.eta[1] != ior) || (b.eta[2] != ior) || (b.eta[3] != ior);
-        float F = fresnel_dielectric(wo.z, 1.f, ior);
+        double Fd = fresnel_dielectric_d(wo.z, 1.0, ior);
+        float F = (float)Fd;
         if (u0 < F) {
             v3 wi = v3(-wo.x, -wo.y, wo.z);
             s.type = BSDF_SPECULAR | BSDF_REFLECTION;
@@ -306,15 +350,22 @@ FL_HD void lobe_sample(const Bsdf &b, v3 wo, float u0, float u1, BsdfSample &s) 
             for (int k = 0; k < 4; ++k) s.f[k] = b.r[k] * F / fabsf(wi.z);
         } else {
             bool entering = wo.z > 0.f;
-            float etaI = entering ? 1.f : ior;
-            float etaT = entering ? ior : 1.f;
-            v3 wi;
-            if (!refract(wo, face_forward(v3(0.f, 0.f, 1.f), wo), etaI / etaT, &wi)) return;
+            double etaI = entering ? 1.0 : (double)ior;
+            double etaT = entering ? (double)ior : 1.0;
+            double eta = etaI / etaT;
+            double nz = entering ? 1.0 : -1.0;   // face_forward((0,0,1), wo)
+            double cosI = (double)wo.z * nz;
+            double sinT2 = eta * eta * fmax(0.0, 1.0 - cosI * cosI);
+            if (sinT2 >= 1.0) return;
+            double cosT = sqrt(1.0 - sinT2);
+            double k2 = eta * cosI - cosT;
+            v3 wi((float)(-eta * wo.x), (float)(-eta * wo.y), (float)(-eta * (double)wo.z + nz * k2));
+            double T = 1.0 - Fd;
+            double sc = (etaI * etaI) / (etaT * etaT) / fabs(-eta * (double)wo.z + nz * k2);
             s.type = BSDF_SPECULAR | BSDF_TRANSMISSION;
             s.wi = wi;
-            s.pdf = 1.f - F;
-            for (int k = 0; k < 4; ++k)
-                s.f[k] = (b.t[k] * (1.f - F)) * ((etaI * etaI) / (etaT * etaT)) / fabsf(wi.z);
+            s.pdf = (float)T;
+            for (int k = 0; k < 4; ++k) s.f[k] = (float)((double)b.t[k] * T * sc);
         }
         s.secondary_terminated = dispersive;
     }

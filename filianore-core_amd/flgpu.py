@@ -225,6 +225,27 @@ class GenSceneData:
         finally:
             L.fl_gen_free(C.byref(g))
 
+    @classmethod
+    def from_arrays(cls, vertices, indices, tri_material, materials, lights, emission, camera, spp=16, depth=8):
+        """A scene from explicit buffers: materials is a list of Material,
+        lights a list of (tri, emission_index, intensity), emission an
+        (n, FL_SPD_N) array, camera a Camera."""
+        self = cls.__new__(cls)
+        self.vertices = np.ascontiguousarray(vertices, np.float32).reshape(-1, 3)
+        self.indices = np.ascontiguousarray(indices, np.uint32).reshape(-1, 3)
+        self.tri_material = np.ascontiguousarray(tri_material, np.uint32)
+        self.materials = (Material * len(materials))(*materials)
+        ls = [AreaLight(t, e, i, 0) for (t, e, i) in lights]
+        self.lights = (AreaLight * max(1, len(ls)))(*ls)
+        self.n_lights = len(ls)
+        em = np.ascontiguousarray(emission, np.float32).reshape(-1, FL_SPD_N)
+        self.emission = em.ravel().copy()
+        self.n_emission = em.shape[0]
+        self.camera = camera
+        self.spp = spp
+        self.depth = depth
+        return self
+
     @property
     def n_triangles(self):
         return self.indices.shape[0]

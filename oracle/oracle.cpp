@@ -449,7 +449,7 @@ struct Bsdf {
         int match = num_components(flags);                                   // bsdf.cpp:52-125
         if (match == 0) { *pdf = 0; *st = 0; return Spec<T, N>(0); }
         int comp = std::min((int)std::floor(u0 * match), match - 1);
-        T ur = std::min(u0 * match - comp, T(1) - T(SHADOW_EPSILON));
+        T ur = std::min(u0 * match - comp, T(1.f - SHADOW_EPSILON));   // float constant, bsdf.cpp:75
         V3<T> wi, wo = to_local(woW);
         if (wo.z == 0) { *pdf = 0; *st = 0; return Spec<T, N>(0); }
         *pdf = 0;

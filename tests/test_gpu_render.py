@@ -28,8 +28,10 @@ def z_stats(gs, gq, os_, oq, n):
     frac = float(np.mean(np.abs(z[valid]) > 3)) if valid.any() else 0.0
     npx = int(valid.sum())
     allowed = 0.0027 + 3 * np.sqrt(0.0027 * 0.9973 / max(npx, 1))
-    # image mean: per-pixel means are independent -> variance adds
-    mean_z = float((gm.mean() - om.mean()) / np.sqrt(np.sum(gv / n + ov / n)) * gm.size)
+    # image mean of Y: pixels are independent, so variances add (X, Y, Z of one
+    # pixel are strongly correlated, so they are not pooled)
+    y = (Ellipsis, 1)
+    mean_z = float((gm[y].mean() - om[y].mean()) / np.sqrt(np.sum(gv[y] / n + ov[y] / n)) * gm[y].size)
     return frac, allowed, mean_z, int((~valid & ~both_zero).sum())
 
 
