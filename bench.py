@@ -34,6 +34,17 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
+def traffic_from_profile():
+    """HBM bytes per k_trace launch measured with rocprofv3 PMC counters
+    (FETCH_SIZE + WRITE_SIZE, scaled by a copy-kernel calibration), recorded
+    in profiles/traffic_k_trace_spheres.json; None until measured."""
+    path = os.path.join(ROOT, "profiles", "traffic_k_trace_spheres.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        return json.load(f).get("hbm_bytes_per_launch")
+
+
 def host_threads():
     n = os.cpu_count() or 1
     try:
@@ -119,19 +130,21 @@ def main():
             cp = flgpu.default_params(spp=spp, depth=gen.depth, mode=flgpu.FL_SPECTRAL_HERO, seed=1,
                                       flags=flgpu.FL_RENDER_VNDF_PDF | flgpu.FL_RENDER_COUNTERS)
             _, _, cst = sc.render(cp, want_film=False)
-            ext_bytes = cst.rays_extension * (32 + 16) + cst.nodes_visited * 128 + cst.tris_tested * 48
+            # k_trace: per bounce, the previous bounce's shadow rays + this bounce's extension rays
+            n_rays = cst.rays_extension + cst.rays_shadow
+            trace_bytes = n_rays * (32 + 16) + cst.nodes_visited * 128 + cst.tris_tested * 48
             launches = max(1, tst.launches_extend)
-            achieved = ext_bytes / (tst.ms_extend * 1e-3) / 1e9
+            achieved = trace_bytes / (tst.ms_extend * 1e-3) / 1e9
             roofline = {
                 "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                "kernel": "k_extend (closest-hit traversal)",
-                "bytes_per_launch": int(ext_bytes / launches),
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic_from_profile(),
+                "kernel": "k_trace (BVH traversal: closest-hit extension + any-hit shadow rays)",
+                "bytes_per_launch": int(trace_bytes / launches),
                 "avg_launch_ms": round(tst.ms_extend / launches, 4),
-                "nodes_per_ray": round(cst.nodes_visited / max(1, cst.rays_extension), 2),
-                "tris_per_ray": round(cst.tris_tested / max(1, cst.rays_extension), 2),
-                "ms_extend": round(tst.ms_extend, 2), "ms_shade": round(tst.ms_shade, 2),
-                "ms_shadow": round(tst.ms_shadow, 2),
+                "nodes_per_ray": round(cst.nodes_visited / max(1, n_rays), 2),
+                "tris_per_ray": round(cst.tris_tested / max(1, n_rays), 2),
+                "ms_trace": round(tst.ms_extend, 2), "ms_shade": round(tst.ms_shade, 2),
+                "launches": launches,
             }
             out["mrays_per_s"] = round(rays * args.steps / wall / 1e6, 1)
             out["rays_per_sample"] = round(rays / samples, 3)

@@ -2,16 +2,20 @@
 // every camera sample, integrators/pathintegrator.cpp:12-87, with
 // uniform_sample_all_lights/estimate_direct, integrators/integrator.cpp:10-110).
 //
-// Kernels per pass:  generate -> { extend -> shade -> shadow } x bounces -> resolve
+// Kernels per pass:   generate -> { trace -> shade } x bounces -> trace -> resolve
 //   generate : camera rays (PhysicalCamera::awaken_ray, cameras/physical.cpp:12-22,
 //              pinhole), hero wavelengths, path state init
-//   extend   : closest hit of the ray queue (persistent, LDS short stack)
+//   trace    : one persistent launch per bounce over the previous bounce's
+//              shadow rays (any hit, emitters transparent: VisibilityEvaluator::
+//              unoccluded, illuminants/illuminant.cpp:40-50) followed by this
+//              bounce's extension rays (closest hit); lanes refill as rays end
 //   shade    : emission (+MIS), NEE light samples -> shadow queue, BSDF
 //              sample -> next ray queue, depth limits, Russian roulette
-//   shadow   : any hit ignoring emitters (VisibilityEvaluator::unoccluded,
-//              illuminants/illuminant.cpp:40-50); occluded NEE terms are zeroed
 //   resolve  : hero lanes -> CIE XYZ, per-pixel sums in a fixed order
 // Queues are SoA arrays appended with one atomic per wave (ballot + popcount).
+// Queue counters live in device memory and are double-buffered by bounce
+// parity, so every kernel reads its work size from the device (the bounce loop
+// needs no host round trip and can be captured in a graph).
 // The MIS BSDF ray of estimate_direct is the next extension ray (same
 // expectation, one traversal fewer per light per vertex).
 #include <algorithm>
@@ -25,7 +29,12 @@ namespace {
 using namespace fl;
 
 constexpr int kBlock = 256;
-constexpr int kStack = 16;
+constexpr int kStack = 12;
+
+// device counter slots
+constexpr int C_EXT = 0;    // [0],[1]: extension queue sizes by bounce parity
+constexpr int C_SH = 2;     // [2],[3]: shadow queue sizes by the parity of the bounce that wrote them
+constexpr int C_FETCH = 4;  // [4],[5]: trace refill cursors by bounce parity
 
 __device__ __forceinline__ uint32_t wave_append(bool want, uint32_t *counter) {
     unsigned long long mask = __ballot(want);
@@ -83,15 +92,15 @@ __global__ void __launch_bounds__(kBlock) k_generate(RenderArgs a) {
             a.st_L[sid] = make_float4(0.f, 0.f, 0.f, 0.f);
             a.st_misc[sid] = make_float4(lam0, 1.f, 0.f, __uint_as_float(0u));
             a.st_key[sid] = make_uint2(pixel, sample);
-            a.pend[sid] = make_float4(0.f, 0.f, 0.f, 0.f);
-            a.pend[a.cap + sid] = make_float4(0.f, 0.f, 0.f, 0.f);
         } else if (sid < a.n_paths) {
             a.st_L[sid] = make_float4(0.f, 0.f, 0.f, 0.f);
             a.st_misc[sid] = make_float4(0.f, 1.f, 0.f, __uint_as_float(PF_DONE));
+        }
+        if (sid < a.n_paths) {
             a.pend[sid] = make_float4(0.f, 0.f, 0.f, 0.f);
             a.pend[a.cap + sid] = make_float4(0.f, 0.f, 0.f, 0.f);
         }
-        uint32_t slot = wave_append(active, &a.counts[0]);
+        uint32_t slot = wave_append(active, &a.counts[C_EXT + 0]);
         if (active) {
             a.rq_o[0][slot] = make_float4(o.x, o.y, o.z, 0.f);       // tMin = time = 0 (physical.cpp:21)
             a.rq_d[0][slot] = make_float4(d.x, d.y, d.z, kInf);
@@ -100,65 +109,60 @@ __global__ void __launch_bounds__(kBlock) k_generate(RenderArgs a) {
     }
 }
 
-// --------------------------------------------------------------- extend ----
-template <bool COUNT>
-__global__ void __launch_bounds__(kBlock) k_extend(RenderArgs a, int cur) {
-    __shared__ uint2 stk[kStack * kBlock];
-    const uint32_t gtid = blockIdx.x * kBlock + threadIdx.x;
-    const uint32_t nthreads = gridDim.x * kBlock;
-    const uint32_t n = a.counts[cur];
-    if (gtid == 0) {
-        a.counts[cur ^ 1] = 0u;
-        a.counts[2] = 0u;
-        atomicAdd(&a.stats[0], (unsigned long long)n);
+// ---------------------------------------------------------------- trace ----
+// Index space: [0, ns) shadow rays written by the previous bounce's shade,
+// [ns, ns + ne) this bounce's extension rays.
+struct TraceSrc {
+    const float4 *so, *sd;
+    const uint32_t *stgt;
+    float4 *pend;
+    const float4 *eo, *ed;
+    float4 *hits;
+    uint32_t ns;
+    __device__ void load(uint32_t i, v3 &ro, v3 &rd, float &tmin, float &tmax, bool &any) const {
+        float4 a, b;
+        if (i < ns) { a = so[i]; b = sd[i]; any = true; }
+        else { a = eo[i - ns]; b = ed[i - ns]; any = false; }
+        ro = v3(a.x, a.y, a.z);
+        rd = v3(b.x, b.y, b.z);
+        tmin = a.w;
+        tmax = b.w;
     }
-    LaneStack ls;
-    ls.lds = stk + threadIdx.x;
-    ls.lds_stride = kBlock;
-    ls.cap = kStack;
-    ls.spill = a.spill + gtid;
-    ls.spill_stride = a.spill_stride;
-    uint32_t cn = 0, ct = 0, cs = 0;
-    for (uint32_t i = gtid; i < n; i += nthreads) {
-        float4 ro = a.rq_o[cur][i], rd = a.rq_d[cur][i];
-        TraceHit h;
-        bool hit = traverse<false, false, COUNT>(a.sc, v3(ro.x, ro.y, ro.z), v3(rd.x, rd.y, rd.z), ro.w, rd.w, ls, h,
-                                                 cn, ct, cs);
-        a.hits[i] = make_float4(h.t, __uint_as_float(hit ? h.rec : kNoHit), h.u, h.v);
+    __device__ void store(uint32_t i, const TravState &s) const {
+        if (i < ns) {
+            if (s.best_id != kNoHit) pend[stgt[i]] = make_float4(0.f, 0.f, 0.f, 0.f);   // occluded NEE term
+        } else {
+            hits[i - ns] = make_float4(s.best_t, __uint_as_float(s.best_id != kNoHit ? s.best_rec : kNoHit), s.bu, s.bv);
+        }
     }
-    if (COUNT) {
-        atomicAdd(&a.stats[3], (unsigned long long)cn);
-        atomicAdd(&a.stats[4], (unsigned long long)ct);
-        atomicAdd(&a.stats[5], (unsigned long long)cs);
-    }
-}
+};
 
-// --------------------------------------------------------------- shadow ----
 template <bool COUNT>
-__global__ void __launch_bounds__(kBlock) k_shadow(RenderArgs a) {
+__global__ void __launch_bounds__(kBlock) k_trace(RenderArgs a, int b) {
     __shared__ uint2 stk[kStack * kBlock];
-    const uint32_t gtid = blockIdx.x * kBlock + threadIdx.x;
-    const uint32_t nthreads = gridDim.x * kBlock;
-    const uint32_t n = a.counts[2];
-    if (gtid == 0) atomicAdd(&a.stats[1], (unsigned long long)n);
+    const int cur = b & 1;
+    const uint32_t ns = a.counts[C_SH + (cur ^ 1)];
+    const uint32_t ne = a.counts[C_EXT + cur];
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        a.counts[C_EXT + (cur ^ 1)] = 0u;        // shade(b) appends the next extension queue here
+        a.counts[C_SH + cur] = 0u;               // and its shadow rays here
+        a.counts[C_FETCH + (cur ^ 1)] = 0u;      // refill cursor of trace(b+1)
+        atomicAdd(&a.stats[0], (unsigned long long)ne);
+        atomicAdd(&a.stats[1], (unsigned long long)ns);
+    }
     LaneStack ls;
     ls.lds = stk + threadIdx.x;
     ls.lds_stride = kBlock;
     ls.cap = kStack;
-    ls.spill = a.spill + gtid;
+    ls.spill = a.spill + blockIdx.x * kBlock + threadIdx.x;
     ls.spill_stride = a.spill_stride;
-    uint32_t cn = 0, ct = 0, cs = 0;
-    for (uint32_t i = gtid; i < n; i += nthreads) {
-        float4 ro = a.sq_o[i], rd = a.sq_d[i];
-        TraceHit h;
-        bool occluded = traverse<true, true, COUNT>(a.sc, v3(ro.x, ro.y, ro.z), v3(rd.x, rd.y, rd.z), ro.w, rd.w, ls,
-                                                    h, cn, ct, cs);
-        if (occluded) a.pend[a.sq_tgt[i]] = make_float4(0.f, 0.f, 0.f, 0.f);
-    }
+    TravCounters c;
+    TraceSrc src{a.sq_o, a.sq_d, a.sq_tgt, a.pend, a.rq_o[cur], a.rq_d[cur], a.hits, ns};
+    trace_persistent<COUNT, true>(a.sc, src, ns + ne, &a.counts[C_FETCH + cur], ls, a.trav_budget, c);
     if (COUNT) {
-        atomicAdd(&a.stats[6], (unsigned long long)cn);
-        atomicAdd(&a.stats[7], (unsigned long long)ct);
-        atomicAdd(&a.stats[5], (unsigned long long)cs);
+        atomicAdd(&a.stats[3], (unsigned long long)c.nodes);
+        atomicAdd(&a.stats[4], (unsigned long long)c.tris);
+        atomicAdd(&a.stats[5], (unsigned long long)c.spills);
     }
 }
 
@@ -173,65 +177,97 @@ __device__ __forceinline__ void setup_bsdf(const RenderArgs &a, const DevMateria
     const float *ta = a.spec + (size_t)m.spec_a * S;
     const float *tb = a.spec + (size_t)m.spec_b * S;
     for (int k = 0; k < 4; ++k) {
-        b.r[k] = 0.f; b.t[k] = 0.f; b.eta[k] = 1.f; b.k[k] = 0.f;
-        if (m.kind == MAT_LAMBERT) {
-            b.r[k] = spec_at(ta, ln, k);
-        } else if (m.kind == MAT_CONDUCTOR) {
-            b.eta[k] = spec_at(ta, ln, k);
-            b.k[k] = spec_at(tb, ln, k);
-        } else if (m.kind == MAT_DIELECTRIC) {
-            b.r[k] = spec_at(ta, ln, k);
-            b.t[k] = spec_at(tb, ln, k);
-            float lum = ln.lam[k] * 1e-3f;                      // um
-            b.eta[k] = m.cauchy_b != 0.f ? m.eta_bsdf + m.cauchy_b / (lum * lum) : m.eta_bsdf;
-        }
+        b.s1[k] = spec_at(ta, ln, k);
+        b.s2[k] = m.kind == MAT_LAMBERT ? 0.f : spec_at(tb, ln, k);
     }
+    // dielectric: Cauchy n(lambda) = A + B / lambda^2 [um] at the hero wavelength
+    float lum = ln.lam[0] * 1e-3f;
+    b.ior = m.cauchy_b != 0.f ? m.eta_bsdf + m.cauchy_b / (lum * lum) : m.eta_bsdf;
+    b.dispersive = m.cauchy_b != 0.f;
 }
 
-__global__ void __launch_bounds__(kBlock) k_shade(RenderArgs a, int cur) {
+// One NEE sample of light `li` (estimate_direct's light half, integrator.cpp:29-59):
+// returns true with the (unoccluded) contribution and the shadow segment.
+__device__ __forceinline__ bool nee_sample(const RenderArgs &a, const Bsdf &b, const Lanes &ln, uint32_t li, float sel,
+                                           float ux, float uy, v3 p, v3 perr, v3 ng, v3 fs, v3 ft, v3 woW, v3 wo,
+                                           const float beta[4], float contrib[4], float4 &sh_o, float4 &sh_d) {
+    const DevLight &Lj = a.lights[li];
+    // Triangle::sample + Shape::sample(ref), triangle.cpp:134-154, shape.cpp:28-41
+    float uu = sqrtf(ux);
+    float u1 = 1.f - uu, u2 = uy * uu;
+    v3 lp0 = ld3(Lj.p0), lp1 = ld3(Lj.p1), lp2 = ld3(Lj.p2), lnn = ld3(Lj.n);
+    v3 ps = lp0 * u1 + lp1 * u2 + lp2 * (1.f - u1 - u2);
+    v3 wun = ps - p;
+    if (length2(wun) == 0.f) return false;
+    v3 wi = normalize(wun);
+    float pdf_l = (1.f / Lj.area) * (length2(p - ps) / absdot(lnn, -wi));
+    if (isinf(pdf_l)) pdf_l = 0.f;
+    pdf_l *= sel;
+    if (!(pdf_l > 0.f)) return false;
+    if (!(dot(lnn, -wi) > 0.f)) return false;                       // DiffuseAreaIlluminant::L
+    v3 wil(dot(wi, fs), dot(wi, ft), dot(wi, ng));
+    bool refl = dot(wi, ng) * dot(woW, ng) > 0.f;
+    float f[4];
+    bsdf_eval_nonspec(b, wo, wil, refl, f);
+    float cw = absdot(wi, ng);
+    float pb = bsdf_pdf_nonspec(b, wo, wil);
+    bool black = true;
+    for (int k = 0; k < 4; ++k) { f[k] = f[k] * cw; black = black && f[k] == 0.f; }
+    if (black) return false;
+    float w = power_heuristic(pdf_l, pb);
+    const float *te = a.spec + (size_t)Lj.em_slot * a.grid.stride;
+    for (int k = 0; k < 4; ++k) contrib[k] = beta[k] * f[k] * (spec_at(te, ln, k) * Lj.intensity) * w / pdf_l;
+    v3 so = offset_point_origin(p, perr, ng, wun);                 // kindle_ray_to
+    sh_o = make_float4(so.x, so.y, so.z, kShadowEps);
+    sh_d = make_float4(wun.x, wun.y, wun.z, 1.f - kShadowEps);
+    return true;
+}
+
+// WAVES: minimum waves per SIMD requested from the register allocator (2 lets
+// it use all the registers it wants; 4 caps at 128 VGPRs with a few spills).
+template <int WAVES>
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WAVES, 8))) k_shade(RenderArgs a, int b) {
     const uint32_t stride = gridDim.x * kBlock;
-    const uint32_t n = a.counts[cur];
+    const int cur = b & 1, nxt = cur ^ 1;
+    const uint32_t n = a.counts[C_EXT + cur];
     const uint32_t nround = (n + kBlock - 1) / kBlock * kBlock;
-    const int nxt = cur ^ 1;
     const int S = a.grid.stride;
     for (uint32_t base = blockIdx.x * kBlock; base < nround; base += stride) {
         const uint32_t i = base + threadIdx.x;
         const bool active = i < n;
-        bool cont = false;
-        bool shadow_ok[2] = {false, false};
-        float4 sh_o[2], sh_d[2];
-        v3 no, nd;
-        uint32_t sid = 0;
+        // -------- phase 1: hit, emission, pass-through, material setup
+        bool cont = false, mat_hit = false;
+        uint32_t sid = 0, bounce = 0, flags = 0, rb = 0, mkind = 0;
+        float beta[4] = {0.f, 0.f, 0.f, 0.f}, L[4] = {0.f, 0.f, 0.f, 0.f};
+        float lam0 = 0.f, eta_scale = 1.f, prev_pdf = 0.f, eta_bsdf = 1.f;
+        v3 p, ng, d, no, nd, perr;
+        Lanes ln;
         if (active) {
             float4 ro = a.rq_o[cur][i], rd = a.rq_d[cur][i];
             sid = a.rq_sid[cur][i];
             float4 hh = a.hits[i];
             float4 beta4 = a.st_beta[sid], L4 = a.st_L[sid], misc = a.st_misc[sid];
-            uint2 key = a.st_key[sid];
             float4 pe0 = a.pend[sid], pe1 = a.pend[a.cap + sid];
-            float beta[4] = {beta4.x, beta4.y, beta4.z, beta4.w};
-            float L[4] = {L4.x + pe0.x + pe1.x, L4.y + pe0.y + pe1.y, L4.z + pe0.z + pe1.z, L4.w + pe0.w + pe1.w};
-            float np0[4] = {0.f, 0.f, 0.f, 0.f}, np1[4] = {0.f, 0.f, 0.f, 0.f};
-            float lam0 = misc.x, eta_scale = misc.y, prev_pdf = misc.z;
+            beta[0] = beta4.x; beta[1] = beta4.y; beta[2] = beta4.z; beta[3] = beta4.w;
+            L[0] = L4.x + pe0.x + pe1.x; L[1] = L4.y + pe0.y + pe1.y;
+            L[2] = L4.z + pe0.z + pe1.z; L[3] = L4.w + pe0.w + pe1.w;
+            lam0 = misc.x; eta_scale = misc.y; prev_pdf = misc.z;
             uint32_t bits = __float_as_uint(misc.w);
-            uint32_t bounce = bits & 0xffu;
-            uint32_t flags = bits & ~0xffu;
+            bounce = bits & 0xffu;
+            flags = bits & ~0xffu;
             const uint32_t rec = __float_as_uint(hh.y);
             const float t = hh.x;
-            bool alive = rec != kNoHit;
-            v3 o(ro.x, ro.y, ro.z), d(rd.x, rd.y, rd.z);
-            if (alive) {
+            d = v3(rd.x, rd.y, rd.z);
+            if (rec != kNoHit) {
                 const uint4 *tp = reinterpret_cast<const uint4 *>(a.sc.tris + rec);
                 uint4 q0 = tp[0], q1 = tp[1], q2 = tp[2];
                 v3 p0(__uint_as_float(q0.x), __uint_as_float(q0.y), __uint_as_float(q0.z));
                 v3 p1(__uint_as_float(q1.x), __uint_as_float(q1.y), __uint_as_float(q1.z));
                 v3 p2(__uint_as_float(q2.x), __uint_as_float(q2.y), __uint_as_float(q2.z));
                 const uint32_t mat = q1.w, light = q2.w;
-                v3 p = o + d * t;                                  // ray.point_at_t(t)
-                v3 ng = normalize(cross(p1 - p0, p2 - p0));
-                Lanes ln = make_lanes(a.grid, lam0);
-                const uint32_t rb = rng_base(a.seed, key.x, key.y, bounce);
-
+                p = v3(ro.x, ro.y, ro.z) + d * t;                  // ray.point_at_t(t)
+                ng = normalize(cross(p1 - p0, p2 - p0));
+                ln = make_lanes(a.grid, lam0);
                 // emission: L += beta * le(-dir) at bounce 0 / after specular; MIS otherwise
                 if (light) {
                     const DevLight &Lj = a.lights[light - 1];
@@ -254,158 +290,131 @@ __global__ void __launch_bounds__(kBlock) k_shade(RenderArgs a, int cur) {
                         }
                     }
                 }
+                perr = vabs(p) * gamma_n(5);
                 if (flags & PF_LAST) {
-                    alive = false;
+                    // the MIS ray of the last vertex: emission counted, path ends
                 } else if (mat == kNoMaterial) {
                     // no BSDF: isect.kindle_ray(ray.dir), bounces-- (pathintegrator.cpp:34-38)
-                    v3 perr = vabs(p) * gamma_n(5);
                     no = offset_point_origin(p, perr, ng, d);
                     nd = d;
                     flags |= PF_PASSED;
                     cont = true;
                 } else {
                     flags &= ~PF_PASSED;
-                    const DevMaterial m = a.mats[mat];
-                    Bsdf b;
-                    setup_bsdf(a, m, ln, b);
-                    // BSDF frame, bsdf.cpp:6-28 (ns = ng: no shading normals)
-                    v3 ns = ng, fs, ft;
-                    coordinate_system(ns, &fs, &ft);
-                    fs = normalize(fs);
-                    v3 woW = -d;
-                    v3 wo(dot(woW, fs), dot(woW, ft), dot(woW, ns));
-                    v3 perr = vabs(p) * gamma_n(5);
-                    const bool nonspec = !is_specular_kind(m.kind);
-                    // ---- NEE, one sample per light (uniform_sample_all_lights)
-                    if (nonspec && a.n_lights > 0) {
-                        const int nslots = a.nee_all ? (int)a.n_lights : 1;
-                        for (int j = 0; j < nslots; ++j) {
-                            uint32_t li = (uint32_t)j;
-                            float sel = 1.f;
-                            if (!a.nee_all) {
-                                li = min((uint32_t)(rng_u(rb, 7) * (float)a.n_lights), a.n_lights - 1u);
-                                sel = 1.f / (float)a.n_lights;
-                            }
-                            const DevLight &Lj = a.lights[li];
-                            float ux = rng_u(rb, 3 + 2 * j), uy = rng_u(rb, 4 + 2 * j);
-                            // Triangle::sample, triangle.cpp:134-154
-                            float uu = sqrtf(ux);
-                            float u1 = 1.f - uu, u2 = uy * uu;
-                            v3 lp0 = ld3(Lj.p0), lp1 = ld3(Lj.p1), lp2 = ld3(Lj.p2), ln_ = ld3(Lj.n);
-                            v3 ps = lp0 * u1 + lp1 * u2 + lp2 * (1.f - u1 - u2);
-                            v3 wun = ps - p;
-                            float dist2 = length2(wun);
-                            if (dist2 == 0.f) continue;
-                            v3 wi = normalize(wun);
-                            float pdf_l = (1.f / Lj.area) * (length2(p - ps) / absdot(ln_, -wi));
-                            if (isinf(pdf_l)) pdf_l = 0.f;
-                            pdf_l *= sel;
-                            if (!(pdf_l > 0.f)) continue;
-                            if (!(dot(ln_, -wi) > 0.f)) continue;           // DiffuseAreaIlluminant::L
-                            v3 wil(dot(wi, fs), dot(wi, ft), dot(wi, ns));
-                            bool refl = dot(wi, ng) * dot(woW, ng) > 0.f;
-                            float f[4];
-                            bsdf_eval_nonspec(b, wo, wil, refl, f);
-                            float cw = absdot(wi, ns);
-                            float pb = bsdf_pdf_nonspec(b, wo, wil);
-                            bool black = true;
-                            for (int k = 0; k < 4; ++k) { f[k] = f[k] * cw; black = black && f[k] == 0.f; }
-                            if (black) continue;
-                            float w = power_heuristic(pdf_l, pb);
-                            const float *te = a.spec + (size_t)Lj.em_slot * S;
-                            float *np = (j == 0) ? np0 : np1;
-                            for (int k = 0; k < 4; ++k)
-                                np[k] = beta[k] * f[k] * (spec_at(te, ln, k) * Lj.intensity) * w / pdf_l;
-                            v3 so = offset_point_origin(p, perr, ng, wun);        // kindle_ray_to
-                            sh_o[j] = make_float4(so.x, so.y, so.z, kShadowEps);
-                            sh_d[j] = make_float4(wun.x, wun.y, wun.z, 1.f - kShadowEps);
-                            shadow_ok[j] = true;
-                        }
-                    }
-                    // ---- BSDF sample (continuation and MIS BSDF ray)
-                    BsdfSample bs;
-                    float u0 = remap_u0(rng_u(rb, 0)), u1 = rng_u(rb, 1);
-                    if (wo.z == 0.f) {
-                        bs.pdf = 0.f;
-                    } else {
-                        lobe_sample(b, wo, u0, u1, bs);
-                    }
-                    bool ok = bs.pdf != 0.f;
-                    v3 wiW;
-                    if (ok) {
-                        wiW = fs * bs.wi.x + ft * bs.wi.y + ns * bs.wi.z;   // to_world
-                        if (!(bs.type & BSDF_SPECULAR)) {
-                            bool refl = dot(wiW, ng) * dot(woW, ng) > 0.f;
-                            if (!refl) {
-                                for (int k = 0; k < 4; ++k) bs.f[k] = 0.f;
-                            } else if (b.kind == MAT_CONDUCTOR) {
-                                microfacet_f(b, wo, bs.wi, bs.f, &bs.wh);
-                            }
-                        }
-                        bool black = true;
-                        for (int k = 0; k < 4; ++k) black = black && bs.f[k] == 0.f;
-                        if (black) ok = false;
-                    }
-                    if (ok) {
-                        const uint32_t ty = bs.type;
-                        bool brk = (a.depth_d == (int)bounce && (ty & BSDF_DIFFUSE)) ||
-                                   (a.depth_r == (int)bounce && (ty & (BSDF_SPECULAR | BSDF_GLOSSY))) ||
-                                   (a.depth_t == (int)bounce && (ty & BSDF_TRANSMISSION));
-                        if (brk && (ty & BSDF_SPECULAR)) ok = false;
-                        if (ok) {
-                            if (brk) flags |= PF_LAST;
-                            float cw = absdot(wiW, ns);
-                            for (int k = 0; k < 4; ++k) beta[k] = beta[k] * bs.f[k] * cw / bs.pdf;
-                            if (ty & BSDF_SPECULAR) flags |= PF_SPECULAR; else flags &= ~PF_SPECULAR;
-                            if ((ty & BSDF_SPECULAR) && (ty & BSDF_TRANSMISSION)) {
-                                float e = m.eta_bsdf;
-                                eta_scale *= (dot(woW, ng) > 0.f) ? (e * e) : 1.f / (e * e);
-                            }
-                            if (bs.secondary_terminated) flags |= PF_SECTERM;
-                            no = offset_point_origin(p, perr, ng, wiW);          // kindle_ray
-                            nd = wiW;
-                            prev_pdf = (ty & BSDF_SPECULAR) ? 0.f : bs.pdf;
-                            cont = true;
-                            if (!brk) {
-                                // Russian roulette, pathintegrator.cpp:77-83
-                                float mx = beta[0] * eta_scale;
-                                if (!(flags & PF_SECTERM))
-                                    for (int k = 1; k < 4; ++k) mx = fmaxf(mx, beta[k] * eta_scale);
-                                if (mx < a.rr_threshold && (int)bounce > (int)a.rr_start) {
-                                    float q = fmaxf(0.05f, 1.f - mx);
-                                    if (rng_u(rb, 2) < q) cont = false;
-                                    else for (int k = 0; k < 4; ++k) beta[k] = beta[k] / (1.f - q);
-                                }
-                            }
-                            bounce += 1;
-                        }
-                    }
+                    mat_hit = true;
+                    mkind = mat;
                 }
             }
+        }
+        // -------- phase 2: NEE, one sample per light (uniform_sample_all_lights)
+        Bsdf bs_;
+        v3 fs, ft, woW, wo;
+        if (mat_hit) {
+            const DevMaterial m = a.mats[mkind];
+            setup_bsdf(a, m, ln, bs_);
+            eta_bsdf = m.eta_bsdf;
+            coordinate_system(ng, &fs, &ft);                       // BSDF frame, bsdf.cpp:6-10 (ns = ng)
+            fs = normalize(fs);
+            woW = -d;
+            wo = v3(dot(woW, fs), dot(woW, ft), dot(woW, ng));
+            rb = rng_base(a.seed, a.st_key[sid].x, a.st_key[sid].y, bounce);
+        }
+        const bool do_nee = mat_hit && !is_specular_kind(bs_.kind) && a.n_lights > 0;
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+            float contrib[4] = {0.f, 0.f, 0.f, 0.f};
+            float4 sh_o, sh_d;
+            bool ok = false;
+            const int nslots = a.nee_all ? (int)a.n_lights : 1;
+            if (do_nee && j < nslots) {
+                uint32_t li = (uint32_t)j;
+                float sel = 1.f;
+                if (!a.nee_all) {
+                    li = min((uint32_t)(rng_u(rb, 7) * (float)a.n_lights), a.n_lights - 1u);
+                    sel = 1.f / (float)a.n_lights;
+                }
+                ok = nee_sample(a, bs_, ln, li, sel, rng_u(rb, 3 + 2 * j), rng_u(rb, 4 + 2 * j), p, perr, ng, fs, ft,
+                                woW, wo, beta, contrib, sh_o, sh_d);
+            }
+            if (active) {   // every live path rewrites both slots (0 when no sample)
+                if (flags & PF_SECTERM) contrib[1] = contrib[2] = contrib[3] = 0.f;
+                a.pend[(size_t)j * a.cap + sid] = make_float4(contrib[0], contrib[1], contrib[2], contrib[3]);
+            }
+            uint32_t ss = wave_append(ok, &a.counts[C_SH + cur]);
+            if (ok) {
+                a.sq_o[ss] = sh_o;
+                a.sq_d[ss] = sh_d;
+                a.sq_tgt[ss] = (uint32_t)j * a.cap + sid;
+            }
+        }
+        // -------- phase 3: BSDF sample (continuation and MIS BSDF ray), RR
+        if (mat_hit) {
+            BsdfSample bsm;
+            float u0 = remap_u0(rng_u(rb, 0)), u1 = rng_u(rb, 1);
+            if (wo.z == 0.f) bsm.pdf = 0.f;
+            else lobe_sample(bs_, wo, u0, u1, bsm);
+            bool ok = bsm.pdf != 0.f;
+            v3 wiW;
+            if (ok) {
+                wiW = fs * bsm.wi.x + ft * bsm.wi.y + ng * bsm.wi.z;   // to_world
+                if (!(bsm.type & BSDF_SPECULAR)) {
+                    bool refl = dot(wiW, ng) * dot(woW, ng) > 0.f;
+                    if (!refl) {
+                        for (int k = 0; k < 4; ++k) bsm.f[k] = 0.f;
+                    } else if (bs_.kind == MAT_CONDUCTOR) {
+                        microfacet_f(bs_, wo, bsm.wi, bsm.f, &bsm.wh);
+                    }
+                }
+                bool black = true;
+                for (int k = 0; k < 4; ++k) black = black && bsm.f[k] == 0.f;
+                if (black) ok = false;
+            }
+            if (ok) {
+                const uint32_t ty = bsm.type;
+                bool brk = (a.depth_d == (int)bounce && (ty & BSDF_DIFFUSE)) ||
+                           (a.depth_r == (int)bounce && (ty & (BSDF_SPECULAR | BSDF_GLOSSY))) ||
+                           (a.depth_t == (int)bounce && (ty & BSDF_TRANSMISSION));
+                if (brk && (ty & BSDF_SPECULAR)) ok = false;
+                if (ok) {
+                    if (brk) flags |= PF_LAST;
+                    float cw = absdot(wiW, ng);
+                    for (int k = 0; k < 4; ++k) beta[k] = beta[k] * bsm.f[k] * cw / bsm.pdf;
+                    if (ty & BSDF_SPECULAR) flags |= PF_SPECULAR; else flags &= ~PF_SPECULAR;
+                    if ((ty & BSDF_SPECULAR) && (ty & BSDF_TRANSMISSION))
+                        eta_scale *= (dot(woW, ng) > 0.f) ? (eta_bsdf * eta_bsdf) : 1.f / (eta_bsdf * eta_bsdf);
+                    if (bsm.secondary_terminated) flags |= PF_SECTERM;
+                    no = offset_point_origin(p, perr, ng, wiW);          // kindle_ray
+                    nd = wiW;
+                    prev_pdf = (ty & BSDF_SPECULAR) ? 0.f : bsm.pdf;
+                    cont = true;
+                    if (!brk) {
+                        // Russian roulette, pathintegrator.cpp:77-83
+                        float mx = beta[0] * eta_scale;
+                        if (!(flags & PF_SECTERM))
+                            for (int k = 1; k < 4; ++k) mx = fmaxf(mx, beta[k] * eta_scale);
+                        if (mx < a.rr_threshold && (int)bounce > (int)a.rr_start) {
+                            float q = fmaxf(0.05f, 1.f - mx);
+                            if (rng_u(rb, 2) < q) cont = false;
+                            else for (int k = 0; k < 4; ++k) beta[k] = beta[k] / (1.f - q);
+                        }
+                    }
+                    bounce += 1;
+                }
+            }
+        }
+        if (active) {
             if (!cont) flags |= PF_DONE;
-            if (flags & PF_SECTERM) { L[1] = L[2] = L[3] = 0.f; np0[1] = np0[2] = np0[3] = 0.f; np1[1] = np1[2] = np1[3] = 0.f; }
+            if (flags & PF_SECTERM) { L[1] = L[2] = L[3] = 0.f; }
             a.st_L[sid] = make_float4(L[0], L[1], L[2], L[3]);
-            a.pend[sid] = make_float4(np0[0], np0[1], np0[2], np0[3]);
-            a.pend[a.cap + sid] = make_float4(np1[0], np1[1], np1[2], np1[3]);
-            if (!shadow_ok[0]) a.pend[sid] = make_float4(0.f, 0.f, 0.f, 0.f);
-            if (!shadow_ok[1]) a.pend[a.cap + sid] = make_float4(0.f, 0.f, 0.f, 0.f);
             a.st_misc[sid] = make_float4(lam0, eta_scale, prev_pdf, __uint_as_float(flags | bounce));
             if (cont) a.st_beta[sid] = make_float4(beta[0], beta[1], beta[2], beta[3]);
         }
-        // ---- compaction: one atomic per wave per queue
-        uint32_t slot = wave_append(cont, &a.counts[nxt]);
+        // -------- compaction of the continuation: one atomic per wave
+        uint32_t slot = wave_append(cont, &a.counts[C_EXT + nxt]);
         if (cont) {
             a.rq_o[nxt][slot] = make_float4(no.x, no.y, no.z, kShadowEps);
             a.rq_d[nxt][slot] = make_float4(nd.x, nd.y, nd.z, kInf);
             a.rq_sid[nxt][slot] = sid;
-        }
-        for (int j = 0; j < 2; ++j) {
-            uint32_t ss = wave_append(shadow_ok[j], &a.counts[2]);
-            if (shadow_ok[j]) {
-                a.sq_o[ss] = sh_o[j];
-                a.sq_d[ss] = sh_d[j];
-                a.sq_tgt[ss] = (uint32_t)j * a.cap + sid;
-            }
         }
     }
 }
@@ -462,7 +471,12 @@ __global__ void k_bsdf_query(const fl_bsdf_query *__restrict__ q, fl_bsdf_result
     b.ggx.ay = Q.alpha_y;
     b.weight = 1.f;
     b.vndf_pdf = (Q.flags & FL_RENDER_VNDF_PDF) != 0;
-    for (int k = 0; k < 4; ++k) { b.r[k] = Q.r[k]; b.t[k] = Q.t[k]; b.eta[k] = Q.eta[k]; b.k[k] = Q.k[k]; }
+    for (int k = 0; k < 4; ++k) {
+        b.s1[k] = Q.kind == MAT_CONDUCTOR ? Q.eta[k] : Q.r[k];
+        b.s2[k] = Q.kind == MAT_CONDUCTOR ? Q.k[k] : Q.t[k];
+    }
+    b.ior = Q.eta[0];
+    b.dispersive = Q.eta[1] != Q.eta[0] || Q.eta[2] != Q.eta[0] || Q.eta[3] != Q.eta[0];
     v3 wo(Q.wo[0], Q.wo[1], Q.wo[2]), wi(Q.wi[0], Q.wi[1], Q.wi[2]);
     fl_bsdf_result R;
     float f[4];
@@ -612,9 +626,16 @@ int render(fl_scene_s *s, const fl_camera *camera, const fl_render_params *p, fl
     RenderBuffers &B = bufs_for(ctx);
     bool err = false;
     const size_t cap = a.cap;
-    int grid_pers = ctx->n_cu * 4;
+    // persistent traversal grid: as many resident blocks as the kernel's
+    // occupancy (LDS stack + VGPRs) allows on every CU
+    int occ = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, reinterpret_cast<const void *>(k_trace<false>), kBlock, 0) !=
+            hipSuccess || occ <= 0)
+        occ = 4;
+    const uint32_t grid_pers = (uint32_t)(ctx->n_cu * occ);
+    a.trav_budget = p->trav_budget ? (int)p->trav_budget : 16;
     size_t spill_per = (size_t)(s->depth > (uint32_t)kStack ? s->depth - kStack + 1 : 1);
-    a.spill_stride = (uint32_t)grid_pers * kBlock;
+    a.spill_stride = grid_pers * kBlock;
     a.st_beta = grow<float4>(B.p[0], B.cap[0], cap * 16, err);
     a.st_L = grow<float4>(B.p[1], B.cap[1], cap * 16, err);
     a.st_misc = grow<float4>(B.p[2], B.cap[2], cap * 16, err);
@@ -638,21 +659,22 @@ int render(fl_scene_s *s, const fl_camera *camera, const fl_render_params *p, fl
     if (err) return set_error(ctx, FL_E_OOM, "render buffers");
 
     const bool counters = (p->flags & FL_RENDER_COUNTERS) != 0;
-    hipEvent_t e0, e1;
+    const bool timing = (p->flags & FL_RENDER_TIMING) != 0;
+    hipEvent_t e0, e1, tev[3];
     FL_HIP_CHECK(ctx, hipEventCreate(&e0));
     FL_HIP_CHECK(ctx, hipEventCreate(&e1));
+    for (int k = 0; k < 3; ++k) FL_HIP_CHECK(ctx, hipEventCreate(&tev[k]));
     auto blocks = [](size_t n) { return (unsigned)((n + kBlock - 1) / kBlock); };
     hipLaunchKernelGGL(k_zero_f32, dim3(blocks((size_t)W * H * 3)), dim3(kBlock), 0, st, a.film_sum, (size_t)W * H * 3);
     hipLaunchKernelGGL(k_zero_f32, dim3(blocks((size_t)W * H * 3)), dim3(kBlock), 0, st, a.film_sq, (size_t)W * H * 3);
     hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, st, (uint32_t *)a.stats, 32u);
     FL_HIP_CHECK(ctx, hipEventRecord(e0, st));
 
-    uint32_t done_spp = 0, passes = 0, max_iter_seen = 0, n_launch = 0;
-    const bool timing = (p->flags & FL_RENDER_TIMING) != 0;
-    hipEvent_t tev[4];
-    for (int k = 0; k < 4; ++k) FL_HIP_CHECK(ctx, hipEventCreate(&tev[k]));
-    double ms_ext = 0.0, ms_sh = 0.0, ms_shd = 0.0;
+    uint32_t done_spp = 0, passes = 0, max_iter_seen = 0, n_trace = 0, n_shade = 0;
+    double ms_trace = 0.0, ms_shade = 0.0;
     const uint32_t max_iter = (uint32_t)std::max({p->diffuse_depth, p->reflection_depth, p->transmission_depth}) * 2 + 8;
+    const uint32_t shade_grid_max = (uint32_t)ctx->n_cu * 8;
+    uint32_t host_counts[16];
     while (done_spp < p->spp) {
         uint32_t this_spp = std::min(spp_pass, p->spp - done_spp);
         a.spp_pass = this_spp;
@@ -660,30 +682,39 @@ int render(fl_scene_s *s, const fl_camera *camera, const fl_render_params *p, fl
         a.sample_base = p->sample_offset + done_spp;
         hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, st, a.counts, 16u);
         hipLaunchKernelGGL(k_generate, dim3(std::min<unsigned>(blocks(a.n_paths), ctx->n_cu * 16)), dim3(kBlock), 0, st, a);
+        uint32_t ne = a.n_paths, ns = 0;   // upper bound for the first launch
         uint32_t iter = 0;
-        for (; iter < max_iter; ++iter) {
-            int cur = iter & 1;
+        for (; iter < max_iter && (ne + ns) > 0; ++iter) {
+            const int cur = iter & 1;
+            uint32_t tg = std::min<uint32_t>(grid_pers, blocks((size_t)ne + ns));
             if (timing) FL_HIP_CHECK(ctx, hipEventRecord(tev[0], st));
-            if (counters) hipLaunchKernelGGL(k_extend<true>, dim3(grid_pers), dim3(kBlock), 0, st, a, cur);
-            else hipLaunchKernelGGL(k_extend<false>, dim3(grid_pers), dim3(kBlock), 0, st, a, cur);
+            if (counters) hipLaunchKernelGGL(k_trace<true>, dim3(tg), dim3(kBlock), 0, st, a, (int)iter);
+            else hipLaunchKernelGGL(k_trace<false>, dim3(tg), dim3(kBlock), 0, st, a, (int)iter);
+            ++n_trace;
             if (timing) FL_HIP_CHECK(ctx, hipEventRecord(tev[1], st));
-            hipLaunchKernelGGL(k_shade, dim3(ctx->n_cu * 8), dim3(kBlock), 0, st, a, cur);
+            if (ne > 0) {
+                dim3 sgrid(std::min<uint32_t>(shade_grid_max, blocks(ne)));
+                if (p->shade_waves >= 4) hipLaunchKernelGGL(k_shade<4>, sgrid, dim3(kBlock), 0, st, a, (int)iter);
+                else hipLaunchKernelGGL(k_shade<2>, sgrid, dim3(kBlock), 0, st, a, (int)iter);
+                ++n_shade;
+            }
             if (timing) FL_HIP_CHECK(ctx, hipEventRecord(tev[2], st));
-            if (counters) hipLaunchKernelGGL(k_shadow<true>, dim3(grid_pers), dim3(kBlock), 0, st, a);
-            else hipLaunchKernelGGL(k_shadow<false>, dim3(grid_pers), dim3(kBlock), 0, st, a);
-            if (timing) FL_HIP_CHECK(ctx, hipEventRecord(tev[3], st));
-            ++n_launch;
-            uint32_t next = 0;
-            FL_HIP_CHECK(ctx, hipMemcpyAsync(&next, a.counts + (cur ^ 1), 4, hipMemcpyDeviceToHost, st));
+            FL_HIP_CHECK(ctx, hipMemcpyAsync(host_counts, a.counts, 64, hipMemcpyDeviceToHost, st));
             FL_HIP_CHECK(ctx, hipStreamSynchronize(st));
             if (timing) {
-                float m0 = 0.f, m1 = 0.f, m2 = 0.f;
+                float m0 = 0.f, m1 = 0.f;
                 (void)hipEventElapsedTime(&m0, tev[0], tev[1]);
                 (void)hipEventElapsedTime(&m1, tev[1], tev[2]);
-                (void)hipEventElapsedTime(&m2, tev[2], tev[3]);
-                ms_ext += m0; ms_sh += m1; ms_shd += m2;
+                ms_trace += m0;
+                ms_shade += m1;
             }
-            if (next == 0) { ++iter; break; }
+            if (ne > 0) {
+                ne = host_counts[C_EXT + (cur ^ 1)];
+                ns = host_counts[C_SH + cur];
+            } else {
+                ne = 0;
+                ns = 0;   // shadow-only trace: nothing follows
+            }
         }
         max_iter_seen = std::max(max_iter_seen, iter);
         hipLaunchKernelGGL(k_resolve, dim3(blocks(a.npix)), dim3(kBlock), 0, st, a, a.spec);
@@ -697,7 +728,7 @@ int render(fl_scene_s *s, const fl_camera *camera, const fl_render_params *p, fl
     (void)hipEventElapsedTime(&ms, e0, e1);
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
-    for (int k = 0; k < 4; ++k) (void)hipEventDestroy(tev[k]);
+    for (int k = 0; k < 3; ++k) (void)hipEventDestroy(tev[k]);
     if (film && film->xyz_sum) {
         if (film->width != W || film->height != H) return set_error(ctx, FL_E_INVALID, "film size mismatch");
         FL_HIP_CHECK(ctx, hipMemcpy(film->xyz_sum, a.film_sum, (size_t)W * H * 12, hipMemcpyDeviceToHost));
@@ -716,12 +747,10 @@ int render(fl_scene_s *s, const fl_camera *camera, const fl_render_params *p, fl
         stats->nodes_visited = h[3];
         stats->tris_tested = h[4];
         stats->stack_spills = h[5];
-        stats->shadow_nodes_visited = h[6];
-        stats->shadow_tris_tested = h[7];
-        stats->ms_extend = ms_ext;
-        stats->ms_shade = ms_sh;
-        stats->ms_shadow = ms_shd;
-        stats->launches_extend = stats->launches_shade = stats->launches_shadow = n_launch;
+        stats->ms_extend = ms_trace;   // combined traversal kernel (extension + shadow rays)
+        stats->ms_shade = ms_shade;
+        stats->launches_extend = n_trace;
+        stats->launches_shade = n_shade;
         stats->passes = passes;
         stats->max_bounce = max_iter_seen;
         stats->paths_per_pass = a.cap;

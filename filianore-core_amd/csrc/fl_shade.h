@@ -229,8 +229,9 @@ struct Bsdf {
     int kind;
     GGX ggx;
     float weight;        // MicrofacetReflection weight (ksweight)
-    float r[4], t[4];    // Lambert R / dielectric R,T per lane
-    float eta[4], k[4];  // conductor eta,k per lane; dielectric eta per lane
+    float s1[4], s2[4];  // per lane: Lambert (R, -), conductor (eta, k), dielectric (R, T)
+    float ior;           // dielectric: eta at the hero wavelength (lane 0)
+    bool dispersive;     // dielectric: eta depends on wavelength (secondary lanes end)
     bool vndf_pdf;
 };
 
@@ -259,7 +260,7 @@ FL_HD void microfacet_f(const Bsdf &b, v3 wo, v3 wi, float f[4], const v3 *whp =
     float D = ggx_D(b.ggx, wh);
     float G = ggx_G(b.ggx, wo, wi);
     for (int k = 0; k < 4; ++k) {
-        float F = fresnel_conductor(cosF, b.eta[k], b.k[k]);
+        float F = fresnel_conductor(cosF, b.s1[k], b.s2[k]);
         f[k] = 1.f * b.weight * D * G * F / (4.f * cO * cI);
     }
 }
@@ -282,7 +283,7 @@ FL_HD void bsdf_eval_nonspec(const Bsdf &b, v3 wo, v3 wi, bool reflect, float f[
     if (wo.z == 0.f) return;
     if (!reflect) return;  // both non-specular lobes are reflection-only
     if (b.kind == MAT_LAMBERT) {
-        for (int k = 0; k < 4; ++k) f[k] = b.r[k] * kInvPi * 1.f;
+        for (int k = 0; k < 4; ++k) f[k] = b.s1[k] * kInvPi * 1.f;
     } else if (b.kind == MAT_CONDUCTOR) {
         microfacet_f(b, wo, wi, f);
     }
@@ -321,7 +322,7 @@ FL_HD void lobe_sample(const Bsdf &b, v3 wo, float u0, float u1, BsdfSample &s) 
         if (wo.z < 0.f) wi.z *= -1.f;
         s.wi = wi;
         s.pdf = same_hemi(wo, wi) ? fabsf(wi.z) * kInvPi : 0.f;
-        for (int k = 0; k < 4; ++k) s.f[k] = b.r[k] * kInvPi * 1.f;
+        for (int k = 0; k < 4; ++k) s.f[k] = b.s1[k] * kInvPi * 1.f;
     } else if (b.kind == MAT_CONDUCTOR) {
         // MicrofacetReflectionBRDF::sample, microfacetreflection.cpp:34-53
         if (wo.z == 0.f) return;
@@ -338,8 +339,8 @@ FL_HD void lobe_sample(const Bsdf &b, v3 wo, float u0, float u1, BsdfSample &s) 
         // (lane 0) decides; a wavelength-dependent eta terminates lanes 1..3.
         // F, 1-F and the refracted direction are formed in fp64: 1-F and cos(theta_t)
         // cancel near grazing incidence / the critical angle.
-        float ior = b.eta[0];
-        bool dispersive = (b.eta[1] != ior) || (b.eta[2] != ior) || (b.eta[3] != ior);
+        float ior = b.ior;
+        bool dispersive = b.dispersive;
         double Fd = fresnel_dielectric_d(wo.z, 1.0, ior);
         float F = (float)Fd;
         if (u0 < F) {
@@ -347,7 +348,7 @@ FL_HD void lobe_sample(const Bsdf &b, v3 wo, float u0, float u1, BsdfSample &s) 
             s.type = BSDF_SPECULAR | BSDF_REFLECTION;
             s.wi = wi;
             s.pdf = F;
-            for (int k = 0; k < 4; ++k) s.f[k] = b.r[k] * F / fabsf(wi.z);
+            for (int k = 0; k < 4; ++k) s.f[k] = b.s1[k] * F / fabsf(wi.z);
         } else {
             bool entering = wo.z > 0.f;
             double etaI = entering ? 1.0 : (double)ior;
@@ -365,7 +366,7 @@ FL_HD void lobe_sample(const Bsdf &b, v3 wo, float u0, float u1, BsdfSample &s) 
             s.type = BSDF_SPECULAR | BSDF_TRANSMISSION;
             s.wi = wi;
             s.pdf = (float)T;
-            for (int k = 0; k < 4; ++k) s.f[k] = (float)((double)b.t[k] * T * sc);
+            for (int k = 0; k < 4; ++k) s.f[k] = (float)((double)b.s2[k] * T * sc);
         }
         s.secondary_terminated = dispersive;
     }

@@ -9,6 +9,11 @@
 // (org + q*step is exact by construction) and widen the slab interval by a
 // relative 2^-20, so no box containing a hit is ever culled: the closest hit
 // (with the smaller-ID tie rule) is then the same as a brute-force search.
+//
+// The traversal is a resumable state machine (TravState + trav_step with an
+// iteration budget) so that persistent kernels can refill lanes whose ray has
+// finished with new rays (one wave-aggregated atomic per refill) instead of
+// idling until the slowest lane of the wave is done.
 #pragma once
 #include "fl_bvh.h"
 
@@ -33,140 +38,229 @@ struct LaneStack {
     size_t spill_stride;
 };
 
+struct TravCounters {
+    uint32_t nodes = 0, tris = 0, spills = 0;
+};
+
 __device__ __forceinline__ float safe_inv(float d) {
     const float eps = 1e-20f;
     float dd = fabsf(d) > eps ? d : copysignf(eps, d);
     return 1.0f / dd;
 }
 
+struct TravState {
+    v3 o, d;
+    float inv[3];
+    float tmin, best_t, bu, bv;
+    uint32_t oct, best_id, best_rec;
+    RayPre rp;
+    uint32_t g_base, g_imask, g_hits, t_base, t_hits;
+    int sp;
+};
+
+__device__ __forceinline__ void trav_init(TravState &s, v3 o, v3 d, float tmin, float tmax) {
+    s.o = o;
+    s.d = d;
+    s.inv[0] = safe_inv(d.x);
+    s.inv[1] = safe_inv(d.y);
+    s.inv[2] = safe_inv(d.z);
+    s.oct = (s.inv[0] < 0.f ? 1u : 0u) | (s.inv[1] < 0.f ? 2u : 0u) | (s.inv[2] < 0.f ? 4u : 0u);
+    s.rp = ray_pre(d);
+    s.tmin = tmin;
+    s.best_t = tmax;
+    s.best_id = kNoHit;
+    s.best_rec = 0;
+    s.bu = s.bv = 0.f;
+    s.g_base = 0;
+    s.g_imask = 1u;
+    s.g_hits = 1u << s.oct;  // the group holding the root (slot 0)
+    s.t_base = 0;
+    s.t_hits = 0;
+    s.sp = 0;
+}
+
+// Runs at most `budget` loop iterations; returns true when the ray is done
+// (state holds the closest hit, or for ANY the first accepted hit).
 template <bool ANY, bool SKIP_LIGHTS, bool COUNT>
-__device__ __forceinline__ bool traverse(const TraceScene &sc, v3 o, v3 d, float tmin, float tmax,
-                                         const LaneStack &ls, TraceHit &hit, uint32_t &cnt_nodes,
-                                         uint32_t &cnt_tris, uint32_t &cnt_spill) {
-    const float ix = safe_inv(d.x), iy = safe_inv(d.y), iz = safe_inv(d.z);
-    const uint32_t oct = (ix < 0.f ? 1u : 0u) | (iy < 0.f ? 2u : 0u) | (iz < 0.f ? 4u : 0u);
-    const RayPre rp = ray_pre(d);
+__device__ __forceinline__ bool trav_step(const TraceScene &sc, TravState &s, const LaneStack &ls, int budget,
+                                          TravCounters &cnt) {
     const float kLo = 1.0f - 0x1p-20f, kHi = 1.0f + 0x1p-20f;
-
-    float best_t = tmax;
-    uint32_t best_id = kNoHit, best_rec = 0;
-    float bu = 0.f, bv = 0.f;
-
-    uint32_t g_base = 0, g_imask = 1u, g_hits = 1u << oct;  // group holding the root
-    uint32_t t_base = 0, t_hits = 0;
-    int sp = 0;
-    bool done = false;
-
-    while (true) {
-        if (g_hits) {
-            uint32_t pos = __ffs(g_hits) - 1;
-            g_hits &= g_hits - 1u;
-            uint32_t slot = pos ^ oct;
-            uint32_t nidx = g_base + __popc(g_imask & ((1u << slot) - 1u));
-            if (g_hits) {
-                uint2 e = make_uint2(g_base, g_imask | (g_hits << 8));
-                if (sp < ls.cap) ls.lds[sp * ls.lds_stride] = e;
-                else { ls.spill[(size_t)(sp - ls.cap) * ls.spill_stride] = e; if (COUNT) ++cnt_spill; }
-                ++sp;
+    for (int it = 0; it < budget; ++it) {
+        if (s.g_hits) {
+            uint32_t pos = __ffs(s.g_hits) - 1;
+            s.g_hits &= s.g_hits - 1u;
+            uint32_t slot = pos ^ s.oct;
+            uint32_t nidx = s.g_base + __popc(s.g_imask & ((1u << slot) - 1u));
+            if (s.g_hits) {
+                uint2 e = make_uint2(s.g_base, s.g_imask | (s.g_hits << 8));
+                if (s.sp < ls.cap) ls.lds[s.sp * ls.lds_stride] = e;
+                else { ls.spill[(size_t)(s.sp - ls.cap) * ls.spill_stride] = e; if (COUNT) ++cnt.spills; }
+                ++s.sp;
             }
             const uint4 *np = reinterpret_cast<const uint4 *>(sc.nodes + nidx);
             uint4 w0 = np[0], w1 = np[1];
             uint4 ql[3] = {np[2], np[3], np[4]};
             uint4 qh[3] = {np[5], np[6], np[7]};
-            if (COUNT) ++cnt_nodes;
+            if (COUNT) ++cnt.nodes;
             float org[3] = {__uint_as_float(w0.x), __uint_as_float(w0.y), __uint_as_float(w0.z)};
             float step[3] = {__uint_as_float((w0.w & 0xffu) << 23), __uint_as_float(((w0.w >> 8) & 0xffu) << 23),
                              __uint_as_float(((w0.w >> 16) & 0xffu) << 23)};
             uint32_t imask = w0.w >> 24;
-            float oo[3] = {o.x, o.y, o.z};
-            float inv[3] = {ix, iy, iz};
+            float oo[3] = {s.o.x, s.o.y, s.o.z};
             uint32_t hits = 0;
 #pragma unroll
-            for (int s = 0; s < 8; ++s) {
-                float bn = tmin, bf = best_t;
+            for (int q = 0; q < 8; ++q) {
                 float tn3[3], tf3[3];
 #pragma unroll
                 for (int k = 0; k < 3; ++k) {
-                    uint32_t wl = (s >> 1) == 0 ? ql[k].x : ((s >> 1) == 1 ? ql[k].y : ((s >> 1) == 2 ? ql[k].z : ql[k].w));
-                    uint32_t wh = (s >> 1) == 0 ? qh[k].x : ((s >> 1) == 1 ? qh[k].y : ((s >> 1) == 2 ? qh[k].z : qh[k].w));
-                    uint32_t qlo = (s & 1) ? (wl >> 16) : (wl & 0xffffu);
-                    uint32_t qhi = (s & 1) ? (wh >> 16) : (wh & 0xffffu);
+                    uint32_t wl = (q >> 1) == 0 ? ql[k].x : ((q >> 1) == 1 ? ql[k].y : ((q >> 1) == 2 ? ql[k].z : ql[k].w));
+                    uint32_t wh = (q >> 1) == 0 ? qh[k].x : ((q >> 1) == 1 ? qh[k].y : ((q >> 1) == 2 ? qh[k].z : qh[k].w));
+                    uint32_t qlo = (q & 1) ? (wl >> 16) : (wl & 0xffffu);
+                    uint32_t qhi = (q & 1) ? (wh >> 16) : (wh & 0xffffu);
                     float plo = org[k] + (float)qlo * step[k];
                     float phi = org[k] + (float)qhi * step[k];
-                    float t0 = (plo - oo[k]) * inv[k];
-                    float t1 = (phi - oo[k]) * inv[k];
-                    bool neg = (oct >> k) & 1u;
+                    float t0 = (plo - oo[k]) * s.inv[k];
+                    float t1 = (phi - oo[k]) * s.inv[k];
+                    bool neg = (s.oct >> k) & 1u;
                     tn3[k] = neg ? t1 : t0;
                     tf3[k] = neg ? t0 : t1;
                 }
                 float n_ = fmaxf(fmaxf(tn3[0], tn3[1]), tn3[2]) * kLo;
                 float f_ = fminf(fminf(tf3[0], tf3[1]), tf3[2]) * kHi;
-                bn = fmaxf(bn, n_);
-                bf = fminf(bf, f_);
-                if (bn <= bf) hits |= 1u << s;
+                float bn = fmaxf(s.tmin, n_);
+                float bf = fminf(s.best_t, f_);
+                if (bn <= bf) hits |= 1u << q;
             }
             uint32_t inner = hits & imask;
             uint32_t leafs = hits & ~imask;
             uint32_t ng = 0;
             while (inner) {
-                uint32_t s = __ffs(inner) - 1;
+                uint32_t q = __ffs(inner) - 1;
                 inner &= inner - 1u;
-                ng |= 1u << (s ^ oct);
+                ng |= 1u << (q ^ s.oct);
             }
             uint32_t tm = 0;
-            uint32_t meta_lo = w1.z, meta_hi = w1.w;
+            const uint32_t meta_lo = w1.z, meta_hi = w1.w;
             while (leafs) {
-                uint32_t s = __ffs(leafs) - 1;
+                uint32_t q = __ffs(leafs) - 1;
                 leafs &= leafs - 1u;
-                uint32_t m = (s < 4 ? (meta_lo >> (8 * s)) : (meta_hi >> (8 * (s - 4)))) & 0xffu;
-                uint32_t off = m & 31u, c = m >> 5;
-                tm |= ((1u << c) - 1u) << off;
+                uint32_t m = (q < 4 ? (meta_lo >> (8 * q)) : (meta_hi >> (8 * (q - 4)))) & 0xffu;
+                tm |= ((1u << (m >> 5)) - 1u) << (m & 31u);
             }
-            g_base = w1.x;
-            g_imask = imask;
-            g_hits = ng;
-            t_base = w1.y;
-            t_hits = tm;
-        } else if (!t_hits) {
-            if (sp == 0) break;
-            --sp;
-            uint2 e = (sp < ls.cap) ? ls.lds[sp * ls.lds_stride] : ls.spill[(size_t)(sp - ls.cap) * ls.spill_stride];
-            g_base = e.x;
-            g_imask = e.y & 0xffu;
-            g_hits = e.y >> 8;
+            s.g_base = w1.x;
+            s.g_imask = imask;
+            s.g_hits = ng;
+            s.t_base = w1.y;
+            s.t_hits = tm;
+        } else if (!s.t_hits) {
+            if (s.sp == 0) return true;
+            --s.sp;
+            uint2 e = (s.sp < ls.cap) ? ls.lds[s.sp * ls.lds_stride]
+                                      : ls.spill[(size_t)(s.sp - ls.cap) * ls.spill_stride];
+            s.g_base = e.x;
+            s.g_imask = e.y & 0xffu;
+            s.g_hits = e.y >> 8;
             continue;
         }
-        while (t_hits) {
-            uint32_t b = __ffs(t_hits) - 1;
-            t_hits &= t_hits - 1u;
-            uint32_t ri = t_base + b;
+        while (s.t_hits) {
+            uint32_t b = __ffs(s.t_hits) - 1;
+            s.t_hits &= s.t_hits - 1u;
+            uint32_t ri = s.t_base + b;
             const uint4 *tp = reinterpret_cast<const uint4 *>(sc.tris + ri);
             uint4 a = tp[0], bq = tp[1], c = tp[2];
-            if (COUNT) ++cnt_tris;
+            if (COUNT) ++cnt.tris;
             if (SKIP_LIGHTS && c.w != 0u) continue;
             v3 p0(__uint_as_float(a.x), __uint_as_float(a.y), __uint_as_float(a.z));
             v3 p1(__uint_as_float(bq.x), __uint_as_float(bq.y), __uint_as_float(bq.z));
             v3 p2(__uint_as_float(c.x), __uint_as_float(c.y), __uint_as_float(c.z));
             float t, u, v;
-            if (tri_intersect(rp, o, p0, p1, p2, tmin, best_t, &t, &u, &v)) {
+            if (tri_intersect(s.rp, s.o, p0, p1, p2, s.tmin, s.best_t, &t, &u, &v)) {
                 if (ANY) {
-                    best_t = t; best_id = a.w; best_rec = ri; bu = u; bv = v;
-                    done = true;
-                    break;
+                    s.best_t = t; s.best_id = a.w; s.best_rec = ri; s.bu = u; s.bv = v;
+                    return true;
                 }
-                if (closer(t, a.w, best_t, best_id)) {
-                    best_t = t; best_id = a.w; best_rec = ri; bu = u; bv = v;
+                if (closer(t, a.w, s.best_t, s.best_id)) {
+                    s.best_t = t; s.best_id = a.w; s.best_rec = ri; s.bu = u; s.bv = v;
                 }
             }
         }
-        if (ANY && done) break;
     }
-    hit.t = best_t;
-    hit.u = bu;
-    hit.v = bv;
-    hit.id = best_id;
-    hit.rec = best_rec;
-    return best_id != kNoHit;
+    return false;
+}
+
+// Whole-ray traversal (no refill), for simple callers.
+template <bool ANY, bool SKIP_LIGHTS, bool COUNT>
+__device__ __forceinline__ bool traverse(const TraceScene &sc, v3 o, v3 d, float tmin, float tmax,
+                                         const LaneStack &ls, TraceHit &hit, TravCounters &cnt) {
+    TravState s;
+    trav_init(s, o, d, tmin, tmax);
+    while (!trav_step<ANY, SKIP_LIGHTS, COUNT>(sc, s, ls, 1 << 30, cnt)) {
+    }
+    hit.t = s.best_t;
+    hit.u = s.bu;
+    hit.v = s.bv;
+    hit.id = s.best_id;
+    hit.rec = s.best_rec;
+    return s.best_id != kNoHit;
+}
+
+// Persistent traversal over n rays with per-lane refill.  Src provides
+//   void load(uint32_t i, v3 &o, v3 &d, float &tmin, float &tmax, bool &any) const
+//   void store(uint32_t i, const TravState &s) const
+// Lane g first takes ray g (no atomic); later rays come from `fetch` (which
+// must be 0 at launch) offset by the grid size, one atomic per wave refill.
+// All lanes of the wave must call this (it uses wave-wide ballots).
+// `any` is per ray: the first hit ends it (emitters skipped when ANY_SKIP).
+template <bool COUNT, bool ANY_SKIP, class Src>
+__device__ __forceinline__ void trace_persistent(const TraceScene &sc, const Src &src, uint32_t n, uint32_t *fetch,
+                                                 const LaneStack &ls, int budget, TravCounters &cnt) {
+    TravState s;
+    const uint32_t lane = __lane_id();
+    const uint32_t nthreads = gridDim.x * blockDim.x;
+    uint32_t first = blockIdx.x * blockDim.x + threadIdx.x;
+    int ray = -1;
+    bool any = false;
+    bool exhausted = false;
+    if (first < n) {
+        ray = (int)first;
+        v3 o, d;
+        float tmin, tmax;
+        src.load(first, o, d, tmin, tmax, any);
+        trav_init(s, o, d, tmin, tmax);
+    } else {
+        exhausted = true;
+    }
+    while (true) {
+        if (__ballot(ray >= 0) == 0ull && __ballot(!exhausted) == 0ull) break;
+        if (ray >= 0) {
+            bool done = any ? trav_step<true, ANY_SKIP, COUNT>(sc, s, ls, budget, cnt)
+                            : trav_step<false, false, COUNT>(sc, s, ls, budget, cnt);
+            if (done) {
+                src.store((uint32_t)ray, s);
+                ray = -1;
+            }
+        }
+        bool need = (ray < 0) && !exhausted;
+        unsigned long long m = __ballot(need);
+        if (m) {
+            int leader = __ffsll(m) - 1;
+            uint32_t base = 0;
+            if (lane == (uint32_t)leader) base = atomicAdd(fetch, (uint32_t)__popcll(m));
+            base = __shfl(base, leader);
+            if (need) {
+                uint32_t idx = nthreads + base + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
+                if (idx >= n) {
+                    exhausted = true;
+                } else {
+                    ray = (int)idx;
+                    v3 o, d;
+                    float tmin, tmax;
+                    src.load(idx, o, d, tmin, tmax, any);
+                    trav_init(s, o, d, tmin, tmax);
+                }
+            }
+        }
+    }
 }
 
 } // namespace fl

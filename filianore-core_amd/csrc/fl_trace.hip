@@ -1,53 +1,88 @@
 // fl_trace.hip -- closest/any-hit queries over ray arrays (Scene::intersect,
 // Scene::intersect_p: core/scene.cpp:25-31, bvh/bvh.cpp:49-88).
 //
-// Persistent grid: each lane loops over rays with a grid stride, so the
-// global spill area of the short stack is bounded by the grid, not by n.
+// Persistent grid sized by occupancy; lanes refill with new rays as their ray
+// finishes, so the global spill area of the short stack is bounded by the
+// grid, not by n.
 #include "fl_internal.h"
 #include "fl_trace.h"
 
 namespace {
 
 constexpr int kBlock = 256;
+constexpr int kStack = 12;
+constexpr int kBudget = 24;
 
-template <bool ANY, bool SKIP, bool COUNT, int S>
-__global__ void __launch_bounds__(kBlock) k_trace_api(fl::TraceScene sc, const fl_ray *__restrict__ rays,
-                                                      fl_hit *__restrict__ hits, uint32_t n, uint2 *spill,
-                                                      unsigned long long *counters) {
-    __shared__ uint2 stk[S * kBlock];
+struct ApiSrc {
+    const fl_ray *rays;
+    fl_hit *hits;
+    bool any;
+    __device__ void load(uint32_t i, fl::v3 &o, fl::v3 &d, float &tmin, float &tmax, bool &is_any) const {
+        is_any = any;
+        const float4 *rp = reinterpret_cast<const float4 *>(rays + i);
+        float4 a = rp[0], b = rp[1];
+        o = fl::v3(a.x, a.y, a.z);
+        d = fl::v3(b.x, b.y, b.z);
+        tmin = a.w;
+        tmax = b.w;
+    }
+    __device__ void store(uint32_t i, const fl::TravState &s) const {
+        bool hit = s.best_id != fl::kNoHit;
+        float4 out;
+        out.x = hit ? s.best_t : __uint_as_float(0x7f800000u);
+        out.y = __uint_as_float(hit ? s.best_id : fl::kNoHit);
+        out.z = hit ? s.bu : 0.f;
+        out.w = hit ? s.bv : 0.f;
+        reinterpret_cast<float4 *>(hits)[i] = out;
+    }
+};
+
+__global__ void k_zero_fetch(uint32_t *p) { *p = 0u; }
+
+template <bool ANY, bool SKIP, bool COUNT>
+__global__ void __launch_bounds__(kBlock) k_trace_api(fl::TraceScene sc, ApiSrc src, uint32_t n, uint2 *spill,
+                                                      uint32_t *fetch, unsigned long long *counters) {
+    __shared__ uint2 stk[kStack * kBlock];
     const uint32_t gtid = blockIdx.x * kBlock + threadIdx.x;
-    const uint32_t nthreads = gridDim.x * kBlock;
     fl::LaneStack ls;
     ls.lds = stk + threadIdx.x;
     ls.lds_stride = kBlock;
-    ls.cap = S;
+    ls.cap = kStack;
     ls.spill = spill + gtid;
-    ls.spill_stride = nthreads;
-    uint32_t cn = 0, ct = 0, cs = 0;
-    for (uint32_t i = gtid; i < n; i += nthreads) {
-        const float4 *rp = reinterpret_cast<const float4 *>(rays + i);
-        float4 a = rp[0], b = rp[1];
-        fl::TraceHit h;
-        bool hit = fl::traverse<ANY, SKIP, COUNT>(sc, fl::v3(a.x, a.y, a.z), fl::v3(b.x, b.y, b.z), a.w, b.w, ls, h,
-                                                   cn, ct, cs);
-        float4 out;
-        out.x = hit ? h.t : __uint_as_float(0x7f800000u);
-        out.y = __uint_as_float(hit ? h.id : fl::kNoHit);
-        out.z = hit ? h.u : 0.f;
-        out.w = hit ? h.v : 0.f;
-        reinterpret_cast<float4 *>(hits)[i] = out;
-    }
+    ls.spill_stride = gridDim.x * kBlock;
+    fl::TravCounters c;
+    fl::trace_persistent<COUNT, SKIP>(sc, src, n, fetch, ls, kBudget, c);
     if (COUNT) {
-        atomicAdd(&counters[0], (unsigned long long)cn);
-        atomicAdd(&counters[1], (unsigned long long)ct);
-        atomicAdd(&counters[2], (unsigned long long)cs);
+        atomicAdd(&counters[0], (unsigned long long)c.nodes);
+        atomicAdd(&counters[1], (unsigned long long)c.tris);
+        atomicAdd(&counters[2], (unsigned long long)c.spills);
     }
 }
 
 template <bool ANY, bool SKIP, bool COUNT>
-void launch(hipStream_t st, int grid, fl::TraceScene sc, const fl_ray *r, fl_hit *h, uint32_t n, uint2 *spill,
-            unsigned long long *c) {
-    hipLaunchKernelGGL((k_trace_api<ANY, SKIP, COUNT, 16>), dim3(grid), dim3(kBlock), 0, st, sc, r, h, n, spill, c);
+int launch(fl_ctx_t ctx, int depth, fl::TraceScene sc, ApiSrc src, uint32_t n, unsigned long long *c) {
+    int occ = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, reinterpret_cast<const void *>(k_trace_api<ANY, SKIP, COUNT>),
+                                                     kBlock, 0) != hipSuccess || occ <= 0)
+        occ = 4;
+    uint32_t grid = (uint32_t)(ctx->n_cu * occ);
+    uint32_t need = (n + kBlock - 1) / kBlock;
+    if (grid > need) grid = need;
+    size_t spill_entries = (size_t)grid * kBlock * (size_t)(depth > kStack ? depth - kStack + 1 : 1);
+    size_t bytes = spill_entries * sizeof(uint2) + 256;
+    if (ctx->scratch_bytes < bytes) {
+        if (ctx->scratch) (void)hipFree(ctx->scratch);
+        ctx->scratch = nullptr;
+        ctx->scratch_bytes = 0;
+        if (hipMalloc(&ctx->scratch, bytes) != hipSuccess) return flx::set_error(ctx, FL_E_OOM, "spill stack");
+        ctx->scratch_bytes = bytes;
+    }
+    uint32_t *fetch = reinterpret_cast<uint32_t *>(ctx->scratch);
+    uint2 *spill = reinterpret_cast<uint2 *>(reinterpret_cast<char *>(ctx->scratch) + 256);
+    hipLaunchKernelGGL(k_zero_fetch, dim3(1), dim3(1), 0, ctx->stream, fetch);
+    hipLaunchKernelGGL((k_trace_api<ANY, SKIP, COUNT>), dim3(grid), dim3(kBlock), 0, ctx->stream, sc, src, n, spill,
+                       fetch, c);
+    return FL_OK;
 }
 
 } // namespace
@@ -57,33 +92,18 @@ namespace flx {
 int trace_rays(fl_ctx_t ctx, fl_scene_s *s, const fl_ray *d_rays, fl_hit *d_hits, uint32_t n, uint32_t flags,
                unsigned long long *d_counters) {
     if (n == 0) return FL_OK;
-    const int S = 16;
-    int grid = ctx->n_cu * 8;
-    uint32_t need = (uint32_t)((n + kBlock - 1) / kBlock);
-    if ((uint32_t)grid > need) grid = (int)need;
-    size_t spill_entries = (size_t)grid * kBlock * (size_t)(s->depth > (uint32_t)S ? s->depth - S + 1 : 1);
-    size_t bytes = spill_entries * sizeof(uint2);
-    if (ctx->scratch_bytes < bytes) {
-        if (ctx->scratch) (void)hipFree(ctx->scratch);
-        ctx->scratch = nullptr;
-        ctx->scratch_bytes = 0;
-        if (hipMalloc(&ctx->scratch, bytes) != hipSuccess) return set_error(ctx, FL_E_OOM, "spill stack");
-        ctx->scratch_bytes = bytes;
-    }
     fl::TraceScene sc{s->d_nodes, s->d_tris};
-    uint2 *spill = (uint2 *)ctx->scratch;
+    ApiSrc src{d_rays, d_hits, (flags & FL_ANY_HIT) != 0};
+    const int depth = (int)s->depth;
     bool any = flags & FL_ANY_HIT, skip = flags & FL_SKIP_LIGHTS, cnt = (flags & FL_COUNTERS) && d_counters;
-    hipStream_t st = ctx->stream;
-    if (!any) {
-        if (cnt) launch<false, false, true>(st, grid, sc, d_rays, d_hits, n, spill, d_counters);
-        else launch<false, false, false>(st, grid, sc, d_rays, d_hits, n, spill, d_counters);
-    } else if (skip) {
-        if (cnt) launch<true, true, true>(st, grid, sc, d_rays, d_hits, n, spill, d_counters);
-        else launch<true, true, false>(st, grid, sc, d_rays, d_hits, n, spill, d_counters);
-    } else {
-        if (cnt) launch<true, false, true>(st, grid, sc, d_rays, d_hits, n, spill, d_counters);
-        else launch<true, false, false>(st, grid, sc, d_rays, d_hits, n, spill, d_counters);
-    }
+    int rc;
+    if (!any) rc = cnt ? launch<false, false, true>(ctx, depth, sc, src, n, d_counters)
+                       : launch<false, false, false>(ctx, depth, sc, src, n, d_counters);
+    else if (skip) rc = cnt ? launch<true, true, true>(ctx, depth, sc, src, n, d_counters)
+                            : launch<true, true, false>(ctx, depth, sc, src, n, d_counters);
+    else rc = cnt ? launch<true, false, true>(ctx, depth, sc, src, n, d_counters)
+                  : launch<true, false, false>(ctx, depth, sc, src, n, d_counters);
+    if (rc) return rc;
     FL_HIP_CHECK(ctx, hipGetLastError());
     return FL_OK;
 }

@@ -74,6 +74,7 @@ struct RenderArgs {
     float *film_sum, *film_sq;
     uint2 *spill;
     uint32_t spill_stride;
+    int trav_budget;         // traversal iterations between lane refills
 };
 
 } // namespace fl
