@@ -85,6 +85,9 @@ def main():
     ap.add_argument("--spp", type=int, default=0)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-cornell", action="store_true", help="skip the small Cornell section")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="also run a plain 1 GiB -> 1 GiB copy kernel (HBM counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this build runs on exactly one GPU (renders shard by pixels/samples; see DESIGN.md)",
@@ -106,7 +109,7 @@ def main():
             info = sc.info()
             log("BVH: %d nodes, depth %d, build %.1f ms" % (info.n_nodes, info.depth, info.build_ms))
             params = flgpu.default_params(spp=spp, depth=gen.depth, mode=flgpu.FL_SPECTRAL_HERO, seed=1,
-                                          flags=flgpu.FL_RENDER_VNDF_PDF)
+                                          flags=flgpu.FL_RENDER_VNDF_PDF | flgpu.FL_RENDER_GRAPH)
             for i in range(args.warmup):
                 _, _, st = sc.render(params, want_film=False)
                 log("warmup %d: %.1f ms" % (i, st.ms_total))
@@ -151,31 +154,39 @@ def main():
             out["bvh_build_ms"] = round(info.build_ms, 2)
             out["nan_samples"] = int(st.nan_samples)
 
+        if args.calibrate_copy:
+            # plain streaming kernel (k_copy_probe), 1 GiB read + 1 GiB written
+            # per launch: the reference point for FETCH_SIZE / WRITE_SIZE
+            out["copy_probe_gbs"] = round(ctx.probe_copy(1 << 30, 3), 1)
+
         # the small Cornell workload (the CPU-runnable one): GPU rate
-        cgen = flgpu.GenSceneData(flgpu.FL_GEN_CORNELL)
-        with ctx.scene(cgen) as csc:
-            cparams = flgpu.default_params(spp=cgen.spp, depth=cgen.depth, mode=flgpu.FL_SPECTRAL_HERO, seed=1)
-            csc.render(cparams, want_film=False)
-            ct = []
-            for _ in range(5):
-                ts = time.perf_counter()
+        cornell = None
+        if not args.no_cornell:
+            cgen = flgpu.GenSceneData(flgpu.FL_GEN_CORNELL)
+            with ctx.scene(cgen) as csc:
+                cparams = flgpu.default_params(spp=cgen.spp, depth=cgen.depth, mode=flgpu.FL_SPECTRAL_HERO, seed=1,
+                                               flags=flgpu.FL_RENDER_GRAPH)
                 csc.render(cparams, want_film=False)
-                ct.append(time.perf_counter() - ts)
-            c_samples = cgen.camera.width * cgen.camera.height * cgen.spp
-            cornell_gpu = c_samples * len(ct) / sum(ct) / 1e6
+                ct = []
+                for _ in range(5):
+                    ts = time.perf_counter()
+                    csc.render(cparams, want_film=False)
+                    ct.append(time.perf_counter() - ts)
+                c_samples = cgen.camera.width * cgen.camera.height * cgen.spp
+                cornell = {"gpu_msamples_s": round(c_samples * len(ct) / sum(ct) / 1e6, 2)}
 
     threads = host_threads()
     cpu = None
-    cornell = {"gpu_msamples_s": round(cornell_gpu, 2)}
     if not args.no_cpu:
         v, ns, secs = cpu_baseline(gen, 1, gen.depth, args.cpu_seconds, threads)
         cpu = {"value": round(v, 4), "unit": "Msamples/s", "cores": threads, "kind": "port",
                "sample": "%d samples (1 spp, rows of the same %dx%d image, depth %d), %.1f s" % (ns, W, H, gen.depth,
                                                                                              secs)}
-        cv, cns, csecs = cpu_baseline(cgen, cgen.spp, cgen.depth, args.cpu_seconds / 3, threads)
-        cornell.update({"cpu_msamples_s": round(cv, 4), "cpu_cores": threads,
-                        "speedup": round(cornell_gpu / cv, 1),
-                        "cpu_sample": "%d samples of the 256x256x16 image" % cns})
+        if cornell is not None:
+            cv, cns, csecs = cpu_baseline(cgen, cgen.spp, cgen.depth, args.cpu_seconds / 3, threads)
+            cornell.update({"cpu_msamples_s": round(cv, 4), "cpu_cores": threads,
+                            "speedup": round(cornell["gpu_msamples_s"] / cv, 1),
+                            "cpu_sample": "%d samples of the 256x256x16 image" % cns})
         out["speedup_vs_cpu"] = round(value / v, 1)
 
     res = {

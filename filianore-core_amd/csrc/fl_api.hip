@@ -1,4 +1,5 @@
 // fl_api.hip -- the C ABI of include/fl_gpu.h.
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <new>
@@ -43,6 +44,11 @@ void free_scene(fl_scene_s *s) {
     delete s;
 }
 
+__global__ void __launch_bounds__(256) k_copy_probe(const uint4 *__restrict__ src, uint4 *__restrict__ dst, size_t n) {
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) dst[i] = src[i];
+}
+
 } // namespace
 
 extern "C" {
@@ -53,6 +59,41 @@ int fl_device_count(void) {
     int n = 0;
     if (hipGetDeviceCount(&n) != hipSuccess) return 0;
     return n;
+}
+
+int fl_probe_copy(fl_ctx_t ctx, uint64_t bytes, uint32_t iters, double *gbps) {
+    if (!ctx || !gbps || bytes < 16 || iters == 0) return set_error(ctx, FL_E_INVALID, "fl_probe_copy: bad argument");
+    (void)hipSetDevice(ctx->device);
+    (void)hipGetLastError();
+    const size_t n = (size_t)(bytes / 16);
+    uint4 *src = nullptr, *dst = nullptr;
+    if (hipMalloc(&src, n * 16) != hipSuccess) return set_error(ctx, FL_E_OOM, "fl_probe_copy: source buffer");
+    if (hipMalloc(&dst, n * 16) != hipSuccess) {
+        (void)hipFree(src);
+        return set_error(ctx, FL_E_OOM, "fl_probe_copy: destination buffer");
+    }
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    int rc = FL_OK;
+    if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) rc = FL_E_HIP;
+    const unsigned grid = (unsigned)std::min<size_t>((n + 255) / 256, (size_t)ctx->n_cu * 16);
+    if (rc == FL_OK) {
+        hipLaunchKernelGGL(k_copy_probe, dim3(grid), dim3(256), 0, ctx->stream, src, dst, n);   // warm-up
+        (void)hipEventRecord(e0, ctx->stream);
+        for (uint32_t i = 0; i < iters; ++i)
+            hipLaunchKernelGGL(k_copy_probe, dim3(grid), dim3(256), 0, ctx->stream, src, dst, n);
+        (void)hipEventRecord(e1, ctx->stream);
+        if (hipStreamSynchronize(ctx->stream) != hipSuccess || hipGetLastError() != hipSuccess) rc = FL_E_HIP;
+    }
+    float ms = 0.f;
+    if (rc == FL_OK && hipEventElapsedTime(&ms, e0, e1) == hipSuccess && ms > 0.f)
+        *gbps = 2.0 * (double)n * 16.0 * iters / (ms * 1e-3) / 1e9;
+    else if (rc == FL_OK)
+        rc = FL_E_HIP;
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    (void)hipFree(src);
+    (void)hipFree(dst);
+    return rc == FL_OK ? FL_OK : set_error(ctx, rc, "fl_probe_copy: HIP failure");
 }
 
 int fl_ctx_create(int device, fl_ctx_t *out) {

@@ -36,6 +36,8 @@ struct SpecTables {
     std::vector<float> host;
 };
 
+struct RenderState;   // fl_render.hip
+
 struct fl_ctx_s {
     int device = 0;
     hipStream_t stream = nullptr;
@@ -45,6 +47,9 @@ struct fl_ctx_s {
     void *scratch = nullptr;
     size_t scratch_bytes = 0;
     hipEvent_t ev[16] = {};
+    // render working set (queues, path state, film, pinned counters, the
+    // captured bounce-loop graph), grown on demand and kept between calls
+    RenderState *render_state = nullptr;
 };
 
 struct fl_scene_s {

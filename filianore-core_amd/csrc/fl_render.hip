@@ -59,21 +59,28 @@ __global__ void k_zero_f32(float *p, size_t n) {
 }
 
 // ------------------------------------------------------------- generate ----
+// Path ids run over sample layers of npix padded pixels in 8x8 tile order;
+// queue slots skip the padding. The slot of an in-image pixel is its rank
+// among in-image pixels in the same order, in closed form (no atomics):
+// full tile rows before it, full tiles before it in its tile row, then rows
+// and columns inside its (possibly clipped) tile.
 __global__ void __launch_bounds__(kBlock) k_generate(RenderArgs a) {
     const uint32_t stride = gridDim.x * kBlock;
-    const uint32_t nround = (a.n_paths + kBlock - 1) / kBlock * kBlock;
-    for (uint32_t base = blockIdx.x * kBlock; base < nround; base += stride) {
-        uint32_t sid = base + threadIdx.x;
-        bool active = sid < a.n_paths;
-        uint32_t x = 0, y = 0, pixel = 0, sample = 0;
+    const uint32_t W = a.cam.W, H = a.cam.H;
+    if (blockIdx.x == 0 && threadIdx.x == 0) a.counts[C_EXT + 0] = W * H * a.spp_pass;
+    for (uint32_t sid = blockIdx.x * kBlock + threadIdx.x; sid < a.n_paths; sid += stride) {
+        uint32_t x = 0, y = 0, pixel = 0, sample = 0, slot = 0;
+        uint32_t s = sid / a.npix, pl = sid - s * a.npix;
+        uint32_t tile = pl >> 6, within = pl & 63u;
+        uint32_t tx = tile % a.tiles_x, ty = tile / a.tiles_x, wx = within & 7u, wy = within >> 3;
+        x = tx * 8u + wx;
+        y = ty * 8u + wy;
+        const bool active = x < W && y < H;
         if (active) {
-            uint32_t s = sid / a.npix, pl = sid - s * a.npix;
-            uint32_t tile = pl >> 6, within = pl & 63u;
-            x = (tile % a.tiles_x) * 8u + (within & 7u);
-            y = (tile / a.tiles_x) * 8u + (within >> 3);
-            active = x < a.cam.W && y < a.cam.H;
-            pixel = y * a.cam.W + x;
+            pixel = y * W + x;
             sample = a.sample_base + s;
+            const uint32_t rows_in = min(8u, H - ty * 8u), cols_in = min(8u, W - tx * 8u);
+            slot = s * (W * H) + 8u * W * ty + 8u * tx * rows_in + wy * cols_in + wx;
         }
         v3 o, d;
         if (active) {
@@ -92,15 +99,12 @@ __global__ void __launch_bounds__(kBlock) k_generate(RenderArgs a) {
             a.st_L[sid] = make_float4(0.f, 0.f, 0.f, 0.f);
             a.st_misc[sid] = make_float4(lam0, 1.f, 0.f, __uint_as_float(0u));
             a.st_key[sid] = make_uint2(pixel, sample);
-        } else if (sid < a.n_paths) {
+        } else {
             a.st_L[sid] = make_float4(0.f, 0.f, 0.f, 0.f);
             a.st_misc[sid] = make_float4(0.f, 1.f, 0.f, __uint_as_float(PF_DONE));
         }
-        if (sid < a.n_paths) {
-            a.pend[sid] = make_float4(0.f, 0.f, 0.f, 0.f);
-            a.pend[a.cap + sid] = make_float4(0.f, 0.f, 0.f, 0.f);
-        }
-        uint32_t slot = wave_append(active, &a.counts[C_EXT + 0]);
+        a.pend[sid] = make_float4(0.f, 0.f, 0.f, 0.f);
+        a.pend[a.cap + sid] = make_float4(0.f, 0.f, 0.f, 0.f);
         if (active) {
             a.rq_o[0][slot] = make_float4(o.x, o.y, o.z, 0.f);       // tMin = time = 0 (physical.cpp:21)
             a.rq_d[0][slot] = make_float4(d.x, d.y, d.z, kInf);
@@ -521,28 +525,58 @@ T *grow(void *&p, size_t &cap, size_t bytes, bool &err) {
 } // namespace
 
 // ----------------------------------------------------------- host driver ---
-struct RenderBuffers {
+struct RenderState {
     void *p[24] = {};
     size_t cap[24] = {};
+    uint32_t *h_counts = nullptr;     // pinned host copies of the queue counters, [2][16]
+    hipEvent_t ev[2] = {};
+    // bounce-loop graph (kChunk bounces), and the arguments it was captured with
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t gexec = nullptr;
+    RenderArgs gkey;
+    int gvariant = -1;
 };
 
 namespace flx {
 
-static RenderBuffers &bufs_for(fl_ctx_t ctx) {
-    static thread_local std::vector<std::pair<fl_ctx_t, RenderBuffers *>> tab;
-    for (auto &e : tab)
-        if (e.first == ctx) return *e.second;
-    tab.push_back({ctx, new RenderBuffers()});
-    return *tab.back().second;
+static RenderState *state_for(fl_ctx_t ctx) {
+    if (!ctx->render_state) {
+        RenderState *r = new (std::nothrow) RenderState();
+        if (!r) return nullptr;
+        memset(&r->gkey, 0, sizeof(r->gkey));
+        if (hipHostMalloc((void **)&r->h_counts, 2 * 16 * sizeof(uint32_t), hipHostMallocDefault) != hipSuccess ||
+            hipEventCreateWithFlags(&r->ev[0], hipEventDisableTiming) != hipSuccess ||
+            hipEventCreateWithFlags(&r->ev[1], hipEventDisableTiming) != hipSuccess) {
+            if (r->h_counts) (void)hipHostFree(r->h_counts);
+            for (auto &e : r->ev)
+                if (e) (void)hipEventDestroy(e);
+            delete r;
+            return nullptr;
+        }
+        ctx->render_state = r;
+    }
+    return ctx->render_state;
+}
+
+static void drop_graph(RenderState &b) {
+    if (b.gexec) (void)hipGraphExecDestroy(b.gexec);
+    if (b.graph) (void)hipGraphDestroy(b.graph);
+    b.gexec = nullptr;
+    b.graph = nullptr;
+    b.gvariant = -1;
 }
 
 void release_render_buffers(fl_ctx_t ctx) {
-    RenderBuffers &b = bufs_for(ctx);
-    for (int i = 0; i < 24; ++i) {
-        if (b.p[i]) (void)hipFree(b.p[i]);
-        b.p[i] = nullptr;
-        b.cap[i] = 0;
-    }
+    RenderState *b = ctx->render_state;
+    if (!b) return;
+    drop_graph(*b);
+    for (int i = 0; i < 24; ++i)
+        if (b->p[i]) (void)hipFree(b->p[i]);
+    if (b->h_counts) (void)hipHostFree(b->h_counts);
+    for (auto &e : b->ev)
+        if (e) (void)hipEventDestroy(e);
+    delete b;
+    ctx->render_state = nullptr;
 }
 
 // PhysicalCamera ctor with flipCameraDirection = true, pinhole (cameras/physical.h:21-43)
@@ -623,7 +657,9 @@ int render(fl_scene_s *s, const fl_camera *camera, const fl_render_params *p, fl
     a.flags = p->flags;
     a.xyz_weight = (tab.grid.hi - tab.grid.lo) / (4.f * 106.856895f);
 
-    RenderBuffers &B = bufs_for(ctx);
+    RenderState *Bp = state_for(ctx);
+    if (!Bp) return set_error(ctx, FL_E_OOM, "render state");
+    RenderState &B = *Bp;
     bool err = false;
     const size_t cap = a.cap;
     // persistent traversal grid: as many resident blocks as the kernel's
@@ -675,6 +711,45 @@ int render(fl_scene_s *s, const fl_camera *camera, const fl_render_params *p, fl
     const uint32_t max_iter = (uint32_t)std::max({p->diffuse_depth, p->reflection_depth, p->transmission_depth}) * 2 + 8;
     const uint32_t shade_grid_max = (uint32_t)ctx->n_cu * 8;
     uint32_t host_counts[16];
+
+    // Pipelined bounce loop: bounces are issued kChunk at a time with the
+    // full persistent grids (every kernel reads its queue size on the
+    // device); the host checks the queue sizes after chunk c while chunk c+1
+    // already runs, so the GPU does not wait for a host round trip. With
+    // FL_RENDER_GRAPH a chunk is one captured graph, replayed.
+    constexpr uint32_t kChunk = 4;   // even: a chunk starts and ends on queue parity 0
+    const int variant = (counters ? 1 : 0) | (p->shade_waves >= 4 ? 2 : 0);
+    auto launch_bounce = [&](const RenderArgs &args, uint32_t j) {
+        if (counters) hipLaunchKernelGGL(k_trace<true>, dim3(grid_pers), dim3(kBlock), 0, st, args, (int)j);
+        else hipLaunchKernelGGL(k_trace<false>, dim3(grid_pers), dim3(kBlock), 0, st, args, (int)j);
+        if (variant & 2) hipLaunchKernelGGL(k_shade<4>, dim3(shade_grid_max), dim3(kBlock), 0, st, args, (int)j);
+        else hipLaunchKernelGGL(k_shade<2>, dim3(shade_grid_max), dim3(kBlock), 0, st, args, (int)j);
+    };
+    const bool use_graph = (p->flags & FL_RENDER_GRAPH) && !timing;
+    if (use_graph) {
+        // trace and shade read only scene, queue and state fields: the
+        // per-pass fields are cleared so one capture serves every pass
+        RenderArgs key = a;
+        key.n_paths = 0;
+        key.spp_pass = 0;
+        key.sample_base = 0;
+        if (!B.gexec || B.gvariant != variant || memcmp(&key, &B.gkey, sizeof(key)) != 0) {
+            drop_graph(B);
+            FL_HIP_CHECK(ctx, hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+            for (uint32_t j = 0; j < kChunk; ++j) launch_bounce(key, j);
+            hipGraph_t g = nullptr;
+            hipError_t ce = hipStreamEndCapture(st, &g);
+            if (ce != hipSuccess || !g) return set_error(ctx, FL_E_HIP, "bounce graph capture failed");
+            B.graph = g;
+            if (hipGraphInstantiate(&B.gexec, B.graph, nullptr, nullptr, 0) != hipSuccess) {
+                drop_graph(B);
+                return set_error(ctx, FL_E_HIP, "bounce graph instantiation failed");
+            }
+            B.gkey = key;
+            B.gvariant = variant;
+        }
+    }
+
     while (done_spp < p->spp) {
         uint32_t this_spp = std::min(spp_pass, p->spp - done_spp);
         a.spp_pass = this_spp;
@@ -682,8 +757,35 @@ int render(fl_scene_s *s, const fl_camera *camera, const fl_render_params *p, fl
         a.sample_base = p->sample_offset + done_spp;
         hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, st, a.counts, 16u);
         hipLaunchKernelGGL(k_generate, dim3(std::min<unsigned>(blocks(a.n_paths), ctx->n_cu * 16)), dim3(kBlock), 0, st, a);
-        uint32_t ne = a.n_paths, ns = 0;   // upper bound for the first launch
+        uint32_t ne = W * H * this_spp, ns = 0;
         uint32_t iter = 0;
+        if (!timing) {
+            uint32_t issued = 0, checked = 0;
+            const uint32_t max_chunks = (max_iter + kChunk - 1) / kChunk;
+            for (;;) {
+                while (issued < checked + 2 && issued < max_chunks) {
+                    if (use_graph) FL_HIP_CHECK(ctx, hipGraphLaunch(B.gexec, st));
+                    else
+                        for (uint32_t j = 0; j < kChunk; ++j) launch_bounce(a, j);
+                    FL_HIP_CHECK(ctx, hipMemcpyAsync(B.h_counts + 16 * (issued & 1), a.counts, 64,
+                                                     hipMemcpyDeviceToHost, st));
+                    FL_HIP_CHECK(ctx, hipEventRecord(B.ev[issued & 1], st));
+                    ++issued;
+                }
+                if (checked == issued) break;
+                FL_HIP_CHECK(ctx, hipEventSynchronize(B.ev[checked & 1]));
+                const uint32_t *hc = B.h_counts + 16 * (checked & 1);
+                ++checked;
+                // after an even number of bounces the next trace reads these
+                if (hc[C_EXT + 0] + hc[C_SH + 1] == 0) break;
+            }
+            // a chunk issued past the end finds empty queues; the stream
+            // orders it before the next pass
+            iter = checked * kChunk;
+            n_trace += issued * kChunk;
+            n_shade += issued * kChunk;
+            ne = ns = 0;
+        }
         for (; iter < max_iter && (ne + ns) > 0; ++iter) {
             const int cur = iter & 1;
             uint32_t tg = std::min<uint32_t>(grid_pers, blocks((size_t)ne + ns));

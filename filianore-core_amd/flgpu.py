@@ -135,7 +135,7 @@ class GenScene(C.Structure):
 
 # exported symbols of include/fl_gpu.h and include/fl_scenegen.h
 EXPORTS = [
-    "fl_abi_version", "fl_device_count", "fl_ctx_create", "fl_ctx_destroy", "fl_last_error",
+    "fl_abi_version", "fl_device_count", "fl_probe_copy", "fl_ctx_create", "fl_ctx_destroy", "fl_last_error",
     "fl_scene_create", "fl_scene_destroy", "fl_scene_get_info", "fl_scene_download_bvh",
     "fl_intersect", "fl_intersect_device", "fl_render", "fl_render_batch", "fl_autotune",
     "fl_guide_create", "fl_guide_destroy", "fl_guide_train_step", "fl_guide_forward",
@@ -156,6 +156,7 @@ def lib():
         vp = C.c_void_p
         L.fl_ctx_create.argtypes = [C.c_int, C.POINTER(vp)]
         L.fl_ctx_destroy.argtypes = [vp]
+        L.fl_probe_copy.argtypes = [vp, C.c_uint64, C.c_uint32, C.POINTER(C.c_double)]
         L.fl_last_error.argtypes = [vp]
         L.fl_last_error.restype = C.c_char_p
         L.fl_scene_create.argtypes = [vp, C.POINTER(SceneDesc), C.POINTER(vp)]
@@ -313,6 +314,12 @@ class Context:
 
     def scene(self, gen):
         return Scene(self, gen)
+
+    def probe_copy(self, nbytes=1 << 30, iters=5):
+        """Device-to-device streaming copy; returns (read + write) GB/s."""
+        g = C.c_double(0.0)
+        _check(self.L.fl_probe_copy(self.h, nbytes, iters, C.byref(g)), self.h, "fl_probe_copy")
+        return g.value
 
     def bsdf_queries(self, queries):
         n = len(queries)

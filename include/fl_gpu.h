@@ -188,6 +188,12 @@ void fl_ctx_destroy(fl_ctx_t ctx);
 const char *fl_last_error(fl_ctx_t ctx);
 int fl_abi_version(void);
 int fl_device_count(void);
+/* HBM copy probe (no counterpart in the reference): a plain streaming kernel
+ * (k_copy_probe) copies `bytes` (rounded down to 16 B) from one device buffer
+ * to another `iters` times; *gbps = (read + write bytes) / time. It is the
+ * known-size reference point for memory-counter calibration and a measured
+ * bandwidth to set beside the published peak. */
+int fl_probe_copy(fl_ctx_t ctx, uint64_t bytes, uint32_t iters, double *gbps);
 
 /* Copies the geometry, builds the 8-wide quantized LBVH on the device. */
 int fl_scene_create(fl_ctx_t ctx, const fl_scene_desc *desc, fl_scene_t *out);

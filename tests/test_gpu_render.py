@@ -37,7 +37,8 @@ def z_stats(gs, gq, os_, oq, n):
 
 @pytest.mark.parametrize("config", [flgpu.FL_GEN_CORNELL])
 def test_parity_image_consistent_with_oracle(ctx, config):
-    gen = flgpu.GenSceneData(config, width=48, height=48)
+    # 46x44: edge tiles of the 8x8 path layout are clipped
+    gen = flgpu.GenSceneData(config, width=46, height=44)
     spp = 1024
     with ctx.scene(gen) as sc:
         p = flgpu.default_params(spp=spp, depth=gen.depth, mode=flgpu.FL_SPECTRAL_PARITY, seed=11)
@@ -70,6 +71,29 @@ def test_render_deterministic(ctx):
         a, _, _ = sc.render(p)
         b, _, _ = sc.render(p)
     assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
+
+
+def test_bounce_loop_variants_identical(ctx):
+    """The pipelined loop, the captured-graph loop (replayed many times over
+    several passes and two calls) and the eager per-bounce loop with host
+    reads of the queue sizes give bit-identical films."""
+    gen = flgpu.GenSceneData(flgpu.FL_GEN_CORNELL, width=60, height=44)
+    npix = 64 * 48   # 8x8 tiles, padded
+    films = {}
+    with ctx.scene(gen) as sc:
+        for name, flags in [("eager", flgpu.FL_RENDER_TIMING), ("pipelined", 0), ("graph", flgpu.FL_RENDER_GRAPH),
+                            ("graph again", flgpu.FL_RENDER_GRAPH),
+                            ("graph counters", flgpu.FL_RENDER_GRAPH | flgpu.FL_RENDER_COUNTERS),
+                            ("eager again", flgpu.FL_RENDER_TIMING)]:
+            p = flgpu.default_params(spp=8, depth=8, seed=9, flags=flags)
+            p.max_paths = 2 * npix   # 2 samples per pixel per pass: 4 passes
+            f, _, st = sc.render(p)
+            assert st.passes == 4 and st.nan_samples == 0
+            assert st.rays_extension >= 60 * 44 * 8   # every camera ray was queued
+            films[name] = f
+    ref = films["eager"].view(np.uint32)
+    for name, f in films.items():
+        assert np.array_equal(f.view(np.uint32), ref), name
 
 
 def test_hero_mode_close_to_parity_mode(ctx):
