@@ -29,7 +29,9 @@ namespace {
 using namespace fl;
 
 constexpr int kBlock = 256;
-constexpr int kStack = 12;
+// device bytes per path slot: state 56, pending NEE 32, two extension queues
+// 72, hit 16, shadow queue (two entries) 72
+constexpr size_t kBytesPerPath = 248;
 
 // device counter slots
 constexpr int C_EXT = 0;    // [0],[1]: extension queue sizes by bounce parity
@@ -141,9 +143,11 @@ struct TraceSrc {
     }
 };
 
-template <bool COUNT>
-__global__ void __launch_bounds__(kBlock) k_trace(RenderArgs a, int b) {
-    __shared__ uint2 stk[kStack * kBlock];
+// STACK: LDS short-stack entries per lane (deeper entries spill to global
+// memory); WAVES: minimum waves per SIMD asked of the register allocator.
+template <bool COUNT, int STACK, int WAVES>
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WAVES))) k_trace(RenderArgs a, int b) {
+    __shared__ unsigned long long stk[STACK * kBlock];
     const int cur = b & 1;
     const uint32_t ns = a.counts[C_SH + (cur ^ 1)];
     const uint32_t ne = a.counts[C_EXT + cur];
@@ -155,9 +159,9 @@ __global__ void __launch_bounds__(kBlock) k_trace(RenderArgs a, int b) {
         atomicAdd(&a.stats[1], (unsigned long long)ns);
     }
     LaneStack ls;
-    ls.lds = stk + threadIdx.x;
+    ls.lds = (lds_u64 *)(stk + threadIdx.x);
     ls.lds_stride = kBlock;
-    ls.cap = kStack;
+    ls.cap = STACK;
     ls.spill = a.spill + blockIdx.x * kBlock + threadIdx.x;
     ls.spill_stride = a.spill_stride;
     TravCounters c;
@@ -169,6 +173,27 @@ __global__ void __launch_bounds__(kBlock) k_trace(RenderArgs a, int b) {
         atomicAdd(&a.stats[5], (unsigned long long)c.spills);
     }
 }
+
+template <int S, int WV>
+void launch_trace_t(bool count, dim3 g, hipStream_t st, const RenderArgs &a, int j) {
+    if (count) hipLaunchKernelGGL((k_trace<true, S, WV>), g, dim3(kBlock), 0, st, a, j);
+    else hipLaunchKernelGGL((k_trace<false, S, WV>), g, dim3(kBlock), 0, st, a, j);
+}
+template <int S, int WV>
+const void *trace_fn() { return reinterpret_cast<const void *>(k_trace<false, S, WV>); }
+
+// traversal variants (LDS stack entries, minimum waves per SIMD); the first
+// is the default, the rest are tuning choices
+struct TraceVariant {
+    int stack, waves;
+    void (*launch)(bool, dim3, hipStream_t, const RenderArgs &, int);
+    const void *(*fn)();
+};
+const TraceVariant kTraceVariants[] = {
+    {12, 1, launch_trace_t<12, 1>, trace_fn<12, 1>}, {8, 1, launch_trace_t<8, 1>, trace_fn<8, 1>},
+    {8, 6, launch_trace_t<8, 6>, trace_fn<8, 6>},    {8, 8, launch_trace_t<8, 8>, trace_fn<8, 8>},
+    {6, 8, launch_trace_t<6, 8>, trace_fn<6, 8>},    {16, 1, launch_trace_t<16, 1>, trace_fn<16, 1>},
+};
 
 // ---------------------------------------------------------------- shade ----
 __device__ __forceinline__ void setup_bsdf(const RenderArgs &a, const DevMaterial &m, const Lanes &ln, Bsdf &b) {
@@ -229,15 +254,46 @@ __device__ __forceinline__ bool nee_sample(const RenderArgs &a, const Bsdf &b, c
 
 // WAVES: minimum waves per SIMD requested from the register allocator (2 lets
 // it use all the registers it wants; 4 caps at 128 VGPRs with a few spills).
+// Shading sort: each block orders its 256 queue entries by material kind
+// (counting sort in LDS) before shading them, so a wave runs one BSDF's code
+// instead of every kind present in its 64 entries. Per-path results do not
+// depend on the order; only the order of the next queue changes.
+constexpr uint32_t kSortKeys = 5;   // miss / no material, Lambert, conductor, dielectric, past the end
+
+__device__ __forceinline__ uint32_t shade_key(const RenderArgs &a, uint32_t i, uint32_t n) {
+    if (i >= n) return kSortKeys - 1;
+    const uint32_t rec = __float_as_uint(a.hits[i].y);
+    if (rec == kNoHit) return 0;
+    const uint32_t mat = reinterpret_cast<const uint4 *>(a.sc.tris + rec)[1].w;
+    if (mat == kNoMaterial) return 0;
+    return (uint32_t)a.mats[mat].kind;   // MAT_LAMBERT 1, MAT_CONDUCTOR 2, MAT_DIELECTRIC 3
+}
+
 template <int WAVES>
 __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WAVES, 8))) k_shade(RenderArgs a, int b) {
+    __shared__ uint32_t s_cnt[kSortKeys];
+    __shared__ uint16_t s_perm[kBlock];
     const uint32_t stride = gridDim.x * kBlock;
     const int cur = b & 1, nxt = cur ^ 1;
     const uint32_t n = a.counts[C_EXT + cur];
     const uint32_t nround = (n + kBlock - 1) / kBlock * kBlock;
     const int S = a.grid.stride;
+    const bool sort = !(a.flags & FL_RENDER_NO_SORT);
     for (uint32_t base = blockIdx.x * kBlock; base < nround; base += stride) {
-        const uint32_t i = base + threadIdx.x;
+        uint32_t i = base + threadIdx.x;
+        if (sort) {
+            const uint32_t key = shade_key(a, i, n);
+            if (threadIdx.x < kSortKeys) s_cnt[threadIdx.x] = 0u;
+            __syncthreads();
+            const uint32_t rank = atomicAdd(&s_cnt[key], 1u);
+            __syncthreads();
+            uint32_t off = 0;
+            for (uint32_t k = 0; k < key; ++k) off += s_cnt[k];
+            s_perm[off + rank] = (uint16_t)threadIdx.x;
+            __syncthreads();
+            i = base + s_perm[threadIdx.x];
+            __syncthreads();   // s_cnt and s_perm are rewritten by the next iteration
+        }
         const bool active = i < n;
         // -------- phase 1: hit, emission, pass-through, material setup
         bool cont = false, mat_hit = false;
@@ -644,7 +700,19 @@ int render(fl_scene_s *s, const fl_camera *camera, const fl_render_params *p, fl
     a.tiles_x = (W + 7) / 8;
     const uint32_t tiles_y = (H + 7) / 8;
     a.npix = a.tiles_x * tiles_y * 64u;
-    uint32_t max_paths = p->max_paths ? p->max_paths : (1u << 22);
+    // paths in flight per pass: every persistent launch ends in a tail where
+    // lanes run out of rays, so passes are made as large as memory allows
+    // (248 B of queues and state per path; 2^26 paths = 16.6 GB), capped at
+    // half the free device memory
+    uint32_t max_paths = p->max_paths;
+    if (!max_paths) {
+        size_t free_b = 0, total_b = 0;
+        max_paths = 1u << 26;
+        if (hipMemGetInfo(&free_b, &total_b) == hipSuccess) {
+            const size_t fit = free_b / 2 / kBytesPerPath;
+            if (fit < max_paths) max_paths = (uint32_t)std::max<size_t>(fit, 1u << 20);
+        }
+    }
     if (max_paths < a.npix) max_paths = a.npix;
     uint32_t spp_pass = std::max(1u, std::min(p->spp, max_paths / a.npix));
     a.cap = a.npix * spp_pass;
@@ -662,15 +730,19 @@ int render(fl_scene_s *s, const fl_camera *camera, const fl_render_params *p, fl
     RenderState &B = *Bp;
     bool err = false;
     const size_t cap = a.cap;
-    // persistent traversal grid: as many resident blocks as the kernel's
-    // occupancy (LDS stack + VGPRs) allows on every CU
+    // traversal variant, then the persistent grid: as many resident blocks
+    // as its occupancy (LDS stack + VGPRs) allows on every CU
+    const int want_stack = p->stack_depth ? (int)p->stack_depth : kTraceVariants[0].stack;
+    const int want_waves = p->trav_waves ? (int)p->trav_waves : 1;
+    const TraceVariant *tv = nullptr;
+    for (const TraceVariant &v : kTraceVariants)
+        if (v.stack == want_stack && v.waves == want_waves) tv = &v;
+    if (!tv) return set_error(ctx, FL_E_INVALID, "fl_render: no traversal variant with this stack_depth/trav_waves");
     int occ = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, reinterpret_cast<const void *>(k_trace<false>), kBlock, 0) !=
-            hipSuccess || occ <= 0)
-        occ = 4;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, tv->fn(), kBlock, 0) != hipSuccess || occ <= 0) occ = 4;
     const uint32_t grid_pers = (uint32_t)(ctx->n_cu * occ);
     a.trav_budget = p->trav_budget ? (int)p->trav_budget : 16;
-    size_t spill_per = (size_t)(s->depth > (uint32_t)kStack ? s->depth - kStack + 1 : 1);
+    size_t spill_per = (size_t)(s->depth > (uint32_t)tv->stack ? s->depth - tv->stack + 1 : 1);
     a.spill_stride = grid_pers * kBlock;
     a.st_beta = grow<float4>(B.p[0], B.cap[0], cap * 16, err);
     a.st_L = grow<float4>(B.p[1], B.cap[1], cap * 16, err);
@@ -718,10 +790,9 @@ int render(fl_scene_s *s, const fl_camera *camera, const fl_render_params *p, fl
     // already runs, so the GPU does not wait for a host round trip. With
     // FL_RENDER_GRAPH a chunk is one captured graph, replayed.
     constexpr uint32_t kChunk = 4;   // even: a chunk starts and ends on queue parity 0
-    const int variant = (counters ? 1 : 0) | (p->shade_waves >= 4 ? 2 : 0);
+    const int variant = (counters ? 1 : 0) | (p->shade_waves >= 4 ? 2 : 0) | ((int)(tv - kTraceVariants) << 2);
     auto launch_bounce = [&](const RenderArgs &args, uint32_t j) {
-        if (counters) hipLaunchKernelGGL(k_trace<true>, dim3(grid_pers), dim3(kBlock), 0, st, args, (int)j);
-        else hipLaunchKernelGGL(k_trace<false>, dim3(grid_pers), dim3(kBlock), 0, st, args, (int)j);
+        tv->launch(counters, dim3(grid_pers), st, args, (int)j);
         if (variant & 2) hipLaunchKernelGGL(k_shade<4>, dim3(shade_grid_max), dim3(kBlock), 0, st, args, (int)j);
         else hipLaunchKernelGGL(k_shade<2>, dim3(shade_grid_max), dim3(kBlock), 0, st, args, (int)j);
     };
@@ -790,8 +861,7 @@ int render(fl_scene_s *s, const fl_camera *camera, const fl_render_params *p, fl
             const int cur = iter & 1;
             uint32_t tg = std::min<uint32_t>(grid_pers, blocks((size_t)ne + ns));
             if (timing) FL_HIP_CHECK(ctx, hipEventRecord(tev[0], st));
-            if (counters) hipLaunchKernelGGL(k_trace<true>, dim3(tg), dim3(kBlock), 0, st, a, (int)iter);
-            else hipLaunchKernelGGL(k_trace<false>, dim3(tg), dim3(kBlock), 0, st, a, (int)iter);
+            tv->launch(counters, dim3(tg), st, a, (int)iter);
             ++n_trace;
             if (timing) FL_HIP_CHECK(ctx, hipEventRecord(tev[1], st));
             if (ne > 0) {
@@ -856,7 +926,7 @@ int render(fl_scene_s *s, const fl_camera *camera, const fl_render_params *p, fl
         stats->passes = passes;
         stats->max_bounce = max_iter_seen;
         stats->paths_per_pass = a.cap;
-        stats->stack_depth = kStack;
+        stats->stack_depth = (uint32_t)tv->stack;
     }
     return FL_OK;
 }

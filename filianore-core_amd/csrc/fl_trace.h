@@ -30,8 +30,12 @@ struct TraceHit {
     uint32_t rec;  // triangle record index (leaf order)
 };
 
+// pointer into LDS (address space 3), so stack accesses compile to
+// ds_read/ds_write; an entry packs (group base, imask | hits << 8) in 64 bits
+typedef __attribute__((address_space(3))) unsigned long long lds_u64;
+
 struct LaneStack {
-    uint2 *lds;        // lane's first entry; entry i at lds[i * lds_stride]
+    lds_u64 *lds;      // lane's first entry; entry i at lds[i * lds_stride]
     int lds_stride;
     int cap;           // entries in LDS
     uint2 *spill;      // lane's first spill entry; entry i at spill[i * spill_stride]
@@ -80,10 +84,12 @@ __device__ __forceinline__ void trav_init(TravState &s, v3 o, v3 d, float tmin, 
 }
 
 // Runs at most `budget` loop iterations; returns true when the ray is done
-// (state holds the closest hit, or for ANY the first accepted hit).
-template <bool ANY, bool SKIP_LIGHTS, bool COUNT>
+// (state holds the closest hit, or with `any` the first accepted hit).
+// `any` and `skip_lights` are per lane, so closest-hit and any-hit rays in one
+// wave run the same instruction stream.
+template <bool COUNT>
 __device__ __forceinline__ bool trav_step(const TraceScene &sc, TravState &s, const LaneStack &ls, int budget,
-                                          TravCounters &cnt) {
+                                          TravCounters &cnt, bool any, bool skip_lights) {
     const float kLo = 1.0f - 0x1p-20f, kHi = 1.0f + 0x1p-20f;
     for (int it = 0; it < budget; ++it) {
         if (s.g_hits) {
@@ -93,37 +99,46 @@ __device__ __forceinline__ bool trav_step(const TraceScene &sc, TravState &s, co
             uint32_t nidx = s.g_base + __popc(s.g_imask & ((1u << slot) - 1u));
             if (s.g_hits) {
                 uint2 e = make_uint2(s.g_base, s.g_imask | (s.g_hits << 8));
-                if (s.sp < ls.cap) ls.lds[s.sp * ls.lds_stride] = e;
+                if (s.sp < ls.cap) ls.lds[s.sp * ls.lds_stride] = ((unsigned long long)e.y << 32) | e.x;
                 else { ls.spill[(size_t)(s.sp - ls.cap) * ls.spill_stride] = e; if (COUNT) ++cnt.spills; }
                 ++s.sp;
             }
             const uint4 *np = reinterpret_cast<const uint4 *>(sc.nodes + nidx);
             uint4 w0 = np[0], w1 = np[1];
-            uint4 ql[3] = {np[2], np[3], np[4]};
-            uint4 qh[3] = {np[5], np[6], np[7]};
+            uint4 qn[3] = {np[2], np[3], np[4]};
+            uint4 qf[3] = {np[5], np[6], np[7]};
             if (COUNT) ++cnt.nodes;
             float org[3] = {__uint_as_float(w0.x), __uint_as_float(w0.y), __uint_as_float(w0.z)};
             float step[3] = {__uint_as_float((w0.w & 0xffu) << 23), __uint_as_float(((w0.w >> 8) & 0xffu) << 23),
                              __uint_as_float(((w0.w >> 16) & 0xffu) << 23)};
             uint32_t imask = w0.w >> 24;
             float oo[3] = {s.o.x, s.o.y, s.o.z};
+            // the ray's near plane per axis is lo for a positive direction and
+            // hi for a negative one: swap the packed plane words once per node
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                if ((s.oct >> k) & 1u) {
+                    uint4 t = qn[k];
+                    qn[k] = qf[k];
+                    qf[k] = t;
+                }
+            }
             uint32_t hits = 0;
 #pragma unroll
             for (int q = 0; q < 8; ++q) {
                 float tn3[3], tf3[3];
 #pragma unroll
                 for (int k = 0; k < 3; ++k) {
-                    uint32_t wl = (q >> 1) == 0 ? ql[k].x : ((q >> 1) == 1 ? ql[k].y : ((q >> 1) == 2 ? ql[k].z : ql[k].w));
-                    uint32_t wh = (q >> 1) == 0 ? qh[k].x : ((q >> 1) == 1 ? qh[k].y : ((q >> 1) == 2 ? qh[k].z : qh[k].w));
-                    uint32_t qlo = (q & 1) ? (wl >> 16) : (wl & 0xffffu);
-                    uint32_t qhi = (q & 1) ? (wh >> 16) : (wh & 0xffffu);
-                    float plo = org[k] + (float)qlo * step[k];
-                    float phi = org[k] + (float)qhi * step[k];
-                    float t0 = (plo - oo[k]) * s.inv[k];
-                    float t1 = (phi - oo[k]) * s.inv[k];
-                    bool neg = (s.oct >> k) & 1u;
-                    tn3[k] = neg ? t1 : t0;
-                    tf3[k] = neg ? t0 : t1;
+                    uint32_t wn = (q >> 1) == 0 ? qn[k].x : ((q >> 1) == 1 ? qn[k].y : ((q >> 1) == 2 ? qn[k].z : qn[k].w));
+                    uint32_t wf = (q >> 1) == 0 ? qf[k].x : ((q >> 1) == 1 ? qf[k].y : ((q >> 1) == 2 ? qf[k].z : qf[k].w));
+                    uint32_t qnear = (q & 1) ? (wn >> 16) : (wn & 0xffffu);
+                    uint32_t qfar = (q & 1) ? (wf >> 16) : (wf & 0xffffu);
+                    // org + q * 2^e is exactly representable: the fused form
+                    // returns the same bits as the separate multiply and add
+                    float pn = __builtin_fmaf((float)qnear, step[k], org[k]);
+                    float pf = __builtin_fmaf((float)qfar, step[k], org[k]);
+                    tn3[k] = (pn - oo[k]) * s.inv[k];
+                    tf3[k] = (pf - oo[k]) * s.inv[k];
                 }
                 float n_ = fmaxf(fmaxf(tn3[0], tn3[1]), tn3[2]) * kLo;
                 float f_ = fminf(fminf(tf3[0], tf3[1]), tf3[2]) * kHi;
@@ -155,8 +170,13 @@ __device__ __forceinline__ bool trav_step(const TraceScene &sc, TravState &s, co
         } else if (!s.t_hits) {
             if (s.sp == 0) return true;
             --s.sp;
-            uint2 e = (s.sp < ls.cap) ? ls.lds[s.sp * ls.lds_stride]
-                                      : ls.spill[(size_t)(s.sp - ls.cap) * ls.spill_stride];
+            uint2 e;
+            if (s.sp < ls.cap) {
+                const unsigned long long v = ls.lds[s.sp * ls.lds_stride];
+                e = make_uint2((uint32_t)v, (uint32_t)(v >> 32));
+            } else {
+                e = ls.spill[(size_t)(s.sp - ls.cap) * ls.spill_stride];
+            }
             s.g_base = e.x;
             s.g_imask = e.y & 0xffu;
             s.g_hits = e.y >> 8;
@@ -169,13 +189,13 @@ __device__ __forceinline__ bool trav_step(const TraceScene &sc, TravState &s, co
             const uint4 *tp = reinterpret_cast<const uint4 *>(sc.tris + ri);
             uint4 a = tp[0], bq = tp[1], c = tp[2];
             if (COUNT) ++cnt.tris;
-            if (SKIP_LIGHTS && c.w != 0u) continue;
+            if (skip_lights && c.w != 0u) continue;
             v3 p0(__uint_as_float(a.x), __uint_as_float(a.y), __uint_as_float(a.z));
             v3 p1(__uint_as_float(bq.x), __uint_as_float(bq.y), __uint_as_float(bq.z));
             v3 p2(__uint_as_float(c.x), __uint_as_float(c.y), __uint_as_float(c.z));
             float t, u, v;
             if (tri_intersect(s.rp, s.o, p0, p1, p2, s.tmin, s.best_t, &t, &u, &v)) {
-                if (ANY) {
+                if (any) {
                     s.best_t = t; s.best_id = a.w; s.best_rec = ri; s.bu = u; s.bv = v;
                     return true;
                 }
@@ -194,7 +214,7 @@ __device__ __forceinline__ bool traverse(const TraceScene &sc, v3 o, v3 d, float
                                          const LaneStack &ls, TraceHit &hit, TravCounters &cnt) {
     TravState s;
     trav_init(s, o, d, tmin, tmax);
-    while (!trav_step<ANY, SKIP_LIGHTS, COUNT>(sc, s, ls, 1 << 30, cnt)) {
+    while (!trav_step<COUNT>(sc, s, ls, 1 << 30, cnt, ANY, SKIP_LIGHTS)) {
     }
     hit.t = s.best_t;
     hit.u = s.bu;
@@ -233,8 +253,7 @@ __device__ __forceinline__ void trace_persistent(const TraceScene &sc, const Src
     while (true) {
         if (__ballot(ray >= 0) == 0ull && __ballot(!exhausted) == 0ull) break;
         if (ray >= 0) {
-            bool done = any ? trav_step<true, ANY_SKIP, COUNT>(sc, s, ls, budget, cnt)
-                            : trav_step<false, false, COUNT>(sc, s, ls, budget, cnt);
+            bool done = trav_step<COUNT>(sc, s, ls, budget, cnt, any, any && ANY_SKIP);
             if (done) {
                 src.store((uint32_t)ray, s);
                 ray = -1;

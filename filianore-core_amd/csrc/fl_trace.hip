@@ -42,10 +42,10 @@ __global__ void k_zero_fetch(uint32_t *p) { *p = 0u; }
 template <bool ANY, bool SKIP, bool COUNT>
 __global__ void __launch_bounds__(kBlock) k_trace_api(fl::TraceScene sc, ApiSrc src, uint32_t n, uint2 *spill,
                                                       uint32_t *fetch, unsigned long long *counters) {
-    __shared__ uint2 stk[kStack * kBlock];
+    __shared__ unsigned long long stk[kStack * kBlock];
     const uint32_t gtid = blockIdx.x * kBlock + threadIdx.x;
     fl::LaneStack ls;
-    ls.lds = stk + threadIdx.x;
+    ls.lds = (fl::lds_u64 *)(stk + threadIdx.x);
     ls.lds_stride = kBlock;
     ls.cap = kStack;
     ls.spill = spill + gtid;

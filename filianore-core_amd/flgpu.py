@@ -63,7 +63,8 @@ class RenderParams(C.Structure):
                 ("transmission_depth", C.c_int32), ("rr_threshold", C.c_float), ("rr_start", C.c_uint32),
                 ("spectral_mode", C.c_uint32), ("flags", C.c_uint32), ("seed", C.c_uint64),
                 ("sample_offset", C.c_uint32), ("max_paths", C.c_uint32), ("stack_depth", C.c_uint32),
-                ("trav_budget", C.c_uint32), ("shade_waves", C.c_uint32), ("guide", C.c_void_p),
+                ("trav_budget", C.c_uint32), ("shade_waves", C.c_uint32), ("trav_waves", C.c_uint32),
+                ("guide", C.c_void_p),
                 ("guide_mix", C.c_float),
                 ("guide_train_fraction", C.c_float)]
 

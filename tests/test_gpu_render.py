@@ -75,8 +75,9 @@ def test_render_deterministic(ctx):
 
 def test_bounce_loop_variants_identical(ctx):
     """The pipelined loop, the captured-graph loop (replayed many times over
-    several passes and two calls) and the eager per-bounce loop with host
-    reads of the queue sizes give bit-identical films."""
+    several passes and two calls), the eager per-bounce loop with host
+    reads of the queue sizes, and shading with and without the per-block
+    material sort give bit-identical films."""
     gen = flgpu.GenSceneData(flgpu.FL_GEN_CORNELL, width=60, height=44)
     npix = 64 * 48   # 8x8 tiles, padded
     films = {}
@@ -84,6 +85,7 @@ def test_bounce_loop_variants_identical(ctx):
         for name, flags in [("eager", flgpu.FL_RENDER_TIMING), ("pipelined", 0), ("graph", flgpu.FL_RENDER_GRAPH),
                             ("graph again", flgpu.FL_RENDER_GRAPH),
                             ("graph counters", flgpu.FL_RENDER_GRAPH | flgpu.FL_RENDER_COUNTERS),
+                            ("unsorted shading", flgpu.FL_RENDER_GRAPH | flgpu.FL_RENDER_NO_SORT),
                             ("eager again", flgpu.FL_RENDER_TIMING)]:
             p = flgpu.default_params(spp=8, depth=8, seed=9, flags=flags)
             p.max_paths = 2 * npix   # 2 samples per pixel per pass: 4 passes
