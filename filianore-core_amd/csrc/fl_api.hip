@@ -333,15 +333,5 @@ int fl_render_batch(fl_ctx_t ctx, const fl_batch_item *, uint32_t, const fl_rend
 int fl_autotune(fl_scene_t s, const fl_camera *, const fl_render_params *, fl_tune_result *) {
     return set_error(s ? s->ctx : nullptr, FL_E_UNSUPPORTED, "fl_autotune: not built");
 }
-int fl_guide_create(fl_ctx_t ctx, const fl_guide_config *, fl_guide_t *) {
-    return set_error(ctx, FL_E_UNSUPPORTED, "guide: not built");
-}
-void fl_guide_destroy(fl_guide_t) {}
-int fl_guide_train_step(fl_guide_t, const float *, uint32_t, int, float *) { return FL_E_UNSUPPORTED; }
-int fl_guide_forward(fl_guide_t, const float *, float *, uint32_t) { return FL_E_UNSUPPORTED; }
-int fl_guide_get_params(fl_guide_t, float *, uint32_t, uint32_t *) { return FL_E_UNSUPPORTED; }
-int fl_guide_set_params(fl_guide_t, const float *, uint32_t) { return FL_E_UNSUPPORTED; }
-int fl_guide_gradient(fl_guide_t, const float *, uint32_t, float *, uint32_t, float *) { return FL_E_UNSUPPORTED; }
-int fl_guide_stats(fl_guide_t, uint32_t *, float *, int *) { return FL_E_UNSUPPORTED; }
 
 } // extern "C"

@@ -1,0 +1,551 @@
+// fl_guide.hip -- training and API of the learned directional proposal
+// (vMF mixture MLP, bf16 on the matrix cores, fp32 master weights, Adam).
+//
+// Loss of one sample (direction w drawn while rendering, weight c):
+//     l = -c log p(w),  p = sum_i w_i vMF_i(w)
+// with c = (radiance the path gathered beyond the vertex) / (throughput at
+// the vertex) = f cos L_i / p_sample: the sample estimate of the KL
+// divergence from the product f cos L_i to the mixture, up to a constant.
+//
+// Gradients are summed per 64-sample block into a partial buffer and reduced
+// in a fixed order, so a training step is deterministic.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <new>
+#include <vector>
+
+#include "fl_guide_mlp.h"
+#include "fl_internal.h"
+
+using flx::set_error;
+
+namespace {
+
+using namespace fl;
+
+constexpr int kRecEnc = 68;      // API record: encoded input 64, direction 3, weight 1
+constexpr int kRecRaw = 16;      // render record (see guide_train_on_records)
+constexpr float kMaxWeight = 20.f;   // weight clamp after normalisation by the mean
+constexpr float kClipNorm = 10.f;    // gradient norm clip (mean loss)
+
+__device__ __forceinline__ uint32_t hash_u32(uint32_t x) {
+    x ^= x >> 16; x *= 0x7feb352du;
+    x ^= x >> 15; x *= 0x846ca68bu;
+    x ^= x >> 16;
+    return x;
+}
+
+// API records: already encoded
+struct EncSrc {
+    const float *rec;
+    uint32_t n;
+    __device__ bool get(uint32_t i, float *x, v3 &dir, float &c) const {
+        if (i >= n) return false;
+        const float *r = rec + (size_t)i * kRecEnc;
+        for (int k = 0; k < 64; ++k) x[k] = r[k];
+        dir = v3(r[64], r[65], r[66]);
+        c = r[67];
+        return true;
+    }
+};
+
+// Render records, drawn uniformly by a counter hash, encoded here
+struct RawSrc {
+    const float *rec;
+    uint32_t n_recs, n_batch, step;
+    uint32_t seed;
+    float inv_mean;
+    __device__ bool get(uint32_t i, float *x, v3 &dir, float &c) const {
+        if (i >= n_batch || n_recs == 0) return false;
+        const uint32_t r = hash_u32(hash_u32(seed ^ (step * 0x9e3779b9u)) ^ (i * 0x85ebca6bu)) % n_recs;
+        const float *R = rec + (size_t)r * kRecRaw;
+        const float pos[3] = {R[0], R[1], R[2]};
+        guide_encode(pos, v3(R[3], R[4], R[5]), R[6], R[7], x);
+        dir = v3(R[8], R[9], R[10]);
+        const float w = R[14] * inv_mean;
+        c = (w > 0.f && w < 3.0e38f) ? fminf(w, kMaxWeight) : 0.f;
+        return true;
+    }
+};
+
+// Forward + backward of 64 samples per block (one wave). partial receives
+// this block's gradient sums, loss_part its loss sum.
+template <class Src>
+__global__ void __launch_bounds__(64) k_guide_grad(const __bf16 *W, const __bf16 *WT, const float *P, Src src,
+                                                   float *partial, float *loss_part) {
+    __shared__ __bf16 sA[kGMats][64 * kGRow];    // X, H1..H4 as [sample][unit]
+    __shared__ __bf16 sAT[kGMats][64 * kGRow];   // the same, [unit][sample]
+    __shared__ __bf16 sD[64 * kGRow], sDT[64 * kGRow];
+    __shared__ float sR[64 * 65];
+    const int q = threadIdx.x;
+    const uint32_t base = blockIdx.x * 64u;
+    float x[64];
+    v3 dir(0.f, 0.f, 1.f);
+    float c = 0.f;
+    const bool ok = src.get(base + q, x, dir, c);
+    if (!ok)
+        for (int i = 0; i < 64; ++i) x[i] = 0.f;
+    for (int i = 0; i < 64; ++i) {
+        const __bf16 h = to_bf16(x[i]);
+        sA[0][q * kGRow + i] = h;
+        sAT[0][i * kGRow + q] = h;
+    }
+    __syncthreads();
+    for (int k = 0; k < 4; ++k) {
+        layer_fwd(W + k * 4096, P + k * kGLayerParams + 4096, sA[k], sA[k + 1], sAT[k + 1], true);
+        __syncthreads();
+    }
+    head_fwd(W + 4 * 4096, P + 4 * kGLayerParams + 4096, sA[4], sR, 65);
+    __syncthreads();
+    float g[kGOut];
+    float loss = 0.f;
+    if (ok && c > 0.f) loss = guide_head_grad(sR + q * 65, dir, c, g);
+    else
+        for (int o = 0; o < kGOut; ++o) g[o] = 0.f;
+    __syncthreads();
+    for (int o = 0; o < 64; ++o) {
+        const float v = o < kGOut ? g[o] : 0.f;
+        sR[q * 65 + o] = v;
+        const __bf16 h = to_bf16(v);
+        sD[q * kGRow + o] = h;
+        sDT[o * kGRow + q] = h;
+    }
+    __syncthreads();
+    float *pb = partial + (size_t)blockIdx.x * kGParams;
+    for (int k = kGMats - 1; k >= 0; --k) {
+        // bias: sum over samples of the fp32 delta; lane = unit
+        float sb = 0.f;
+        for (int s = 0; s < 64; ++s) sb += sR[s * 65 + q];
+        pb[k * kGLayerParams + 4096 + q] = sb;
+        // weight: dW[o][i] = sum_q D[q][o] A[q][i]  (P = D^T [o][q], Q = A^T [i][q])
+        f32x4 acc[4][4];
+        acc_zero(acc);
+        mm64(sDT, kGRow, sAT[k], kGRow, acc);
+#pragma unroll
+        for (int mt = 0; mt < 4; ++mt)
+#pragma unroll
+            for (int nt = 0; nt < 4; ++nt)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) pb[k * kGLayerParams + acc_m(mt, r) * 64 + acc_n(nt)] = acc[mt][nt][r];
+        if (k == 0) break;
+        // data: dA^T[i][q] = sum_o WT[i][o] D[q][o], then the ReLU mask of A = relu(Z)
+        acc_zero(acc);
+        mm64(WT + k * 4096, kGH, sD, kGRow, acc);
+        __syncthreads();
+#pragma unroll
+        for (int mt = 0; mt < 4; ++mt)
+#pragma unroll
+            for (int nt = 0; nt < 4; ++nt) {
+                const int s = acc_n(nt);
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const int i = acc_m(mt, r);
+                    const float a = (float)sA[k][s * kGRow + i];
+                    const float d = a > 0.f ? acc[mt][nt][r] : 0.f;
+                    sR[s * 65 + i] = d;
+                    const __bf16 h = to_bf16(d);
+                    sD[s * kGRow + i] = h;
+                    sDT[i * kGRow + s] = h;
+                }
+            }
+        __syncthreads();
+    }
+    // loss: wave sum
+    for (int off = 32; off > 0; off >>= 1) loss += __shfl_xor(loss, off);
+    if (q == 0) loss_part[blockIdx.x] = loss;
+}
+
+// grad[p] = sum over blocks of partial[b][p] (fixed order)
+__global__ void k_grad_reduce(const float *partial, uint32_t nblocks, float *grad) {
+    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= (uint32_t)kGParams) return;
+    float s = 0.f;
+    for (uint32_t b = 0; b < nblocks; ++b) s += partial[(size_t)b * kGParams + p];
+    grad[p] = s;
+}
+
+__device__ __forceinline__ float block_sum(float v, float *red) {
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+    const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+    if (l == 0) red[w] = v;
+    __syncthreads();
+    float t = 0.f;
+    if (threadIdx.x == 0) {
+        for (int i = 0; i < (int)(blockDim.x >> 6); ++i) t += red[i];
+        red[32] = t;
+    }
+    __syncthreads();
+    t = red[32];
+    __syncthreads();
+    return t;
+}
+
+// bf16 copies of the master weights: W[k][o][i] and WT[k][i][o]
+__device__ __forceinline__ void publish_one(const float *P, __bf16 *W, __bf16 *WT, uint32_t e) {
+    const uint32_t k = e / 4096u, oi = e % 4096u, o = oi / 64u, i = oi % 64u;
+    const __bf16 h = to_bf16(P[k * kGLayerParams + oi]);
+    W[k * 4096u + o * 64u + i] = h;
+    WT[k * 4096u + i * 64u + o] = h;
+}
+
+__global__ void k_guide_publish(const float *P, __bf16 *W, __bf16 *WT) {
+    const uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e < (uint32_t)(kGMats * 4096)) publish_one(P, W, WT, e);
+}
+
+// One Adam step from the summed gradient (one block). Skips the update and
+// raises the diverged flag when the loss or the gradient is not finite.
+__global__ void __launch_bounds__(1024) k_adam(float *P, float *M, float *V, const float *grad, const float *loss_part,
+                                               uint32_t nblocks, float inv_n, float lr, float b1, float b2, float eps,
+                                               uint32_t t, float *stat, __bf16 *W, __bf16 *WT) {
+    __shared__ float red[33];
+    float l = 0.f;
+    for (uint32_t b = threadIdx.x; b < nblocks; b += blockDim.x) l += loss_part[b];
+    const float loss = block_sum(l, red) * inv_n;
+    float s = 0.f;
+    for (int p = threadIdx.x; p < kGParams; p += blockDim.x) {
+        const float g = grad[p] * inv_n;
+        s += g * g;
+    }
+    const float norm = sqrtf(block_sum(s, red));
+    if (!isfinite(loss) || !isfinite(norm)) {
+        if (threadIdx.x == 0) { stat[0] = loss; stat[1] = norm; stat[2] = 1.f; }
+        return;
+    }
+    const float scale = norm > kClipNorm ? kClipNorm / norm : 1.f;
+    const float c1 = 1.f - powf(b1, (float)t), c2 = 1.f - powf(b2, (float)t);
+    for (int p = threadIdx.x; p < kGParams; p += blockDim.x) {
+        const float g = grad[p] * inv_n * scale;
+        const float m = b1 * M[p] + (1.f - b1) * g;
+        const float v = b2 * V[p] + (1.f - b2) * g * g;
+        M[p] = m;
+        V[p] = v;
+        P[p] = P[p] - lr * (m / c1) / (sqrtf(v / c2) + eps);
+    }
+    __syncthreads();
+    for (uint32_t e = threadIdx.x; e < (uint32_t)(kGMats * 4096); e += blockDim.x) publish_one(P, W, WT, e);
+    if (threadIdx.x == 0) { stat[0] = loss; stat[1] = norm; }
+}
+
+// Forward of encoded inputs (n x 64) to raw head outputs (n x 40).
+__global__ void __launch_bounds__(64) k_guide_fwd(const __bf16 *W, const float *P, const float *in, float *out,
+                                                  uint32_t n) {
+    __shared__ __bf16 sH[2][64 * kGRow];
+    __shared__ float sR[64 * 65];
+    const int q = threadIdx.x;
+    const uint32_t s = blockIdx.x * 64u + q;
+    for (int i = 0; i < 64; ++i) sH[0][q * kGRow + i] = to_bf16(s < n ? in[(size_t)s * 64 + i] : 0.f);
+    __syncthreads();
+    for (int k = 0; k < 4; ++k) {
+        layer_fwd(W + k * 4096, P + k * kGLayerParams + 4096, sH[k & 1], sH[(k + 1) & 1], nullptr, true);
+        __syncthreads();
+    }
+    head_fwd(W + 4 * 4096, P + 4 * kGLayerParams + 4096, sH[0], sR, 65);
+    __syncthreads();
+    if (s < n)
+        for (int o = 0; o < kGOut; ++o) out[(size_t)s * kGOut + o] = sR[q * 65 + o];
+}
+
+// mean of the positive, finite record weights: per-block partial sums
+__global__ void k_rec_weight_sum(const float *rec, uint32_t n, float *part_sum, float *part_cnt) {
+    __shared__ float red[33];
+    float s = 0.f, c = 0.f;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const float w = rec[(size_t)i * kRecRaw + 14];
+        if (w > 0.f && w < 3.0e38f) { s += w; c += 1.f; }
+    }
+    s = block_sum(s, red);
+    c = block_sum(c, red);
+    if (threadIdx.x == 0) { part_sum[blockIdx.x] = s; part_cnt[blockIdx.x] = c; }
+}
+
+// ---------------------------------------------------------------- host ----
+struct Pcg {
+    uint64_t s, inc;
+    explicit Pcg(uint64_t seed) : s(0), inc(0xda3e39cb94b95bdbULL) { next(); s += seed; next(); }
+    uint32_t next() {
+        uint64_t o = s;
+        s = o * 6364136223846793005ULL + inc;
+        uint32_t x = (uint32_t)(((o >> 18u) ^ o) >> 27u), r = (uint32_t)(o >> 59u);
+        return (x >> r) | (x << ((-r) & 31));
+    }
+    float uniform() { return (float)(next() >> 8) * (1.f / 16777216.f); }
+};
+
+// He-uniform hidden layers; the head starts small with its lobe means on the
+// 8 cube diagonals, concentration 1 and equal weights.
+void init_params(std::vector<float> &P, uint64_t seed) {
+    P.assign(kGParams, 0.f);
+    Pcg rng(seed);
+    for (int k = 0; k < kGMats; ++k) {
+        const int fan_in = k == 0 ? kGIn : kGH;
+        const float a = k < 4 ? std::sqrt(6.f / (float)fan_in) : 0.1f * std::sqrt(6.f / (float)fan_in);
+        const int rows = k < 4 ? kGH : kGOut;
+        for (int o = 0; o < rows; ++o)
+            for (int i = 0; i < fan_in; ++i) P[k * kGLayerParams + o * 64 + i] = a * (2.f * rng.uniform() - 1.f);
+    }
+    float *b4 = P.data() + 4 * kGLayerParams + 4096;
+    for (int l = 0; l < kGLobes; ++l) {
+        b4[5 * l + 0] = (l & 1) ? 1.f : -1.f;
+        b4[5 * l + 1] = (l & 2) ? 1.f : -1.f;
+        b4[5 * l + 2] = (l & 4) ? 1.f : -1.f;
+    }
+}
+
+int ensure(fl_ctx_t ctx, float *&p, size_t &cap, size_t bytes) {
+    if (cap >= bytes) return FL_OK;
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    cap = 0;
+    if (hipMalloc(&p, bytes) != hipSuccess) return set_error(ctx, FL_E_OOM, "guide staging buffer");
+    cap = bytes;
+    return FL_OK;
+}
+
+int ensure_partial(fl_guide_s *g, uint32_t blocks) {
+    if (g->partial_blocks >= blocks) return FL_OK;
+    if (g->d_partial) (void)hipFree(g->d_partial);
+    if (g->d_loss) (void)hipFree(g->d_loss);
+    g->d_partial = nullptr;
+    g->d_loss = nullptr;
+    g->partial_blocks = 0;
+    if (hipMalloc(&g->d_partial, (size_t)blocks * kGParams * 4) != hipSuccess ||
+        hipMalloc(&g->d_loss, (size_t)blocks * 4) != hipSuccess)
+        return set_error(g->ctx, FL_E_OOM, "guide gradient buffers");
+    g->partial_blocks = blocks;
+    return FL_OK;
+}
+
+// summed gradient of n samples into g->d_grad, loss sum partials in d_loss
+template <class Src>
+int run_grad(fl_guide_s *g, const Src &src, uint32_t n, hipStream_t st) {
+    const uint32_t blocks = (n + 63) / 64;
+    int rc = ensure_partial(g, blocks);
+    if (rc != FL_OK) return rc;
+    hipLaunchKernelGGL(k_guide_grad<Src>, dim3(blocks), dim3(64), 0, st, g->d_w, g->d_wt, g->d_params, src,
+                       g->d_partial, g->d_loss);
+    hipLaunchKernelGGL(k_grad_reduce, dim3((kGParams + 255) / 256), dim3(256), 0, st, g->d_partial, blocks, g->d_grad);
+    return FL_OK;
+}
+
+void adam(fl_guide_s *g, uint32_t blocks, uint32_t n, hipStream_t st) {
+    g->steps += 1;
+    hipLaunchKernelGGL(k_adam, dim3(1), dim3(1024), 0, st, g->d_params, g->d_m, g->d_v, g->d_grad, g->d_loss, blocks,
+                       1.f / (float)std::max(1u, n), g->cfg.lr, g->cfg.beta1, g->cfg.beta2, g->cfg.eps, g->steps,
+                       g->d_stat, g->d_w, g->d_wt);
+}
+
+int read_stats(fl_guide_s *g) {
+    float h[3];
+    if (hipMemcpy(h, g->d_stat, sizeof(h), hipMemcpyDeviceToHost) != hipSuccess)
+        return set_error(g->ctx, FL_E_HIP, "guide stats");
+    g->last_loss = h[0];
+    if (h[2] != 0.f) g->diverged = 1;
+    return FL_OK;
+}
+
+} // namespace
+
+namespace flx {
+
+int guide_train_on_records(fl_guide_s *g, const float *d_recs, uint32_t n_recs, uint32_t steps, uint32_t batch,
+                           uint64_t seed, hipStream_t st) {
+    if (!g || n_recs == 0 || steps == 0 || g->diverged) return FL_OK;
+    // normalise the weights by their mean over this set of records
+    const uint32_t wb = 256;
+    int rc = ensure(g->ctx, g->d_scratch_out, g->scratch_out_bytes, 2 * wb * 4);
+    if (rc != FL_OK) return rc;
+    hipLaunchKernelGGL(k_rec_weight_sum, dim3(wb), dim3(256), 0, st, d_recs, n_recs, g->d_scratch_out,
+                       g->d_scratch_out + wb);
+    std::vector<float> h(2 * wb);
+    if (hipMemcpyAsync(h.data(), g->d_scratch_out, 2 * wb * 4, hipMemcpyDeviceToHost, st) != hipSuccess ||
+        hipStreamSynchronize(st) != hipSuccess)
+        return set_error(g->ctx, FL_E_HIP, "guide record weights");
+    double s = 0.0, c = 0.0;
+    for (uint32_t i = 0; i < wb; ++i) { s += h[i]; c += h[wb + i]; }
+    if (c < 1.0 || !(s > 0.0)) return FL_OK;   // nothing to learn from yet
+    const float inv_mean = (float)(c / s);
+    const uint32_t blocks = (batch + 63) / 64;
+    for (uint32_t k = 0; k < steps; ++k) {
+        RawSrc src{d_recs, n_recs, batch, k, (uint32_t)(seed * 0x9e3779b97f4a7c15ULL >> 32) ^ g->steps, inv_mean};
+        rc = run_grad(g, src, batch, st);
+        if (rc != FL_OK) return rc;
+        adam(g, blocks, batch, st);
+    }
+    if (hipGetLastError() != hipSuccess) return set_error(g->ctx, FL_E_HIP, "guide training launch");
+    return read_stats(g);
+}
+
+} // namespace flx
+
+extern "C" {
+
+int fl_guide_create(fl_ctx_t ctx, const fl_guide_config *cfg, fl_guide_t *out) {
+    if (!ctx || !out) return set_error(ctx, FL_E_INVALID, "fl_guide_create: null argument");
+    *out = nullptr;
+    fl_guide_config c{};
+    if (cfg) c = *cfg;
+    if ((c.hidden && c.hidden != (uint32_t)kGH) || (c.layers && c.layers != 4u) ||
+        (c.lobes && c.lobes != (uint32_t)kGLobes) || (c.bands && c.bands != (uint32_t)kGBands))
+        return set_error(ctx, FL_E_UNSUPPORTED, "fl_guide_create: the network is 4 x 64 hidden, 8 lobes, 8 bands");
+    c.hidden = kGH;
+    c.layers = 4;
+    c.lobes = kGLobes;
+    c.bands = kGBands;
+    if (!(c.lr > 0.f)) c.lr = 5e-3f;
+    if (!(c.beta1 > 0.f)) c.beta1 = 0.9f;
+    if (!(c.beta2 > 0.f)) c.beta2 = 0.99f;
+    if (!(c.eps > 0.f)) c.eps = 1e-8f;
+    if (!c.batch) c.batch = 16384;
+    fl_guide_s *g = new (std::nothrow) fl_guide_s();
+    if (!g) return set_error(ctx, FL_E_OOM, "fl_guide_create");
+    g->ctx = ctx;
+    g->cfg = c;
+    (void)hipSetDevice(ctx->device);
+    bool ok = hipMalloc(&g->d_params, kGParams * 4) == hipSuccess && hipMalloc(&g->d_m, kGParams * 4) == hipSuccess &&
+              hipMalloc(&g->d_v, kGParams * 4) == hipSuccess && hipMalloc(&g->d_grad, kGParams * 4) == hipSuccess &&
+              hipMalloc(&g->d_w, kGMats * 4096 * 2) == hipSuccess &&
+              hipMalloc(&g->d_wt, kGMats * 4096 * 2) == hipSuccess && hipMalloc(&g->d_stat, 16) == hipSuccess;
+    if (!ok) {
+        fl_guide_destroy(g);
+        return set_error(ctx, FL_E_OOM, "fl_guide_create: device buffers");
+    }
+    std::vector<float> P;
+    init_params(P, c.seed ? c.seed : 1u);
+    std::vector<float> z(kGParams, 0.f);
+    ok = hipMemcpy(g->d_params, P.data(), kGParams * 4, hipMemcpyHostToDevice) == hipSuccess &&
+         hipMemcpy(g->d_m, z.data(), kGParams * 4, hipMemcpyHostToDevice) == hipSuccess &&
+         hipMemcpy(g->d_v, z.data(), kGParams * 4, hipMemcpyHostToDevice) == hipSuccess &&
+         hipMemcpy(g->d_stat, z.data(), 16, hipMemcpyHostToDevice) == hipSuccess;
+    if (ok) {
+        hipLaunchKernelGGL(k_guide_publish, dim3((kGMats * 4096 + 255) / 256), dim3(256), 0, ctx->stream, g->d_params,
+                           g->d_w, g->d_wt);
+        ok = hipStreamSynchronize(ctx->stream) == hipSuccess && hipGetLastError() == hipSuccess;
+    }
+    if (!ok) {
+        fl_guide_destroy(g);
+        return set_error(ctx, FL_E_HIP, "fl_guide_create: upload");
+    }
+    *out = g;
+    return FL_OK;
+}
+
+void fl_guide_destroy(fl_guide_t g) {
+    if (!g) return;
+    if (g->ctx) (void)hipStreamSynchronize(g->ctx->stream);
+    for (void *p : {(void *)g->d_params, (void *)g->d_m, (void *)g->d_v, (void *)g->d_grad, (void *)g->d_w,
+                    (void *)g->d_wt, (void *)g->d_partial, (void *)g->d_loss, (void *)g->d_stat,
+                    (void *)g->d_scratch_in, (void *)g->d_scratch_out})
+        if (p) (void)hipFree(p);
+    delete g;
+}
+
+int fl_guide_forward(fl_guide_t g, const float *inputs, float *outputs, uint32_t n) {
+    if (!g || (!inputs && n) || (!outputs && n)) return set_error(g ? g->ctx : nullptr, FL_E_INVALID, "fl_guide_forward");
+    if (n == 0) return FL_OK;
+    fl_ctx_t ctx = g->ctx;
+    int rc = ensure(ctx, g->d_scratch_in, g->scratch_in_bytes, (size_t)n * 64 * 4);
+    if (rc == FL_OK) rc = ensure(ctx, g->d_scratch_out, g->scratch_out_bytes, (size_t)n * kGOut * 4);
+    if (rc != FL_OK) return rc;
+    if (hipMemcpy(g->d_scratch_in, inputs, (size_t)n * 64 * 4, hipMemcpyHostToDevice) != hipSuccess)
+        return set_error(ctx, FL_E_HIP, "fl_guide_forward: upload");
+    hipLaunchKernelGGL(k_guide_fwd, dim3((n + 63) / 64), dim3(64), 0, ctx->stream, g->d_w, g->d_params,
+                       g->d_scratch_in, g->d_scratch_out, n);
+    if (hipMemcpyAsync(outputs, g->d_scratch_out, (size_t)n * kGOut * 4, hipMemcpyDeviceToHost, ctx->stream) !=
+            hipSuccess ||
+        hipStreamSynchronize(ctx->stream) != hipSuccess || hipGetLastError() != hipSuccess)
+        return set_error(ctx, FL_E_HIP, "fl_guide_forward");
+    return FL_OK;
+}
+
+int fl_guide_gradient(fl_guide_t g, const float *records, uint32_t n, float *grad, uint32_t n_params, float *loss) {
+    if (!g || !records || !grad || n == 0) return set_error(g ? g->ctx : nullptr, FL_E_INVALID, "fl_guide_gradient");
+    if (n_params < (uint32_t)kGParams) return set_error(g->ctx, FL_E_INVALID, "fl_guide_gradient: grad too small");
+    fl_ctx_t ctx = g->ctx;
+    int rc = ensure(ctx, g->d_scratch_in, g->scratch_in_bytes, (size_t)n * kRecEnc * 4);
+    if (rc != FL_OK) return rc;
+    if (hipMemcpy(g->d_scratch_in, records, (size_t)n * kRecEnc * 4, hipMemcpyHostToDevice) != hipSuccess)
+        return set_error(ctx, FL_E_HIP, "fl_guide_gradient: upload");
+    rc = run_grad(g, EncSrc{g->d_scratch_in, n}, n, ctx->stream);
+    if (rc != FL_OK) return rc;
+    const uint32_t blocks = (n + 63) / 64;
+    std::vector<float> lp(blocks);
+    if (hipMemcpyAsync(grad, g->d_grad, kGParams * 4, hipMemcpyDeviceToHost, ctx->stream) != hipSuccess ||
+        hipMemcpyAsync(lp.data(), g->d_loss, blocks * 4, hipMemcpyDeviceToHost, ctx->stream) != hipSuccess ||
+        hipStreamSynchronize(ctx->stream) != hipSuccess || hipGetLastError() != hipSuccess)
+        return set_error(ctx, FL_E_HIP, "fl_guide_gradient");
+    if (loss) {
+        double s = 0.0;
+        for (float v : lp) s += v;
+        *loss = (float)s;
+    }
+    return FL_OK;
+}
+
+int fl_guide_train_step(fl_guide_t g, const float *records, uint32_t n, int records_on_device, float *loss) {
+    if (!g || !records || n == 0) return set_error(g ? g->ctx : nullptr, FL_E_INVALID, "fl_guide_train_step");
+    fl_ctx_t ctx = g->ctx;
+    const float *d_rec = records;
+    if (!records_on_device) {
+        int rc = ensure(ctx, g->d_scratch_in, g->scratch_in_bytes, (size_t)n * kRecEnc * 4);
+        if (rc != FL_OK) return rc;
+        if (hipMemcpy(g->d_scratch_in, records, (size_t)n * kRecEnc * 4, hipMemcpyHostToDevice) != hipSuccess)
+            return set_error(ctx, FL_E_HIP, "fl_guide_train_step: upload");
+        d_rec = g->d_scratch_in;
+    }
+    if (g->diverged) return set_error(ctx, FL_E_NUMERIC, "guide diverged; training stopped");
+    int rc = run_grad(g, EncSrc{d_rec, n}, n, ctx->stream);
+    if (rc != FL_OK) return rc;
+    adam(g, (n + 63) / 64, n, ctx->stream);
+    if (hipStreamSynchronize(ctx->stream) != hipSuccess || hipGetLastError() != hipSuccess)
+        return set_error(ctx, FL_E_HIP, "fl_guide_train_step");
+    rc = read_stats(g);
+    if (rc != FL_OK) return rc;
+    if (loss) *loss = g->last_loss;
+    if (g->diverged) return set_error(ctx, FL_E_NUMERIC, "guide diverged (non-finite loss or gradient)");
+    return FL_OK;
+}
+
+int fl_guide_get_params(fl_guide_t g, float *params, uint32_t n_params, uint32_t *n_out) {
+    if (!g) return FL_E_INVALID;
+    if (n_out) *n_out = 3u * kGParams;
+    if (!params) return FL_OK;
+    if (n_params < 3u * kGParams) return set_error(g->ctx, FL_E_INVALID, "fl_guide_get_params: buffer too small");
+    if (hipMemcpy(params, g->d_params, kGParams * 4, hipMemcpyDeviceToHost) != hipSuccess ||
+        hipMemcpy(params + kGParams, g->d_m, kGParams * 4, hipMemcpyDeviceToHost) != hipSuccess ||
+        hipMemcpy(params + 2 * kGParams, g->d_v, kGParams * 4, hipMemcpyDeviceToHost) != hipSuccess)
+        return set_error(g->ctx, FL_E_HIP, "fl_guide_get_params");
+    return FL_OK;
+}
+
+int fl_guide_set_params(fl_guide_t g, const float *params, uint32_t n_params) {
+    if (!g || !params || (n_params != (uint32_t)kGParams && n_params != 3u * kGParams))
+        return set_error(g ? g->ctx : nullptr, FL_E_INVALID, "fl_guide_set_params: need 20800 or 62400 floats");
+    fl_ctx_t ctx = g->ctx;
+    if (hipMemcpy(g->d_params, params, kGParams * 4, hipMemcpyHostToDevice) != hipSuccess)
+        return set_error(ctx, FL_E_HIP, "fl_guide_set_params");
+    if (n_params == 3u * kGParams &&
+        (hipMemcpy(g->d_m, params + kGParams, kGParams * 4, hipMemcpyHostToDevice) != hipSuccess ||
+         hipMemcpy(g->d_v, params + 2 * kGParams, kGParams * 4, hipMemcpyHostToDevice) != hipSuccess))
+        return set_error(ctx, FL_E_HIP, "fl_guide_set_params");
+    const float zero[4] = {0.f, 0.f, 0.f, 0.f};
+    if (hipMemcpy(g->d_stat, zero, sizeof(zero), hipMemcpyHostToDevice) != hipSuccess)
+        return set_error(ctx, FL_E_HIP, "fl_guide_set_params");
+    hipLaunchKernelGGL(k_guide_publish, dim3((kGMats * 4096 + 255) / 256), dim3(256), 0, ctx->stream, g->d_params,
+                       g->d_w, g->d_wt);
+    if (hipStreamSynchronize(ctx->stream) != hipSuccess || hipGetLastError() != hipSuccess)
+        return set_error(ctx, FL_E_HIP, "fl_guide_set_params: publish");
+    g->diverged = 0;
+    return FL_OK;
+}
+
+int fl_guide_stats(fl_guide_t g, uint32_t *steps, float *last_loss, int *diverged) {
+    if (!g) return FL_E_INVALID;
+    if (steps) *steps = g->steps;
+    if (last_loss) *last_loss = g->last_loss;
+    if (diverged) *diverged = g->diverged;
+    return FL_OK;
+}
+
+} // extern "C"

@@ -1,0 +1,162 @@
+// fl_guide_mlp.h -- matrix-core building blocks of the guiding network on
+// gfx950: one wave owns 64 samples; every product is 64 x 64 x 64 and runs
+// as 4 x 4 output tiles of v_mfma_f32_16x16x32_bf16 over 2 K steps.
+//
+// Operand layout of v_mfma_f32_16x16x32_bf16 (D = A B, A 16x32, B 32x16):
+//   lane l supplies A[l % 16][8 (l / 16) + j] and B[8 (l / 16) + j][l % 16],
+//   j = 0..7 (one 16-byte bf16x8 each), and receives D[4 (l / 16) + r][l % 16]
+//   in result register r = 0..3.
+// Every product here is written as Out^T = P Q^T with P stored [m][k] and Q
+// stored [n][k] (both k-contiguous), so each fragment is one 16-byte load and
+// each lane's 4 results are 4 consecutive m for one n.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "fl_guide.h"
+
+namespace fl {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+constexpr int kGRow = 72;   // bf16 row stride of 64-wide tiles in LDS (144 B: spreads rows over banks)
+
+__device__ __forceinline__ __bf16 to_bf16(float x) { return (__bf16)x; }   // round to nearest even
+
+// acc[mt][nt] (+)= P[m][k] * Q[n][k] summed over k = 0..63, for m, n in 0..63.
+// P row stride ps, Q row stride qs (in bf16 elements); both may be LDS or global.
+__device__ __forceinline__ void mm64(const __bf16 *P, int ps, const __bf16 *Q, int qs, f32x4 acc[4][4]) {
+    const int l = threadIdx.x & 63;
+    const int r16 = l & 15, kq = (l >> 4) * 8;
+#pragma unroll
+    for (int ks = 0; ks < 2; ++ks) {
+        bf16x8 a[4], b[4];
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+            a[t] = *reinterpret_cast<const bf16x8 *>(P + (t * 16 + r16) * ps + ks * 32 + kq);
+            b[t] = *reinterpret_cast<const bf16x8 *>(Q + (t * 16 + r16) * qs + ks * 32 + kq);
+        }
+#pragma unroll
+        for (int mt = 0; mt < 4; ++mt)
+#pragma unroll
+            for (int nt = 0; nt < 4; ++nt)
+                acc[mt][nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[mt], b[nt], acc[mt][nt], 0, 0, 0);
+    }
+}
+
+__device__ __forceinline__ void acc_zero(f32x4 acc[4][4]) {
+#pragma unroll
+    for (int mt = 0; mt < 4; ++mt)
+#pragma unroll
+        for (int nt = 0; nt < 4; ++nt) acc[mt][nt] = f32x4{0.f, 0.f, 0.f, 0.f};
+}
+
+// Row m of acc element (mt, nt, r) for this lane, and its column n.
+__device__ __forceinline__ int acc_m(int mt, int r) { return mt * 16 + ((threadIdx.x & 63) >> 4) * 4 + r; }
+__device__ __forceinline__ int acc_n(int nt) { return nt * 16 + (threadIdx.x & 15); }
+
+// Input encoding of one sample (53 used values, zero-padded to 64):
+//   [16 a + 2 k] = sin(2^k pi x_a), [16 a + 2 k + 1] = cos(2^k pi x_a)  (a = 0..2, k = 0..7)
+//   [48..50] = normal, [51] = wavelength in [0,1], [52] = roughness, rest 0.
+__device__ __forceinline__ void guide_encode(const float pos01[3], v3 n, float lam01, float rough, float *x) {
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        float f = pos01[a];
+#pragma unroll
+        for (int k = 0; k < kGBands; ++k) {
+            x[16 * a + 2 * k] = sinpif(f);
+            x[16 * a + 2 * k + 1] = cospif(f);
+            f = f * 2.f;
+        }
+    }
+    x[48] = n.x;
+    x[49] = n.y;
+    x[50] = n.z;
+    x[51] = lam01;
+    x[52] = rough;
+#pragma unroll
+    for (int i = 53; i < 64; ++i) x[i] = 0.f;
+}
+
+// One hidden layer for the wave's 64 samples: Hout[q][o] = relu(sum_i W[o][i] Hin[q][i] + b[o]),
+// W global bf16 [64][64], Hin/Hout LDS [64][kGRow]. With HoutT non-null the
+// transposed copy HoutT[o][q] is written too (for the weight gradient).
+__device__ __forceinline__ void layer_fwd(const __bf16 *W, const float *b, const __bf16 *Hin, __bf16 *Hout,
+                                          __bf16 *HoutT, bool relu) {
+    f32x4 acc[4][4];
+    acc_zero(acc);
+    mm64(W, kGH, Hin, kGRow, acc);
+#pragma unroll
+    for (int mt = 0; mt < 4; ++mt)
+#pragma unroll
+        for (int nt = 0; nt < 4; ++nt) {
+            const int q = acc_n(nt);
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int o = acc_m(mt, r);
+                float v = acc[mt][nt][r] + b[o];
+                if (relu) v = fmaxf(v, 0.f);
+                const __bf16 h = to_bf16(v);
+                Hout[q * kGRow + o] = h;
+                if (HoutT) HoutT[o * kGRow + q] = h;
+            }
+        }
+}
+
+// Head: raw[q][o] = sum_i W[o][i] H[q][i] + b[o] in fp32, raw row stride rs.
+__device__ __forceinline__ void head_fwd(const __bf16 *W, const float *b, const __bf16 *H, float *raw, int rs) {
+    f32x4 acc[4][4];
+    acc_zero(acc);
+    mm64(W, kGH, H, kGRow, acc);
+#pragma unroll
+    for (int mt = 0; mt < 4; ++mt)
+#pragma unroll
+        for (int nt = 0; nt < 4; ++nt) {
+            const int q = acc_n(nt);
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int o = acc_m(mt, r);
+                raw[q * rs + o] = acc[mt][nt][r] + b[o];
+            }
+        }
+}
+
+// d(-c log p)/d raw for one sample: p = sum_i w_i v_i(dir) over the decoded
+// lobes. Returns the loss; g receives 40 values.
+FL_HD float guide_head_grad(const float *raw, v3 dir, float c, float *g) {
+    VmfLobe L[kGLobes];
+    guide_decode(raw, L);
+    float v[kGLobes], p = 0.f;
+    for (int i = 0; i < kGLobes; ++i) {
+        v[i] = vmf_pdf(L[i], dir);
+        p += L[i].w * v[i];
+    }
+    p = fmaxf(p, 1e-30f);
+    for (int i = 0; i < kGLobes; ++i) {
+        const float *r = raw + 5 * i;
+        float *gi = g + 5 * i;
+        const float resp = L[i].w * v[i] / p;
+        // weight logit (softmax)
+        gi[4] = -c * (resp - L[i].w);
+        // concentration: kappa = exp(clamp(raw)), dlog v/dkappa = 1/kappa - 2/expm1(2 kappa) + (mu.w - 1)
+        const float k = L[i].kappa;
+        const float md = dot(L[i].mu, dir);
+        const bool inside = r[3] > kGKappaLo && r[3] < kGKappaHi;
+        const float dk = 1.f / k - 2.f / expm1f(2.f * k) + (md - 1.f);
+        gi[3] = inside ? -c * resp * dk * k : 0.f;
+        // mean direction mu = m / |m|: dlog v/dm = kappa (w - (mu.w) mu) / |m|
+        const float len = sqrtf(r[0] * r[0] + r[1] * r[1] + r[2] * r[2]);
+        if (len > 1e-6f) {
+            const float s = -c * resp * k / len;
+            gi[0] = s * (dir.x - md * L[i].mu.x);
+            gi[1] = s * (dir.y - md * L[i].mu.y);
+            gi[2] = s * (dir.z - md * L[i].mu.z);
+        } else {
+            gi[0] = gi[1] = gi[2] = 0.f;
+        }
+    }
+    return -c * logf(p);
+}
+
+} // namespace fl

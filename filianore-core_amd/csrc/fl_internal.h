@@ -66,6 +66,31 @@ struct fl_scene_s {
     std::vector<DevLight> h_lights;
 };
 
+// Learned proposal: fp32 master parameters + Adam moments, and the bf16
+// copies (row-major W[o][i] for the forward, transposed WT[i][o] for the data
+// gradient) that the matrix-core kernels read. Parameter order per matrix k:
+// W_k (64 x 64, out-major) then b_k (64).
+struct fl_guide_s {
+    fl_ctx_t ctx = nullptr;
+    fl_guide_config cfg{};
+    float *d_params = nullptr;       // kGParams fp32 master
+    float *d_m = nullptr, *d_v = nullptr;
+    __bf16 *d_w = nullptr;           // kGMats x 64 x 64
+    __bf16 *d_wt = nullptr;          // kGMats x 64 x 64, transposed
+    float *d_grad = nullptr;         // kGParams (sum over the batch)
+    float *d_partial = nullptr;      // per-block partial gradients
+    uint32_t partial_blocks = 0;
+    float *d_loss = nullptr;         // per-block partial losses
+    float *d_stat = nullptr;         // [0] last loss, [1] gradient norm, [2] diverged flag
+    float *d_scratch_in = nullptr;   // staging of host records / inputs
+    size_t scratch_in_bytes = 0;
+    float *d_scratch_out = nullptr;
+    size_t scratch_out_bytes = 0;
+    uint32_t steps = 0;
+    float last_loss = 0.f;
+    int diverged = 0;
+};
+
 namespace flx {
 
 int set_error(fl_ctx_t ctx, int code, const std::string &msg);
@@ -92,5 +117,11 @@ int trace_rays(fl_ctx_t ctx, fl_scene_s *s, const fl_ray *d_rays, fl_hit *d_hits
 
 // BSDF queries (fl_render.hip)
 int bsdf_queries(fl_ctx_t ctx, const fl_bsdf_query *d_q, fl_bsdf_result *d_r, uint32_t n);
+
+// Online training of the guide on render records (device, 16 floats each:
+// position in the unit box 3, normal 3, wavelength 1, roughness 1, direction
+// 3, throughput 1, radiance before 1, path id 1, weight 1, unused 1).
+int guide_train_on_records(fl_guide_s *g, const float *d_recs, uint32_t n_recs, uint32_t steps, uint32_t batch,
+                           uint64_t seed, hipStream_t st);
 
 } // namespace flx

@@ -316,6 +316,9 @@ class Context:
     def scene(self, gen):
         return Scene(self, gen)
 
+    def guide(self, seed=1, lr=0.0, batch=0, scene_min=(0, 0, 0), scene_max=(1, 1, 1)):
+        return Guide(self, seed=seed, lr=lr, batch=batch, scene_min=scene_min, scene_max=scene_max)
+
     def probe_copy(self, nbytes=1 << 30, iters=5):
         """Device-to-device streaming copy; returns (read + write) GB/s."""
         g = C.c_double(0.0)
@@ -394,3 +397,69 @@ class Scene:
         if want_film:
             return s, q, st
         return None, None, st
+
+
+GUIDE_PARAMS = 5 * (64 * 64 + 64)
+
+
+class Guide:
+    """The learned vMF-mixture proposal (fl_guide_*)."""
+
+    def __init__(self, ctx, seed=1, lr=0.0, batch=0, scene_min=(0, 0, 0), scene_max=(1, 1, 1)):
+        self.ctx = ctx
+        cfg = GuideConfig()
+        cfg.seed, cfg.lr, cfg.batch = seed, lr, batch
+        cfg.scene_min = f3(*scene_min)
+        cfg.scene_max = f3(*scene_max)
+        h = C.c_void_p()
+        _check(ctx.L.fl_guide_create(ctx.h, C.byref(cfg), C.byref(h)), ctx.h, "fl_guide_create")
+        self.h = h
+
+    def close(self):
+        if self.h:
+            self.ctx.L.fl_guide_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def forward(self, x):
+        x = np.ascontiguousarray(x, np.float32)
+        out = np.zeros((x.shape[0], 40), np.float32)
+        _check(self.ctx.L.fl_guide_forward(self.h, x.ctypes.data, out.ctypes.data, x.shape[0]), self.ctx.h,
+               "fl_guide_forward")
+        return out
+
+    def gradient(self, records):
+        r = np.ascontiguousarray(records, np.float32)
+        g = np.zeros(GUIDE_PARAMS, np.float32)
+        loss = C.c_float(0)
+        _check(self.ctx.L.fl_guide_gradient(self.h, r.ctypes.data, r.shape[0], g.ctypes.data, GUIDE_PARAMS,
+                                            C.byref(loss)), self.ctx.h, "fl_guide_gradient")
+        return loss.value, g
+
+    def train_step(self, records):
+        r = np.ascontiguousarray(records, np.float32)
+        loss = C.c_float(0)
+        rc = self.ctx.L.fl_guide_train_step(self.h, r.ctypes.data, r.shape[0], 0, C.byref(loss))
+        return rc, loss.value
+
+    def params(self):
+        P = np.zeros(3 * GUIDE_PARAMS, np.float32)
+        n = C.c_uint32(0)
+        _check(self.ctx.L.fl_guide_get_params(self.h, P.ctypes.data, P.size, C.byref(n)), self.ctx.h,
+               "fl_guide_get_params")
+        return P[:GUIDE_PARAMS], P[GUIDE_PARAMS:2 * GUIDE_PARAMS], P[2 * GUIDE_PARAMS:]
+
+    def set_params(self, P):
+        P = np.ascontiguousarray(P, np.float32)
+        _check(self.ctx.L.fl_guide_set_params(self.h, P.ctypes.data, P.size), self.ctx.h, "fl_guide_set_params")
+
+    def stats(self):
+        steps, loss, div = C.c_uint32(0), C.c_float(0), C.c_int(0)
+        _check(self.ctx.L.fl_guide_stats(self.h, C.byref(steps), C.byref(loss), C.byref(div)), self.ctx.h,
+               "fl_guide_stats")
+        return steps.value, loss.value, div.value

@@ -71,6 +71,17 @@ typedef struct {
 int or_render(or_scene_t s, const fl_camera *cam, const or_render_params *p, float *film_sum, float *film_sq,
               or_render_stats *st);
 
+/* learned proposal (oracle_guide.cpp): parameters are 5 x (64 x 64 + 64)
+ * doubles in the GPU's order; records are n x 68 floats (encoded input 64,
+ * direction 3, weight 1); bf16 = 1 rounds where the GPU rounds to bf16 */
+double or_bf16_round(double x);
+void or_guide_encode(const float pos01[3], const float n[3], float lam01, float rough, float *x);
+void or_guide_forward(const double *P, const float *x, uint32_t n, int bf16, double *raw);
+double or_guide_gradient(const double *P, const float *rec, uint32_t n, int bf16, double *grad);
+double or_vmf_mixture_pdf(const double *raw, const double *dir);
+double or_vmf_norm(double kappa);
+double or_vmf_sample_cos(double kappa, double u);
+
 #ifdef __cplusplus
 }
 #endif

@@ -53,6 +53,19 @@ def lib():
         L.or_intersect2.argtypes = [vp, vp, vp, vp, C.c_uint32, C.c_int]
         L.or_bsdf_batch.argtypes = [C.POINTER(flgpu.BsdfQuery), C.POINTER(flgpu.BsdfResult), C.c_uint32, C.c_int]
         L.or_render.argtypes = [vp, C.POINTER(flgpu.Camera), C.POINTER(RenderParams), fp, fp, C.POINTER(RenderStats)]
+        dp = C.POINTER(C.c_double)
+        L.or_bf16_round.argtypes = [C.c_double]
+        L.or_bf16_round.restype = C.c_double
+        L.or_guide_encode.argtypes = [fp, fp, C.c_float, C.c_float, fp]
+        L.or_guide_forward.argtypes = [dp, fp, C.c_uint32, C.c_int, dp]
+        L.or_guide_gradient.argtypes = [dp, fp, C.c_uint32, C.c_int, dp]
+        L.or_guide_gradient.restype = C.c_double
+        L.or_vmf_mixture_pdf.argtypes = [dp, dp]
+        L.or_vmf_mixture_pdf.restype = C.c_double
+        L.or_vmf_norm.argtypes = [C.c_double]
+        L.or_vmf_norm.restype = C.c_double
+        L.or_vmf_sample_cos.argtypes = [C.c_double, C.c_double]
+        L.or_vmf_sample_cos.restype = C.c_double
         _lib = L
     return _lib
 
@@ -177,3 +190,54 @@ class Scene:
         if rc != 0:
             raise RuntimeError("or_render failed %d" % rc)
         return s, q, st
+
+
+# ---- learned proposal (oracle_guide.cpp) ----------------------------------
+GUIDE_PARAMS = 5 * (64 * 64 + 64)
+
+
+def _d(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def bf16_round(x):
+    return lib().or_bf16_round(float(x))
+
+
+def guide_encode(pos01, normal, lam01, rough):
+    x = np.zeros(64, np.float32)
+    lib().or_guide_encode(_f(np.asarray(pos01, np.float32)), _f(np.asarray(normal, np.float32)), lam01, rough, _f(x))
+    return x
+
+
+def guide_forward(params, x, bf16=True):
+    P = np.ascontiguousarray(params, np.float64)
+    x = np.ascontiguousarray(x, np.float32)
+    n = x.shape[0]
+    raw = np.zeros((n, 40), np.float64)
+    lib().or_guide_forward(_d(P), _f(x), n, int(bf16), _d(raw))
+    return raw
+
+
+def guide_gradient(params, records, bf16=True, want_grad=True):
+    """records: n x 68 float32 (encoded input, direction, weight); returns
+    (loss sum, gradient sum or None)."""
+    P = np.ascontiguousarray(params, np.float64)
+    r = np.ascontiguousarray(records, np.float32)
+    g = np.zeros(GUIDE_PARAMS, np.float64) if want_grad else None
+    loss = lib().or_guide_gradient(_d(P), _f(r), r.shape[0], int(bf16), _d(g) if want_grad else None)
+    return loss, g
+
+
+def vmf_mixture_pdf(raw40, dirs):
+    raw = np.ascontiguousarray(raw40, np.float64)
+    dirs = np.ascontiguousarray(np.atleast_2d(dirs), np.float64)
+    return np.array([lib().or_vmf_mixture_pdf(_d(raw), _d(d)) for d in dirs])
+
+
+def vmf_norm(kappa):
+    return lib().or_vmf_norm(float(kappa))
+
+
+def vmf_sample_cos(kappa, u):
+    return lib().or_vmf_sample_cos(float(kappa), float(u))
