@@ -3,9 +3,10 @@
 //
 // Loss of one sample (direction w drawn while rendering, weight c):
 //     l = -c log p(w),  p = sum_i w_i vMF_i(w)
-// with c = (radiance the path gathered beyond the vertex) / (throughput at
-// the vertex) = f cos L_i / p_sample: the sample estimate of the KL
-// divergence from the product f cos L_i to the mixture, up to a constant.
+// with c = radiance the path gathered beyond the vertex = beta f cos L_i /
+// p_sample: the sample estimate of the KL divergence from the product
+// f cos L_i to the mixture at the vertex, weighted by the vertex's share of
+// the pixel (its throughput beta).
 //
 // Gradients are summed per 64-sample block into a partial buffer and reduced
 // in a fixed order, so a training step is deterministic.
@@ -156,27 +157,54 @@ __global__ void __launch_bounds__(64) k_guide_grad(const __bf16 *W, const __bf16
     if (q == 0) loss_part[blockIdx.x] = loss;
 }
 
-// grad[p] = sum over blocks of partial[b][p] (fixed order)
-__global__ void k_grad_reduce(const float *partial, uint32_t nblocks, float *grad) {
-    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= (uint32_t)kGParams) return;
+// grad[p] = sum over blocks of partial[b][p] in a fixed order. A block owns
+// 64 parameters; its 4 waves each sum a quarter of the partials (lane =
+// parameter, coalesced rows), then combine in LDS. sq[blk] receives the sum
+// of squares of the block's 64 sums (for the gradient norm).
+constexpr int kRedParams = 64;
+constexpr int kRedBlocks = (kGParams + kRedParams - 1) / kRedParams;   // 325
+
+__global__ void __launch_bounds__(256) k_grad_reduce(const float *partial, uint32_t nblocks, float *grad, float *sq) {
+    __shared__ float red[4][kRedParams];
+    const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+    const uint32_t p = blockIdx.x * kRedParams + l;
     float s = 0.f;
-    for (uint32_t b = 0; b < nblocks; ++b) s += partial[(size_t)b * kGParams + p];
-    grad[p] = s;
+    if (p < (uint32_t)kGParams) {
+        const uint32_t per = (nblocks + 3) / 4, b0 = w * per, b1 = min(nblocks, b0 + per);
+        float s0 = 0.f, s1 = 0.f, s2 = 0.f, s3 = 0.f;
+        uint32_t b = b0;
+        for (; b + 4 <= b1; b += 4) {
+            s0 += partial[(size_t)b * kGParams + p];
+            s1 += partial[(size_t)(b + 1) * kGParams + p];
+            s2 += partial[(size_t)(b + 2) * kGParams + p];
+            s3 += partial[(size_t)(b + 3) * kGParams + p];
+        }
+        for (; b < b1; ++b) s0 += partial[(size_t)b * kGParams + p];
+        s = (s0 + s1) + (s2 + s3);
+    }
+    red[w][l] = s;
+    __syncthreads();
+    if (w == 0) {
+        const float t = (red[0][l] + red[1][l]) + (red[2][l] + red[3][l]);
+        float q = 0.f;
+        if (p < (uint32_t)kGParams) {
+            grad[p] = t;
+            q = t * t;
+        }
+        for (int off = 32; off > 0; off >>= 1) q += __shfl_xor(q, off);
+        if (l == 0) sq[blockIdx.x] = q;
+    }
 }
 
-__device__ __forceinline__ float block_sum(float v, float *red) {
-    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+// sum of n floats by one 256-thread block, fixed order
+__device__ __forceinline__ float block_sum_array(const float *v, uint32_t n, float *red) {
+    float s = 0.f;
+    for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) s += v[i];
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off);
     const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
-    if (l == 0) red[w] = v;
+    if (l == 0) red[w] = s;
     __syncthreads();
-    float t = 0.f;
-    if (threadIdx.x == 0) {
-        for (int i = 0; i < (int)(blockDim.x >> 6); ++i) t += red[i];
-        red[32] = t;
-    }
-    __syncthreads();
-    t = red[32];
+    const float t = (red[0] + red[1]) + (red[2] + red[3]);
     __syncthreads();
     return t;
 }
@@ -194,38 +222,42 @@ __global__ void k_guide_publish(const float *P, __bf16 *W, __bf16 *WT) {
     if (e < (uint32_t)(kGMats * 4096)) publish_one(P, W, WT, e);
 }
 
-// One Adam step from the summed gradient (one block). Skips the update and
-// raises the diverged flag when the loss or the gradient is not finite.
-__global__ void __launch_bounds__(1024) k_adam(float *P, float *M, float *V, const float *grad, const float *loss_part,
-                                               uint32_t nblocks, float inv_n, float lr, float b1, float b2, float eps,
-                                               uint32_t t, float *stat, __bf16 *W, __bf16 *WT) {
-    __shared__ float red[33];
-    float l = 0.f;
-    for (uint32_t b = threadIdx.x; b < nblocks; b += blockDim.x) l += loss_part[b];
-    const float loss = block_sum(l, red) * inv_n;
-    float s = 0.f;
-    for (int p = threadIdx.x; p < kGParams; p += blockDim.x) {
-        const float g = grad[p] * inv_n;
-        s += g * g;
-    }
-    const float norm = sqrtf(block_sum(s, red));
+// One Adam step from the summed gradient: every block recomputes the loss and
+// the gradient norm from the partial sums (the same fixed order in every
+// block), then updates its 256 parameters and their bf16 copies. The update
+// is skipped, with the diverged flag raised, when either is not finite; a
+// parameter that is not finite after its update raises the flag too.
+__global__ void __launch_bounds__(256) k_adam(float *P, float *M, float *V, const float *grad, const float *sq,
+                                              const float *loss_part, uint32_t nblocks, float inv_n, float lr,
+                                              float b1, float b2, float eps, uint32_t t, float *stat, __bf16 *W,
+                                              __bf16 *WT) {
+    __shared__ float red[4];
+    const float loss = block_sum_array(loss_part, nblocks, red) * inv_n;
+    const float norm = sqrtf(block_sum_array(sq, kRedBlocks, red)) * inv_n;
     if (!isfinite(loss) || !isfinite(norm)) {
-        if (threadIdx.x == 0) { stat[0] = loss; stat[1] = norm; stat[2] = 1.f; }
+        if (blockIdx.x == 0 && threadIdx.x == 0) { stat[0] = loss; stat[1] = norm; stat[2] = 1.f; }
         return;
     }
     const float scale = norm > kClipNorm ? kClipNorm / norm : 1.f;
     const float c1 = 1.f - powf(b1, (float)t), c2 = 1.f - powf(b2, (float)t);
-    for (int p = threadIdx.x; p < kGParams; p += blockDim.x) {
+    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p < (uint32_t)kGParams) {
         const float g = grad[p] * inv_n * scale;
         const float m = b1 * M[p] + (1.f - b1) * g;
         const float v = b2 * V[p] + (1.f - b2) * g * g;
         M[p] = m;
         V[p] = v;
-        P[p] = P[p] - lr * (m / c1) / (sqrtf(v / c2) + eps);
+        const float w = P[p] - lr * (m / c1) / (sqrtf(v / c2) + eps);
+        P[p] = w;
+        if (!isfinite(w)) stat[2] = 1.f;   // a non-finite parameter (e.g. set from outside) stops training
+        const uint32_t k = p / (uint32_t)kGLayerParams, r = p % (uint32_t)kGLayerParams;
+        if (r < 4096u) {
+            const __bf16 h = to_bf16(w);
+            W[k * 4096u + r] = h;
+            WT[k * 4096u + (r % 64u) * 64u + r / 64u] = h;
+        }
     }
-    __syncthreads();
-    for (uint32_t e = threadIdx.x; e < (uint32_t)(kGMats * 4096); e += blockDim.x) publish_one(P, W, WT, e);
-    if (threadIdx.x == 0) { stat[0] = loss; stat[1] = norm; }
+    if (blockIdx.x == 0 && threadIdx.x == 0) { stat[0] = loss; stat[1] = norm; }
 }
 
 // Forward of encoded inputs (n x 64) to raw head outputs (n x 40).
@@ -245,6 +277,22 @@ __global__ void __launch_bounds__(64) k_guide_fwd(const __bf16 *W, const float *
     __syncthreads();
     if (s < n)
         for (int o = 0; o < kGOut; ++o) out[(size_t)s * kGOut + o] = sR[q * 65 + o];
+}
+
+__device__ __forceinline__ float block_sum(float v, float *red) {
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+    const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+    if (l == 0) red[w] = v;
+    __syncthreads();
+    float t = 0.f;
+    if (threadIdx.x == 0) {
+        for (int i = 0; i < (int)(blockDim.x >> 6); ++i) t += red[i];
+        red[32] = t;
+    }
+    __syncthreads();
+    t = red[32];
+    __syncthreads();
+    return t;
 }
 
 // mean of the positive, finite record weights: per-block partial sums
@@ -325,20 +373,21 @@ int run_grad(fl_guide_s *g, const Src &src, uint32_t n, hipStream_t st) {
     if (rc != FL_OK) return rc;
     hipLaunchKernelGGL(k_guide_grad<Src>, dim3(blocks), dim3(64), 0, st, g->d_w, g->d_wt, g->d_params, src,
                        g->d_partial, g->d_loss);
-    hipLaunchKernelGGL(k_grad_reduce, dim3((kGParams + 255) / 256), dim3(256), 0, st, g->d_partial, blocks, g->d_grad);
+    hipLaunchKernelGGL(k_grad_reduce, dim3(kRedBlocks), dim3(256), 0, st, g->d_partial, blocks, g->d_grad, g->d_sq);
     return FL_OK;
 }
 
 void adam(fl_guide_s *g, uint32_t blocks, uint32_t n, hipStream_t st) {
     g->steps += 1;
-    hipLaunchKernelGGL(k_adam, dim3(1), dim3(1024), 0, st, g->d_params, g->d_m, g->d_v, g->d_grad, g->d_loss, blocks,
-                       1.f / (float)std::max(1u, n), g->cfg.lr, g->cfg.beta1, g->cfg.beta2, g->cfg.eps, g->steps,
-                       g->d_stat, g->d_w, g->d_wt);
+    hipLaunchKernelGGL(k_adam, dim3((kGParams + 255) / 256), dim3(256), 0, st, g->d_params, g->d_m, g->d_v, g->d_grad,
+                       g->d_sq, g->d_loss, blocks, 1.f / (float)std::max(1u, n), g->cfg.lr, g->cfg.beta1,
+                       g->cfg.beta2, g->cfg.eps, g->steps, g->d_stat, g->d_w, g->d_wt);
 }
 
 int read_stats(fl_guide_s *g) {
     float h[3];
-    if (hipMemcpy(h, g->d_stat, sizeof(h), hipMemcpyDeviceToHost) != hipSuccess)
+    if (hipMemcpyAsync(h, g->d_stat, sizeof(h), hipMemcpyDeviceToHost, g->ctx->stream) != hipSuccess ||
+        hipStreamSynchronize(g->ctx->stream) != hipSuccess)
         return set_error(g->ctx, FL_E_HIP, "guide stats");
     g->last_loss = h[0];
     if (h[2] != 0.f) g->diverged = 1;
@@ -374,7 +423,7 @@ int guide_train_on_records(fl_guide_s *g, const float *d_recs, uint32_t n_recs, 
         adam(g, blocks, batch, st);
     }
     if (hipGetLastError() != hipSuccess) return set_error(g->ctx, FL_E_HIP, "guide training launch");
-    return read_stats(g);
+    return read_stats(g);   // synchronises: the caller's timing covers the steps
 }
 
 } // namespace flx
@@ -398,6 +447,7 @@ int fl_guide_create(fl_ctx_t ctx, const fl_guide_config *cfg, fl_guide_t *out) {
     if (!(c.beta2 > 0.f)) c.beta2 = 0.99f;
     if (!(c.eps > 0.f)) c.eps = 1e-8f;
     if (!c.batch) c.batch = 16384;
+    if (!c.steps_per_pass) c.steps_per_pass = 64;
     fl_guide_s *g = new (std::nothrow) fl_guide_s();
     if (!g) return set_error(ctx, FL_E_OOM, "fl_guide_create");
     g->ctx = ctx;
@@ -406,7 +456,8 @@ int fl_guide_create(fl_ctx_t ctx, const fl_guide_config *cfg, fl_guide_t *out) {
     bool ok = hipMalloc(&g->d_params, kGParams * 4) == hipSuccess && hipMalloc(&g->d_m, kGParams * 4) == hipSuccess &&
               hipMalloc(&g->d_v, kGParams * 4) == hipSuccess && hipMalloc(&g->d_grad, kGParams * 4) == hipSuccess &&
               hipMalloc(&g->d_w, kGMats * 4096 * 2) == hipSuccess &&
-              hipMalloc(&g->d_wt, kGMats * 4096 * 2) == hipSuccess && hipMalloc(&g->d_stat, 16) == hipSuccess;
+              hipMalloc(&g->d_wt, kGMats * 4096 * 2) == hipSuccess && hipMalloc(&g->d_stat, 16) == hipSuccess &&
+              hipMalloc(&g->d_sq, kRedBlocks * 4) == hipSuccess;
     if (!ok) {
         fl_guide_destroy(g);
         return set_error(ctx, FL_E_OOM, "fl_guide_create: device buffers");
@@ -434,7 +485,7 @@ int fl_guide_create(fl_ctx_t ctx, const fl_guide_config *cfg, fl_guide_t *out) {
 void fl_guide_destroy(fl_guide_t g) {
     if (!g) return;
     if (g->ctx) (void)hipStreamSynchronize(g->ctx->stream);
-    for (void *p : {(void *)g->d_params, (void *)g->d_m, (void *)g->d_v, (void *)g->d_grad, (void *)g->d_w,
+    for (void *p : {(void *)g->d_params, (void *)g->d_m, (void *)g->d_v, (void *)g->d_grad, (void *)g->d_sq, (void *)g->d_w,
                     (void *)g->d_wt, (void *)g->d_partial, (void *)g->d_loss, (void *)g->d_stat,
                     (void *)g->d_scratch_in, (void *)g->d_scratch_out})
         if (p) (void)hipFree(p);

@@ -78,6 +78,7 @@ struct fl_guide_s {
     __bf16 *d_w = nullptr;           // kGMats x 64 x 64
     __bf16 *d_wt = nullptr;          // kGMats x 64 x 64, transposed
     float *d_grad = nullptr;         // kGParams (sum over the batch)
+    float *d_sq = nullptr;           // per-reduce-block sums of squared gradients
     float *d_partial = nullptr;      // per-block partial gradients
     uint32_t partial_blocks = 0;
     float *d_loss = nullptr;         // per-block partial losses

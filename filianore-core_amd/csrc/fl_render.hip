@@ -19,9 +19,11 @@
 // The MIS BSDF ray of estimate_direct is the next extension ray (same
 // expectation, one traversal fewer per light per vertex).
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstring>
 
+#include "fl_guide_mlp.h"
 #include "fl_wavefront.h"
 
 namespace {
@@ -107,6 +109,7 @@ __global__ void __launch_bounds__(kBlock) k_generate(RenderArgs a) {
         }
         a.pend[sid] = make_float4(0.f, 0.f, 0.f, 0.f);
         a.pend[a.cap + sid] = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (a.st_rec) a.st_rec[sid] = 0u;
         if (active) {
             a.rq_o[0][slot] = make_float4(o.x, o.y, o.z, 0.f);       // tMin = time = 0 (physical.cpp:21)
             a.rq_d[0][slot] = make_float4(d.x, d.y, d.z, kInf);
@@ -195,6 +198,129 @@ const TraceVariant kTraceVariants[] = {
     {6, 8, launch_trace_t<6, 8>, trace_fn<6, 8>},    {16, 1, launch_trace_t<16, 1>, trace_fn<16, 1>},
 };
 
+// ---------------------------------------------------------------- guide ----
+// Decoded lobes of one queue slot: 8 x (mu xyz, kappa, weight), world space.
+__device__ __forceinline__ float guide_pdf_at(const float *lb, v3 w) {
+    float p = 0.f;
+    for (int i = 0; i < kGLobes; ++i) {
+        const float *l = lb + 5 * i;
+        const float k = l[3];
+        p += l[4] * vmf_norm(k) * expf(k * (l[0] * w.x + l[1] * w.y + l[2] * w.z - 1.f));
+    }
+    return p;
+}
+
+__device__ __forceinline__ v3 guide_sample_at(const float *lb, float u_lobe, float u1, float u2) {
+    int sel = kGLobes - 1;
+    float c = 0.f;
+    for (int i = 0; i < kGLobes - 1; ++i) {
+        c += lb[5 * i + 4];
+        if (u_lobe < c) { sel = i; break; }
+    }
+    const float *l = lb + 5 * sel;
+    VmfLobe L;
+    L.mu = v3(l[0], l[1], l[2]);
+    L.kappa = l[3];
+    L.w = l[4];
+    return vmf_sample(L, u1, u2);
+}
+
+// Materials the guide serves: the non-specular ones.
+__device__ __forceinline__ bool guidable(int kind) { return kind == MAT_LAMBERT || kind == MAT_CONDUCTOR; }
+
+// Network inputs of a shading point: position in the unit scene box, the
+// geometric normal turned toward the incoming ray, hero wavelength in the
+// spectral range, roughness (GGX alpha; 1 for Lambert).
+__device__ __forceinline__ void guide_features(const RenderArgs &a, v3 p, v3 ng, v3 d, float lam0, const DevMaterial &m,
+                                               float pos01[3], v3 &n_in, float &lam01, float &rough) {
+    const float pp[3] = {p.x, p.y, p.z};
+    for (int k = 0; k < 3; ++k) pos01[k] = fminf(fmaxf((pp[k] - a.box_lo[k]) * a.box_inv[k], 0.f), 1.f);
+    n_in = dot(ng, d) < 0.f ? ng : -ng;
+    lam01 = (lam0 - a.grid.lo) / (a.grid.hi - a.grid.lo);
+    rough = m.kind == MAT_CONDUCTOR ? m.ax : 1.f;
+}
+
+// Network evaluation for every guidable hit of extension queue `b & 1`: one
+// wave per block, 64 slots per chunk, lobes written per queue slot.
+__global__ void __launch_bounds__(64) k_guide_infer(RenderArgs a, int b) {
+    __shared__ __bf16 sH[2][64 * kGRow];
+    __shared__ float sR[64 * 65];
+    const int q = threadIdx.x;
+    const uint32_t n = a.counts[C_EXT + (b & 1)];
+    const uint32_t nchunks = (n + 63) / 64;
+    for (uint32_t ch = blockIdx.x; ch < nchunks; ch += gridDim.x) {
+        const uint32_t i = ch * 64u + q;
+        float x[64];
+        bool want = false;
+        if (i < n) {
+            const float4 hh = a.hits[i];
+            const uint32_t rec = __float_as_uint(hh.y);
+            if (rec != kNoHit) {
+                const uint4 *tp = reinterpret_cast<const uint4 *>(a.sc.tris + rec);
+                const uint4 q0 = tp[0], q1 = tp[1], q2 = tp[2];
+                const uint32_t mat = q1.w;
+                if (mat != kNoMaterial && guidable(a.mats[mat].kind)) {
+                    want = true;
+                    const float4 ro = a.rq_o[b & 1][i], rd = a.rq_d[b & 1][i];
+                    const v3 d(rd.x, rd.y, rd.z);
+                    const v3 p = v3(ro.x, ro.y, ro.z) + d * hh.x;
+                    const v3 p0(__uint_as_float(q0.x), __uint_as_float(q0.y), __uint_as_float(q0.z));
+                    const v3 p1(__uint_as_float(q1.x), __uint_as_float(q1.y), __uint_as_float(q1.z));
+                    const v3 p2(__uint_as_float(q2.x), __uint_as_float(q2.y), __uint_as_float(q2.z));
+                    const v3 ng = normalize(cross(p1 - p0, p2 - p0));
+                    const float lam0 = a.st_misc[a.rq_sid[b & 1][i]].x;
+                    float pos01[3], lam01, rough;
+                    v3 n_in;
+                    guide_features(a, p, ng, d, lam0, a.mats[mat], pos01, n_in, lam01, rough);
+                    guide_encode(pos01, n_in, lam01, rough, x);
+                }
+            }
+        }
+        if (!want)
+            for (int k = 0; k < 64; ++k) x[k] = 0.f;
+        for (int k = 0; k < 64; ++k) sH[0][q * kGRow + k] = to_bf16(x[k]);
+        __syncthreads();
+        for (int k = 0; k < 4; ++k) {
+            layer_fwd(a.gW + k * 4096, a.gP + k * kGLayerParams + 4096, sH[k & 1], sH[(k + 1) & 1], nullptr, true);
+            __syncthreads();
+        }
+        head_fwd(a.gW + 4 * 4096, a.gP + 4 * kGLayerParams + 4096, sH[0], sR, 65);
+        __syncthreads();
+        if (want) {
+            VmfLobe L[kGLobes];
+            guide_decode(sR + q * 65, L);
+            float *lb = a.lobes + (size_t)i * kGOut;
+            for (int l = 0; l < kGLobes; ++l) {
+                lb[5 * l + 0] = L[l].mu.x;
+                lb[5 * l + 1] = L[l].mu.y;
+                lb[5 * l + 2] = L[l].mu.z;
+                lb[5 * l + 3] = L[l].kappa;
+                lb[5 * l + 4] = L[l].w;
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// Training record weights after a pass: the radiance the path gathered beyond
+// the vertex, plus the light-sampled share of the emission its continuation
+// hit, hero lane = beta f cos L_i / p_sample with the full incident radiance.
+// The throughput beta before the vertex is kept in the weight: a vertex's
+// share of the loss is its share of the pixel's value, so vertices the image
+// hardly sees (deep in a path) do not steer the network. Records whose
+// continuation was never shaded keep weight 0.
+__global__ void k_finish_records(RenderArgs a, uint32_t n) {
+    const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= n) return;
+    float *R = a.recs + (size_t)r * 16;
+    const uint32_t sid = __float_as_uint(R[13]);
+    const float Lf = a.st_L[sid].x + a.pend[sid].x + a.pend[a.cap + sid].x;
+    const float Lb = R[12];
+    float w = 0.f;
+    if (Lb >= 0.f) w = fmaxf(Lf - Lb, 0.f) + R[15];
+    R[14] = (w < 3.0e38f) ? w : 0.f;
+}
+
 // ---------------------------------------------------------------- shade ----
 __device__ __forceinline__ void setup_bsdf(const RenderArgs &a, const DevMaterial &m, const Lanes &ln, Bsdf &b) {
     b.kind = m.kind;
@@ -219,7 +345,8 @@ __device__ __forceinline__ void setup_bsdf(const RenderArgs &a, const DevMateria
 // returns true with the (unoccluded) contribution and the shadow segment.
 __device__ __forceinline__ bool nee_sample(const RenderArgs &a, const Bsdf &b, const Lanes &ln, uint32_t li, float sel,
                                            float ux, float uy, v3 p, v3 perr, v3 ng, v3 fs, v3 ft, v3 woW, v3 wo,
-                                           const float beta[4], float contrib[4], float4 &sh_o, float4 &sh_d) {
+                                           const float beta[4], float contrib[4], float4 &sh_o, float4 &sh_d,
+                                           const float *lb, float mix) {
     const DevLight &Lj = a.lights[li];
     // Triangle::sample + Shape::sample(ref), triangle.cpp:134-154, shape.cpp:28-41
     float uu = sqrtf(ux);
@@ -240,6 +367,9 @@ __device__ __forceinline__ bool nee_sample(const RenderArgs &a, const Bsdf &b, c
     bsdf_eval_nonspec(b, wo, wil, refl, f);
     float cw = absdot(wi, ng);
     float pb = bsdf_pdf_nonspec(b, wo, wil);
+    // guided vertices sample the mixture (1 - mix) BSDF + mix guide: that is
+    // the density the light sample's MIS weight must compete with
+    if (lb) pb = (1.f - mix) * pb + mix * guide_pdf_at(lb, wi);
     bool black = true;
     for (int k = 0; k < 4; ++k) { f[k] = f[k] * cw; black = black && f[k] == 0.f; }
     if (black) return false;
@@ -296,7 +426,9 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WAV
         }
         const bool active = i < n;
         // -------- phase 1: hit, emission, pass-through, material setup
-        bool cont = false, mat_hit = false;
+        bool cont = false, mat_hit = false, rec_want = false;
+        uint32_t rec_prev = 0;   // index + 1 of the previous vertex's training record
+        float beta0 = 0.f;
         uint32_t sid = 0, bounce = 0, flags = 0, rb = 0, mkind = 0;
         float beta[4] = {0.f, 0.f, 0.f, 0.f}, L[4] = {0.f, 0.f, 0.f, 0.f};
         float lam0 = 0.f, eta_scale = 1.f, prev_pdf = 0.f, eta_bsdf = 1.f;
@@ -311,6 +443,15 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WAV
             beta[0] = beta4.x; beta[1] = beta4.y; beta[2] = beta4.z; beta[3] = beta4.w;
             L[0] = L4.x + pe0.x + pe1.x; L[1] = L4.y + pe0.y + pe1.y;
             L[2] = L4.z + pe0.z + pe1.z; L[3] = L4.w + pe0.w + pe1.w;
+            if (a.st_rec) {
+                // the previous vertex's training record: radiance gathered up to
+                // and including that vertex's own light samples
+                rec_prev = a.st_rec[sid];
+                if (rec_prev) {
+                    a.recs[(size_t)(rec_prev - 1u) * 16 + 12] = L[0];
+                    a.st_rec[sid] = 0u;
+                }
+            }
             lam0 = misc.x; eta_scale = misc.y; prev_pdf = misc.z;
             uint32_t bits = __float_as_uint(misc.w);
             bounce = bits & 0xffu;
@@ -344,10 +485,15 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WAV
                             if (!a.nee_all) pl = pl / (float)a.n_lights;
                             if (pl > 0.f) w = power_heuristic(prev_pdf, pl);
                         }
-                        if (w > 0.f) {
-                            const float *te = a.spec + (size_t)Lj.em_slot * S;
+                        const float *te = a.spec + (size_t)Lj.em_slot * S;
+                        if (w > 0.f)
                             for (int k = 0; k < 4; ++k) L[k] += beta[k] * (spec_at(te, ln, k) * Lj.intensity) * w;
-                        }
+                        // the record of the previous vertex learns the full incident
+                        // radiance: the share of this emission that light sampling
+                        // covers under MIS is added to its target separately
+                        if (rec_prev && w < 1.f)
+                            a.recs[(size_t)(rec_prev - 1u) * 16 + 15] =
+                                beta[0] * (spec_at(te, ln, 0) * Lj.intensity) * (1.f - w);
                     }
                 }
                 perr = vabs(p) * gamma_n(5);
@@ -379,6 +525,7 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WAV
             wo = v3(dot(woW, fs), dot(woW, ft), dot(woW, ng));
             rb = rng_base(a.seed, a.st_key[sid].x, a.st_key[sid].y, bounce);
         }
+        const float *lb = (a.guided && mat_hit && guidable(bs_.kind)) ? a.lobes + (size_t)i * kGOut : nullptr;
         const bool do_nee = mat_hit && !is_specular_kind(bs_.kind) && a.n_lights > 0;
 #pragma unroll
         for (int j = 0; j < 2; ++j) {
@@ -394,7 +541,7 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WAV
                     sel = 1.f / (float)a.n_lights;
                 }
                 ok = nee_sample(a, bs_, ln, li, sel, rng_u(rb, 3 + 2 * j), rng_u(rb, 4 + 2 * j), p, perr, ng, fs, ft,
-                                woW, wo, beta, contrib, sh_o, sh_d);
+                                woW, wo, beta, contrib, sh_o, sh_d, lb, a.guide_mix);
             }
             if (active) {   // every live path rewrites both slots (0 when no sample)
                 if (flags & PF_SECTERM) contrib[1] = contrib[2] = contrib[3] = 0.f;
@@ -410,24 +557,62 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WAV
         // -------- phase 3: BSDF sample (continuation and MIS BSDF ray), RR
         if (mat_hit) {
             BsdfSample bsm;
-            float u0 = remap_u0(rng_u(rb, 0)), u1 = rng_u(rb, 1);
-            if (wo.z == 0.f) bsm.pdf = 0.f;
-            else lobe_sample(bs_, wo, u0, u1, bsm);
-            bool ok = bsm.pdf != 0.f;
+            bool ok;
             v3 wiW;
-            if (ok) {
-                wiW = fs * bsm.wi.x + ft * bsm.wi.y + ng * bsm.wi.z;   // to_world
-                if (!(bsm.type & BSDF_SPECULAR)) {
-                    bool refl = dot(wiW, ng) * dot(woW, ng) > 0.f;
-                    if (!refl) {
-                        for (int k = 0; k < 4; ++k) bsm.f[k] = 0.f;
-                    } else if (bs_.kind == MAT_CONDUCTOR) {
-                        microfacet_f(bs_, wo, bsm.wi, bsm.f, &bsm.wh);
+            beta0 = beta[0];
+            if (lb) {
+                // one-sample mixture: the guide with probability mix, else the
+                // BSDF; the throughput divides by the mixture density at the
+                // chosen direction, so the estimate stays unbiased wherever
+                // the BSDF is non-zero
+                bsm.pdf = 0.f;
+                bsm.type = lobe_type(bs_.kind);
+                bsm.secondary_terminated = false;
+                v3 wl;
+                bool have = wo.z != 0.f;
+                if (have) {
+                    if (rng_u(rb, 8) < a.guide_mix) {
+                        wiW = guide_sample_at(lb, rng_u(rb, 9), rng_u(rb, 10), rng_u(rb, 11));
+                        wl = v3(dot(wiW, fs), dot(wiW, ft), dot(wiW, ng));
+                    } else {
+                        BsdfSample bs2;
+                        lobe_sample(bs_, wo, remap_u0(rng_u(rb, 0)), rng_u(rb, 1), bs2);
+                        have = bs2.pdf != 0.f;
+                        wl = bs2.wi;
+                        wiW = fs * wl.x + ft * wl.y + ng * wl.z;
                     }
                 }
-                bool black = true;
-                for (int k = 0; k < 4; ++k) black = black && bsm.f[k] == 0.f;
-                if (black) ok = false;
+                if (have) {
+                    const bool refl = dot(wiW, ng) * dot(woW, ng) > 0.f;
+                    bsdf_eval_nonspec(bs_, wo, wl, refl, bsm.f);
+                    bsm.pdf = (1.f - a.guide_mix) * bsdf_pdf_nonspec(bs_, wo, wl) + a.guide_mix * guide_pdf_at(lb, wiW);
+                    bsm.wi = wl;
+                }
+                ok = bsm.pdf > 0.f;
+                if (ok) {
+                    bool black = true;
+                    for (int k = 0; k < 4; ++k) black = black && bsm.f[k] == 0.f;
+                    if (black) ok = false;
+                }
+            } else {
+                float u0 = remap_u0(rng_u(rb, 0)), u1 = rng_u(rb, 1);
+                if (wo.z == 0.f) bsm.pdf = 0.f;
+                else lobe_sample(bs_, wo, u0, u1, bsm);
+                ok = bsm.pdf != 0.f;
+                if (ok) {
+                    wiW = fs * bsm.wi.x + ft * bsm.wi.y + ng * bsm.wi.z;   // to_world
+                    if (!(bsm.type & BSDF_SPECULAR)) {
+                        bool refl = dot(wiW, ng) * dot(woW, ng) > 0.f;
+                        if (!refl) {
+                            for (int k = 0; k < 4; ++k) bsm.f[k] = 0.f;
+                        } else if (bs_.kind == MAT_CONDUCTOR) {
+                            microfacet_f(bs_, wo, bsm.wi, bsm.f, &bsm.wh);
+                        }
+                    }
+                    bool black = true;
+                    for (int k = 0; k < 4; ++k) black = black && bsm.f[k] == 0.f;
+                    if (black) ok = false;
+                }
             }
             if (ok) {
                 const uint32_t ty = bsm.type;
@@ -459,6 +644,7 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WAV
                         }
                     }
                     bounce += 1;
+                    rec_want = a.recs != nullptr && cont && guidable(bs_.kind);
                 }
             }
         }
@@ -468,6 +654,25 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WAV
             a.st_L[sid] = make_float4(L[0], L[1], L[2], L[3]);
             a.st_misc[sid] = make_float4(lam0, eta_scale, prev_pdf, __uint_as_float(flags | bounce));
             if (cont) a.st_beta[sid] = make_float4(beta[0], beta[1], beta[2], beta[3]);
+        }
+        // -------- training record of this vertex (training passes only)
+        if (a.recs) {
+            const uint32_t ridx = wave_append(rec_want, a.rec_count);
+            if (rec_want && ridx < a.rec_cap) {
+                float *R = a.recs + (size_t)ridx * 16;
+                float pos01[3], lam01, rough;
+                v3 n_in;
+                guide_features(a, p, ng, d, lam0, a.mats[mkind], pos01, n_in, lam01, rough);
+                R[0] = pos01[0]; R[1] = pos01[1]; R[2] = pos01[2];
+                R[3] = n_in.x; R[4] = n_in.y; R[5] = n_in.z;
+                R[6] = lam01; R[7] = rough;
+                R[8] = nd.x; R[9] = nd.y; R[10] = nd.z;
+                R[11] = beta0;
+                R[12] = -1.f;                      // radiance before: written at the next vertex
+                R[13] = __uint_as_float(sid);
+                R[14] = 0.f; R[15] = 0.f;
+                a.st_rec[sid] = ridx + 1u;
+            }
         }
         // -------- compaction of the continuation: one atomic per wave
         uint32_t slot = wave_append(cont, &a.counts[C_EXT + nxt]);
@@ -584,7 +789,7 @@ T *grow(void *&p, size_t &cap, size_t bytes, bool &err) {
 struct RenderState {
     void *p[24] = {};
     size_t cap[24] = {};
-    uint32_t *h_counts = nullptr;     // pinned host copies of the queue counters, [2][16]
+    uint32_t *h_counts = nullptr;     // pinned host copies of the queue counters, [2][16], then the record count
     hipEvent_t ev[2] = {};
     // bounce-loop graph (kChunk bounces), and the arguments it was captured with
     hipGraph_t graph = nullptr;
@@ -600,7 +805,7 @@ static RenderState *state_for(fl_ctx_t ctx) {
         RenderState *r = new (std::nothrow) RenderState();
         if (!r) return nullptr;
         memset(&r->gkey, 0, sizeof(r->gkey));
-        if (hipHostMalloc((void **)&r->h_counts, 2 * 16 * sizeof(uint32_t), hipHostMallocDefault) != hipSuccess ||
+        if (hipHostMalloc((void **)&r->h_counts, 3 * 16 * sizeof(uint32_t), hipHostMallocDefault) != hipSuccess ||
             hipEventCreateWithFlags(&r->ev[0], hipEventDisableTiming) != hipSuccess ||
             hipEventCreateWithFlags(&r->ev[1], hipEventDisableTiming) != hipSuccess) {
             if (r->h_counts) (void)hipHostFree(r->h_counts);
@@ -681,7 +886,9 @@ int render(fl_scene_s *s, const fl_camera *camera, const fl_render_params *p, fl
     hipStream_t st = ctx->stream;
     if (!camera || !p || camera->width == 0 || camera->height == 0 || p->spp == 0)
         return set_error(ctx, FL_E_INVALID, "fl_render: bad camera/params");
-    if (p->flags & FL_RENDER_GUIDED) return set_error(ctx, FL_E_UNSUPPORTED, "guided rendering not built");
+    fl_guide_s *guide = (p->flags & FL_RENDER_GUIDED) ? p->guide : nullptr;
+    if ((p->flags & FL_RENDER_GUIDED) && !guide) return set_error(ctx, FL_E_INVALID, "FL_RENDER_GUIDED needs params.guide");
+    if (guide && guide->ctx != ctx) return set_error(ctx, FL_E_INVALID, "guide belongs to another context");
     (void)hipGetLastError();  // do not report a stale error of an earlier call
     const int mode = p->spectral_mode == FL_SPECTRAL_PARITY ? 1 : 0;
     const SpecTables &tab = s->spec[mode];
@@ -705,6 +912,7 @@ int render(fl_scene_s *s, const fl_camera *camera, const fl_render_params *p, fl
     // (248 B of queues and state per path; 2^26 paths = 16.6 GB), capped at
     // half the free device memory
     uint32_t max_paths = p->max_paths;
+    if (!max_paths && guide) max_paths = 1u << 24;   // guided passes also hold 160 B of lobes per path
     if (!max_paths) {
         size_t free_b = 0, total_b = 0;
         max_paths = 1u << 26;
@@ -766,6 +974,34 @@ int render(fl_scene_s *s, const fl_camera *camera, const fl_render_params *p, fl
     a.spill = grow<uint2>(B.p[19], B.cap[19], (size_t)a.spill_stride * spill_per * 8, err);
     if (err) return set_error(ctx, FL_E_OOM, "render buffers");
 
+    // learned proposal: schedule, network pointers, lobes and training records
+    const bool train = guide && (p->flags & FL_RENDER_TRAIN_GUIDE) && !guide->diverged;
+    uint32_t train_spp = 0, train_rounds = 0;
+    float *recs = nullptr;
+    uint32_t *st_rec = nullptr, rec_cap = 0;
+    if (guide) {
+        a.guide_mix = p->guide_mix > 0.f && p->guide_mix < 1.f ? p->guide_mix : 0.5f;
+        a.gW = guide->d_w;
+        a.gP = guide->d_params;
+        for (int k = 0; k < 3; ++k) {
+            const float pad = 0.01f * std::max(1e-6f, s->bmax[k] - s->bmin[k]);
+            a.box_lo[k] = s->bmin[k] - pad;
+            a.box_inv[k] = 1.f / (s->bmax[k] - s->bmin[k] + 2.f * pad);
+        }
+        a.lobes = grow<float>(B.p[20], B.cap[20], cap * kGOut * sizeof(float), err);
+        if (train) {
+            const float fr = p->guide_train_fraction > 0.f && p->guide_train_fraction <= 1.f ? p->guide_train_fraction
+                                                                                              : 0.25f;
+            train_spp = std::min(p->spp, std::max(1u, (uint32_t)std::lround(fr * (float)p->spp)));
+            train_rounds = std::min(8u, train_spp);
+            const uint32_t round_spp = (train_spp + train_rounds - 1) / train_rounds;
+            rec_cap = (uint32_t)std::min<size_t>((size_t)a.npix * std::min(round_spp, spp_pass) * 6, 1u << 24);
+            st_rec = grow<uint32_t>(B.p[21], B.cap[21], cap * 4, err);
+            recs = grow<float>(B.p[22], B.cap[22], (size_t)rec_cap * 16 * sizeof(float), err);
+        }
+        if (err) return set_error(ctx, FL_E_OOM, "guide buffers");
+    }
+
     const bool counters = (p->flags & FL_RENDER_COUNTERS) != 0;
     const bool timing = (p->flags & FL_RENDER_TIMING) != 0;
     hipEvent_t e0, e1, tev[3];
@@ -791,41 +1027,65 @@ int render(fl_scene_s *s, const fl_camera *camera, const fl_render_params *p, fl
     // FL_RENDER_GRAPH a chunk is one captured graph, replayed.
     constexpr uint32_t kChunk = 4;   // even: a chunk starts and ends on queue parity 0
     const int variant = (counters ? 1 : 0) | (p->shade_waves >= 4 ? 2 : 0) | ((int)(tv - kTraceVariants) << 2);
+    const uint32_t infer_grid = (uint32_t)ctx->n_cu * 4;
+    auto launch_shade = [&](const RenderArgs &args, dim3 g, int j) {
+        if (args.guided) hipLaunchKernelGGL(k_guide_infer, dim3(infer_grid), dim3(64), 0, st, args, j);
+        if (variant & 2) hipLaunchKernelGGL(k_shade<4>, g, dim3(kBlock), 0, st, args, j);
+        else hipLaunchKernelGGL(k_shade<2>, g, dim3(kBlock), 0, st, args, j);
+    };
     auto launch_bounce = [&](const RenderArgs &args, uint32_t j) {
         tv->launch(counters, dim3(grid_pers), st, args, (int)j);
-        if (variant & 2) hipLaunchKernelGGL(k_shade<4>, dim3(shade_grid_max), dim3(kBlock), 0, st, args, (int)j);
-        else hipLaunchKernelGGL(k_shade<2>, dim3(shade_grid_max), dim3(kBlock), 0, st, args, (int)j);
+        launch_shade(args, dim3(shade_grid_max), (int)j);
     };
     const bool use_graph = (p->flags & FL_RENDER_GRAPH) && !timing;
-    if (use_graph) {
-        // trace and shade read only scene, queue and state fields: the
-        // per-pass fields are cleared so one capture serves every pass
+    // One captured chunk serves every pass with the same arguments: trace,
+    // guide and shade read only scene, queue and state fields, so the
+    // per-pass fields are cleared from the key.
+    auto ensure_graph = [&]() -> int {
         RenderArgs key = a;
         key.n_paths = 0;
         key.spp_pass = 0;
         key.sample_base = 0;
-        if (!B.gexec || B.gvariant != variant || memcmp(&key, &B.gkey, sizeof(key)) != 0) {
+        if (B.gexec && B.gvariant == variant && memcmp(&key, &B.gkey, sizeof(key)) == 0) return FL_OK;
+        if (B.gexec) FL_HIP_CHECK(ctx, hipStreamSynchronize(st));   // no replay of the old graph still queued
+        drop_graph(B);
+        FL_HIP_CHECK(ctx, hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+        for (uint32_t j = 0; j < kChunk; ++j) launch_bounce(key, j);
+        hipGraph_t g = nullptr;
+        hipError_t ce = hipStreamEndCapture(st, &g);
+        if (ce != hipSuccess || !g) return set_error(ctx, FL_E_HIP, "bounce graph capture failed");
+        B.graph = g;
+        if (hipGraphInstantiate(&B.gexec, B.graph, nullptr, nullptr, 0) != hipSuccess) {
             drop_graph(B);
-            FL_HIP_CHECK(ctx, hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-            for (uint32_t j = 0; j < kChunk; ++j) launch_bounce(key, j);
-            hipGraph_t g = nullptr;
-            hipError_t ce = hipStreamEndCapture(st, &g);
-            if (ce != hipSuccess || !g) return set_error(ctx, FL_E_HIP, "bounce graph capture failed");
-            B.graph = g;
-            if (hipGraphInstantiate(&B.gexec, B.graph, nullptr, nullptr, 0) != hipSuccess) {
-                drop_graph(B);
-                return set_error(ctx, FL_E_HIP, "bounce graph instantiation failed");
-            }
-            B.gkey = key;
-            B.gvariant = variant;
+            return set_error(ctx, FL_E_HIP, "bounce graph instantiation failed");
         }
-    }
+        B.gkey = key;
+        B.gvariant = variant;
+        return FL_OK;
+    };
 
+    double ms_guide = 0.0;
+    uint32_t round = 0;
     while (done_spp < p->spp) {
         uint32_t this_spp = std::min(spp_pass, p->spp - done_spp);
+        const bool training_pass = train && done_spp < train_spp && !guide->diverged;
+        if (training_pass) {
+            // spread the training samples over train_rounds passes
+            const uint32_t left = train_rounds > round ? train_rounds - round : 1u;
+            this_spp = std::min(this_spp, std::max(1u, (train_spp - done_spp + left - 1) / left));
+        }
+        a.guided = (guide && !guide->diverged && !(training_pass && round == 0)) ? 1u : 0u;
+        a.recs = training_pass ? recs : nullptr;
+        a.st_rec = training_pass ? st_rec : nullptr;
+        a.rec_count = training_pass ? a.counts + 8 : nullptr;
+        a.rec_cap = training_pass ? rec_cap : 0u;
         a.spp_pass = this_spp;
         a.n_paths = a.npix * this_spp;
         a.sample_base = p->sample_offset + done_spp;
+        if (use_graph) {
+            const int grc = ensure_graph();
+            if (grc != FL_OK) return grc;
+        }
         hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, st, a.counts, 16u);
         hipLaunchKernelGGL(k_generate, dim3(std::min<unsigned>(blocks(a.n_paths), ctx->n_cu * 16)), dim3(kBlock), 0, st, a);
         uint32_t ne = W * H * this_spp, ns = 0;
@@ -865,9 +1125,7 @@ int render(fl_scene_s *s, const fl_camera *camera, const fl_render_params *p, fl
             ++n_trace;
             if (timing) FL_HIP_CHECK(ctx, hipEventRecord(tev[1], st));
             if (ne > 0) {
-                dim3 sgrid(std::min<uint32_t>(shade_grid_max, blocks(ne)));
-                if (p->shade_waves >= 4) hipLaunchKernelGGL(k_shade<4>, sgrid, dim3(kBlock), 0, st, a, (int)iter);
-                else hipLaunchKernelGGL(k_shade<2>, sgrid, dim3(kBlock), 0, st, a, (int)iter);
+                launch_shade(a, dim3(std::min<uint32_t>(shade_grid_max, blocks(ne))), (int)iter);
                 ++n_shade;
             }
             if (timing) FL_HIP_CHECK(ctx, hipEventRecord(tev[2], st));
@@ -890,6 +1148,22 @@ int render(fl_scene_s *s, const fl_camera *camera, const fl_render_params *p, fl
         }
         max_iter_seen = std::max(max_iter_seen, iter);
         hipLaunchKernelGGL(k_resolve, dim3(blocks(a.npix)), dim3(kBlock), 0, st, a, a.spec);
+        if (training_pass) {
+            // finish this pass's records, then train on them before the next pass
+            const auto t0 = std::chrono::steady_clock::now();
+            uint32_t nrec = 0;
+            FL_HIP_CHECK(ctx, hipMemcpyAsync(&B.h_counts[32], a.counts + 8, 4, hipMemcpyDeviceToHost, st));
+            FL_HIP_CHECK(ctx, hipStreamSynchronize(st));
+            nrec = std::min(B.h_counts[32], rec_cap);
+            if (nrec) {
+                hipLaunchKernelGGL(k_finish_records, dim3(blocks(nrec)), dim3(kBlock), 0, st, a, nrec);
+                const int trc = guide_train_on_records(guide, recs, nrec, guide->cfg.steps_per_pass, guide->cfg.batch,
+                                                       p->seed * 1315423911ull + round, st);
+                if (trc != FL_OK) return trc;
+            }
+            ms_guide += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+            ++round;
+        }
         done_spp += this_spp;
         ++passes;
     }
@@ -921,6 +1195,7 @@ int render(fl_scene_s *s, const fl_camera *camera, const fl_render_params *p, fl
         stats->stack_spills = h[5];
         stats->ms_extend = ms_trace;   // combined traversal kernel (extension + shadow rays)
         stats->ms_shade = ms_shade;
+        stats->ms_guide = ms_guide;
         stats->launches_extend = n_trace;
         stats->launches_shade = n_shade;
         stats->passes = passes;

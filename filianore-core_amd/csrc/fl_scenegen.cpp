@@ -352,10 +352,23 @@ void guided_room(Builder &B, const fl_gen_options *o, fl_gen_scene *out) {
     ceil_rect(sx1, X, -Z, Z);
     ceil_rect(sx0, sx1, -Z, sz0);
     ceil_rect(sx0, sx1, sz1, Z);
+    // the outer light: in a closed attic above the ceiling ([-X,X] x [Yh, Yh+Ya]
+    // x [-Z,Z], Lambert, inward normals), lying on the attic floor beside the
+    // slot and facing up. No point of the room sees it, so the room is lit
+    // only by light that bounced in the attic and came down through the slot:
+    // light sampling from inside the room finds nothing, and paths have to
+    // find the slot themselves (hard indirect transport).
+    const float Ya = 3.f;
+    uint32_t ma = B.add_material(lambert(0.8f, 0.8f, 0.8f));
+    B.add_quad_n({0, Yh + Ya, 0}, {0, -1, 0}, {1, 0, 0}, X, Z, ma);                    // attic top
+    B.add_quad_n({0, Yh + Ya * 0.5f, -Z}, {0, 0, 1}, {1, 0, 0}, X, Ya * 0.5f, ma);    // attic back
+    B.add_quad_n({0, Yh + Ya * 0.5f, Z}, {0, 0, -1}, {1, 0, 0}, X, Ya * 0.5f, ma);     // attic front
+    B.add_quad_n({-X, Yh + Ya * 0.5f, 0}, {1, 0, 0}, {0, 0, 1}, Z, Ya * 0.5f, ma);    // attic left
+    B.add_quad_n({X, Yh + Ya * 0.5f, 0}, {-1, 0, 0}, {0, 0, 1}, Z, Ya * 0.5f, ma);    // attic right
     uint32_t em = add_blackbody(B, 6500.f);
     uint32_t mn = B.add_material(nomat());
-    uint32_t lt = B.add_quad_n({0.5f * (sx0 + sx1), Yh + 1.f, 0.f}, {0, -1, 0}, {1, 0, 0}, 1.5f, 3.5f, mn);
-    B.add_light_quad(lt, em, 17.f * 20.f);
+    uint32_t lt = B.add_quad_n({2.5f, Yh + 0.01f, 0.f}, {0, 1, 0}, {1, 0, 0}, 1.f, 1.5f, mn);
+    B.add_light_quad(lt, em, 17.f * 40.f);
     out->camera = fl_camera{{0.f, 3.f, 3.9f}, {0.f, 1.f, 0.f}, {0.f, 1.f, 0.f}, 60.f, 1024, 1024};
     out->spp = 64;
     out->depth = 16;

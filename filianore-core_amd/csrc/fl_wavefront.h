@@ -75,6 +75,18 @@ struct RenderArgs {
     uint2 *spill;
     uint32_t spill_stride;
     int trav_budget;         // traversal iterations between lane refills
+    // learned proposal (guided passes)
+    uint32_t guided;         // 1: mix the guide's vMF lobes with BSDF sampling
+    float guide_mix;         // probability of sampling the guide
+    const __bf16 *gW;        // bf16 weights [5][64][64]
+    const float *gP;         // fp32 parameters (biases)
+    float *lobes;            // [cap][40]: decoded lobes per extension-queue slot
+    float box_lo[3], box_inv[3];   // scene box -> unit cube
+    // training records (training passes): 16 floats each
+    float *recs;
+    uint32_t *rec_count;
+    uint32_t rec_cap;
+    uint32_t *st_rec;        // per path: index + 1 of its last record, 0 = none
 };
 
 } // namespace fl

@@ -105,7 +105,8 @@ class TuneResult(C.Structure):
 class GuideConfig(C.Structure):
     _fields_ = [("hidden", C.c_uint32), ("layers", C.c_uint32), ("lobes", C.c_uint32), ("bands", C.c_uint32),
                 ("lr", C.c_float), ("beta1", C.c_float), ("beta2", C.c_float), ("eps", C.c_float),
-                ("batch", C.c_uint32), ("seed", C.c_uint64), ("scene_min", f3), ("scene_max", f3)]
+                ("batch", C.c_uint32), ("seed", C.c_uint64), ("scene_min", f3), ("scene_max", f3),
+                ("steps_per_pass", C.c_uint32)]
 
 
 class BsdfQuery(C.Structure):
