@@ -327,8 +327,15 @@ int fl_bsdf_query_batch(fl_ctx_t ctx, const fl_bsdf_query *q, fl_bsdf_result *r,
 }
 
 // Not yet available entry points fail loudly.
-int fl_render_batch(fl_ctx_t ctx, const fl_batch_item *, uint32_t, const fl_render_params *, fl_stats *) {
-    return set_error(ctx, FL_E_UNSUPPORTED, "fl_render_batch: not built");
+int fl_render_batch(fl_ctx_t ctx, const fl_batch_item *items, uint32_t n, const fl_render_params *params,
+                    fl_stats *stats) {
+    if (!ctx) return FL_E_INVALID;
+    if (!items || n == 0 || !params) return set_error(ctx, FL_E_INVALID, "fl_render_batch: no items or params");
+    std::vector<flx::HostItem> hi(n);
+    for (uint32_t k = 0; k < n; ++k)
+        hi[k] = flx::HostItem{items[k].scene, items[k].camera, items[k].film, items[k].spp ? items[k].spp : params->spp};
+    (void)hipSetDevice(ctx->device);
+    return flx::render_items(ctx, hi.data(), n, params, stats);
 }
 int fl_autotune(fl_scene_t s, const fl_camera *, const fl_render_params *, fl_tune_result *) {
     return set_error(s ? s->ctx : nullptr, FL_E_UNSUPPORTED, "fl_autotune: not built");

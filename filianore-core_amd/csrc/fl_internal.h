@@ -119,6 +119,15 @@ int trace_rays(fl_ctx_t ctx, fl_scene_s *s, const fl_ray *d_rays, fl_hit *d_hits
 // BSDF queries (fl_render.hip)
 int bsdf_queries(fl_ctx_t ctx, const fl_bsdf_query *d_q, fl_bsdf_result *d_r, uint32_t n);
 
+// One (scene, camera, film, spp) of a render; fl_render is a batch of one.
+struct HostItem {
+    fl_scene_s *scene;
+    fl_camera cam;
+    fl_film *film;
+    uint32_t spp;
+};
+int render_items(fl_ctx_t ctx, const HostItem *items, uint32_t n, const fl_render_params *p, fl_stats *stats);
+
 // Online training of the guide on render records (device, 16 floats each:
 // position in the unit box 3, normal 3, wavelength 1, roughness 1, direction
 // 3, throughput 1, radiance before 1, path id 1, weight 1, unused 1).

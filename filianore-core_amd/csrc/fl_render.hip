@@ -63,40 +63,44 @@ __global__ void k_zero_f32(float *p, size_t n) {
 }
 
 // ------------------------------------------------------------- generate ----
-// Path ids run over sample layers of npix padded pixels in 8x8 tile order;
-// queue slots skip the padding. The slot of an in-image pixel is its rank
-// among in-image pixels in the same order, in closed form (no atomics):
-// full tile rows before it, full tiles before it in its tile row, then rows
-// and columns inside its (possibly clipped) tile.
+// Path ids run item after item; inside an item over sample layers of npix
+// padded pixels in 8x8 tile order. Queue slots skip the padding. The slot of
+// an in-image pixel is its item's slot base plus its rank among the item's
+// in-image pixels in the same order, in closed form (no atomics): full tile
+// rows before it, full tiles before it in its tile row, then rows and columns
+// inside its (possibly clipped) tile.
 __global__ void __launch_bounds__(kBlock) k_generate(RenderArgs a) {
     const uint32_t stride = gridDim.x * kBlock;
-    const uint32_t W = a.cam.W, H = a.cam.H;
-    if (blockIdx.x == 0 && threadIdx.x == 0) a.counts[C_EXT + 0] = W * H * a.spp_pass;
+    if (blockIdx.x == 0 && threadIdx.x == 0) a.counts[C_EXT + 0] = a.n_cam_rays;
     for (uint32_t sid = blockIdx.x * kBlock + threadIdx.x; sid < a.n_paths; sid += stride) {
+        const uint32_t item = a.n_items > 1 ? item_search(a, sid, [](const ItemDev &it) { return it.path_base; }) : 0u;
+        const ItemDev &it = a.items[item];
+        const uint32_t W = it.cam.W, H = it.cam.H;
         uint32_t x = 0, y = 0, pixel = 0, sample = 0, slot = 0;
-        uint32_t s = sid / a.npix, pl = sid - s * a.npix;
+        const uint32_t local = sid - it.path_base;
+        uint32_t s = local / it.npix, pl = local - s * it.npix;
         uint32_t tile = pl >> 6, within = pl & 63u;
-        uint32_t tx = tile % a.tiles_x, ty = tile / a.tiles_x, wx = within & 7u, wy = within >> 3;
+        uint32_t tx = tile % it.tiles_x, ty = tile / it.tiles_x, wx = within & 7u, wy = within >> 3;
         x = tx * 8u + wx;
         y = ty * 8u + wy;
         const bool active = x < W && y < H;
         if (active) {
             pixel = y * W + x;
-            sample = a.sample_base + s;
+            sample = it.sample_base + s;
             const uint32_t rows_in = min(8u, H - ty * 8u), cols_in = min(8u, W - tx * 8u);
-            slot = s * (W * H) + 8u * W * ty + 8u * tx * rows_in + wy * cols_in + wx;
+            slot = it.slot_base + s * (W * H) + 8u * W * ty + 8u * tx * rows_in + wy * cols_in + wx;
         }
         v3 o, d;
         if (active) {
-            uint32_t rb = rng_base(a.seed, pixel, sample, 0xffu);
+            uint32_t rb = rng_base(it.seed, pixel, sample, 0xffu);
             float jx = rng_u(rb, 0), jy = rng_u(rb, 1), ul = rng_u(rb, 2);
             float lam0 = a.grid.lo + ul * (a.grid.hi - a.grid.lo);
             if (lam0 >= a.grid.hi) lam0 = a.grid.lo;
             // pixel column px -> s = 1 - (px+jx)/W, row py (top=0) -> t = 1 - (py+jy)/H
-            float su = 1.f - ((float)x + jx) / (float)a.cam.W;
-            float tv = 1.f - ((float)y + jy) / (float)a.cam.H;
-            v3 llc = ld3(a.cam.llc), hor = ld3(a.cam.hor), ver = ld3(a.cam.ver);
-            o = ld3(a.cam.o);
+            float su = 1.f - ((float)x + jx) / (float)W;
+            float tv = 1.f - ((float)y + jy) / (float)H;
+            v3 llc = ld3(it.cam.llc), hor = ld3(it.cam.hor), ver = ld3(it.cam.ver);
+            o = ld3(it.cam.o);
             v3 dir = llc + hor * su + ver * tv - o - v3(0.f, 0.f, 0.f);
             d = normalize(dir);
             a.st_beta[sid] = make_float4(1.f, 1.f, 1.f, 1.f);
@@ -107,6 +111,7 @@ __global__ void __launch_bounds__(kBlock) k_generate(RenderArgs a) {
             a.st_L[sid] = make_float4(0.f, 0.f, 0.f, 0.f);
             a.st_misc[sid] = make_float4(0.f, 1.f, 0.f, __uint_as_float(PF_DONE));
         }
+        if (a.n_items > 1) a.st_item[sid] = item;
         a.pend[sid] = make_float4(0.f, 0.f, 0.f, 0.f);
         a.pend[a.cap + sid] = make_float4(0.f, 0.f, 0.f, 0.f);
         if (a.st_rec) a.st_rec[sid] = 0u;
@@ -121,17 +126,35 @@ __global__ void __launch_bounds__(kBlock) k_generate(RenderArgs a) {
 // ---------------------------------------------------------------- trace ----
 // Index space: [0, ns) shadow rays written by the previous bounce's shade,
 // [ns, ns + ne) this bounce's extension rays.
+// In batches each ray traverses the BVH of its path's item.
 struct TraceSrc {
     const float4 *so, *sd;
     const uint32_t *stgt;
     float4 *pend;
     const float4 *eo, *ed;
+    const uint32_t *esid;
     float4 *hits;
-    uint32_t ns;
-    __device__ void load(uint32_t i, v3 &ro, v3 &rd, float &tmin, float &tmax, bool &any) const {
+    uint32_t ns, cap;
+    const ItemDev *items;
+    const uint32_t *st_item;
+    uint32_t n_items;
+    TraceScene sc0;
+    __device__ void load(uint32_t i, v3 &ro, v3 &rd, float &tmin, float &tmax, bool &any, TraceScene &sc) const {
         float4 a, b;
         if (i < ns) { a = so[i]; b = sd[i]; any = true; }
         else { a = eo[i - ns]; b = ed[i - ns]; any = false; }
+        if (n_items > 1) {
+            uint32_t sid;
+            if (i < ns) {
+                const uint32_t t = stgt[i];
+                sid = t >= cap ? t - cap : t;   // target slot j * cap + path id
+            } else {
+                sid = esid[i - ns];
+            }
+            sc = items[st_item[sid]].sc;
+        } else {
+            sc = sc0;
+        }
         ro = v3(a.x, a.y, a.z);
         rd = v3(b.x, b.y, b.z);
         tmin = a.w;
@@ -168,8 +191,9 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WAV
     ls.spill = a.spill + blockIdx.x * kBlock + threadIdx.x;
     ls.spill_stride = a.spill_stride;
     TravCounters c;
-    TraceSrc src{a.sq_o, a.sq_d, a.sq_tgt, a.pend, a.rq_o[cur], a.rq_d[cur], a.hits, ns};
-    trace_persistent<COUNT, true>(a.sc, src, ns + ne, &a.counts[C_FETCH + cur], ls, a.trav_budget, c);
+    TraceSrc src{a.sq_o,    a.sq_d, a.sq_tgt, a.pend,    a.rq_o[cur], a.rq_d[cur], a.rq_sid[cur],
+                 a.hits,    ns,     a.cap,    a.items,   a.st_item,   a.n_items,   a.sc};
+    trace_persistent<COUNT, true>(src, ns + ne, &a.counts[C_FETCH + cur], ls, a.trav_budget, c);
     if (COUNT) {
         atomicAdd(&a.stats[3], (unsigned long long)c.nodes);
         atomicAdd(&a.stats[4], (unsigned long long)c.tris);
@@ -255,11 +279,13 @@ __global__ void __launch_bounds__(64) k_guide_infer(RenderArgs a, int b) {
         if (i < n) {
             const float4 hh = a.hits[i];
             const uint32_t rec = __float_as_uint(hh.y);
+            const uint32_t sid = a.rq_sid[b & 1][i];
+            const SceneRef S = scene_of(a, item_of(a, sid));
             if (rec != kNoHit) {
-                const uint4 *tp = reinterpret_cast<const uint4 *>(a.sc.tris + rec);
+                const uint4 *tp = reinterpret_cast<const uint4 *>(S.sc.tris + rec);
                 const uint4 q0 = tp[0], q1 = tp[1], q2 = tp[2];
                 const uint32_t mat = q1.w;
-                if (mat != kNoMaterial && guidable(a.mats[mat].kind)) {
+                if (mat != kNoMaterial && guidable(S.mats[mat].kind)) {
                     want = true;
                     const float4 ro = a.rq_o[b & 1][i], rd = a.rq_d[b & 1][i];
                     const v3 d(rd.x, rd.y, rd.z);
@@ -268,10 +294,10 @@ __global__ void __launch_bounds__(64) k_guide_infer(RenderArgs a, int b) {
                     const v3 p1(__uint_as_float(q1.x), __uint_as_float(q1.y), __uint_as_float(q1.z));
                     const v3 p2(__uint_as_float(q2.x), __uint_as_float(q2.y), __uint_as_float(q2.z));
                     const v3 ng = normalize(cross(p1 - p0, p2 - p0));
-                    const float lam0 = a.st_misc[a.rq_sid[b & 1][i]].x;
+                    const float lam0 = a.st_misc[sid].x;
                     float pos01[3], lam01, rough;
                     v3 n_in;
-                    guide_features(a, p, ng, d, lam0, a.mats[mat], pos01, n_in, lam01, rough);
+                    guide_features(a, p, ng, d, lam0, S.mats[mat], pos01, n_in, lam01, rough);
                     guide_encode(pos01, n_in, lam01, rough, x);
                 }
             }
@@ -322,15 +348,16 @@ __global__ void k_finish_records(RenderArgs a, uint32_t n) {
 }
 
 // ---------------------------------------------------------------- shade ----
-__device__ __forceinline__ void setup_bsdf(const RenderArgs &a, const DevMaterial &m, const Lanes &ln, Bsdf &b) {
+__device__ __forceinline__ void setup_bsdf(const RenderArgs &a, const float *spec, const DevMaterial &m,
+                                           const Lanes &ln, Bsdf &b) {
     b.kind = m.kind;
     b.ggx.ax = m.ax;
     b.ggx.ay = m.ay;
     b.weight = m.weight;
     b.vndf_pdf = (a.flags & FL_RENDER_VNDF_PDF) != 0;
     const int S = a.grid.stride;
-    const float *ta = a.spec + (size_t)m.spec_a * S;
-    const float *tb = a.spec + (size_t)m.spec_b * S;
+    const float *ta = spec + (size_t)m.spec_a * S;
+    const float *tb = spec + (size_t)m.spec_b * S;
     for (int k = 0; k < 4; ++k) {
         b.s1[k] = spec_at(ta, ln, k);
         b.s2[k] = m.kind == MAT_LAMBERT ? 0.f : spec_at(tb, ln, k);
@@ -343,11 +370,12 @@ __device__ __forceinline__ void setup_bsdf(const RenderArgs &a, const DevMateria
 
 // One NEE sample of light `li` (estimate_direct's light half, integrator.cpp:29-59):
 // returns true with the (unoccluded) contribution and the shadow segment.
-__device__ __forceinline__ bool nee_sample(const RenderArgs &a, const Bsdf &b, const Lanes &ln, uint32_t li, float sel,
+__device__ __forceinline__ bool nee_sample(const RenderArgs &a, const SceneRef &S, const Bsdf &b, const Lanes &ln,
+                                           uint32_t li, float sel,
                                            float ux, float uy, v3 p, v3 perr, v3 ng, v3 fs, v3 ft, v3 woW, v3 wo,
                                            const float beta[4], float contrib[4], float4 &sh_o, float4 &sh_d,
                                            const float *lb, float mix) {
-    const DevLight &Lj = a.lights[li];
+    const DevLight &Lj = S.lights[li];
     // Triangle::sample + Shape::sample(ref), triangle.cpp:134-154, shape.cpp:28-41
     float uu = sqrtf(ux);
     float u1 = 1.f - uu, u2 = uy * uu;
@@ -374,7 +402,7 @@ __device__ __forceinline__ bool nee_sample(const RenderArgs &a, const Bsdf &b, c
     for (int k = 0; k < 4; ++k) { f[k] = f[k] * cw; black = black && f[k] == 0.f; }
     if (black) return false;
     float w = power_heuristic(pdf_l, pb);
-    const float *te = a.spec + (size_t)Lj.em_slot * a.grid.stride;
+    const float *te = S.spec + (size_t)Lj.em_slot * a.grid.stride;
     for (int k = 0; k < 4; ++k) contrib[k] = beta[k] * f[k] * (spec_at(te, ln, k) * Lj.intensity) * w / pdf_l;
     v3 so = offset_point_origin(p, perr, ng, wun);                 // kindle_ray_to
     sh_o = make_float4(so.x, so.y, so.z, kShadowEps);
@@ -390,13 +418,14 @@ __device__ __forceinline__ bool nee_sample(const RenderArgs &a, const Bsdf &b, c
 // depend on the order; only the order of the next queue changes.
 constexpr uint32_t kSortKeys = 5;   // miss / no material, Lambert, conductor, dielectric, past the end
 
-__device__ __forceinline__ uint32_t shade_key(const RenderArgs &a, uint32_t i, uint32_t n) {
+__device__ __forceinline__ uint32_t shade_key(const RenderArgs &a, int cur, uint32_t i, uint32_t n) {
     if (i >= n) return kSortKeys - 1;
     const uint32_t rec = __float_as_uint(a.hits[i].y);
     if (rec == kNoHit) return 0;
-    const uint32_t mat = reinterpret_cast<const uint4 *>(a.sc.tris + rec)[1].w;
+    const SceneRef S = scene_of(a, item_of(a, a.rq_sid[cur][i]));
+    const uint32_t mat = reinterpret_cast<const uint4 *>(S.sc.tris + rec)[1].w;
     if (mat == kNoMaterial) return 0;
-    return (uint32_t)a.mats[mat].kind;   // MAT_LAMBERT 1, MAT_CONDUCTOR 2, MAT_DIELECTRIC 3
+    return (uint32_t)S.mats[mat].kind;   // MAT_LAMBERT 1, MAT_CONDUCTOR 2, MAT_DIELECTRIC 3
 }
 
 template <int WAVES>
@@ -407,12 +436,12 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WAV
     const int cur = b & 1, nxt = cur ^ 1;
     const uint32_t n = a.counts[C_EXT + cur];
     const uint32_t nround = (n + kBlock - 1) / kBlock * kBlock;
-    const int S = a.grid.stride;
+    const int GS = a.grid.stride;
     const bool sort = !(a.flags & FL_RENDER_NO_SORT);
     for (uint32_t base = blockIdx.x * kBlock; base < nround; base += stride) {
         uint32_t i = base + threadIdx.x;
         if (sort) {
-            const uint32_t key = shade_key(a, i, n);
+            const uint32_t key = shade_key(a, cur, i, n);
             if (threadIdx.x < kSortKeys) s_cnt[threadIdx.x] = 0u;
             __syncthreads();
             const uint32_t rank = atomicAdd(&s_cnt[key], 1u);
@@ -425,6 +454,7 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WAV
             __syncthreads();   // s_cnt and s_perm are rewritten by the next iteration
         }
         const bool active = i < n;
+        const SceneRef S = scene_of(a, active ? item_of(a, a.rq_sid[cur][i]) : 0u);
         // -------- phase 1: hit, emission, pass-through, material setup
         bool cont = false, mat_hit = false, rec_want = false;
         uint32_t rec_prev = 0;   // index + 1 of the previous vertex's training record
@@ -460,7 +490,7 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WAV
             const float t = hh.x;
             d = v3(rd.x, rd.y, rd.z);
             if (rec != kNoHit) {
-                const uint4 *tp = reinterpret_cast<const uint4 *>(a.sc.tris + rec);
+                const uint4 *tp = reinterpret_cast<const uint4 *>(S.sc.tris + rec);
                 uint4 q0 = tp[0], q1 = tp[1], q2 = tp[2];
                 v3 p0(__uint_as_float(q0.x), __uint_as_float(q0.y), __uint_as_float(q0.z));
                 v3 p1(__uint_as_float(q1.x), __uint_as_float(q1.y), __uint_as_float(q1.z));
@@ -471,7 +501,7 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WAV
                 ln = make_lanes(a.grid, lam0);
                 // emission: L += beta * le(-dir) at bounce 0 / after specular; MIS otherwise
                 if (light) {
-                    const DevLight &Lj = a.lights[light - 1];
+                    const DevLight &Lj = S.lights[light - 1];
                     if (dot(ng, -d) > 0.f) {
                         float w = 0.f;
                         if (bounce == 0 || (flags & PF_SPECULAR)) {
@@ -482,10 +512,10 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WAV
                             float cosl = fabsf(dot(ng, d)) / dl;
                             float pl = dist2 / (cosl * Lj.area);
                             if (isinf(pl)) pl = 0.f;
-                            if (!a.nee_all) pl = pl / (float)a.n_lights;
+                            if (!S.nee_all) pl = pl / (float)S.n_lights;
                             if (pl > 0.f) w = power_heuristic(prev_pdf, pl);
                         }
-                        const float *te = a.spec + (size_t)Lj.em_slot * S;
+                        const float *te = S.spec + (size_t)Lj.em_slot * GS;
                         if (w > 0.f)
                             for (int k = 0; k < 4; ++k) L[k] += beta[k] * (spec_at(te, ln, k) * Lj.intensity) * w;
                         // the record of the previous vertex learns the full incident
@@ -516,31 +546,31 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WAV
         Bsdf bs_;
         v3 fs, ft, woW, wo;
         if (mat_hit) {
-            const DevMaterial m = a.mats[mkind];
-            setup_bsdf(a, m, ln, bs_);
+            const DevMaterial m = S.mats[mkind];
+            setup_bsdf(a, S.spec, m, ln, bs_);
             eta_bsdf = m.eta_bsdf;
             coordinate_system(ng, &fs, &ft);                       // BSDF frame, bsdf.cpp:6-10 (ns = ng)
             fs = normalize(fs);
             woW = -d;
             wo = v3(dot(woW, fs), dot(woW, ft), dot(woW, ng));
-            rb = rng_base(a.seed, a.st_key[sid].x, a.st_key[sid].y, bounce);
+            rb = rng_base(S.seed, a.st_key[sid].x, a.st_key[sid].y, bounce);
         }
         const float *lb = (a.guided && mat_hit && guidable(bs_.kind)) ? a.lobes + (size_t)i * kGOut : nullptr;
-        const bool do_nee = mat_hit && !is_specular_kind(bs_.kind) && a.n_lights > 0;
+        const bool do_nee = mat_hit && !is_specular_kind(bs_.kind) && S.n_lights > 0;
 #pragma unroll
         for (int j = 0; j < 2; ++j) {
             float contrib[4] = {0.f, 0.f, 0.f, 0.f};
             float4 sh_o, sh_d;
             bool ok = false;
-            const int nslots = a.nee_all ? (int)a.n_lights : 1;
+            const int nslots = S.nee_all ? (int)S.n_lights : 1;
             if (do_nee && j < nslots) {
                 uint32_t li = (uint32_t)j;
                 float sel = 1.f;
-                if (!a.nee_all) {
-                    li = min((uint32_t)(rng_u(rb, 7) * (float)a.n_lights), a.n_lights - 1u);
-                    sel = 1.f / (float)a.n_lights;
+                if (!S.nee_all) {
+                    li = min((uint32_t)(rng_u(rb, 7) * (float)S.n_lights), S.n_lights - 1u);
+                    sel = 1.f / (float)S.n_lights;
                 }
-                ok = nee_sample(a, bs_, ln, li, sel, rng_u(rb, 3 + 2 * j), rng_u(rb, 4 + 2 * j), p, perr, ng, fs, ft,
+                ok = nee_sample(a, S, bs_, ln, li, sel, rng_u(rb, 3 + 2 * j), rng_u(rb, 4 + 2 * j), p, perr, ng, fs, ft,
                                 woW, wo, beta, contrib, sh_o, sh_d, lb, a.guide_mix);
             }
             if (active) {   // every live path rewrites both slots (0 when no sample)
@@ -662,7 +692,7 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WAV
                 float *R = a.recs + (size_t)ridx * 16;
                 float pos01[3], lam01, rough;
                 v3 n_in;
-                guide_features(a, p, ng, d, lam0, a.mats[mkind], pos01, n_in, lam01, rough);
+                guide_features(a, p, ng, d, lam0, S.mats[mkind], pos01, n_in, lam01, rough);
                 R[0] = pos01[0]; R[1] = pos01[1]; R[2] = pos01[2];
                 R[3] = n_in.x; R[4] = n_in.y; R[5] = n_in.z;
                 R[6] = lam01; R[7] = rough;
@@ -685,20 +715,24 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WAV
 }
 
 // -------------------------------------------------------------- resolve ----
+// One thread per padded pixel of all items; sums its samples of this pass in
+// sample order and adds them to the item's film.
 __global__ void __launch_bounds__(kBlock) k_resolve(RenderArgs a, const float *cmf) {
-    uint32_t pl = blockIdx.x * kBlock + threadIdx.x;
-    if (pl >= a.npix) return;
+    const uint32_t pg = blockIdx.x * kBlock + threadIdx.x;
+    if (pg >= a.total_pix) return;
+    const uint32_t item = a.n_items > 1 ? item_search(a, pg, [](const ItemDev &it) { return it.ppix_base; }) : 0u;
+    const ItemDev &it = a.items[item];
+    const uint32_t pl = pg - it.ppix_base;
     uint32_t tile = pl >> 6, within = pl & 63u;
-    uint32_t x = (tile % a.tiles_x) * 8u + (within & 7u);
-    uint32_t y = (tile / a.tiles_x) * 8u + (within >> 3);
-    if (x >= a.cam.W || y >= a.cam.H) return;
+    uint32_t x = (tile % it.tiles_x) * 8u + (within & 7u);
+    uint32_t y = (tile / it.tiles_x) * 8u + (within >> 3);
+    if (x >= it.cam.W || y >= it.cam.H) return;
     const int S = a.grid.stride;
     const float *cx = cmf, *cy = cmf + S, *cz = cmf + 2 * S;
     float sx = 0.f, sy = 0.f, sz = 0.f, qx = 0.f, qy = 0.f, qz = 0.f;
     uint32_t nan_count = 0;
-    for (uint32_t s = 0; s < a.spp_pass; ++s) {
-        uint32_t sid = s * a.npix + pl;
-        if (sid >= a.n_paths) break;
+    for (uint32_t s = 0; s < it.spp_pass; ++s) {
+        const uint32_t sid = it.path_base + s * it.npix + pl;
         float4 L4 = a.st_L[sid], p0 = a.pend[sid], p1 = a.pend[a.cap + sid], misc = a.st_misc[sid];
         float L[4] = {L4.x + p0.x + p1.x, L4.y + p0.y + p1.y, L4.z + p0.z + p1.z, L4.w + p0.w + p1.w};
         uint32_t flags = __float_as_uint(misc.w);
@@ -715,7 +749,7 @@ __global__ void __launch_bounds__(kBlock) k_resolve(RenderArgs a, const float *c
         sx += X; sy += Y; sz += Z;
         qx += X * X; qy += Y * Y; qz += Z * Z;
     }
-    uint32_t pix = y * a.cam.W + x;
+    const size_t pix = (size_t)it.pix_base + y * it.cam.W + x;
     a.film_sum[3 * pix + 0] += sx;
     a.film_sum[3 * pix + 1] += sy;
     a.film_sum[3 * pix + 2] += sz;
@@ -787,8 +821,13 @@ T *grow(void *&p, size_t &cap, size_t bytes, bool &err) {
 
 // ----------------------------------------------------------- host driver ---
 struct RenderState {
-    void *p[24] = {};
-    size_t cap[24] = {};
+    void *p[32] = {};
+    size_t cap[32] = {};
+    // pinned staging of the per-pass item table (two slots, each reused
+    // only after the copy that read it has run)
+    ItemDev *h_items[2] = {};
+    uint32_t h_items_cap = 0;
+    hipEvent_t item_ev[2] = {};
     uint32_t *h_counts = nullptr;     // pinned host copies of the queue counters, [2][16], then the record count
     hipEvent_t ev[2] = {};
     // bounce-loop graph (kChunk bounces), and the arguments it was captured with
@@ -831,8 +870,12 @@ void release_render_buffers(fl_ctx_t ctx) {
     RenderState *b = ctx->render_state;
     if (!b) return;
     drop_graph(*b);
-    for (int i = 0; i < 24; ++i)
+    for (int i = 0; i < 32; ++i)
         if (b->p[i]) (void)hipFree(b->p[i]);
+    for (int k = 0; k < 2; ++k) {
+        if (b->h_items[k]) (void)hipHostFree(b->h_items[k]);
+        if (b->item_ev[k]) (void)hipEventDestroy(b->item_ev[k]);
+    }
     if (b->h_counts) (void)hipHostFree(b->h_counts);
     for (auto &e : b->ev)
         if (e) (void)hipEventDestroy(e);
@@ -881,20 +924,31 @@ static CameraDev make_camera(const fl_camera *c) {
     return cam;
 }
 
-int render(fl_scene_s *s, const fl_camera *camera, const fl_render_params *p, fl_film *film, fl_stats *stats) {
-    fl_ctx_t ctx = s->ctx;
+int render_items(fl_ctx_t ctx, const HostItem *hi, uint32_t n_items, const fl_render_params *p, fl_stats *stats) {
     hipStream_t st = ctx->stream;
-    if (!camera || !p || camera->width == 0 || camera->height == 0 || p->spp == 0)
-        return set_error(ctx, FL_E_INVALID, "fl_render: bad camera/params");
+    if (!p || n_items == 0 || !hi) return set_error(ctx, FL_E_INVALID, "fl_render: bad params");
+    for (uint32_t k = 0; k < n_items; ++k) {
+        if (!hi[k].scene || hi[k].scene->ctx != ctx)
+            return set_error(ctx, FL_E_INVALID, "fl_render_batch: scene missing or of another context");
+        if (hi[k].cam.width == 0 || hi[k].cam.height == 0 || hi[k].spp == 0)
+            return set_error(ctx, FL_E_INVALID, "fl_render: bad camera or spp");
+        if ((uint64_t)hi[k].cam.width * hi[k].cam.height > (1u << 26))
+            return set_error(ctx, FL_E_INVALID, "fl_render: image larger than 2^26 pixels");
+    }
     fl_guide_s *guide = (p->flags & FL_RENDER_GUIDED) ? p->guide : nullptr;
     if ((p->flags & FL_RENDER_GUIDED) && !guide) return set_error(ctx, FL_E_INVALID, "FL_RENDER_GUIDED needs params.guide");
     if (guide && guide->ctx != ctx) return set_error(ctx, FL_E_INVALID, "guide belongs to another context");
+    for (uint32_t k = 1; guide && k < n_items; ++k)
+        if (hi[k].scene != hi[0].scene)
+            return set_error(ctx, FL_E_UNSUPPORTED, "guided batches render one scene (many cameras)");
     (void)hipGetLastError();  // do not report a stale error of an earlier call
     const int mode = p->spectral_mode == FL_SPECTRAL_PARITY ? 1 : 0;
+    fl_scene_s *s = hi[0].scene;
     const SpecTables &tab = s->spec[mode];
 
     RenderArgs a;
     memset(&a, 0, sizeof(a));
+    a.n_items = n_items;
     a.sc = TraceScene{s->d_nodes, s->d_tris};
     a.mats = s->d_mats;
     a.lights = s->d_lights;
@@ -902,15 +956,42 @@ int render(fl_scene_s *s, const fl_camera *camera, const fl_render_params *p, fl
     a.nee_all = s->n_lights <= 2 ? 1u : 0u;
     a.spec = tab.d_tab;
     a.grid = tab.grid;
-    a.cam = make_camera(camera);
-    const uint32_t W = camera->width, H = camera->height;
-    a.tiles_x = (W + 7) / 8;
-    const uint32_t tiles_y = (H + 7) / 8;
-    a.npix = a.tiles_x * tiles_y * 64u;
+    a.seed = p->seed;
+    // items: cameras, padded pixel layers, film and padded-pixel bases
+    std::vector<ItemDev> items(n_items);
+    uint64_t total_pix = 0, film_pix = 0, samples = 0;
+    uint32_t max_spp = 0, max_depth = 0;
+    for (uint32_t k = 0; k < n_items; ++k) {
+        ItemDev &it = items[k];
+        memset(&it, 0, sizeof(it));
+        fl_scene_s *sk = hi[k].scene;
+        it.sc = TraceScene{sk->d_nodes, sk->d_tris};
+        it.mats = sk->d_mats;
+        it.lights = sk->d_lights;
+        it.spec = sk->spec[mode].d_tab;
+        it.n_lights = sk->n_lights;
+        it.nee_all = sk->n_lights <= 2 ? 1u : 0u;
+        it.cam = make_camera(&hi[k].cam);
+        it.tiles_x = (hi[k].cam.width + 7) / 8;
+        it.npix = it.tiles_x * ((hi[k].cam.height + 7) / 8) * 64u;
+        it.ppix_base = (uint32_t)total_pix;
+        it.pix_base = (uint32_t)film_pix;
+        it.seed = p->seed + (uint64_t)k * 0x9e3779b97f4a7c15ull;   // item 0 keeps the render's seed
+        total_pix += it.npix;
+        film_pix += (uint64_t)hi[k].cam.width * hi[k].cam.height;
+        samples += (uint64_t)hi[k].cam.width * hi[k].cam.height * hi[k].spp;
+        max_spp = std::max(max_spp, hi[k].spp);
+        max_depth = std::max(max_depth, sk->depth);
+        if (sk->spec[mode].grid.stride != tab.grid.stride)
+            return set_error(ctx, FL_E_INVALID, "fl_render_batch: scenes with different spectral tables");
+    }
+    if (total_pix > (1u << 30)) return set_error(ctx, FL_E_INVALID, "fl_render_batch: too many pixels");
+    a.total_pix = (uint32_t)total_pix;
     // paths in flight per pass: every persistent launch ends in a tail where
     // lanes run out of rays, so passes are made as large as memory allows
     // (248 B of queues and state per path; 2^26 paths = 16.6 GB), capped at
-    // half the free device memory
+    // half the free device memory. Every item advances by the same number of
+    // samples per pixel per pass.
     uint32_t max_paths = p->max_paths;
     if (!max_paths && guide) max_paths = 1u << 24;   // guided passes also hold 160 B of lobes per path
     if (!max_paths) {
@@ -921,10 +1002,9 @@ int render(fl_scene_s *s, const fl_camera *camera, const fl_render_params *p, fl
             if (fit < max_paths) max_paths = (uint32_t)std::max<size_t>(fit, 1u << 20);
         }
     }
-    if (max_paths < a.npix) max_paths = a.npix;
-    uint32_t spp_pass = std::max(1u, std::min(p->spp, max_paths / a.npix));
-    a.cap = a.npix * spp_pass;
-    a.seed = p->seed;
+    if (max_paths < a.total_pix) max_paths = a.total_pix;
+    uint32_t spp_pass = std::max(1u, std::min(max_spp, max_paths / a.total_pix));
+    a.cap = a.total_pix * spp_pass;
     a.depth_d = p->diffuse_depth;
     a.depth_r = p->reflection_depth;
     a.depth_t = p->transmission_depth;
@@ -950,7 +1030,7 @@ int render(fl_scene_s *s, const fl_camera *camera, const fl_render_params *p, fl
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, tv->fn(), kBlock, 0) != hipSuccess || occ <= 0) occ = 4;
     const uint32_t grid_pers = (uint32_t)(ctx->n_cu * occ);
     a.trav_budget = p->trav_budget ? (int)p->trav_budget : 16;
-    size_t spill_per = (size_t)(s->depth > (uint32_t)tv->stack ? s->depth - tv->stack + 1 : 1);
+    size_t spill_per = (size_t)(max_depth > (uint32_t)tv->stack ? max_depth - tv->stack + 1 : 1);
     a.spill_stride = grid_pers * kBlock;
     a.st_beta = grow<float4>(B.p[0], B.cap[0], cap * 16, err);
     a.st_L = grow<float4>(B.p[1], B.cap[1], cap * 16, err);
@@ -969,8 +1049,21 @@ int render(fl_scene_s *s, const fl_camera *camera, const fl_render_params *p, fl
     a.sq_tgt = grow<uint32_t>(B.p[14], B.cap[14], 2 * cap * 4, err);
     a.counts = grow<uint32_t>(B.p[15], B.cap[15], 64, err);
     a.stats = grow<unsigned long long>(B.p[16], B.cap[16], 256, err);
-    a.film_sum = grow<float>(B.p[17], B.cap[17], (size_t)W * H * 12, err);
-    a.film_sq = grow<float>(B.p[18], B.cap[18], (size_t)W * H * 12, err);
+    a.film_sum = grow<float>(B.p[17], B.cap[17], (size_t)film_pix * 12, err);
+    a.film_sq = grow<float>(B.p[18], B.cap[18], (size_t)film_pix * 12, err);
+    a.items = grow<ItemDev>(B.p[23], B.cap[23], (size_t)n_items * sizeof(ItemDev), err);
+    if (n_items > 1) a.st_item = grow<uint32_t>(B.p[24], B.cap[24], cap * 4, err);
+    if (B.h_items_cap < n_items) {
+        for (int k = 0; k < 2; ++k) {
+            if (B.h_items[k]) (void)hipHostFree(B.h_items[k]);
+            B.h_items[k] = nullptr;
+            if (hipHostMalloc((void **)&B.h_items[k], (size_t)n_items * sizeof(ItemDev), hipHostMallocDefault) != hipSuccess)
+                err = true;
+        }
+        B.h_items_cap = err ? 0u : n_items;
+    }
+    for (int k = 0; k < 2 && !err; ++k)
+        if (!B.item_ev[k] && hipEventCreateWithFlags(&B.item_ev[k], hipEventDisableTiming) != hipSuccess) err = true;
     a.spill = grow<uint2>(B.p[19], B.cap[19], (size_t)a.spill_stride * spill_per * 8, err);
     if (err) return set_error(ctx, FL_E_OOM, "render buffers");
 
@@ -992,10 +1085,10 @@ int render(fl_scene_s *s, const fl_camera *camera, const fl_render_params *p, fl
         if (train) {
             const float fr = p->guide_train_fraction > 0.f && p->guide_train_fraction <= 1.f ? p->guide_train_fraction
                                                                                               : 0.25f;
-            train_spp = std::min(p->spp, std::max(1u, (uint32_t)std::lround(fr * (float)p->spp)));
+            train_spp = std::min(max_spp, std::max(1u, (uint32_t)std::lround(fr * (float)max_spp)));
             train_rounds = std::min(8u, train_spp);
             const uint32_t round_spp = (train_spp + train_rounds - 1) / train_rounds;
-            rec_cap = (uint32_t)std::min<size_t>((size_t)a.npix * std::min(round_spp, spp_pass) * 6, 1u << 24);
+            rec_cap = (uint32_t)std::min<size_t>((size_t)a.total_pix * std::min(round_spp, spp_pass) * 6, 1u << 24);
             st_rec = grow<uint32_t>(B.p[21], B.cap[21], cap * 4, err);
             recs = grow<float>(B.p[22], B.cap[22], (size_t)rec_cap * 16 * sizeof(float), err);
         }
@@ -1009,8 +1102,8 @@ int render(fl_scene_s *s, const fl_camera *camera, const fl_render_params *p, fl
     FL_HIP_CHECK(ctx, hipEventCreate(&e1));
     for (int k = 0; k < 3; ++k) FL_HIP_CHECK(ctx, hipEventCreate(&tev[k]));
     auto blocks = [](size_t n) { return (unsigned)((n + kBlock - 1) / kBlock); };
-    hipLaunchKernelGGL(k_zero_f32, dim3(blocks((size_t)W * H * 3)), dim3(kBlock), 0, st, a.film_sum, (size_t)W * H * 3);
-    hipLaunchKernelGGL(k_zero_f32, dim3(blocks((size_t)W * H * 3)), dim3(kBlock), 0, st, a.film_sq, (size_t)W * H * 3);
+    hipLaunchKernelGGL(k_zero_f32, dim3(blocks((size_t)film_pix * 3)), dim3(kBlock), 0, st, a.film_sum, (size_t)film_pix * 3);
+    hipLaunchKernelGGL(k_zero_f32, dim3(blocks((size_t)film_pix * 3)), dim3(kBlock), 0, st, a.film_sq, (size_t)film_pix * 3);
     hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, st, (uint32_t *)a.stats, 32u);
     FL_HIP_CHECK(ctx, hipEventRecord(e0, st));
 
@@ -1044,8 +1137,7 @@ int render(fl_scene_s *s, const fl_camera *camera, const fl_render_params *p, fl
     auto ensure_graph = [&]() -> int {
         RenderArgs key = a;
         key.n_paths = 0;
-        key.spp_pass = 0;
-        key.sample_base = 0;
+        key.n_cam_rays = 0;
         if (B.gexec && B.gvariant == variant && memcmp(&key, &B.gkey, sizeof(key)) == 0) return FL_OK;
         if (B.gexec) FL_HIP_CHECK(ctx, hipStreamSynchronize(st));   // no replay of the old graph still queued
         drop_graph(B);
@@ -1066,8 +1158,9 @@ int render(fl_scene_s *s, const fl_camera *camera, const fl_render_params *p, fl
 
     double ms_guide = 0.0;
     uint32_t round = 0;
-    while (done_spp < p->spp) {
-        uint32_t this_spp = std::min(spp_pass, p->spp - done_spp);
+    uint32_t item_slot = 0;
+    while (done_spp < max_spp) {
+        uint32_t this_spp = std::min(spp_pass, max_spp - done_spp);
         const bool training_pass = train && done_spp < train_spp && !guide->diverged;
         if (training_pass) {
             // spread the training samples over train_rounds passes
@@ -1079,16 +1172,35 @@ int render(fl_scene_s *s, const fl_camera *camera, const fl_render_params *p, fl
         a.st_rec = training_pass ? st_rec : nullptr;
         a.rec_count = training_pass ? a.counts + 8 : nullptr;
         a.rec_cap = training_pass ? rec_cap : 0u;
-        a.spp_pass = this_spp;
-        a.n_paths = a.npix * this_spp;
-        a.sample_base = p->sample_offset + done_spp;
+        // this pass's item table: every item advances by up to this_spp samples
+        {
+            ItemDev *hit = B.h_items[item_slot];
+            FL_HIP_CHECK(ctx, hipEventSynchronize(B.item_ev[item_slot]));   // its last copy has run
+            uint32_t path_base = 0, slot_base = 0;
+            for (uint32_t k = 0; k < n_items; ++k) {
+                ItemDev it = items[k];
+                it.spp_pass = hi[k].spp > done_spp ? std::min(this_spp, hi[k].spp - done_spp) : 0u;
+                it.sample_base = p->sample_offset + done_spp;
+                it.path_base = path_base;
+                it.slot_base = slot_base;
+                path_base += it.npix * it.spp_pass;
+                slot_base += it.cam.W * it.cam.H * it.spp_pass;
+                hit[k] = it;
+            }
+            a.n_paths = path_base;
+            a.n_cam_rays = slot_base;
+            FL_HIP_CHECK(ctx, hipMemcpyAsync(const_cast<ItemDev *>(a.items), hit, (size_t)n_items * sizeof(ItemDev),
+                                             hipMemcpyHostToDevice, st));
+            FL_HIP_CHECK(ctx, hipEventRecord(B.item_ev[item_slot], st));
+            item_slot ^= 1u;
+        }
         if (use_graph) {
             const int grc = ensure_graph();
             if (grc != FL_OK) return grc;
         }
         hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, st, a.counts, 16u);
         hipLaunchKernelGGL(k_generate, dim3(std::min<unsigned>(blocks(a.n_paths), ctx->n_cu * 16)), dim3(kBlock), 0, st, a);
-        uint32_t ne = W * H * this_spp, ns = 0;
+        uint32_t ne = a.n_cam_rays, ns = 0;
         uint32_t iter = 0;
         if (!timing) {
             uint32_t issued = 0, checked = 0;
@@ -1147,7 +1259,7 @@ int render(fl_scene_s *s, const fl_camera *camera, const fl_render_params *p, fl
             }
         }
         max_iter_seen = std::max(max_iter_seen, iter);
-        hipLaunchKernelGGL(k_resolve, dim3(blocks(a.npix)), dim3(kBlock), 0, st, a, a.spec);
+        hipLaunchKernelGGL(k_resolve, dim3(blocks(a.total_pix)), dim3(kBlock), 0, st, a, a.spec);
         if (training_pass) {
             // finish this pass's records, then train on them before the next pass
             const auto t0 = std::chrono::steady_clock::now();
@@ -1175,18 +1287,22 @@ int render(fl_scene_s *s, const fl_camera *camera, const fl_render_params *p, fl
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     for (int k = 0; k < 3; ++k) (void)hipEventDestroy(tev[k]);
-    if (film && film->xyz_sum) {
+    for (uint32_t k = 0; k < n_items; ++k) {
+        fl_film *film = hi[k].film;
+        if (!film || !film->xyz_sum) continue;
+        const uint32_t W = hi[k].cam.width, H = hi[k].cam.height;
         if (film->width != W || film->height != H) return set_error(ctx, FL_E_INVALID, "film size mismatch");
-        FL_HIP_CHECK(ctx, hipMemcpy(film->xyz_sum, a.film_sum, (size_t)W * H * 12, hipMemcpyDeviceToHost));
+        const size_t off = (size_t)items[k].pix_base * 3;
+        FL_HIP_CHECK(ctx, hipMemcpy(film->xyz_sum, a.film_sum + off, (size_t)W * H * 12, hipMemcpyDeviceToHost));
         if (film->xyz_sqsum)
-            FL_HIP_CHECK(ctx, hipMemcpy(film->xyz_sqsum, a.film_sq, (size_t)W * H * 12, hipMemcpyDeviceToHost));
+            FL_HIP_CHECK(ctx, hipMemcpy(film->xyz_sqsum, a.film_sq + off, (size_t)W * H * 12, hipMemcpyDeviceToHost));
     }
     if (stats) {
         unsigned long long h[10];
         FL_HIP_CHECK(ctx, hipMemcpy(h, a.stats, sizeof(h), hipMemcpyDeviceToHost));
         memset(stats, 0, sizeof(*stats));
         stats->ms_total = ms;
-        stats->samples = (uint64_t)W * H * p->spp;
+        stats->samples = samples;
         stats->rays_extension = h[0];
         stats->rays_shadow = h[1];
         stats->nan_samples = h[2];
@@ -1204,6 +1320,12 @@ int render(fl_scene_s *s, const fl_camera *camera, const fl_render_params *p, fl
         stats->stack_depth = (uint32_t)tv->stack;
     }
     return FL_OK;
+}
+
+int render(fl_scene_s *s, const fl_camera *camera, const fl_render_params *p, fl_film *film, fl_stats *stats) {
+    if (!camera || !p) return set_error(s->ctx, FL_E_INVALID, "fl_render: null camera/params");
+    HostItem h{s, *camera, film, p->spp};
+    return render_items(s->ctx, &h, 1, p, stats);
 }
 
 int bsdf_queries(fl_ctx_t ctx, const fl_bsdf_query *d_q, fl_bsdf_result *d_r, uint32_t n) {

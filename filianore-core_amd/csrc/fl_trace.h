@@ -225,15 +225,17 @@ __device__ __forceinline__ bool traverse(const TraceScene &sc, v3 o, v3 d, float
 }
 
 // Persistent traversal over n rays with per-lane refill.  Src provides
-//   void load(uint32_t i, v3 &o, v3 &d, float &tmin, float &tmax, bool &any) const
+//   void load(uint32_t i, v3 &o, v3 &d, float &tmin, float &tmax, bool &any,
+//             TraceScene &sc) const     (sc: the BVH this ray traverses)
 //   void store(uint32_t i, const TravState &s) const
 // Lane g first takes ray g (no atomic); later rays come from `fetch` (which
 // must be 0 at launch) offset by the grid size, one atomic per wave refill.
 // All lanes of the wave must call this (it uses wave-wide ballots).
 // `any` is per ray: the first hit ends it (emitters skipped when ANY_SKIP).
 template <bool COUNT, bool ANY_SKIP, class Src>
-__device__ __forceinline__ void trace_persistent(const TraceScene &sc, const Src &src, uint32_t n, uint32_t *fetch,
-                                                 const LaneStack &ls, int budget, TravCounters &cnt) {
+__device__ __forceinline__ void trace_persistent(const Src &src, uint32_t n, uint32_t *fetch, const LaneStack &ls,
+                                                 int budget, TravCounters &cnt) {
+    TraceScene sc{};
     TravState s;
     const uint32_t lane = __lane_id();
     const uint32_t nthreads = gridDim.x * blockDim.x;
@@ -245,7 +247,7 @@ __device__ __forceinline__ void trace_persistent(const TraceScene &sc, const Src
         ray = (int)first;
         v3 o, d;
         float tmin, tmax;
-        src.load(first, o, d, tmin, tmax, any);
+        src.load(first, o, d, tmin, tmax, any, sc);
         trav_init(s, o, d, tmin, tmax);
     } else {
         exhausted = true;
@@ -274,7 +276,7 @@ __device__ __forceinline__ void trace_persistent(const TraceScene &sc, const Src
                     ray = (int)idx;
                     v3 o, d;
                     float tmin, tmax;
-                    src.load(idx, o, d, tmin, tmax, any);
+                    src.load(idx, o, d, tmin, tmax, any, sc);
                     trav_init(s, o, d, tmin, tmax);
                 }
             }

@@ -17,8 +17,11 @@ struct ApiSrc {
     const fl_ray *rays;
     fl_hit *hits;
     bool any;
-    __device__ void load(uint32_t i, fl::v3 &o, fl::v3 &d, float &tmin, float &tmax, bool &is_any) const {
+    fl::TraceScene sc;
+    __device__ void load(uint32_t i, fl::v3 &o, fl::v3 &d, float &tmin, float &tmax, bool &is_any,
+                         fl::TraceScene &ray_sc) const {
         is_any = any;
+        ray_sc = sc;
         const float4 *rp = reinterpret_cast<const float4 *>(rays + i);
         float4 a = rp[0], b = rp[1];
         o = fl::v3(a.x, a.y, a.z);
@@ -51,7 +54,7 @@ __global__ void __launch_bounds__(kBlock) k_trace_api(fl::TraceScene sc, ApiSrc 
     ls.spill = spill + gtid;
     ls.spill_stride = gridDim.x * kBlock;
     fl::TravCounters c;
-    fl::trace_persistent<COUNT, SKIP>(sc, src, n, fetch, ls, kBudget, c);
+    fl::trace_persistent<COUNT, SKIP>(src, n, fetch, ls, kBudget, c);
     if (COUNT) {
         atomicAdd(&counters[0], (unsigned long long)c.nodes);
         atomicAdd(&counters[1], (unsigned long long)c.tris);
@@ -93,7 +96,7 @@ int trace_rays(fl_ctx_t ctx, fl_scene_s *s, const fl_ray *d_rays, fl_hit *d_hits
                unsigned long long *d_counters) {
     if (n == 0) return FL_OK;
     fl::TraceScene sc{s->d_nodes, s->d_tris};
-    ApiSrc src{d_rays, d_hits, (flags & FL_ANY_HIT) != 0};
+    ApiSrc src{d_rays, d_hits, (flags & FL_ANY_HIT) != 0, sc};
     const int depth = (int)s->depth;
     bool any = flags & FL_ANY_HIT, skip = flags & FL_SKIP_LIGHTS, cnt = (flags & FL_COUNTERS) && d_counters;
     int rc;

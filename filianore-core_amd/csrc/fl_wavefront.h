@@ -37,20 +37,48 @@ struct CameraDev {
     uint32_t W, H;
 };
 
-struct RenderArgs {
+// One render item: a (scene, camera, film) triple of a batch. fl_render is a
+// batch of one. Path ids, queue slots and film pixels of the items are laid
+// out item after item; the per-pass fields change between passes.
+struct ItemDev {
     TraceScene sc;
+    const DevMaterial *mats;
+    const DevLight *lights;
+    const float *spec;       // spectral tables of the render's mode
+    uint32_t n_lights, nee_all;
+    CameraDev cam;
+    uint32_t npix, tiles_x;  // padded pixels per sample layer (8x8 tiles), tiles per row
+    uint32_t spp_pass, sample_base;   // this pass: samples per pixel, index of the first
+    uint32_t path_base, slot_base;    // this pass: first path id, first camera-ray slot
+    uint32_t pix_base;                // first pixel of this item's film in the film arrays
+    uint32_t ppix_base;               // first padded pixel of this item (resolve's index space)
+    uint64_t seed;
+};
+
+// Scene data one path needs, from its item.
+struct SceneRef {
+    TraceScene sc;
+    const DevMaterial *mats;
+    const DevLight *lights;
+    const float *spec;
+    uint32_t n_lights, nee_all;
+    uint64_t seed;
+};
+
+struct RenderArgs {
+    const ItemDev *items;    // device table, n_items entries
+    uint32_t n_items;
+    uint32_t *st_item;       // per path: its item (batches only)
+    uint32_t total_pix;      // padded pixels of all items (one sample layer each)
+    uint32_t n_cam_rays;     // this pass: camera rays of all items (in-image pixels x samples)
+    TraceScene sc;           // item 0 (the whole render when n_items == 1)
     const DevMaterial *mats;
     const DevLight *lights;
     uint32_t n_lights;
     uint32_t nee_all;        // 1: one NEE sample per light (uniform_sample_all_lights)
     const float *spec;
     SpecGrid grid;
-    CameraDev cam;
-    uint32_t npix;           // padded pixels per sample layer (8x8 tiles)
-    uint32_t tiles_x;
-    uint32_t spp_pass;
-    uint32_t sample_base;
-    uint32_t n_paths;        // npix * spp_pass
+    uint32_t n_paths;        // this pass: path ids of all items
     uint32_t cap;            // path capacity (pend stride)
     uint64_t seed;
     int depth_d, depth_r, depth_t;
@@ -70,7 +98,7 @@ struct RenderArgs {
     uint32_t *sq_tgt;
     uint32_t *counts;        // [0],[1] ray queues, [2] shadow queue
     unsigned long long *stats; // [0] ext rays, [1] shadow rays, [2] nan samples, [3] nodes, [4] tris, [5] spills
-    // film
+    // films of all items, concatenated (pix_base of each item)
     float *film_sum, *film_sq;
     uint2 *spill;
     uint32_t spill_stride;
@@ -88,5 +116,47 @@ struct RenderArgs {
     uint32_t rec_cap;
     uint32_t *st_rec;        // per path: index + 1 of its last record, 0 = none
 };
+
+// The item of path `sid` (0 in single-item renders, read from the path
+// state in batches).
+__device__ __forceinline__ uint32_t item_of(const RenderArgs &a, uint32_t sid) {
+    return a.n_items > 1 ? a.st_item[sid] : 0u;
+}
+
+__device__ __forceinline__ SceneRef scene_of(const RenderArgs &a, uint32_t item) {
+    SceneRef r;
+    if (a.n_items > 1) {
+        const ItemDev &it = a.items[item];
+        r.sc = it.sc;
+        r.mats = it.mats;
+        r.lights = it.lights;
+        r.spec = it.spec;
+        r.n_lights = it.n_lights;
+        r.nee_all = it.nee_all;
+        r.seed = it.seed;
+    } else {
+        r.sc = a.sc;
+        r.mats = a.mats;
+        r.lights = a.lights;
+        r.spec = a.spec;
+        r.n_lights = a.n_lights;
+        r.nee_all = a.nee_all;
+        r.seed = a.seed;
+    }
+    return r;
+}
+
+// Item owning a position in a concatenated range (path ids or padded
+// pixels): the last item whose base is <= v. Bases ascend with the item index.
+template <typename F>
+__device__ __forceinline__ uint32_t item_search(const RenderArgs &a, uint32_t v, F base) {
+    uint32_t lo = 0, hi = a.n_items;   // invariant: base(lo) <= v, answer in [lo, hi)
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) / 2;
+        if (base(a.items[mid]) <= v) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
 
 } // namespace fl

@@ -317,6 +317,28 @@ class Context:
     def scene(self, gen):
         return Scene(self, gen)
 
+    def render_batch(self, entries, params, want_film=True):
+        """entries: [(Scene, Camera or None, spp or 0)]; one batched render.
+        Returns ([(sum, sq)] per entry, Stats)."""
+        n = len(entries)
+        items = (BatchItem * n)()
+        keep = []
+        for k, (sc, cam, spp) in enumerate(entries):
+            cam = cam if cam is not None else sc.gen.camera
+            items[k].scene = sc.h
+            items[k].camera = cam
+            items[k].spp = spp
+            if want_film:
+                W, H = cam.width, cam.height
+                s = np.zeros((H, W, 3), np.float32)
+                q = np.zeros((H, W, 3), np.float32)
+                f = Film(s.ctypes.data_as(C.POINTER(C.c_float)), q.ctypes.data_as(C.POINTER(C.c_float)), W, H)
+                keep.append((s, q, f))
+                items[k].film = C.pointer(f)
+        st = Stats()
+        _check(self.L.fl_render_batch(self.h, items, n, C.byref(params), C.byref(st)), self.h, "fl_render_batch")
+        return [(s, q) for s, q, _ in keep], st
+
     def guide(self, seed=1, lr=0.0, batch=0, scene_min=(0, 0, 0), scene_max=(1, 1, 1)):
         return Guide(self, seed=seed, lr=lr, batch=batch, scene_min=scene_min, scene_max=scene_max)
 
@@ -401,6 +423,7 @@ class Scene:
 
 
 GUIDE_PARAMS = 5 * (64 * 64 + 64)
+ITEM_SEED_STRIDE = 0x9E3779B97F4A7C15   # item k of a batch renders with seed + k * stride (mod 2^64)
 
 
 class Guide:
