@@ -326,7 +326,6 @@ int fl_bsdf_query_batch(fl_ctx_t ctx, const fl_bsdf_query *q, fl_bsdf_result *r,
     return rc;
 }
 
-// Not yet available entry points fail loudly.
 int fl_render_batch(fl_ctx_t ctx, const fl_batch_item *items, uint32_t n, const fl_render_params *params,
                     fl_stats *stats) {
     if (!ctx) return FL_E_INVALID;
@@ -337,8 +336,23 @@ int fl_render_batch(fl_ctx_t ctx, const fl_batch_item *items, uint32_t n, const 
     (void)hipSetDevice(ctx->device);
     return flx::render_items(ctx, hi.data(), n, params, stats);
 }
-int fl_autotune(fl_scene_t s, const fl_camera *, const fl_render_params *, fl_tune_result *) {
-    return set_error(s ? s->ctx : nullptr, FL_E_UNSUPPORTED, "fl_autotune: not built");
+int fl_autotune(fl_scene_t s, const fl_camera *camera, const fl_render_params *params, fl_tune_result *out) {
+    if (!s) return FL_E_INVALID;
+    if (!camera || !params) return set_error(s->ctx, FL_E_INVALID, "fl_autotune: null argument");
+    (void)hipSetDevice(s->ctx->device);
+    flx::HostItem h{s, *camera, nullptr, params->spp};
+    return flx::autotune_items(s->ctx, &h, 1, params, out);
+}
+
+int fl_autotune_batch(fl_ctx_t ctx, const fl_batch_item *items, uint32_t n, const fl_render_params *params,
+                      fl_tune_result *out) {
+    if (!ctx) return FL_E_INVALID;
+    if (!items || n == 0 || !params) return set_error(ctx, FL_E_INVALID, "fl_autotune_batch: no items or params");
+    std::vector<flx::HostItem> hi(n);
+    for (uint32_t k = 0; k < n; ++k)
+        hi[k] = flx::HostItem{items[k].scene, items[k].camera, nullptr, items[k].spp ? items[k].spp : params->spp};
+    (void)hipSetDevice(ctx->device);
+    return flx::autotune_items(ctx, hi.data(), n, params, out);
 }
 
 } // extern "C"

@@ -127,6 +127,7 @@ struct HostItem {
     uint32_t spp;
 };
 int render_items(fl_ctx_t ctx, const HostItem *items, uint32_t n, const fl_render_params *p, fl_stats *stats);
+int autotune_items(fl_ctx_t ctx, const HostItem *items, uint32_t n, const fl_render_params *p, fl_tune_result *out);
 
 // Online training of the guide on render records (device, 16 floats each:
 // position in the unit box 3, normal 3, wavelength 1, roughness 1, direction

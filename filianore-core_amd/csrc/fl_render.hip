@@ -1328,6 +1328,80 @@ int render(fl_scene_s *s, const fl_camera *camera, const fl_render_params *p, fl
     return render_items(s->ctx, &h, 1, p, stats);
 }
 
+// Coordinate descent over the render knobs with probe renders of the same
+// items (films not downloaded, guide not trained). Each probe runs twice;
+// the faster device time counts, so one-time costs (allocation, graph
+// capture) do not decide.
+int autotune_items(fl_ctx_t ctx, const HostItem *hi, uint32_t n, const fl_render_params *p0, fl_tune_result *out) {
+    if (!p0 || !out || !hi || n == 0) return set_error(ctx, FL_E_INVALID, "fl_autotune: null argument");
+    std::vector<HostItem> items(hi, hi + n);
+    for (HostItem &h : items) {
+        h.film = nullptr;
+        h.spp = std::min(h.spp, 64u);
+    }
+    fl_render_params base = *p0;
+    base.flags &= ~(uint32_t)(FL_RENDER_TIMING | FL_RENDER_COUNTERS | FL_RENDER_TRAIN_GUIDE);
+    base.flags |= FL_RENDER_GRAPH;
+    uint32_t tried = 0;
+    auto probe = [&](const fl_render_params &q, double *ms) -> int {
+        double best = 1e300;
+        for (int r = 0; r < 2; ++r) {
+            fl_stats st;
+            const int rc = render_items(ctx, items.data(), n, &q, &st);
+            if (rc != FL_OK) return rc;
+            best = std::min(best, st.ms_total);
+        }
+        ++tried;
+        *ms = best;
+        return FL_OK;
+    };
+    double ms_default = 0.0;
+    int rc = probe(base, &ms_default);
+    if (rc != FL_OK) return rc;
+    fl_render_params cur = base;
+    double ms_cur = ms_default;
+    auto try_values = [&](auto set, const std::vector<uint32_t> &vals) -> int {
+        fl_render_params keep = cur;
+        for (uint32_t v : vals) {
+            fl_render_params q = keep;
+            set(q, v);
+            double ms = 0.0;
+            const int r = probe(q, &ms);
+            if (r == FL_E_INVALID) { (void)hipGetLastError(); continue; }   // value not valid here
+            if (r != FL_OK) return r;
+            if (ms < ms_cur) { ms_cur = ms; cur = q; }
+        }
+        return FL_OK;
+    };
+    // paths in flight per pass (the largest fitting is the default)
+    rc = try_values([](fl_render_params &q, uint32_t v) { q.max_paths = v; }, {1u << 22, 1u << 24});
+    if (rc == FL_OK)   // traversal variant: (stack, waves) pairs of kTraceVariants
+        for (const TraceVariant &v : kTraceVariants) {
+            if (&v == &kTraceVariants[0]) continue;
+            const uint32_t packed = (uint32_t)v.stack | ((uint32_t)v.waves << 16);
+            rc = try_values([](fl_render_params &q, uint32_t pk) { q.stack_depth = pk & 0xffffu; q.trav_waves = pk >> 16; },
+                            {packed});
+            if (rc != FL_OK) break;
+        }
+    if (rc == FL_OK) rc = try_values([](fl_render_params &q, uint32_t v) { q.trav_budget = v; }, {8u, 32u});
+    if (rc == FL_OK)
+        rc = try_values([](fl_render_params &q, uint32_t v) { q.flags = v ? (q.flags | FL_RENDER_NO_SORT) : q.flags; },
+                        {1u});
+    if (rc == FL_OK) rc = try_values([](fl_render_params &q, uint32_t v) { q.shade_waves = v; }, {4u});
+    if (rc != FL_OK) return rc;
+    memset(out, 0, sizeof(*out));
+    out->max_paths = cur.max_paths;
+    out->stack_depth = cur.stack_depth ? cur.stack_depth : (uint32_t)kTraceVariants[0].stack;
+    out->trav_waves = cur.trav_waves;
+    out->trav_budget = cur.trav_budget ? cur.trav_budget : 16u;
+    out->shade_waves = cur.shade_waves;
+    out->sort_shading = (cur.flags & FL_RENDER_NO_SORT) ? 0u : 1u;
+    out->ms_best = ms_cur;
+    out->ms_default = ms_default;
+    out->n_tried = tried;
+    return FL_OK;
+}
+
 int bsdf_queries(fl_ctx_t ctx, const fl_bsdf_query *d_q, fl_bsdf_result *d_r, uint32_t n) {
     if (n == 0) return FL_OK;
     hipLaunchKernelGGL(k_bsdf_query, dim3((n + 255) / 256), dim3(256), 0, ctx->stream, d_q, d_r, n);

@@ -97,9 +97,19 @@ class BatchItem(C.Structure):
 
 
 class TuneResult(C.Structure):
-    _fields_ = [("max_paths", C.c_uint32), ("stack_depth", C.c_uint32), ("block_size", C.c_uint32),
-                ("sort_shading", C.c_uint32), ("ms_best", C.c_double), ("ms_default", C.c_double),
-                ("n_tried", C.c_uint32)]
+    _fields_ = [("max_paths", C.c_uint32), ("stack_depth", C.c_uint32), ("trav_waves", C.c_uint32),
+                ("trav_budget", C.c_uint32), ("shade_waves", C.c_uint32), ("sort_shading", C.c_uint32),
+                ("ms_best", C.c_double), ("ms_default", C.c_double), ("n_tried", C.c_uint32), ("pad", C.c_uint32)]
+
+    def apply(self, params):
+        """Copies the tuned values into fl_render_params."""
+        params.max_paths, params.stack_depth, params.trav_waves = self.max_paths, self.stack_depth, self.trav_waves
+        params.trav_budget, params.shade_waves = self.trav_budget, self.shade_waves
+        if self.sort_shading:
+            params.flags &= ~FL_RENDER_NO_SORT
+        else:
+            params.flags |= FL_RENDER_NO_SORT
+        return params
 
 
 class GuideConfig(C.Structure):
@@ -139,7 +149,7 @@ class GenScene(C.Structure):
 EXPORTS = [
     "fl_abi_version", "fl_device_count", "fl_probe_copy", "fl_ctx_create", "fl_ctx_destroy", "fl_last_error",
     "fl_scene_create", "fl_scene_destroy", "fl_scene_get_info", "fl_scene_download_bvh",
-    "fl_intersect", "fl_intersect_device", "fl_render", "fl_render_batch", "fl_autotune",
+    "fl_intersect", "fl_intersect_device", "fl_render", "fl_render_batch", "fl_autotune", "fl_autotune_batch",
     "fl_guide_create", "fl_guide_destroy", "fl_guide_train_step", "fl_guide_forward",
     "fl_guide_get_params", "fl_guide_set_params", "fl_guide_gradient", "fl_guide_stats",
     "fl_bsdf_query_batch",
@@ -171,6 +181,8 @@ def lib():
         L.fl_render_batch.argtypes = [vp, C.POINTER(BatchItem), C.c_uint32, C.POINTER(RenderParams),
                                       C.POINTER(Stats)]
         L.fl_autotune.argtypes = [vp, C.POINTER(Camera), C.POINTER(RenderParams), C.POINTER(TuneResult)]
+        L.fl_autotune_batch.argtypes = [vp, C.POINTER(BatchItem), C.c_uint32, C.POINTER(RenderParams),
+                                        C.POINTER(TuneResult)]
         L.fl_guide_create.argtypes = [vp, C.POINTER(GuideConfig), C.POINTER(vp)]
         L.fl_guide_destroy.argtypes = [vp]
         L.fl_guide_train_step.argtypes = [vp, vp, C.c_uint32, C.c_int, C.POINTER(C.c_float)]
@@ -316,6 +328,12 @@ class Context:
 
     def scene(self, gen):
         return Scene(self, gen)
+
+    def autotune(self, scene, params, camera=None):
+        r = TuneResult()
+        cam = camera if camera is not None else scene.gen.camera
+        _check(self.L.fl_autotune(scene.h, C.byref(cam), C.byref(params), C.byref(r)), self.h, "fl_autotune")
+        return r
 
     def render_batch(self, entries, params, want_film=True):
         """entries: [(Scene, Camera or None, spp or 0)]; one batched render.

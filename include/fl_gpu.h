@@ -233,14 +233,25 @@ int fl_render_batch(fl_ctx_t ctx, const fl_batch_item *items, uint32_t n,
                     const fl_render_params *params, fl_stats *stats);
 
 /* ---- autotuner -------------------------------------------------------------- */
+/* Probe renders of the same job (the given spp, at most 64) choose, one knob
+ * after the other, the values of fl_render_params that minimise device time:
+ * paths in flight per pass (queue chunk), traversal variant (LDS stack depth x
+ * minimum waves per SIMD), traversal steps between lane refills, shading sort
+ * on/off, shading-kernel occupancy. Node width is not a knob: nodes are
+ * 8-wide by construction (one 128-byte line). No counterpart in the
+ * reference. */
 typedef struct {
-    uint32_t max_paths, stack_depth, block_size, sort_shading;
-    double ms_best, ms_default;
-    uint32_t n_tried;
+    uint32_t max_paths, stack_depth, trav_waves, trav_budget, shade_waves;
+    uint32_t sort_shading;        /* 1: per-block material sort on (FL_RENDER_NO_SORT clear) */
+    double ms_best, ms_default;   /* device ms of one render with the chosen and the default values */
+    uint32_t n_tried;             /* probe configurations measured */
+    uint32_t pad;
 } fl_tune_result;
 
 int fl_autotune(fl_scene_t scene, const fl_camera *camera, const fl_render_params *params,
                 fl_tune_result *out);
+int fl_autotune_batch(fl_ctx_t ctx, const fl_batch_item *items, uint32_t n, const fl_render_params *params,
+                      fl_tune_result *out);
 
 /* ---- learned proposal (vMF mixture MLP) ----------------------------------- */
 typedef struct {
