@@ -76,6 +76,123 @@ def cpu_baseline(gen, spp, depth, budget_s, threads):
     return done_samples / secs / 1e6, done_samples, secs
 
 
+MFMA_BF16_PEAK_TFLOPS = 2500.0
+GUIDE_FWD_FLOP = 2 * 18240          # algorithmic FLOP per network query (53*64 + 3*64*64 + 64*40 MAC)
+GUIDE_TRAIN_FLOP = 3 * GUIDE_FWD_FLOP
+
+
+def rel_mse(img, ref):
+    import numpy as np
+    return float(np.mean((img - ref) ** 2 / (ref ** 2 + 0.01)))
+
+
+def batch_section(flgpu, ctx, n_scenes, tune):
+    """64 scenes (log-uniform 1e3..2e6 triangles, 128^2..1024^2, 16 spp, depth
+    12) in one fl_render_batch, against the same renders one by one."""
+    t0 = time.time()
+    gens = [flgpu.GenSceneData(flgpu.FL_GEN_BATCH, scene_index=k) for k in range(n_scenes)]
+    t_gen = time.time() - t0
+    scenes = [ctx.scene(g) for g in gens]
+    try:
+        tris = sum(g.n_triangles for g in gens)
+        samples = sum(g.camera.width * g.camera.height * g.spp for g in gens)
+        p = flgpu.default_params(spp=16, depth=12, seed=1, flags=flgpu.FL_RENDER_VNDF_PDF | flgpu.FL_RENDER_GRAPH)
+        entries = [(sc, None, g.spp) for sc, g in zip(scenes, gens)]
+        tuned = None
+        if tune:
+            n = len(entries)
+            items = (flgpu.BatchItem * n)()
+            for k, (sc, _, spp) in enumerate(entries):
+                items[k].scene, items[k].camera, items[k].spp = sc.h, sc.gen.camera, spp
+            r = flgpu.TuneResult()
+            flgpu._check(ctx.L.fl_autotune_batch(ctx.h, items, n, flgpu.C.byref(p), flgpu.C.byref(r)), ctx.h,
+                         "fl_autotune_batch")
+            r.apply(p)
+            tuned = {"max_paths": r.max_paths, "stack_depth": r.stack_depth, "trav_waves": r.trav_waves,
+                     "trav_budget": r.trav_budget, "shade_waves": r.shade_waves, "sort_shading": r.sort_shading,
+                     "ms_default": round(r.ms_default, 2), "ms_tuned": round(r.ms_best, 2), "probes": r.n_tried}
+        ctx.render_batch(entries, p, want_film=False)
+        ts = time.perf_counter()
+        _, st = ctx.render_batch(entries, p, want_film=False)
+        t_batch = time.perf_counter() - ts
+        t_seq = 0.0
+        for sc, g in zip(scenes, gens):
+            q = flgpu.default_params(spp=g.spp, depth=12, seed=1, flags=p.flags)
+            sc.render(q, want_film=False)
+            ts = time.perf_counter()
+            sc.render(q, want_film=False)
+            t_seq += time.perf_counter() - ts
+    finally:
+        for sc in scenes:
+            sc.close()
+    return {"scenes": n_scenes, "triangles": tris, "samples": samples, "gen_s": round(t_gen, 1),
+            "ms_batch": round(t_batch * 1e3, 2), "msamples_s": round(samples / t_batch / 1e6, 1),
+            "ms_one_by_one": round(t_seq * 1e3, 2), "batch_speedup": round(t_seq / t_batch, 2),
+            "passes": st.passes, "autotune": tuned}
+
+
+def guided_section(flgpu, ctx, res, ref_spp):
+    """Room lit only through a ceiling slot (light in a closed attic): plain
+    vs guided (trained online on the first 25% of the samples) at equal wall
+    time, relMSE against a plain reference of ref_spp samples per pixel."""
+    import numpy as np
+    gen = flgpu.GenSceneData(flgpu.FL_GEN_GUIDED, width=res, height=res)
+    base = flgpu.FL_RENDER_VNDF_PDF | flgpu.FL_RENDER_GRAPH
+    out = {"resolution": res, "triangles": gen.n_triangles, "reference_spp": ref_spp}
+    with ctx.scene(gen) as sc:
+        p = flgpu.default_params(spp=ref_spp, depth=gen.depth, seed=1234, flags=base)
+        ts = time.perf_counter()
+        ref, _, _ = sc.render(p)
+        ref = ref / ref_spp
+        out["reference_s"] = round(time.perf_counter() - ts, 2)
+        plain = []
+        for spp in (64, 128, 256):
+            p = flgpu.default_params(spp=spp, depth=gen.depth, seed=7, flags=base)
+            sc.render(p, want_film=False)
+            ts = time.perf_counter()
+            img, _, _ = sc.render(p)
+            plain.append((spp, time.perf_counter() - ts, rel_mse(img / spp, ref)))
+        guided = []
+        mfma = None
+        for spp in (32, 64, 128, 256):
+            with ctx.guide(seed=1) as g:
+                p = flgpu.default_params(spp=spp, depth=gen.depth, seed=7,
+                                         flags=base | flgpu.FL_RENDER_GUIDED | flgpu.FL_RENDER_TRAIN_GUIDE)
+                p.guide = g.h
+                ts = time.perf_counter()
+                img, _, st = sc.render(p)
+                guided.append((spp, time.perf_counter() - ts, rel_mse(img / spp, ref), st.ms_guide))
+                if spp == 64:
+                    steps = g.stats()[0]
+                    tp = flgpu.default_params(spp=spp, depth=gen.depth, seed=8,
+                                              flags=flgpu.FL_RENDER_VNDF_PDF | flgpu.FL_RENDER_GUIDED | flgpu.FL_RENDER_TIMING)
+                    tp.guide = g.h
+                    _, _, tst = sc.render(tp, want_film=False)
+                    infer_tf = tst.guide_queries * GUIDE_FWD_FLOP / (tst.ms_infer * 1e-3) / 1e12
+                    train_tf = steps * 16384 * GUIDE_TRAIN_FLOP / (st.ms_guide * 1e-3) / 1e12
+                    mfma = {"infer_queries": int(tst.guide_queries), "infer_ms": round(tst.ms_infer, 2),
+                            "infer_tflops": round(infer_tf, 2), "infer_mfma_util": round(infer_tf / MFMA_BF16_PEAK_TFLOPS, 5),
+                            "train_steps": steps, "train_ms": round(st.ms_guide, 1),
+                            "train_tflops": round(train_tf, 2), "train_mfma_util": round(train_tf / MFMA_BF16_PEAK_TFLOPS, 5)}
+    out["plain"] = [{"spp": s, "s": round(t, 4), "relmse": round(e, 5)} for s, t, e in plain]
+    out["guided"] = [{"spp": s, "s": round(t, 4), "relmse": round(e, 5), "train_ms": round(tm, 1)} for s, t, e, tm in guided]
+    # wall time the guided render needs to reach the plain 64-spp relMSE (log-log
+    # interpolation between guided points), over the plain render's time
+    target_t, target_e = plain[0][1], plain[0][2]
+    need = None
+    pts = sorted((t, e) for _, t, e, _ in guided)
+    for (t0, e0), (t1, e1) in zip(pts, pts[1:]):
+        if e0 >= target_e >= e1 and e0 > 0 and e1 > 0 and e0 != e1:
+            f = (np.log(target_e) - np.log(e0)) / (np.log(e1) - np.log(e0))
+            need = float(np.exp(np.log(t0) + f * (np.log(t1) - np.log(t0))))
+            break
+    if need is None and pts and pts[0][1] <= target_e:
+        need = pts[0][0]
+    out["time_ratio_to_plain_relmse"] = round(need / target_t, 3) if need else None
+    out["mfma"] = mfma
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -86,6 +203,11 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-cornell", action="store_true", help="skip the small Cornell section")
+    ap.add_argument("--no-batch", action="store_true", help="skip the 64-scene batch section")
+    ap.add_argument("--no-guided", action="store_true", help="skip the guided (learned proposal) section")
+    ap.add_argument("--batch-scenes", type=int, default=64)
+    ap.add_argument("--guided-res", type=int, default=256)
+    ap.add_argument("--guided-ref-spp", type=int, default=16384)
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run a plain 1 GiB -> 1 GiB copy kernel (HBM counter calibration)")
     args = ap.parse_args()
@@ -153,6 +275,15 @@ def main():
             out["rays_per_sample"] = round(rays / samples, 3)
             out["bvh_build_ms"] = round(info.build_ms, 2)
             out["nan_samples"] = int(st.nan_samples)
+
+        if not args.no_batch:
+            log("batch section: %d scenes" % args.batch_scenes)
+            out["batch"] = batch_section(flgpu, ctx, args.batch_scenes, tune=True)
+            log("batch: %s" % out["batch"])
+        if not args.no_guided:
+            log("guided section: %d^2, reference %d spp" % (args.guided_res, args.guided_ref_spp))
+            out["guided"] = guided_section(flgpu, ctx, args.guided_res, args.guided_ref_spp)
+            log("guided: %s" % out["guided"])
 
         if args.calibrate_copy:
             # plain streaming kernel (k_copy_probe), 1 GiB read + 1 GiB written

@@ -304,6 +304,8 @@ __global__ void __launch_bounds__(64) k_guide_infer(RenderArgs a, int b) {
         }
         if (!want)
             for (int k = 0; k < 64; ++k) x[k] = 0.f;
+        const unsigned long long wm = __ballot(want);
+        if (q == 0 && wm) atomicAdd(&a.stats[6], (unsigned long long)__popcll(wm));
         for (int k = 0; k < 64; ++k) sH[0][q * kGRow + k] = to_bf16(x[k]);
         __syncthreads();
         for (int k = 0; k < 4; ++k) {
@@ -1097,10 +1099,10 @@ int render_items(fl_ctx_t ctx, const HostItem *hi, uint32_t n_items, const fl_re
 
     const bool counters = (p->flags & FL_RENDER_COUNTERS) != 0;
     const bool timing = (p->flags & FL_RENDER_TIMING) != 0;
-    hipEvent_t e0, e1, tev[3];
+    hipEvent_t e0, e1, tev[4];
     FL_HIP_CHECK(ctx, hipEventCreate(&e0));
     FL_HIP_CHECK(ctx, hipEventCreate(&e1));
-    for (int k = 0; k < 3; ++k) FL_HIP_CHECK(ctx, hipEventCreate(&tev[k]));
+    for (int k = 0; k < 4; ++k) FL_HIP_CHECK(ctx, hipEventCreate(&tev[k]));
     auto blocks = [](size_t n) { return (unsigned)((n + kBlock - 1) / kBlock); };
     hipLaunchKernelGGL(k_zero_f32, dim3(blocks((size_t)film_pix * 3)), dim3(kBlock), 0, st, a.film_sum, (size_t)film_pix * 3);
     hipLaunchKernelGGL(k_zero_f32, dim3(blocks((size_t)film_pix * 3)), dim3(kBlock), 0, st, a.film_sq, (size_t)film_pix * 3);
@@ -1108,7 +1110,7 @@ int render_items(fl_ctx_t ctx, const HostItem *hi, uint32_t n_items, const fl_re
     FL_HIP_CHECK(ctx, hipEventRecord(e0, st));
 
     uint32_t done_spp = 0, passes = 0, max_iter_seen = 0, n_trace = 0, n_shade = 0;
-    double ms_trace = 0.0, ms_shade = 0.0;
+    double ms_trace = 0.0, ms_shade = 0.0, ms_infer = 0.0;
     const uint32_t max_iter = (uint32_t)std::max({p->diffuse_depth, p->reflection_depth, p->transmission_depth}) * 2 + 8;
     const uint32_t shade_grid_max = (uint32_t)ctx->n_cu * 8;
     uint32_t host_counts[16];
@@ -1121,8 +1123,8 @@ int render_items(fl_ctx_t ctx, const HostItem *hi, uint32_t n_items, const fl_re
     constexpr uint32_t kChunk = 4;   // even: a chunk starts and ends on queue parity 0
     const int variant = (counters ? 1 : 0) | (p->shade_waves >= 4 ? 2 : 0) | ((int)(tv - kTraceVariants) << 2);
     const uint32_t infer_grid = (uint32_t)ctx->n_cu * 4;
-    auto launch_shade = [&](const RenderArgs &args, dim3 g, int j) {
-        if (args.guided) hipLaunchKernelGGL(k_guide_infer, dim3(infer_grid), dim3(64), 0, st, args, j);
+    auto launch_shade = [&](const RenderArgs &args, dim3 g, int j, bool with_infer = true) {
+        if (with_infer && args.guided) hipLaunchKernelGGL(k_guide_infer, dim3(infer_grid), dim3(64), 0, st, args, j);
         if (variant & 2) hipLaunchKernelGGL(k_shade<4>, g, dim3(kBlock), 0, st, args, j);
         else hipLaunchKernelGGL(k_shade<2>, g, dim3(kBlock), 0, st, args, j);
     };
@@ -1237,17 +1239,24 @@ int render_items(fl_ctx_t ctx, const HostItem *hi, uint32_t n_items, const fl_re
             ++n_trace;
             if (timing) FL_HIP_CHECK(ctx, hipEventRecord(tev[1], st));
             if (ne > 0) {
-                launch_shade(a, dim3(std::min<uint32_t>(shade_grid_max, blocks(ne))), (int)iter);
+                // guide inference and shading, timed apart
+                if (a.guided) hipLaunchKernelGGL(k_guide_infer, dim3(infer_grid), dim3(64), 0, st, a, (int)iter);
+                if (timing) FL_HIP_CHECK(ctx, hipEventRecord(tev[3], st));
+                launch_shade(a, dim3(std::min<uint32_t>(shade_grid_max, blocks(ne))), (int)iter, false);
                 ++n_shade;
+            } else if (timing) {
+                FL_HIP_CHECK(ctx, hipEventRecord(tev[3], st));
             }
             if (timing) FL_HIP_CHECK(ctx, hipEventRecord(tev[2], st));
             FL_HIP_CHECK(ctx, hipMemcpyAsync(host_counts, a.counts, 64, hipMemcpyDeviceToHost, st));
             FL_HIP_CHECK(ctx, hipStreamSynchronize(st));
             if (timing) {
-                float m0 = 0.f, m1 = 0.f;
+                float m0 = 0.f, m1 = 0.f, m2 = 0.f;
                 (void)hipEventElapsedTime(&m0, tev[0], tev[1]);
-                (void)hipEventElapsedTime(&m1, tev[1], tev[2]);
+                (void)hipEventElapsedTime(&m2, tev[1], tev[3]);
+                (void)hipEventElapsedTime(&m1, tev[3], tev[2]);
                 ms_trace += m0;
+                ms_infer += m2;
                 ms_shade += m1;
             }
             if (ne > 0) {
@@ -1286,7 +1295,7 @@ int render_items(fl_ctx_t ctx, const HostItem *hi, uint32_t n_items, const fl_re
     (void)hipEventElapsedTime(&ms, e0, e1);
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
-    for (int k = 0; k < 3; ++k) (void)hipEventDestroy(tev[k]);
+    for (int k = 0; k < 4; ++k) (void)hipEventDestroy(tev[k]);
     for (uint32_t k = 0; k < n_items; ++k) {
         fl_film *film = hi[k].film;
         if (!film || !film->xyz_sum) continue;
@@ -1298,7 +1307,7 @@ int render_items(fl_ctx_t ctx, const HostItem *hi, uint32_t n_items, const fl_re
             FL_HIP_CHECK(ctx, hipMemcpy(film->xyz_sqsum, a.film_sq + off, (size_t)W * H * 12, hipMemcpyDeviceToHost));
     }
     if (stats) {
-        unsigned long long h[10];
+        unsigned long long h[12];
         FL_HIP_CHECK(ctx, hipMemcpy(h, a.stats, sizeof(h), hipMemcpyDeviceToHost));
         memset(stats, 0, sizeof(*stats));
         stats->ms_total = ms;
@@ -1312,6 +1321,8 @@ int render_items(fl_ctx_t ctx, const HostItem *hi, uint32_t n_items, const fl_re
         stats->ms_extend = ms_trace;   // combined traversal kernel (extension + shadow rays)
         stats->ms_shade = ms_shade;
         stats->ms_guide = ms_guide;
+        stats->ms_infer = ms_infer;
+        stats->guide_queries = h[6];
         stats->launches_extend = n_trace;
         stats->launches_shade = n_shade;
         stats->passes = passes;

@@ -85,7 +85,8 @@ class Stats(C.Structure):
                 ("pad0", C.c_uint32),
                 ("nan_samples", C.c_uint64), ("stack_spills", C.c_uint64),
                 ("passes", C.c_uint32), ("max_bounce", C.c_uint32),
-                ("paths_per_pass", C.c_uint32), ("stack_depth", C.c_uint32)]
+                ("paths_per_pass", C.c_uint32), ("stack_depth", C.c_uint32),
+                ("ms_infer", C.c_double), ("guide_queries", C.c_uint64)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

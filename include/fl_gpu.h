@@ -182,6 +182,8 @@ typedef struct {
     uint64_t nan_samples, stack_spills;
     uint32_t passes, max_bounce;
     uint32_t paths_per_pass, stack_depth;
+    double ms_infer;                /* with FL_RENDER_TIMING: guide inference kernels */
+    uint64_t guide_queries;         /* network evaluations (guided hits) */
 } fl_stats;
 
 /* ---- context / scene ---------------------------------------------------- */
