@@ -193,6 +193,36 @@ def guided_section(flgpu, ctx, res, ref_spp):
     return out
 
 
+def terrain_section(flgpu, ctx, spp, res_div):
+    """The full terrain config: 4096 x 2048 fBm heightfield (16.8 M
+    triangles) + 200 Cauchy prisms + 24 GGX metals, 8 cameras at
+    3840x2160 / res_div rendered as one guided batch (trained online)."""
+    t0 = time.time()
+    gen = flgpu.GenSceneData(flgpu.FL_GEN_TERRAIN)
+    W, H = 3840 // res_div, 2160 // res_div
+    cams = []
+    for k in range(8):
+        c = flgpu.GenSceneData(flgpu.FL_GEN_TERRAIN, grid_x=2, grid_y=2, camera_index=k, width=W, height=H).camera
+        cams.append(c)
+    t_gen = time.time() - t0
+    out = {"triangles": gen.n_triangles, "cameras": 8, "width": W, "height": H, "spp": spp, "gen_s": round(t_gen, 1)}
+    with ctx.scene(gen) as sc, ctx.guide(seed=1) as g:
+        out["bvh_build_ms"] = round(sc.info().build_ms, 1)
+        p = flgpu.default_params(spp=spp, depth=gen.depth, seed=1,
+                                 flags=flgpu.FL_RENDER_VNDF_PDF | flgpu.FL_RENDER_GRAPH | flgpu.FL_RENDER_GUIDED |
+                                 flgpu.FL_RENDER_TRAIN_GUIDE)
+        p.guide = g.h
+        ts = time.perf_counter()
+        _, st = ctx.render_batch([(sc, c, spp) for c in cams], p, want_film=False)
+        dt = time.perf_counter() - ts
+        steps, loss, div = g.stats()
+    samples = 8 * W * H * spp
+    out.update({"s": round(dt, 2), "samples": samples, "msamples_s": round(samples / dt / 1e6, 1),
+                "rays": int(st.rays_extension + st.rays_shadow), "passes": st.passes, "guide_steps": steps,
+                "guide_train_ms": round(st.ms_guide, 1), "guide_diverged": div, "nan_samples": int(st.nan_samples)})
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -208,6 +238,9 @@ def main():
     ap.add_argument("--batch-scenes", type=int, default=64)
     ap.add_argument("--guided-res", type=int, default=256)
     ap.add_argument("--guided-ref-spp", type=int, default=16384)
+    ap.add_argument("--terrain", action="store_true", help="also render the full terrain config (8 x 4K x 256 spp)")
+    ap.add_argument("--terrain-spp", type=int, default=256)
+    ap.add_argument("--terrain-res-div", type=int, default=1)
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run a plain 1 GiB -> 1 GiB copy kernel (HBM counter calibration)")
     args = ap.parse_args()
@@ -284,6 +317,11 @@ def main():
             log("guided section: %d^2, reference %d spp" % (args.guided_res, args.guided_ref_spp))
             out["guided"] = guided_section(flgpu, ctx, args.guided_res, args.guided_ref_spp)
             log("guided: %s" % out["guided"])
+
+        if args.terrain:
+            log("terrain section")
+            out["terrain"] = terrain_section(flgpu, ctx, args.terrain_spp, args.terrain_res_div)
+            log("terrain: %s" % out["terrain"])
 
         if args.calibrate_copy:
             # plain streaming kernel (k_copy_probe), 1 GiB read + 1 GiB written
