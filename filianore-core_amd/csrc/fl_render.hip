@@ -193,7 +193,7 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WAV
     TravCounters c;
     TraceSrc src{a.sq_o,    a.sq_d, a.sq_tgt, a.pend,    a.rq_o[cur], a.rq_d[cur], a.rq_sid[cur],
                  a.hits,    ns,     a.cap,    a.items,   a.st_item,   a.n_items,   a.sc};
-    trace_persistent<COUNT, true>(src, ns + ne, &a.counts[C_FETCH + cur], ls, a.trav_budget, c);
+    trace_persistent<COUNT, true>(src, ns + ne, &a.counts[C_FETCH + cur], ls, a.trav_budget, a.refill_lanes, c);
     if (COUNT) {
         atomicAdd(&a.stats[3], (unsigned long long)c.nodes);
         atomicAdd(&a.stats[4], (unsigned long long)c.tris);
@@ -1032,6 +1032,7 @@ int render_items(fl_ctx_t ctx, const HostItem *hi, uint32_t n_items, const fl_re
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, tv->fn(), kBlock, 0) != hipSuccess || occ <= 0) occ = 4;
     const uint32_t grid_pers = (uint32_t)(ctx->n_cu * occ);
     a.trav_budget = p->trav_budget ? (int)p->trav_budget : 16;
+    a.refill_lanes = p->refill_lanes ? (int)std::min(64u, p->refill_lanes) : 16;
     size_t spill_per = (size_t)(max_depth > (uint32_t)tv->stack ? max_depth - tv->stack + 1 : 1);
     a.spill_stride = grid_pers * kBlock;
     a.st_beta = grow<float4>(B.p[0], B.cap[0], cap * 16, err);
@@ -1395,6 +1396,7 @@ int autotune_items(fl_ctx_t ctx, const HostItem *hi, uint32_t n, const fl_render
             if (rc != FL_OK) break;
         }
     if (rc == FL_OK) rc = try_values([](fl_render_params &q, uint32_t v) { q.trav_budget = v; }, {8u, 32u});
+    if (rc == FL_OK) rc = try_values([](fl_render_params &q, uint32_t v) { q.refill_lanes = v; }, {1u, 32u});
     if (rc == FL_OK)
         rc = try_values([](fl_render_params &q, uint32_t v) { q.flags = v ? (q.flags | FL_RENDER_NO_SORT) : q.flags; },
                         {1u});
@@ -1405,6 +1407,7 @@ int autotune_items(fl_ctx_t ctx, const HostItem *hi, uint32_t n, const fl_render
     out->stack_depth = cur.stack_depth ? cur.stack_depth : (uint32_t)kTraceVariants[0].stack;
     out->trav_waves = cur.trav_waves;
     out->trav_budget = cur.trav_budget ? cur.trav_budget : 16u;
+    out->refill_lanes = cur.refill_lanes ? cur.refill_lanes : 16u;
     out->shade_waves = cur.shade_waves;
     out->sort_shading = (cur.flags & FL_RENDER_NO_SORT) ? 0u : 1u;
     out->ms_best = ms_cur;

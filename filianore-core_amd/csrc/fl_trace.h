@@ -230,11 +230,13 @@ __device__ __forceinline__ bool traverse(const TraceScene &sc, v3 o, v3 d, float
 //   void store(uint32_t i, const TravState &s) const
 // Lane g first takes ray g (no atomic); later rays come from `fetch` (which
 // must be 0 at launch) offset by the grid size, one atomic per wave refill.
+// A lane runs `budget` traversal steps between refill checks; idle lanes are
+// refilled together once `refill_lanes` of them (or all) are idle.
 // All lanes of the wave must call this (it uses wave-wide ballots).
 // `any` is per ray: the first hit ends it (emitters skipped when ANY_SKIP).
 template <bool COUNT, bool ANY_SKIP, class Src>
 __device__ __forceinline__ void trace_persistent(const Src &src, uint32_t n, uint32_t *fetch, const LaneStack &ls,
-                                                 int budget, TravCounters &cnt) {
+                                                 int budget, int refill_lanes, TravCounters &cnt) {
     TraceScene sc{};
     TravState s;
     const uint32_t lane = __lane_id();
@@ -261,9 +263,11 @@ __device__ __forceinline__ void trace_persistent(const Src &src, uint32_t n, uin
                 ray = -1;
             }
         }
+        // refill once at least refill_lanes lanes are idle (or all are): one
+        // atomic and one round of ray loads serve many lanes
         bool need = (ray < 0) && !exhausted;
         unsigned long long m = __ballot(need);
-        if (m) {
+        if (m && (__popcll(m) >= (unsigned)refill_lanes || __ballot(ray >= 0) == 0ull)) {
             int leader = __ffsll(m) - 1;
             uint32_t base = 0;
             if (lane == (uint32_t)leader) base = atomicAdd(fetch, (uint32_t)__popcll(m));

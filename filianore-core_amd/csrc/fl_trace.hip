@@ -54,7 +54,7 @@ __global__ void __launch_bounds__(kBlock) k_trace_api(fl::TraceScene sc, ApiSrc 
     ls.spill = spill + gtid;
     ls.spill_stride = gridDim.x * kBlock;
     fl::TravCounters c;
-    fl::trace_persistent<COUNT, SKIP>(src, n, fetch, ls, kBudget, c);
+    fl::trace_persistent<COUNT, SKIP>(src, n, fetch, ls, kBudget, 1, c);
     if (COUNT) {
         atomicAdd(&counters[0], (unsigned long long)c.nodes);
         atomicAdd(&counters[1], (unsigned long long)c.tris);

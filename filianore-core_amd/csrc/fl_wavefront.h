@@ -102,7 +102,8 @@ struct RenderArgs {
     float *film_sum, *film_sq;
     uint2 *spill;
     uint32_t spill_stride;
-    int trav_budget;         // traversal iterations between lane refills
+    int trav_budget;         // traversal iterations between lane refill checks
+    int refill_lanes;        // idle lanes that trigger a refill (or all lanes idle)
     // learned proposal (guided passes)
     uint32_t guided;         // 1: mix the guide's vMF lobes with BSDF sampling
     float guide_mix;         // probability of sampling the guide

@@ -66,7 +66,7 @@ class RenderParams(C.Structure):
                 ("trav_budget", C.c_uint32), ("shade_waves", C.c_uint32), ("trav_waves", C.c_uint32),
                 ("guide", C.c_void_p),
                 ("guide_mix", C.c_float),
-                ("guide_train_fraction", C.c_float)]
+                ("guide_train_fraction", C.c_float), ("refill_lanes", C.c_uint32), ("pad1", C.c_uint32)]
 
 
 class Film(C.Structure):
@@ -100,12 +100,14 @@ class BatchItem(C.Structure):
 class TuneResult(C.Structure):
     _fields_ = [("max_paths", C.c_uint32), ("stack_depth", C.c_uint32), ("trav_waves", C.c_uint32),
                 ("trav_budget", C.c_uint32), ("shade_waves", C.c_uint32), ("sort_shading", C.c_uint32),
-                ("ms_best", C.c_double), ("ms_default", C.c_double), ("n_tried", C.c_uint32), ("pad", C.c_uint32)]
+                ("ms_best", C.c_double), ("ms_default", C.c_double), ("n_tried", C.c_uint32),
+                ("refill_lanes", C.c_uint32)]
 
     def apply(self, params):
         """Copies the tuned values into fl_render_params."""
         params.max_paths, params.stack_depth, params.trav_waves = self.max_paths, self.stack_depth, self.trav_waves
         params.trav_budget, params.shade_waves = self.trav_budget, self.shade_waves
+        params.refill_lanes = self.refill_lanes
         if self.sort_shading:
             params.flags &= ~FL_RENDER_NO_SORT
         else:

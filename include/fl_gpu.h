@@ -163,6 +163,8 @@ typedef struct {
     fl_guide_t guide;            /* guiding network when FL_RENDER_GUIDED */
     float guide_mix;             /* proposal probability of the guide (0.5) */
     float guide_train_fraction;  /* with TRAIN_GUIDE: fraction of spp used to train */
+    uint32_t refill_lanes;       /* traversal: idle lanes of a wave that trigger a refill (0: auto) */
+    uint32_t pad1;
 } fl_render_params;
 
 /* Film: per-pixel sums of linear CIE XYZ (3 floats per pixel) and of its
@@ -238,8 +240,8 @@ int fl_render_batch(fl_ctx_t ctx, const fl_batch_item *items, uint32_t n,
 /* Probe renders of the same job (the given spp, at most 64) choose, one knob
  * after the other, the values of fl_render_params that minimise device time:
  * paths in flight per pass (queue chunk), traversal variant (LDS stack depth x
- * minimum waves per SIMD), traversal steps between lane refills, shading sort
- * on/off, shading-kernel occupancy. Node width is not a knob: nodes are
+ * minimum waves per SIMD), traversal steps between lane refills, refill
+ * threshold, shading sort on/off, shading-kernel occupancy. Node width is not a knob: nodes are
  * 8-wide by construction (one 128-byte line). No counterpart in the
  * reference. */
 typedef struct {
@@ -247,7 +249,7 @@ typedef struct {
     uint32_t sort_shading;        /* 1: per-block material sort on (FL_RENDER_NO_SORT clear) */
     double ms_best, ms_default;   /* device ms of one render with the chosen and the default values */
     uint32_t n_tried;             /* probe configurations measured */
-    uint32_t pad;
+    uint32_t refill_lanes;        /* traversal refill threshold */
 } fl_tune_result;
 
 int fl_autotune(fl_scene_t scene, const fl_camera *camera, const fl_render_params *params,

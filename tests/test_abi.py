@@ -40,4 +40,4 @@ def test_struct_sizes_match_header():
     assert C.sizeof(flgpu.BsdfResult) == 56
     assert C.sizeof(flgpu.Camera) == 48
     assert C.sizeof(flgpu.Stats) == 176
-    assert C.sizeof(flgpu.RenderParams) == 80
+    assert C.sizeof(flgpu.RenderParams) == 88
