@@ -11,7 +11,7 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 def declared(header):
     text = open(os.path.join(ROOT, "include", header)).read()
     text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
-    return sorted(set(re.findall(r"\b(fl_[a-z0-9_]+)\s*\(", text)))
+    return sorted(set(re.findall(r"\b((?:fl|rtc)[A-Za-z0-9_]+)\s*\(", text)))
 
 
 def test_all_declared_symbols_exported():
@@ -21,6 +21,16 @@ def test_all_declared_symbols_exported():
     missing = [n for n in names if not hasattr(L, n)]
     assert not missing, missing
     assert set(flgpu.EXPORTS) <= set(names)
+
+
+def test_embree_subset_exported():
+    """include/fl_rtc.h (the Embree 4 subset the reference's BVH calls) is
+    implemented in the same library."""
+    L = flgpu.lib()
+    names = declared("fl_rtc.h")
+    assert len(names) == 17
+    missing = [n for n in names if not hasattr(L, n)]
+    assert not missing, missing
 
 
 def test_abi_version_and_no_device_behaviour():

@@ -12,10 +12,11 @@ import flgpu
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 EXE = os.path.join(ROOT, "filianore-core_amd", "host", "fl_render_example")
+RTC_EXE = os.path.join(ROOT, "filianore-core_amd", "host", "fl_rtc_example")
 
 
-def run_example(*args):
-    p = subprocess.run([EXE, *args], capture_output=True, text=True, timeout=120)
+def run_example(*args, exe=EXE):
+    p = subprocess.run([exe, *args], capture_output=True, text=True, timeout=120)
     line = p.stdout.strip().splitlines()[-1] if p.stdout.strip() else "{}"
     return p.returncode, json.loads(line)
 
@@ -44,3 +45,25 @@ def test_cpp_layer_matches_c_abi_render(tmp_path):
                                  flags=flgpu.FL_RENDER_GRAPH | flgpu.FL_RENDER_VNDF_PDF)
         ref, _, _ = sc.render(p)
     assert np.array_equal(got.view(np.uint32), ref.view(np.uint32))
+
+
+def test_rtc_example_built_and_reports_missing_gpu():
+    assert os.access(RTC_EXE, os.X_OK), "build() did not produce the Embree-API example"
+    if flgpu.lib().fl_device_count() > 0:
+        pytest.skip("a GPU is present")
+    rc, out = run_example(exe=RTC_EXE)
+    assert rc == 2 and out["ok"] is False and "no usable GPU" in out["error"]
+
+
+@pytest.mark.gpu
+def test_embree_pattern_matches_c_abi():
+    """Code in the reference's Embree call pattern (one geometry per
+    triangle, rtcIntersect1 per ray), built against include/fl_rtc.h: hits
+    equal fl_intersect bit for bit with geomID as the triangle, Ng is the
+    unnormalized face normal, rtcOccluded1 agrees, multi-triangle meshes map
+    (geomID, primID) to the same triangles, and misuse reports errors."""
+    rc, out = run_example("2048", exe=RTC_EXE)
+    print(out)
+    assert rc == 0 and out["ok"], out
+    assert out["hits"] > 500 and out["mismatches"] == 0 and out["mesh_mismatches"] == 0
+    assert out["occluded_mismatches"] == 0 and out["ng_mismatches"] == 0
