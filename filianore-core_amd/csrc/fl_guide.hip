@@ -16,6 +16,8 @@
 #include <new>
 #include <vector>
 
+#include <hipcub/hipcub.hpp>
+
 #include "fl_guide_mlp.h"
 #include "fl_internal.h"
 
@@ -27,7 +29,6 @@ using namespace fl;
 
 constexpr int kRecEnc = 68;      // API record: encoded input 64, direction 3, weight 1
 constexpr int kRecRaw = 16;      // render record (see guide_train_on_records)
-constexpr float kMaxWeight = 20.f;   // weight clamp after normalisation by the mean
 constexpr float kClipNorm = 10.f;    // gradient norm clip (mean loss)
 
 __device__ __forceinline__ uint32_t hash_u32(uint32_t x) {
@@ -51,22 +52,45 @@ struct EncSrc {
     }
 };
 
-// Render records, drawn uniformly by a counter hash, encoded here
+// Render records, drawn with probability proportional to their weight (the
+// inclusive prefix sum `cdf` of the weights, in record order) by a counter
+// hash, encoded here. Each drawn sample then carries unit weight: the batch
+// gradient has the expectation of the weighted gradient over all records (up
+// to the constant total weight), and no batch entry is spent on the many
+// records whose path gathered nothing beyond the vertex.
 struct RawSrc {
     const float *rec;
+    const double *cdf;
     uint32_t n_recs, n_batch, step;
     uint32_t seed;
-    float inv_mean;
+    double total;
     __device__ bool get(uint32_t i, float *x, v3 &dir, float &c) const {
-        if (i >= n_batch || n_recs == 0) return false;
-        const uint32_t r = hash_u32(hash_u32(seed ^ (step * 0x9e3779b9u)) ^ (i * 0x85ebca6bu)) % n_recs;
-        const float *R = rec + (size_t)r * kRecRaw;
+        if (i >= n_batch || n_recs == 0 || !(total > 0.0)) return false;
+        const uint32_t h = hash_u32(hash_u32(seed ^ (step * 0x9e3779b9u)) ^ (i * 0x85ebca6bu));
+        const double u = ((double)h + 0.5) * (1.0 / 4294967296.0) * total;
+        // first record whose prefix sum exceeds u (zero-weight records never are)
+        uint32_t lo = 0, hi = n_recs - 1;
+        while (lo < hi) {
+            const uint32_t mid = (lo + hi) >> 1;
+            if (cdf[mid] > u) hi = mid;
+            else lo = mid + 1;
+        }
+        const float *R = rec + (size_t)lo * kRecRaw;
         const float pos[3] = {R[0], R[1], R[2]};
         guide_encode(pos, v3(R[3], R[4], R[5]), R[6], R[7], x);
         dir = v3(R[8], R[9], R[10]);
-        const float w = R[14] * inv_mean;
-        c = (w > 0.f && w < 3.0e38f) ? fminf(w, kMaxWeight) : 0.f;
+        c = 1.f;
         return true;
+    }
+};
+
+// Clean weight of render record i (0 unless positive and finite), as double
+// for the prefix sum.
+struct RecWeight {
+    const float *rec;
+    __host__ __device__ double operator()(uint32_t i) const {
+        const float w = rec[(size_t)i * kRecRaw + 14];
+        return (w > 0.f && w < 3.0e38f) ? (double)w : 0.0;
     }
 };
 
@@ -295,19 +319,6 @@ __device__ __forceinline__ float block_sum(float v, float *red) {
     return t;
 }
 
-// mean of the positive, finite record weights: per-block partial sums
-__global__ void k_rec_weight_sum(const float *rec, uint32_t n, float *part_sum, float *part_cnt) {
-    __shared__ float red[33];
-    float s = 0.f, c = 0.f;
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-        const float w = rec[(size_t)i * kRecRaw + 14];
-        if (w > 0.f && w < 3.0e38f) { s += w; c += 1.f; }
-    }
-    s = block_sum(s, red);
-    c = block_sum(c, red);
-    if (threadIdx.x == 0) { part_sum[blockIdx.x] = s; part_cnt[blockIdx.x] = c; }
-}
-
 // ---------------------------------------------------------------- host ----
 struct Pcg {
     uint64_t s, inc;
@@ -347,6 +358,18 @@ int ensure(fl_ctx_t ctx, float *&p, size_t &cap, size_t bytes) {
     p = nullptr;
     cap = 0;
     if (hipMalloc(&p, bytes) != hipSuccess) return set_error(ctx, FL_E_OOM, "guide staging buffer");
+    cap = bytes;
+    return FL_OK;
+}
+
+template <class T>
+int ensure_bytes(fl_ctx_t ctx, T *&p, size_t &cap, size_t bytes) {
+    if (cap >= bytes) return FL_OK;
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    cap = 0;
+    if (hipMalloc(reinterpret_cast<void **>(&p), bytes) != hipSuccess)
+        return set_error(ctx, FL_E_OOM, "guide record buffer");
     cap = bytes;
     return FL_OK;
 }
@@ -401,23 +424,27 @@ namespace flx {
 int guide_train_on_records(fl_guide_s *g, const float *d_recs, uint32_t n_recs, uint32_t steps, uint32_t batch,
                            uint64_t seed, hipStream_t st) {
     if (!g || n_recs == 0 || steps == 0 || g->diverged) return FL_OK;
-    // normalise the weights by their mean over this set of records
-    const uint32_t wb = 256;
-    int rc = ensure(g->ctx, g->d_scratch_out, g->scratch_out_bytes, 2 * wb * 4);
+    // prefix sum of the record weights, for weight-proportional draws
+    int rc = ensure_bytes(g->ctx, g->d_cdf, g->cdf_bytes, (size_t)n_recs * sizeof(double));
     if (rc != FL_OK) return rc;
-    hipLaunchKernelGGL(k_rec_weight_sum, dim3(wb), dim3(256), 0, st, d_recs, n_recs, g->d_scratch_out,
-                       g->d_scratch_out + wb);
-    std::vector<float> h(2 * wb);
-    if (hipMemcpyAsync(h.data(), g->d_scratch_out, 2 * wb * 4, hipMemcpyDeviceToHost, st) != hipSuccess ||
+    hipcub::TransformInputIterator<double, RecWeight, hipcub::CountingInputIterator<uint32_t>> wit(
+        hipcub::CountingInputIterator<uint32_t>(0u), RecWeight{d_recs});
+    size_t tmp_bytes = 0;
+    if (hipcub::DeviceScan::InclusiveSum(nullptr, tmp_bytes, wit, g->d_cdf, n_recs, st) != hipSuccess)
+        return set_error(g->ctx, FL_E_HIP, "guide record scan");
+    rc = ensure_bytes(g->ctx, g->d_scan_tmp, g->scan_tmp_bytes, std::max<size_t>(tmp_bytes, 256));
+    if (rc != FL_OK) return rc;
+    if (hipcub::DeviceScan::InclusiveSum(g->d_scan_tmp, tmp_bytes, wit, g->d_cdf, n_recs, st) != hipSuccess)
+        return set_error(g->ctx, FL_E_HIP, "guide record scan");
+    double total = 0.0;
+    if (hipMemcpyAsync(&total, g->d_cdf + (n_recs - 1), sizeof(double), hipMemcpyDeviceToHost, st) != hipSuccess ||
         hipStreamSynchronize(st) != hipSuccess)
         return set_error(g->ctx, FL_E_HIP, "guide record weights");
-    double s = 0.0, c = 0.0;
-    for (uint32_t i = 0; i < wb; ++i) { s += h[i]; c += h[wb + i]; }
-    if (c < 1.0 || !(s > 0.0)) return FL_OK;   // nothing to learn from yet
-    const float inv_mean = (float)(c / s);
+    if (!(total > 0.0) || !std::isfinite(total)) return FL_OK;   // nothing to learn from yet
     const uint32_t blocks = (batch + 63) / 64;
     for (uint32_t k = 0; k < steps; ++k) {
-        RawSrc src{d_recs, n_recs, batch, k, (uint32_t)(seed * 0x9e3779b97f4a7c15ULL >> 32) ^ g->steps, inv_mean};
+        RawSrc src{d_recs, g->d_cdf, n_recs, batch, k, (uint32_t)(seed * 0x9e3779b97f4a7c15ULL >> 32) ^ g->steps,
+                   total};
         rc = run_grad(g, src, batch, st);
         if (rc != FL_OK) return rc;
         adam(g, blocks, batch, st);
@@ -487,7 +514,7 @@ void fl_guide_destroy(fl_guide_t g) {
     if (g->ctx) (void)hipStreamSynchronize(g->ctx->stream);
     for (void *p : {(void *)g->d_params, (void *)g->d_m, (void *)g->d_v, (void *)g->d_grad, (void *)g->d_sq, (void *)g->d_w,
                     (void *)g->d_wt, (void *)g->d_partial, (void *)g->d_loss, (void *)g->d_stat,
-                    (void *)g->d_scratch_in, (void *)g->d_scratch_out})
+                    (void *)g->d_scratch_in, (void *)g->d_scratch_out, (void *)g->d_cdf, g->d_scan_tmp})
         if (p) (void)hipFree(p);
     delete g;
 }

@@ -87,6 +87,10 @@ struct fl_guide_s {
     size_t scratch_in_bytes = 0;
     float *d_scratch_out = nullptr;
     size_t scratch_out_bytes = 0;
+    double *d_cdf = nullptr;         // prefix sums of render-record weights
+    size_t cdf_bytes = 0;
+    void *d_scan_tmp = nullptr;      // hipCUB scan workspace
+    size_t scan_tmp_bytes = 0;
     uint32_t steps = 0;
     float last_loss = 0.f;
     int diverged = 0;

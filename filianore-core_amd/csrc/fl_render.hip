@@ -592,6 +592,7 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WAV
             bool ok;
             v3 wiW;
             beta0 = beta[0];
+            float rr_guide = 1.f;
             if (lb) {
                 // one-sample mixture: the guide with probability mix, else the
                 // BSDF; the throughput divides by the mixture density at the
@@ -617,8 +618,15 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WAV
                 if (have) {
                     const bool refl = dot(wiW, ng) * dot(woW, ng) > 0.f;
                     bsdf_eval_nonspec(bs_, wo, wl, refl, bsm.f);
-                    bsm.pdf = (1.f - a.guide_mix) * bsdf_pdf_nonspec(bs_, wo, wl) + a.guide_mix * guide_pdf_at(lb, wiW);
+                    const float pb_dir = bsdf_pdf_nonspec(bs_, wo, wl);
+                    bsm.pdf = (1.f - a.guide_mix) * pb_dir + a.guide_mix * guide_pdf_at(lb, wiW);
                     bsm.wi = wl;
+                    // Russian roulette judges the throughput BSDF sampling would
+                    // have given this direction (beta * f cos / p_bsdf), not the
+                    // guided one: a direction the guide favours because much
+                    // light arrives from it gets a small beta, and rouletting on
+                    // that beta would end exactly the paths about to gather it
+                    rr_guide = pb_dir > 0.f ? fminf(bsm.pdf / pb_dir, 1.0e4f) : 1.0e4f;
                 }
                 ok = bsm.pdf > 0.f;
                 if (ok) {
@@ -659,6 +667,7 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WAV
                     if (ty & BSDF_SPECULAR) flags |= PF_SPECULAR; else flags &= ~PF_SPECULAR;
                     if ((ty & BSDF_SPECULAR) && (ty & BSDF_TRANSMISSION))
                         eta_scale *= (dot(woW, ng) > 0.f) ? (eta_bsdf * eta_bsdf) : 1.f / (eta_bsdf * eta_bsdf);
+                    if (lb) eta_scale *= rr_guide;   // eta_scale only scales the roulette's throughput
                     if (bsm.secondary_terminated) flags |= PF_SECTERM;
                     no = offset_point_origin(p, perr, ng, wiW);          // kindle_ray
                     nd = wiW;

@@ -374,6 +374,36 @@ void guided_room(Builder &B, const fl_gen_options *o, fl_gen_scene *out) {
     out->depth = 16;
 }
 
+// A second reading of the guided room: the spheres inside a closed
+// 12-triangle room [-6,6] x [0,6] x [-4,4] (Lambert, inward normals). The
+// ceiling quad stops 0.4 short of the back wall: that strip, 12 x 0.4 = 4.8 = 5% of the ceiling, is the only opening.
+// The outer quad light hangs above the strip facing down, so light enters
+// only through the strip: it falls on a band of the back wall and floor, and
+// the rest of the room is lit indirectly (hard indirect transport). Light
+// sampling from most room points hits the ceiling.
+void skylight_room(Builder &B, const fl_gen_options *o, fl_gen_scene *out) {
+    Pcg32 rng(o->seed ? o->seed : 3u);
+    int level = o->subdiv >= 0 ? o->subdiv : 7;
+    int n = o->n_spheres > 0 ? o->n_spheres : 8;
+    spheres_on_ground(B, rng, n, level, 7.5f, 2.5f, 1);
+    uint32_t mw = B.add_material(lambert(0.7f, 0.7f, 0.7f));
+    const float X = 6.f, Yh = 6.f, Z = 4.f, slot = 0.4f;
+    B.add_quad_n({0, 0.001f, 0}, {0, 1, 0}, {1, 0, 0}, X, Z, mw);          // floor (just above ground)
+    B.add_quad_n({0, Yh * 0.5f, -Z}, {0, 0, 1}, {1, 0, 0}, X, Yh * 0.5f, mw); // back
+    B.add_quad_n({0, Yh * 0.5f, Z}, {0, 0, -1}, {1, 0, 0}, X, Yh * 0.5f, mw); // front
+    B.add_quad_n({-X, Yh * 0.5f, 0}, {1, 0, 0}, {0, 0, 1}, Z, Yh * 0.5f, mw); // left
+    B.add_quad_n({X, Yh * 0.5f, 0}, {-1, 0, 0}, {0, 0, 1}, Z, Yh * 0.5f, mw); // right
+    // ceiling: z in [-Z + slot, Z]; the strip z in [-Z, -Z + slot] stays open
+    B.add_quad_n({0, Yh, 0.5f * slot}, {0, -1, 0}, {1, 0, 0}, X, Z - 0.5f * slot, mw);
+    uint32_t em = add_blackbody(B, 6500.f);
+    uint32_t mn = B.add_material(nomat());
+    uint32_t lt = B.add_quad_n({0.f, Yh + 0.6f, -Z + 0.2f}, {0, -1, 0}, {1, 0, 0}, X, 0.6f, mn);
+    B.add_light_quad(lt, em, 17.f * 2.5f);
+    out->camera = fl_camera{{0.f, 3.f, 3.9f}, {0.f, 1.f, 0.f}, {0.f, 1.f, 0.f}, 60.f, 1024, 1024};
+    out->spp = 64;
+    out->depth = 16;
+}
+
 void batch_scene(Builder &B, const fl_gen_options *o, fl_gen_scene *out) {
     uint32_t k = o->scene_index;
     Pcg32 rng((o->seed ? o->seed : 5u) * 1000003u + k, 7u);
@@ -498,6 +528,7 @@ int fl_gen_config(int config, const fl_gen_options *opt, fl_gen_scene *out) {
     case FL_GEN_BATCH: batch_scene(B, o, out); break;
     case FL_GEN_GUIDED: guided_room(B, o, out); break;
     case FL_GEN_TERRAIN: terrain(B, o, out); break;
+    case FL_GEN_GUIDED_SKYLIGHT: skylight_room(B, o, out); break;
     default: return FL_E_INVALID;
     }
     if (o->width) out->camera.width = o->width;

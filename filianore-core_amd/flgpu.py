@@ -21,7 +21,7 @@ FL_RENDER_GUIDED, FL_RENDER_COUNTERS, FL_RENDER_VNDF_PDF, FL_RENDER_GRAPH = 1, 2
 FL_RENDER_TRAIN_GUIDE, FL_RENDER_NO_SORT, FL_RENDER_TIMING = 16, 32, 64
 FL_NO_HIT = 0xFFFFFFFF
 FL_SPD_N = 471
-FL_GEN_CORNELL, FL_GEN_SPHERES, FL_GEN_BATCH, FL_GEN_GUIDED, FL_GEN_TERRAIN = 0, 1, 2, 3, 4
+FL_GEN_CORNELL, FL_GEN_SPHERES, FL_GEN_BATCH, FL_GEN_GUIDED, FL_GEN_TERRAIN, FL_GEN_GUIDED_SKYLIGHT = 0, 1, 2, 3, 4, 5
 BSDF_REFLECTION, BSDF_TRANSMISSION, BSDF_DIFFUSE, BSDF_GLOSSY, BSDF_SPECULAR = 1, 2, 4, 8, 16
 
 f3 = C.c_float * 3
@@ -360,8 +360,9 @@ class Context:
         _check(self.L.fl_render_batch(self.h, items, n, C.byref(params), C.byref(st)), self.h, "fl_render_batch")
         return [(s, q) for s, q, _ in keep], st
 
-    def guide(self, seed=1, lr=0.0, batch=0, scene_min=(0, 0, 0), scene_max=(1, 1, 1)):
-        return Guide(self, seed=seed, lr=lr, batch=batch, scene_min=scene_min, scene_max=scene_max)
+    def guide(self, seed=1, lr=0.0, batch=0, scene_min=(0, 0, 0), scene_max=(1, 1, 1), steps_per_pass=0):
+        return Guide(self, seed=seed, lr=lr, batch=batch, scene_min=scene_min, scene_max=scene_max,
+                     steps_per_pass=steps_per_pass)
 
     def probe_copy(self, nbytes=1 << 30, iters=5):
         """Device-to-device streaming copy; returns (read + write) GB/s."""
@@ -450,10 +451,10 @@ ITEM_SEED_STRIDE = 0x9E3779B97F4A7C15   # item k of a batch renders with seed + 
 class Guide:
     """The learned vMF-mixture proposal (fl_guide_*)."""
 
-    def __init__(self, ctx, seed=1, lr=0.0, batch=0, scene_min=(0, 0, 0), scene_max=(1, 1, 1)):
+    def __init__(self, ctx, seed=1, lr=0.0, batch=0, scene_min=(0, 0, 0), scene_max=(1, 1, 1), steps_per_pass=0):
         self.ctx = ctx
         cfg = GuideConfig()
-        cfg.seed, cfg.lr, cfg.batch = seed, lr, batch
+        cfg.seed, cfg.lr, cfg.batch, cfg.steps_per_pass = seed, lr, batch, steps_per_pass
         cfg.scene_min = f3(*scene_min)
         cfg.scene_max = f3(*scene_max)
         h = C.c_void_p()

@@ -20,8 +20,9 @@ enum {
     FL_GEN_CORNELL = 0,   /* small Cornell box: 36 tris, 256^2, 16 spp, depth 8 */
     FL_GEN_SPHERES = 1,   /* 8 displaced icospheres (subdiv 7) + ground: 2,621,442 tris */
     FL_GEN_BATCH = 2,     /* one scene of the 64-scene batch (index = opt.scene_index) */
-    FL_GEN_GUIDED = 3,    /* spheres inside a closed room lit through a slot */
-    FL_GEN_TERRAIN = 4    /* fBm heightfield + 200 dispersive prisms + metals */
+    FL_GEN_GUIDED = 3,    /* spheres inside a closed room lit through a ceiling slot from an attic */
+    FL_GEN_TERRAIN = 4,   /* fBm heightfield + 200 dispersive prisms + metals */
+    FL_GEN_GUIDED_SKYLIGHT = 5 /* spheres in a closed 12-tri room under a 5% ceiling strip, light above it */
 };
 
 typedef struct {

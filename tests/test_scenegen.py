@@ -55,3 +55,25 @@ def test_guided_room_slot_is_five_percent():
     area = 0.5 * np.linalg.norm(np.cross(t[:, 1] - t[:, 0], t[:, 2] - t[:, 0]), axis=1)
     ceiling = (np.abs(c[:, 1] - 6.0) < 1e-4) & np.all(np.isclose(n, [0, -1, 0], atol=1e-6), axis=1)
     assert abs(area[ceiling].sum() - 0.95 * 96.0) < 1e-3
+
+
+def test_skylight_room_is_twelve_triangles_with_a_five_percent_strip():
+    g = flgpu.GenSceneData(flgpu.FL_GEN_GUIDED_SKYLIGHT, subdiv=1)
+    t = g.triangles().astype(np.float64)
+    n, c = normals(g)
+    area = 0.5 * np.linalg.norm(np.cross(t[:, 1] - t[:, 0], t[:, 2] - t[:, 0]), axis=1)
+    lt = [g.lights[i].tri for i in range(g.n_lights)]
+    # the room: the 12 triangles after the spheres and ground, before the light
+    room = np.arange(lt[0] - 12, lt[0])
+    wall_mat = g.tri_material[room[0]]
+    assert np.all(g.tri_material[room] == wall_mat)
+    # every room face looks into the room [-6,6] x [0,6] x [-4,4]
+    centre = np.array([0.0, 3.0, 0.0])
+    assert np.all(np.sum(n[room] * (centre - c[room]), axis=1) > 0)
+    ceiling = room[np.abs(c[room, 1] - 6.0) < 1e-4]
+    assert len(ceiling) == 2 and abs(area[ceiling].sum() - 0.95 * 96.0) < 1e-3
+    # the strip left open is z in [-4, -3.6]; the light above it faces down
+    assert np.isclose(t[ceiling][:, :, 2].min(), -3.6, atol=1e-5)
+    assert np.allclose(n[lt], [0, -1, 0]) and np.all(t[lt][:, :, 1] > 6.0)
+    lz = t[lt][:, :, 2]
+    assert lz.min() < -4.0 and lz.max() > -3.6
