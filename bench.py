@@ -132,63 +132,107 @@ def batch_section(flgpu, ctx, n_scenes, tune):
             "passes": st.passes, "autotune": tuned}
 
 
-def guided_section(flgpu, ctx, res, ref_spp):
-    """Room lit only through a ceiling slot (light in a closed attic): plain
-    vs guided (trained online on the first 25% of the samples) at equal wall
-    time, relMSE against a plain reference of ref_spp samples per pixel."""
+NO_RR = 1 << 20   # rr_start beyond any depth: no Russian roulette
+
+
+def guided_section(flgpu, ctx, config, res, ref_spp, spp=64):
+    """Plain vs guided at equal wall time on a room lit only through a 5%
+    ceiling slot. Each method runs with and without Russian roulette (the
+    reference rule after bounce 3); the better setting of each (lower relMSE x
+    time) is compared. Guided renders train the proposal online on their
+    first 25% of samples, inside the timed render. relMSE is against a plain
+    reference of ref_spp samples per pixel, rendered in 4096-spp parts, with
+    the reference's own noise (its per-pixel variance of the mean, from the
+    film's sums of squares) subtracted so that the figure estimates the error
+    against the converged image."""
     import numpy as np
-    gen = flgpu.GenSceneData(flgpu.FL_GEN_GUIDED, width=res, height=res)
+    gen = flgpu.GenSceneData(config, width=res, height=res)
     base = flgpu.FL_RENDER_VNDF_PDF | flgpu.FL_RENDER_GRAPH
-    out = {"resolution": res, "triangles": gen.n_triangles, "reference_spp": ref_spp}
+    out = {"scene": {flgpu.FL_GEN_GUIDED: "attic slot", flgpu.FL_GEN_GUIDED_SKYLIGHT: "skylight strip"}[config],
+           "resolution": res, "triangles": gen.n_triangles, "depth": gen.depth, "reference_spp": ref_spp}
     with ctx.scene(gen) as sc:
-        p = flgpu.default_params(spp=ref_spp, depth=gen.depth, seed=1234, flags=base)
         ts = time.perf_counter()
-        ref, _, _ = sc.render(p)
-        ref = ref / ref_spp
-        out["reference_s"] = round(time.perf_counter() - ts, 2)
-        plain = []
-        for spp in (64, 128, 256):
+        rs = rq = None
+        parts = max(1, ref_spp // 4096)
+        for k in range(parts):
+            p = flgpu.default_params(spp=ref_spp // parts, depth=gen.depth, seed=1000 + k, flags=base)
+            s_, q_, _ = sc.render(p)
+            rs = s_.astype(np.float64) if rs is None else rs + s_
+            rq = q_.astype(np.float64) if rq is None else rq + q_
+            log("  reference part %d/%d (%.0f s)" % (k + 1, parts, time.perf_counter() - ts))
+        ref = rs / ref_spp
+        ref_var = np.maximum(rq / ref_spp - ref * ref, 0.0) / ref_spp
+        den = ref * ref + 0.01
+        out["reference_s"] = round(time.perf_counter() - ts, 1)
+        out["reference_noise_relmse"] = round(float(np.mean(ref_var / den)), 5)
+
+        def relmse(img):
+            return float(np.mean(((img - ref) ** 2 - ref_var) / den))
+
+        def timed(p):
+            t0 = time.perf_counter()
+            img, _, st = sc.render(p)
+            return time.perf_counter() - t0, img, st
+
+        plain = {}
+        for name, rr in (("rr3", 3), ("no_rr", NO_RR)):
             p = flgpu.default_params(spp=spp, depth=gen.depth, seed=7, flags=base)
+            p.rr_start = rr
             sc.render(p, want_film=False)
-            ts = time.perf_counter()
-            img, _, _ = sc.render(p)
-            plain.append((spp, time.perf_counter() - ts, rel_mse(img / spp, ref)))
-        guided = []
+            t, img, _ = timed(p)
+            e = relmse(img / spp)
+            plain[name] = {"spp": spp, "s": round(t, 4), "relmse": round(e, 5), "relmse_x_s": round(e * t, 5)}
+        guided = {}
         mfma = None
-        for spp in (32, 64, 128, 256):
-            with ctx.guide(seed=1) as g:
-                p = flgpu.default_params(spp=spp, depth=gen.depth, seed=7,
-                                         flags=base | flgpu.FL_RENDER_GUIDED | flgpu.FL_RENDER_TRAIN_GUIDE)
-                p.guide = g.h
-                ts = time.perf_counter()
-                img, _, st = sc.render(p)
-                guided.append((spp, time.perf_counter() - ts, rel_mse(img / spp, ref), st.ms_guide))
-                if spp == 64:
-                    steps = g.stats()[0]
-                    tp = flgpu.default_params(spp=spp, depth=gen.depth, seed=8,
-                                              flags=flgpu.FL_RENDER_VNDF_PDF | flgpu.FL_RENDER_GUIDED | flgpu.FL_RENDER_TIMING)
-                    tp.guide = g.h
-                    _, _, tst = sc.render(tp, want_film=False)
-                    infer_tf = tst.guide_queries * GUIDE_FWD_FLOP / (tst.ms_infer * 1e-3) / 1e12
-                    train_tf = steps * 16384 * GUIDE_TRAIN_FLOP / (st.ms_guide * 1e-3) / 1e12
-                    mfma = {"infer_queries": int(tst.guide_queries), "infer_ms": round(tst.ms_infer, 2),
-                            "infer_tflops": round(infer_tf, 2), "infer_mfma_util": round(infer_tf / MFMA_BF16_PEAK_TFLOPS, 5),
-                            "train_steps": steps, "train_ms": round(st.ms_guide, 1),
-                            "train_tflops": round(train_tf, 2), "train_mfma_util": round(train_tf / MFMA_BF16_PEAK_TFLOPS, 5)}
-    out["plain"] = [{"spp": s, "s": round(t, 4), "relmse": round(e, 5)} for s, t, e in plain]
-    out["guided"] = [{"spp": s, "s": round(t, 4), "relmse": round(e, 5), "train_ms": round(tm, 1)} for s, t, e, tm in guided]
-    # wall time the guided render needs to reach the plain 64-spp relMSE (log-log
-    # interpolation between guided points), over the plain render's time
-    target_t, target_e = plain[0][1], plain[0][2]
+        for name, rr in (("no_rr", NO_RR), ("rr3", 3)):
+            pts = []
+            for g_spp in ((8, 16, 32, spp) if name == "no_rr" else (spp,)):
+                with ctx.guide(seed=1) as g:
+                    p = flgpu.default_params(spp=g_spp, depth=gen.depth, seed=7,
+                                             flags=base | flgpu.FL_RENDER_GUIDED | flgpu.FL_RENDER_TRAIN_GUIDE)
+                    p.rr_start = rr
+                    p.guide = g.h
+                    t, img, st = timed(p)
+                    e = relmse(img / g_spp)
+                    pts.append({"spp": g_spp, "s": round(t, 4), "relmse": round(e, 5), "relmse_x_s": round(e * t, 5),
+                                "train_ms": round(st.ms_guide, 1), "passes": st.passes})
+                    if name == "no_rr" and g_spp == spp:
+                        steps = g.stats()[0]
+                        tp = flgpu.default_params(spp=spp, depth=gen.depth, seed=8,
+                                                  flags=flgpu.FL_RENDER_VNDF_PDF | flgpu.FL_RENDER_GUIDED |
+                                                  flgpu.FL_RENDER_TIMING)
+                        tp.rr_start = rr
+                        tp.guide = g.h
+                        _, _, tst = sc.render(tp, want_film=False)
+                        infer_tf = tst.guide_queries * GUIDE_FWD_FLOP / (tst.ms_infer * 1e-3) / 1e12
+                        train_tf = steps * 16384 * GUIDE_TRAIN_FLOP / (st.ms_guide * 1e-3) / 1e12
+                        mfma = {"infer_queries": int(tst.guide_queries), "infer_ms": round(tst.ms_infer, 2),
+                                "infer_tflops": round(infer_tf, 2),
+                                "infer_mfma_util": round(infer_tf / MFMA_BF16_PEAK_TFLOPS, 5),
+                                "render_ms": round(tst.ms_total, 1), "train_steps": steps,
+                                "train_ms": round(st.ms_guide, 1), "train_tflops": round(train_tf, 2),
+                                "train_mfma_util": round(train_tf / MFMA_BF16_PEAK_TFLOPS, 5)}
+            guided[name] = pts
+    out["plain"] = plain
+    out["guided"] = guided
+    # the better setting of each method at the config's spp, by relMSE x time
+    best_plain = min(plain, key=lambda k: plain[k]["relmse_x_s"])
+    best_guided = min(guided, key=lambda k: guided[k][-1]["relmse_x_s"])
+    target_t, target_e = plain[best_plain]["s"], plain[best_plain]["relmse"]
+    # wall time the guided render needs to reach that relMSE: log-log
+    # interpolation between guided points of the chosen setting
     need = None
-    pts = sorted((t, e) for _, t, e, _ in guided)
+    pts = sorted((q["s"], q["relmse"]) for q in guided[best_guided])
     for (t0, e0), (t1, e1) in zip(pts, pts[1:]):
         if e0 >= target_e >= e1 and e0 > 0 and e1 > 0 and e0 != e1:
             f = (np.log(target_e) - np.log(e0)) / (np.log(e1) - np.log(e0))
             need = float(np.exp(np.log(t0) + f * (np.log(t1) - np.log(t0))))
             break
     if need is None and pts and pts[0][1] <= target_e:
-        need = pts[0][0]
+        need = pts[0][0]   # the cheapest guided point already beats it: an upper bound
+    out["plain_setting"] = best_plain
+    out["guided_setting"] = best_guided
+    out["equal_spp_relmse_ratio"] = round(guided[best_guided][-1]["relmse"] / target_e, 4)
     out["time_ratio_to_plain_relmse"] = round(need / target_t, 3) if need else None
     out["mfma"] = mfma
     return out
@@ -237,8 +281,10 @@ def main():
     ap.add_argument("--no-batch", action="store_true", help="skip the 64-scene batch section")
     ap.add_argument("--no-guided", action="store_true", help="skip the guided (learned proposal) section")
     ap.add_argument("--batch-scenes", type=int, default=64)
-    ap.add_argument("--guided-res", type=int, default=256)
+    ap.add_argument("--guided-res", type=int, default=1024)
     ap.add_argument("--guided-ref-spp", type=int, default=16384)
+    ap.add_argument("--guided-skylight", action="store_true",
+                    help="also run the guided comparison on the skylight reading of the room")
     ap.add_argument("--terrain", action="store_true", help="also render the full terrain config (8 x 4K x 256 spp)")
     ap.add_argument("--terrain-spp", type=int, default=256)
     ap.add_argument("--terrain-res-div", type=int, default=1)
@@ -316,8 +362,12 @@ def main():
             log("batch: %s" % out["batch"])
         if not args.no_guided:
             log("guided section: %d^2, reference %d spp" % (args.guided_res, args.guided_ref_spp))
-            out["guided"] = guided_section(flgpu, ctx, args.guided_res, args.guided_ref_spp)
+            out["guided"] = guided_section(flgpu, ctx, flgpu.FL_GEN_GUIDED, args.guided_res, args.guided_ref_spp)
             log("guided: %s" % out["guided"])
+            if args.guided_skylight:
+                out["guided_skylight"] = guided_section(flgpu, ctx, flgpu.FL_GEN_GUIDED_SKYLIGHT, args.guided_res,
+                                                        args.guided_ref_spp)
+                log("guided skylight: %s" % out["guided_skylight"])
 
         if args.terrain:
             log("terrain section")

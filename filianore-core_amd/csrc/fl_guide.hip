@@ -474,7 +474,7 @@ int fl_guide_create(fl_ctx_t ctx, const fl_guide_config *cfg, fl_guide_t *out) {
     if (!(c.beta2 > 0.f)) c.beta2 = 0.99f;
     if (!(c.eps > 0.f)) c.eps = 1e-8f;
     if (!c.batch) c.batch = 16384;
-    if (!c.steps_per_pass) c.steps_per_pass = 64;
+    if (!c.steps_per_pass) c.steps_per_pass = 128;
     fl_guide_s *g = new (std::nothrow) fl_guide_s();
     if (!g) return set_error(ctx, FL_E_OOM, "fl_guide_create");
     g->ctx = ctx;

@@ -59,15 +59,23 @@ __device__ __forceinline__ int acc_n(int nt) { return nt * 16 + (threadIdx.x & 1
 // Input encoding of one sample (53 used values, zero-padded to 64):
 //   [16 a + 2 k] = sin(2^k pi x_a), [16 a + 2 k + 1] = cos(2^k pi x_a)  (a = 0..2, k = 0..7)
 //   [48..50] = normal, [51] = wavelength in [0,1], [52] = roughness, rest 0.
+// Band k + 1 comes from band k by the double-angle identities
+// sin 2t = 2 sin t cos t, cos 2t = (cos t - sin t)(cos t + sin t): one sincos
+// per axis instead of eight. Each doubling at most doubles the absolute error,
+// so band 7 is within ~1e-5 of sin/cos of 2^7 pi x; the network sees bf16
+// inputs (spacing 2^-8 near 1), so the difference does not reach it.
 __device__ __forceinline__ void guide_encode(const float pos01[3], v3 n, float lam01, float rough, float *x) {
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
-        float f = pos01[a];
+        float sn, cs;
+        sincospif(pos01[a], &sn, &cs);
 #pragma unroll
         for (int k = 0; k < kGBands; ++k) {
-            x[16 * a + 2 * k] = sinpif(f);
-            x[16 * a + 2 * k + 1] = cospif(f);
-            f = f * 2.f;
+            x[16 * a + 2 * k] = sn;
+            x[16 * a + 2 * k + 1] = cs;
+            const float s2 = 2.f * sn * cs;
+            cs = (cs - sn) * (cs + sn);
+            sn = s2;
         }
     }
     x[48] = n.x;
@@ -77,6 +85,56 @@ __device__ __forceinline__ void guide_encode(const float pos01[3], v3 n, float l
     x[52] = rough;
 #pragma unroll
     for (int i = 53; i < 64; ++i) x[i] = 0.f;
+}
+
+// Stores a sample's 64 encoded inputs as bf16 into row q of a [64][kGRow]
+// LDS tile (eight 16-byte stores; kGRow * 2 bytes is a multiple of 16).
+__device__ __forceinline__ void store_row_bf16(__bf16 *tile, int q, const float *x) {
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+        bf16x8 v;
+#pragma unroll
+        for (int e = 0; e < 8; ++e) v[e] = to_bf16(x[8 * j + e]);
+        *reinterpret_cast<bf16x8 *>(tile + q * kGRow + 8 * j) = v;
+    }
+}
+
+// guide_encode followed by store_row_bf16 without the 64-float array: the
+// row is written eight values at a time (an axis' bands 0-3, then 4-7), so
+// only a few of them are live at once. Zeros for a sample without a query.
+__device__ __forceinline__ void encode_row_bf16(__bf16 *tile, int q, bool want, const float pos01[3], v3 n,
+                                                float lam01, float rough) {
+    bf16x8 *row = reinterpret_cast<bf16x8 *>(tile + q * kGRow);
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        float sn = 0.f, cs = 0.f;
+        if (want) sincospif(pos01[a], &sn, &cs);
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            bf16x8 v;
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                v[2 * k] = to_bf16(sn);
+                v[2 * k + 1] = to_bf16(cs);
+                const float s2 = 2.f * sn * cs;
+                cs = (cs - sn) * (cs + sn);
+                sn = s2;
+            }
+            row[2 * a + h] = v;
+        }
+    }
+    bf16x8 t0, t1;
+#pragma unroll
+    for (int e = 0; e < 8; ++e) t0[e] = t1[e] = to_bf16(0.f);
+    if (want) {
+        t0[0] = to_bf16(n.x);
+        t0[1] = to_bf16(n.y);
+        t0[2] = to_bf16(n.z);
+        t0[3] = to_bf16(lam01);
+        t0[4] = to_bf16(rough);
+    }
+    row[6] = t0;
+    row[7] = t1;
 }
 
 // One hidden layer for the wave's 64 samples: Hout[q][o] = relu(sum_i W[o][i] Hin[q][i] + b[o]),
@@ -104,8 +162,10 @@ __device__ __forceinline__ void layer_fwd(const __bf16 *W, const float *b, const
         }
 }
 
-// Head: raw[q][o] = sum_i W[o][i] H[q][i] + b[o] in fp32, raw row stride rs.
-__device__ __forceinline__ void head_fwd(const __bf16 *W, const float *b, const __bf16 *H, float *raw, int rs) {
+// Head: raw[q][o] = sum_i W[o][i] H[q][i] + b[o] in fp32 for o < nout, raw
+// row stride rs.
+__device__ __forceinline__ void head_fwd(const __bf16 *W, const float *b, const __bf16 *H, float *raw, int rs,
+                                         int nout = kGH) {
     f32x4 acc[4][4];
     acc_zero(acc);
     mm64(W, kGH, H, kGRow, acc);
@@ -117,7 +177,7 @@ __device__ __forceinline__ void head_fwd(const __bf16 *W, const float *b, const 
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
                 const int o = acc_m(mt, r);
-                raw[q * rs + o] = acc[mt][nt][r] + b[o];
+                if (o < nout) raw[q * rs + o] = acc[mt][nt][r] + b[o];
             }
         }
 }

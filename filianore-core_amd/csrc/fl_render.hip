@@ -265,16 +265,23 @@ __device__ __forceinline__ void guide_features(const RenderArgs &a, v3 p, v3 ng,
 }
 
 // Network evaluation for every guidable hit of extension queue `b & 1`: one
-// wave per block, 64 slots per chunk, lobes written per queue slot.
-__global__ void __launch_bounds__(64) k_guide_infer(RenderArgs a, int b) {
-    __shared__ __bf16 sH[2][64 * kGRow];
-    __shared__ float sR[64 * 65];
+// wave per block, 64 slots per chunk, lobes written per queue slot. LDS holds
+// one tile, updated in place: a layer's fragments are all loaded (LDS returns
+// a wave's operations in order) before its epilogue overwrites the tile, and
+// the block is that one wave. The head's fp32 outputs (40 per sample, odd row
+// stride against bank conflicts) take the same bytes. 10.5 KB per wave.
+constexpr int kInferRaw = 41;
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3, 8))) k_guide_infer(RenderArgs a, int b) {
+    __shared__ __attribute__((aligned(16))) float sB[64 * kInferRaw];
+    static_assert(sizeof(float) * 64 * kInferRaw >= sizeof(__bf16) * 64 * kGRow, "sB holds a hidden tile");
+    __bf16 *sT = reinterpret_cast<__bf16 *>(sB);
     const int q = threadIdx.x;
     const uint32_t n = a.counts[C_EXT + (b & 1)];
     const uint32_t nchunks = (n + 63) / 64;
     for (uint32_t ch = blockIdx.x; ch < nchunks; ch += gridDim.x) {
         const uint32_t i = ch * 64u + q;
-        float x[64];
+        float pos01[3] = {0.f, 0.f, 0.f}, lam01 = 0.f, rough = 0.f;
+        v3 n_in(0.f, 0.f, 0.f);
         bool want = false;
         if (i < n) {
             const float4 hh = a.hits[i];
@@ -295,39 +302,47 @@ __global__ void __launch_bounds__(64) k_guide_infer(RenderArgs a, int b) {
                     const v3 p2(__uint_as_float(q2.x), __uint_as_float(q2.y), __uint_as_float(q2.z));
                     const v3 ng = normalize(cross(p1 - p0, p2 - p0));
                     const float lam0 = a.st_misc[sid].x;
-                    float pos01[3], lam01, rough;
-                    v3 n_in;
                     guide_features(a, p, ng, d, lam0, S.mats[mat], pos01, n_in, lam01, rough);
-                    guide_encode(pos01, n_in, lam01, rough, x);
                 }
             }
         }
-        if (!want)
-            for (int k = 0; k < 64; ++k) x[k] = 0.f;
         const unsigned long long wm = __ballot(want);
         if (q == 0 && wm) atomicAdd(&a.stats[6], (unsigned long long)__popcll(wm));
-        for (int k = 0; k < 64; ++k) sH[0][q * kGRow + k] = to_bf16(x[k]);
+        encode_row_bf16(sT, q, want, pos01, n_in, lam01, rough);
         __syncthreads();
         for (int k = 0; k < 4; ++k) {
-            layer_fwd(a.gW + k * 4096, a.gP + k * kGLayerParams + 4096, sH[k & 1], sH[(k + 1) & 1], nullptr, true);
+            layer_fwd(a.gW + k * 4096, a.gP + k * kGLayerParams + 4096, sT, sT, nullptr, true);
             __syncthreads();
         }
-        head_fwd(a.gW + 4 * 4096, a.gP + 4 * kGLayerParams + 4096, sH[0], sR, 65);
+        head_fwd(a.gW + 4 * 4096, a.gP + 4 * kGLayerParams + 4096, sT, sB, kInferRaw, kGOut);
         __syncthreads();
         if (want) {
             VmfLobe L[kGLobes];
-            guide_decode(sR + q * 65, L);
-            float *lb = a.lobes + (size_t)i * kGOut;
+            guide_decode(sB + q * kInferRaw, L);
+            float4 *lb = reinterpret_cast<float4 *>(a.lobes + (size_t)i * kGOut);
+            float o[kGOut];
             for (int l = 0; l < kGLobes; ++l) {
-                lb[5 * l + 0] = L[l].mu.x;
-                lb[5 * l + 1] = L[l].mu.y;
-                lb[5 * l + 2] = L[l].mu.z;
-                lb[5 * l + 3] = L[l].kappa;
-                lb[5 * l + 4] = L[l].w;
+                o[5 * l + 0] = L[l].mu.x;
+                o[5 * l + 1] = L[l].mu.y;
+                o[5 * l + 2] = L[l].mu.z;
+                o[5 * l + 3] = L[l].kappa;
+                o[5 * l + 4] = L[l].w;
             }
+#pragma unroll
+            for (int j = 0; j < kGOut / 4; ++j) lb[j] = make_float4(o[4 * j], o[4 * j + 1], o[4 * j + 2], o[4 * j + 3]);
         }
         __syncthreads();
     }
+}
+
+// Online training runs in kTrainRounds passes of doubling size over the
+// first train_spp samples per pixel; pass r ends after
+// train_spp (2^(r+1) - 1) / (2^R - 1) samples (r = R - 1 ends at train_spp).
+constexpr uint32_t kTrainRounds = 4;
+inline uint32_t train_round_end(uint32_t train_spp, uint32_t rounds, int r) {
+    if (r < 0) return 0u;
+    const uint64_t den = (1ull << rounds) - 1ull, num = (2ull << r) - 1ull;
+    return (uint32_t)std::max<uint64_t>((uint64_t)r + 1u, (uint64_t)train_spp * num / den);
 }
 
 // Training record weights after a pass: the radiance the path gathered beyond
@@ -1004,12 +1019,13 @@ int render_items(fl_ctx_t ctx, const HostItem *hi, uint32_t n_items, const fl_re
     // half the free device memory. Every item advances by the same number of
     // samples per pixel per pass.
     uint32_t max_paths = p->max_paths;
-    if (!max_paths && guide) max_paths = 1u << 24;   // guided passes also hold 160 B of lobes per path
     if (!max_paths) {
+        // guided passes also hold 160 B of lobes per path (2^26 paths: 10.7 GB)
+        const size_t per_path = kBytesPerPath + (guide ? (size_t)kGOut * sizeof(float) : 0u);
         size_t free_b = 0, total_b = 0;
         max_paths = 1u << 26;
         if (hipMemGetInfo(&free_b, &total_b) == hipSuccess) {
-            const size_t fit = free_b / 2 / kBytesPerPath;
+            const size_t fit = free_b / 2 / per_path;
             if (fit < max_paths) max_paths = (uint32_t)std::max<size_t>(fit, 1u << 20);
         }
     }
@@ -1098,9 +1114,10 @@ int render_items(fl_ctx_t ctx, const HostItem *hi, uint32_t n_items, const fl_re
             const float fr = p->guide_train_fraction > 0.f && p->guide_train_fraction <= 1.f ? p->guide_train_fraction
                                                                                               : 0.25f;
             train_spp = std::min(max_spp, std::max(1u, (uint32_t)std::lround(fr * (float)max_spp)));
-            train_rounds = std::min(8u, train_spp);
-            const uint32_t round_spp = (train_spp + train_rounds - 1) / train_rounds;
-            rec_cap = (uint32_t)std::min<size_t>((size_t)a.total_pix * std::min(round_spp, spp_pass) * 6, 1u << 24);
+            train_rounds = std::min(kTrainRounds, train_spp);
+            const uint32_t round_spp = train_round_end(train_spp, train_rounds, train_rounds - 1) -
+                                       train_round_end(train_spp, train_rounds, train_rounds - 2);
+            rec_cap = (uint32_t)std::min<size_t>((size_t)a.total_pix * std::min(round_spp, spp_pass) * 6, 1u << 26);
             st_rec = grow<uint32_t>(B.p[21], B.cap[21], cap * 4, err);
             recs = grow<float>(B.p[22], B.cap[22], (size_t)rec_cap * 16 * sizeof(float), err);
         }
@@ -1132,7 +1149,7 @@ int render_items(fl_ctx_t ctx, const HostItem *hi, uint32_t n_items, const fl_re
     // FL_RENDER_GRAPH a chunk is one captured graph, replayed.
     constexpr uint32_t kChunk = 4;   // even: a chunk starts and ends on queue parity 0
     const int variant = (counters ? 1 : 0) | (p->shade_waves >= 4 ? 2 : 0) | ((int)(tv - kTraceVariants) << 2);
-    const uint32_t infer_grid = (uint32_t)ctx->n_cu * 4;
+    const uint32_t infer_grid = (uint32_t)ctx->n_cu * 16;   // one-wave blocks of 10.5 KB LDS
     auto launch_shade = [&](const RenderArgs &args, dim3 g, int j, bool with_infer = true) {
         if (with_infer && args.guided) hipLaunchKernelGGL(k_guide_infer, dim3(infer_grid), dim3(64), 0, st, args, j);
         if (variant & 2) hipLaunchKernelGGL(k_shade<4>, g, dim3(kBlock), 0, st, args, j);
@@ -1175,9 +1192,10 @@ int render_items(fl_ctx_t ctx, const HostItem *hi, uint32_t n_items, const fl_re
         uint32_t this_spp = std::min(spp_pass, max_spp - done_spp);
         const bool training_pass = train && done_spp < train_spp && !guide->diverged;
         if (training_pass) {
-            // spread the training samples over train_rounds passes
-            const uint32_t left = train_rounds > round ? train_rounds - round : 1u;
-            this_spp = std::min(this_spp, std::max(1u, (train_spp - done_spp + left - 1) / left));
+            // training passes of doubling size: each trains the guide that
+            // samples the next, larger one
+            const uint32_t end = train_round_end(train_spp, train_rounds, std::min(round, train_rounds - 1));
+            this_spp = std::min(this_spp, std::max(1u, end > done_spp ? end - done_spp : 1u));
         }
         a.guided = (guide && !guide->diverged && !(training_pass && round == 0)) ? 1u : 0u;
         a.recs = training_pass ? recs : nullptr;

@@ -268,7 +268,7 @@ typedef struct {
     uint32_t batch;         /* training batch (samples per step) */
     uint64_t seed;
     float scene_min[3], scene_max[3]; /* position normalisation box (fl_render uses the scene's bounds) */
-    uint32_t steps_per_pass;  /* Adam steps after each training pass of a guided render (64) */
+    uint32_t steps_per_pass;  /* Adam steps after each of the 4 training passes of a guided render (128) */
 } fl_guide_config;
 
 int fl_guide_create(fl_ctx_t ctx, const fl_guide_config *cfg, fl_guide_t *out);
