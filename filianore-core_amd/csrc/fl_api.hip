@@ -137,6 +137,19 @@ int fl_scene_create(fl_ctx_t ctx, const fl_scene_desc *desc, fl_scene_t *out) {
     if (desc->n_materials == 0 || !desc->materials) return set_error(ctx, FL_E_INVALID, "fl_scene_create: no materials");
     if (desc->n_lights && (!desc->lights || !desc->emission_spd || desc->n_emission == 0))
         return set_error(ctx, FL_E_INVALID, "fl_scene_create: lights need emission spectra");
+    if (desc->n_delta_lights && (!desc->delta_lights || !desc->emission_spd || desc->n_emission == 0))
+        return set_error(ctx, FL_E_INVALID, "fl_scene_create: delta lights need emission spectra");
+    for (uint32_t l = 0; l < desc->n_delta_lights; ++l) {
+        const fl_delta_light &dl = desc->delta_lights[l];
+        const float *v = dl.direction;
+        const bool has_dir = v[0] != 0.f || v[1] != 0.f || v[2] != 0.f;
+        if (dl.type < FL_LIGHT_POINT || dl.type > FL_LIGHT_DIRECTIONAL || dl.emission >= desc->n_emission ||
+            dl.shadow_emission > desc->n_emission || (dl.type != FL_LIGHT_POINT && !has_dir))
+            return set_error(ctx, FL_E_INVALID, "fl_scene_create: bad delta light");
+    }
+    for (uint32_t l = 0; l < desc->n_lights; ++l)
+        if (desc->lights[l].shadow_emission > desc->n_emission)
+            return set_error(ctx, FL_E_INVALID, "fl_scene_create: bad light shadow color");
     for (uint32_t t = 0; t < 3ull * desc->n_triangles; ++t)
         if (desc->indices[t] >= desc->n_vertices) return set_error(ctx, FL_E_INVALID, "fl_scene_create: index out of range");
     if (desc->tri_material)
@@ -150,11 +163,11 @@ int fl_scene_create(fl_ctx_t ctx, const fl_scene_desc *desc, fl_scene_t *out) {
     s->n_tris = desc->n_triangles;
     s->n_verts = desc->n_vertices;
     s->n_mats = desc->n_materials;
-    s->n_lights = desc->n_lights;
+    s->n_lights = desc->n_lights + desc->n_delta_lights;
 
     // per-triangle light index + 1, lights' geometry (Triangle::sample / area)
     std::vector<uint32_t> tri_light(desc->n_triangles, 0u);
-    s->h_lights.resize(desc->n_lights);
+    s->h_lights.assign(s->n_lights, DevLight{});
     for (uint32_t l = 0; l < desc->n_lights; ++l) {
         const fl_area_light &al = desc->lights[l];
         if (al.tri >= desc->n_triangles || al.emission >= desc->n_emission) {
@@ -182,8 +195,33 @@ int fl_scene_create(fl_ctx_t ctx, const fl_scene_desc *desc, fl_scene_t *out) {
     flx::build_spectral_tables(s, desc, FL_SPECTRAL_PARITY, mats);
     flx::build_spectral_tables(s, desc, FL_SPECTRAL_HERO, mats);
     s->h_mats = mats;
-    for (uint32_t l = 0; l < desc->n_lights; ++l)
-        s->h_lights[l].em_slot = 3 + 2 * desc->n_materials + desc->lights[l].emission;
+    const uint32_t em0 = 3 + 2 * desc->n_materials;   // table slot of emission 0
+    for (uint32_t l = 0; l < desc->n_lights; ++l) {
+        DevLight &d = s->h_lights[l];
+        d.em_slot = em0 + desc->lights[l].emission;
+        d.shadow_slot = desc->lights[l].shadow_emission ? em0 + desc->lights[l].shadow_emission - 1 : 0u;
+    }
+    // delta lights: point.cpp:6-10, spot.cpp:6-21, directional.cpp:6-13
+    for (uint32_t l = 0; l < desc->n_delta_lights; ++l) {
+        const fl_delta_light &dl = desc->delta_lights[l];
+        DevLight &d = s->h_lights[desc->n_lights + l];
+        d.type = (uint32_t)dl.type;
+        d.tri = fl::kNoHit;
+        d.intensity = dl.intensity;
+        d.em_slot = em0 + dl.emission;
+        d.shadow_slot = dl.shadow_emission ? em0 + dl.shadow_emission - 1 : 0u;
+        if (dl.type == FL_LIGHT_DIRECTIONAL) {
+            // dirIllum = normalize(-vector_transform(dir)): toward the light
+            const float v[3] = {-dl.direction[0], -dl.direction[1], -dl.direction[2]};
+            const float len = std::sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+            for (int k = 0; k < 3; ++k) d.p0[k] = v[k] / len;
+        } else {
+            for (int k = 0; k < 3; ++k) { d.p0[k] = dl.position[k]; d.p1[k] = dl.direction[k]; }
+            d.cos_cone = dl.cos_cone;
+            d.cos_penumbra = dl.cos_penumbra;
+        }
+    }
+    for (const DevLight &d : s->h_lights) s->shadow_colors = s->shadow_colors || d.shadow_slot != 0u;
 
     int rc;
     float *d_verts = nullptr;

@@ -19,13 +19,20 @@ struct DevMaterial {        // 32 bytes
     float cauchy_b;         // dielectric Cauchy B (um^2), 0 = scalar eta
 };
 
-struct DevLight {           // 64 bytes: one DiffuseAreaIlluminant over one triangle
-    float p0[3], p1[3], p2[3];
-    float n[3];             // normalize(cross(p1-p0, p2-p0)), triangle.cpp:146
-    float area;             // 0.5*|cross(p1-p0, p2-p0)|, triangle.cpp:130-132
+// One illuminant (80 bytes). Area lights (type 0) come first, in
+// fl_scene_desc.lights order, then the delta lights.
+struct DevLight {
+    float p0[3], p1[3], p2[3];   // area: the triangle. point, spot: p0 = posIllum;
+                                 // spot: p1 = the cone axis (dirIllum);
+                                 // directional: p0 = dirIllum = normalize(-direction)
+    float n[3];             // area: normalize(cross(p1-p0, p2-p0)), triangle.cpp:146
+    float area;             // area: 0.5*|cross(p1-p0, p2-p0)|, triangle.cpp:130-132
     float intensity;
-    uint32_t em_slot;       // emission table slot
-    uint32_t tri;
+    uint32_t em_slot;       // emission table slot (the light's color)
+    uint32_t tri;           // area: emitting triangle; delta: kNoHit
+    uint32_t type;          // 0 = area, else FL_LIGHT_*
+    uint32_t shadow_slot;   // shadowColor: 0 = black, else its table slot
+    float cos_cone, cos_penumbra;   // spot
 };
 
 // Spectral tables of one scene in one grid: [cmf x, y, z, materials (2 each),
@@ -55,6 +62,8 @@ struct fl_ctx_s {
 struct fl_scene_s {
     fl_ctx_t ctx = nullptr;
     uint32_t n_tris = 0, n_verts = 0, n_nodes = 0, depth = 0, n_mats = 0, n_lights = 0;
+    uint32_t rgb_basis = 0;         // table slot of the first of the 7 RGB->spectrum basis curves
+    bool shadow_colors = false;     // some light has a non-black shadowColor
     fl::Node8 *d_nodes = nullptr;
     fl::TriRec *d_tris = nullptr;
     DevMaterial *d_mats = nullptr;

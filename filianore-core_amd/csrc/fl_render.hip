@@ -127,6 +127,16 @@ __global__ void __launch_bounds__(kBlock) k_generate(RenderArgs a) {
 // Index space: [0, ns) shadow rays written by the previous bounce's shade,
 // [ns, ns + ne) this bounce's extension rays.
 // In batches each ray traverses the BVH of its path's item.
+//
+// Shadow rays follow VisibilityEvaluator::unoccluded (illuminant.cpp:40-50):
+// the closest hit on the segment decides, and the segment counts as
+// unoccluded when that hit is an emitter. An area light's own segment ends on
+// the light, so any hit before it that is not an emitter decides the same
+// way: those rays run any-hit with emitters skipped. A delta light's segment
+// can pass through an emitter and go on; its shadow rays (target slot tagged
+// kVisClosest) run closest-hit and test the hit's light flag.
+constexpr uint32_t kVisClosest = 0x80000000u;
+
 struct TraceSrc {
     const float4 *so, *sd;
     const uint32_t *stgt;
@@ -139,22 +149,23 @@ struct TraceSrc {
     const uint32_t *st_item;
     uint32_t n_items;
     TraceScene sc0;
+    const float4 *socc;      // NEE term of an occluded shadow ray (shadowColor), or null: 0
+    __device__ TraceScene scene_of_ray(uint32_t i) const {
+        if (n_items <= 1) return sc0;
+        uint32_t sid;
+        if (i < ns) {
+            const uint32_t t = stgt[i] & ~kVisClosest;
+            sid = t >= cap ? t - cap : t;   // target slot j * cap + path id
+        } else {
+            sid = esid[i - ns];
+        }
+        return items[st_item[sid]].sc;
+    }
     __device__ void load(uint32_t i, v3 &ro, v3 &rd, float &tmin, float &tmax, bool &any, TraceScene &sc) const {
         float4 a, b;
-        if (i < ns) { a = so[i]; b = sd[i]; any = true; }
+        if (i < ns) { a = so[i]; b = sd[i]; any = (stgt[i] & kVisClosest) == 0u; }
         else { a = eo[i - ns]; b = ed[i - ns]; any = false; }
-        if (n_items > 1) {
-            uint32_t sid;
-            if (i < ns) {
-                const uint32_t t = stgt[i];
-                sid = t >= cap ? t - cap : t;   // target slot j * cap + path id
-            } else {
-                sid = esid[i - ns];
-            }
-            sc = items[st_item[sid]].sc;
-        } else {
-            sc = sc0;
-        }
+        sc = scene_of_ray(i);
         ro = v3(a.x, a.y, a.z);
         rd = v3(b.x, b.y, b.z);
         tmin = a.w;
@@ -162,7 +173,15 @@ struct TraceSrc {
     }
     __device__ void store(uint32_t i, const TravState &s) const {
         if (i < ns) {
-            if (s.best_id != kNoHit) pend[stgt[i]] = make_float4(0.f, 0.f, 0.f, 0.f);   // occluded NEE term
+            const uint32_t t = stgt[i];
+            bool occluded = s.best_id != kNoHit;
+            if (occluded && (t & kVisClosest)) {
+                // closest hit on the segment: an emitter there leaves it unoccluded
+                const uint4 *tp = reinterpret_cast<const uint4 *>(scene_of_ray(i).tris + s.best_rec);
+                occluded = tp[2].w == 0u;
+            }
+            if (occluded)   // Li = shadowColor (integrator.cpp:47-50)
+                pend[t & ~kVisClosest] = socc ? socc[i] : make_float4(0.f, 0.f, 0.f, 0.f);
         } else {
             hits[i - ns] = make_float4(s.best_t, __uint_as_float(s.best_id != kNoHit ? s.best_rec : kNoHit), s.bu, s.bv);
         }
@@ -192,7 +211,7 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WAV
     ls.spill_stride = a.spill_stride;
     TravCounters c;
     TraceSrc src{a.sq_o,    a.sq_d, a.sq_tgt, a.pend,    a.rq_o[cur], a.rq_d[cur], a.rq_sid[cur],
-                 a.hits,    ns,     a.cap,    a.items,   a.st_item,   a.n_items,   a.sc};
+                 a.hits,    ns,     a.cap,    a.items,   a.st_item,   a.n_items,   a.sc, a.sq_occ};
     trace_persistent<COUNT, true>(src, ns + ne, &a.counts[C_FETCH + cur], ls, a.trav_budget, a.refill_lanes, c);
     if (COUNT) {
         atomicAdd(&a.stats[3], (unsigned long long)c.nodes);
@@ -385,14 +404,75 @@ __device__ __forceinline__ void setup_bsdf(const RenderArgs &a, const float *spe
     b.dispersive = m.cauchy_b != 0.f;
 }
 
+// The value of light Lj toward the shading point for delta lights
+// (sample_li of point.cpp:12-20, spot.cpp:23-31, directional.cpp:15-24):
+// the direction wi toward the light, the unnormalized segment to its
+// visibility point and the factor intensity * falloff / decay that scales
+// the light's color. Returns false when the light gives nothing there.
+__device__ __forceinline__ bool delta_light_sample(const DevLight &Lj, v3 p, v3 &wi, v3 &seg, float &scale) {
+    if (Lj.type == FL_LIGHT_DIRECTIONAL) {
+        wi = ld3(Lj.p0);
+        seg = wi * (2.f * 1000.f);          // pOutside = p + dirIllum * 2 worldRadius
+        scale = Lj.intensity;
+        return true;
+    }
+    const v3 pos = ld3(Lj.p0);
+    seg = pos - p;
+    if (length2(seg) == 0.f) return false;
+    wi = normalize(seg);
+    const float dl = length(seg);
+    float s_ = Lj.intensity / (dl * dl);    // evaluate_decay_rate: d^2 whatever the decay rate
+    if (Lj.type == FL_LIGHT_SPOT) {
+        // SpotIlluminant::Falloff(-wi), spot.cpp:41-55
+        const float ct = dot(normalize(-wi), normalize(ld3(Lj.p1)));
+        float fo;
+        if (ct < Lj.cos_cone) fo = 0.f;
+        else if (ct > Lj.cos_penumbra) fo = 1.f;
+        else {
+            const float t = fminf(fmaxf((ct - Lj.cos_cone) / (Lj.cos_penumbra - Lj.cos_cone), 0.f), 1.f);
+            fo = t * t * (3.f - 2.f * t);
+        }
+        s_ = Lj.intensity * fo / (dl * dl);
+    }
+    scale = s_;
+    return scale != 0.f;
+}
+
 // One NEE sample of light `li` (estimate_direct's light half, integrator.cpp:29-59):
-// returns true with the (unoccluded) contribution and the shadow segment.
+// returns true with the (unoccluded) contribution, the contribution if the
+// segment turns out occluded (Li = shadowColor) and the shadow segment.
 __device__ __forceinline__ bool nee_sample(const RenderArgs &a, const SceneRef &S, const Bsdf &b, const Lanes &ln,
                                            uint32_t li, float sel,
                                            float ux, float uy, v3 p, v3 perr, v3 ng, v3 fs, v3 ft, v3 woW, v3 wo,
-                                           const float beta[4], float contrib[4], float4 &sh_o, float4 &sh_d,
-                                           const float *lb, float mix) {
+                                           const float beta[4], float contrib[4], float occ[4], float4 &sh_o,
+                                           float4 &sh_d, const float *lb, float mix) {
     const DevLight &Lj = S.lights[li];
+    const float *te = S.spec + (size_t)Lj.em_slot * a.grid.stride;
+    const float *ts = Lj.shadow_slot ? S.spec + (size_t)Lj.shadow_slot * a.grid.stride : nullptr;
+    if (Lj.type != 0u) {
+        // delta light: pdf 1, no MIS (integrator.cpp:53-54)
+        v3 wi, seg;
+        float scale;
+        if (!delta_light_sample(Lj, p, wi, seg, scale)) return false;
+        v3 wil(dot(wi, fs), dot(wi, ft), dot(wi, ng));
+        bool refl = dot(wi, ng) * dot(woW, ng) > 0.f;
+        float f[4];
+        bsdf_eval_nonspec(b, wo, wil, refl, f);
+        const float cw = absdot(wi, ng);
+        bool black = true;
+        for (int k = 0; k < 4; ++k) { f[k] = f[k] * cw; black = black && f[k] == 0.f; }
+        if (black) return false;
+        for (int k = 0; k < 4; ++k) {
+            contrib[k] = beta[k] * f[k] * (spec_at(te, ln, k) * scale) / sel;
+            occ[k] = ts ? beta[k] * f[k] * spec_at(ts, ln, k) / sel : 0.f;
+        }
+        // kindle_ray_to(target): d = target - p, origin offset along d
+        const v3 sd = Lj.type == FL_LIGHT_DIRECTIONAL ? (p + seg) - p : seg;
+        v3 so = offset_point_origin(p, perr, ng, sd);
+        sh_o = make_float4(so.x, so.y, so.z, kShadowEps);
+        sh_d = make_float4(sd.x, sd.y, sd.z, 1.f - kShadowEps);
+        return true;
+    }
     // Triangle::sample + Shape::sample(ref), triangle.cpp:134-154, shape.cpp:28-41
     float uu = sqrtf(ux);
     float u1 = 1.f - uu, u2 = uy * uu;
@@ -419,12 +499,52 @@ __device__ __forceinline__ bool nee_sample(const RenderArgs &a, const SceneRef &
     for (int k = 0; k < 4; ++k) { f[k] = f[k] * cw; black = black && f[k] == 0.f; }
     if (black) return false;
     float w = power_heuristic(pdf_l, pb);
-    const float *te = S.spec + (size_t)Lj.em_slot * a.grid.stride;
-    for (int k = 0; k < 4; ++k) contrib[k] = beta[k] * f[k] * (spec_at(te, ln, k) * Lj.intensity) * w / pdf_l;
+    for (int k = 0; k < 4; ++k) {
+        contrib[k] = beta[k] * f[k] * (spec_at(te, ln, k) * Lj.intensity) * w / pdf_l;
+        occ[k] = ts ? beta[k] * f[k] * spec_at(ts, ln, k) * w / pdf_l : 0.f;
+    }
     v3 so = offset_point_origin(p, perr, ng, wun);                 // kindle_ray_to
     sh_o = make_float4(so.x, so.y, so.z, kShadowEps);
     sh_d = make_float4(wun.x, wun.y, wun.z, 1.f - kShadowEps);
     return true;
+}
+
+// NormalIntegrator::Li (normalintegrator.cpp:11-26): from_reflectance_rgb of
+// the hit's normal (spectrumoperations.cpp:113-147) at the path's four
+// wavelengths. basis: the 7 basis curves (white, cyan, magenta, yellow, red,
+// green, blue), GS floats apart. The combination is formed, scaled by .94
+// and clamped at the two table entries around each wavelength, as the host
+// forms a material's spectrum on the table grid, then interpolated.
+__device__ __forceinline__ void normal_spectrum(const float *basis, int GS, v3 n, const Lanes &ln, float L[4]) {
+    const float r = n.x, g = n.y, bl = n.z;
+    int b1, b2;
+    float c0, c1, c2;
+    if (r <= g && r <= bl) {
+        c0 = r;
+        if (g <= bl) { b1 = 1; c1 = g - r; b2 = 6; c2 = bl - g; }
+        else { b1 = 1; c1 = bl - r; b2 = 5; c2 = g - bl; }
+    } else if (g <= r && g <= bl) {
+        c0 = g;
+        if (r <= bl) { b1 = 2; c1 = r - g; b2 = 6; c2 = bl - r; }
+        else { b1 = 2; c1 = bl - g; b2 = 4; c2 = r - bl; }
+    } else {
+        c0 = bl;
+        if (r <= g) { b1 = 3; c1 = r - bl; b2 = 5; c2 = g - r; }
+        else { b1 = 3; c1 = g - bl; b2 = 4; c2 = r - g; }
+    }
+    const float *w = basis, *s1 = basis + (size_t)b1 * GS, *s2 = basis + (size_t)b2 * GS;
+    for (int k = 0; k < 4; ++k) {
+        float v[2];
+        for (int e = 0; e < 2; ++e) {
+            const int i = ln.idx[k] + e;
+            float x = 0.f + w[i] * c0;
+            x = x + s1[i] * c1;
+            x = x + s2[i] * c2;
+            x = x * 0.94f;
+            v[e] = x < 0.f ? 0.f : x;   // spectrum_clamp
+        }
+        L[k] = v[0] + ln.frac[k] * (v[1] - v[0]);
+    }
 }
 
 // WAVES: minimum waves per SIMD requested from the register allocator (2 lets
@@ -517,7 +637,10 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WAV
                 ng = normalize(cross(p1 - p0, p2 - p0));
                 ln = make_lanes(a.grid, lam0);
                 // emission: L += beta * le(-dir) at bounce 0 / after specular; MIS otherwise
-                if (light) {
+                if (a.flags & FL_RENDER_NORMALS) {
+                    normal_spectrum(S.spec + (size_t)S.rgb_basis * GS, GS, ng, ln, L);
+                    flags |= PF_LAST;   // NormalIntegrator: the first hit ends the path
+                } else if (light) {
                     const DevLight &Lj = S.lights[light - 1];
                     if (dot(ng, -d) > 0.f) {
                         float w = 0.f;
@@ -576,9 +699,9 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WAV
         const bool do_nee = mat_hit && !is_specular_kind(bs_.kind) && S.n_lights > 0;
 #pragma unroll
         for (int j = 0; j < 2; ++j) {
-            float contrib[4] = {0.f, 0.f, 0.f, 0.f};
+            float contrib[4] = {0.f, 0.f, 0.f, 0.f}, occ[4] = {0.f, 0.f, 0.f, 0.f};
             float4 sh_o, sh_d;
-            bool ok = false;
+            bool ok = false, nee_delta = false;
             const int nslots = S.nee_all ? (int)S.n_lights : 1;
             if (do_nee && j < nslots) {
                 uint32_t li = (uint32_t)j;
@@ -588,17 +711,19 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WAV
                     sel = 1.f / (float)S.n_lights;
                 }
                 ok = nee_sample(a, S, bs_, ln, li, sel, rng_u(rb, 3 + 2 * j), rng_u(rb, 4 + 2 * j), p, perr, ng, fs, ft,
-                                woW, wo, beta, contrib, sh_o, sh_d, lb, a.guide_mix);
+                                woW, wo, beta, contrib, occ, sh_o, sh_d, lb, a.guide_mix);
+                nee_delta = S.lights[li].type != 0u;
             }
             if (active) {   // every live path rewrites both slots (0 when no sample)
-                if (flags & PF_SECTERM) contrib[1] = contrib[2] = contrib[3] = 0.f;
+                if (flags & PF_SECTERM) contrib[1] = contrib[2] = contrib[3] = occ[1] = occ[2] = occ[3] = 0.f;
                 a.pend[(size_t)j * a.cap + sid] = make_float4(contrib[0], contrib[1], contrib[2], contrib[3]);
             }
             uint32_t ss = wave_append(ok, &a.counts[C_SH + cur]);
             if (ok) {
                 a.sq_o[ss] = sh_o;
                 a.sq_d[ss] = sh_d;
-                a.sq_tgt[ss] = (uint32_t)j * a.cap + sid;
+                a.sq_tgt[ss] = ((uint32_t)j * a.cap + sid) | (nee_delta ? kVisClosest : 0u);
+                if (a.sq_occ) a.sq_occ[ss] = make_float4(occ[0], occ[1], occ[2], occ[3]);
             }
         }
         // -------- phase 3: BSDF sample (continuation and MIS BSDF ray), RR
@@ -980,6 +1105,7 @@ int render_items(fl_ctx_t ctx, const HostItem *hi, uint32_t n_items, const fl_re
     a.lights = s->d_lights;
     a.n_lights = s->n_lights;
     a.nee_all = s->n_lights <= 2 ? 1u : 0u;
+    a.rgb_basis = s->rgb_basis;
     a.spec = tab.d_tab;
     a.grid = tab.grid;
     a.seed = p->seed;
@@ -997,6 +1123,7 @@ int render_items(fl_ctx_t ctx, const HostItem *hi, uint32_t n_items, const fl_re
         it.spec = sk->spec[mode].d_tab;
         it.n_lights = sk->n_lights;
         it.nee_all = sk->n_lights <= 2 ? 1u : 0u;
+        it.rgb_basis = sk->rgb_basis;
         it.cam = make_camera(&hi[k].cam);
         it.tiles_x = (hi[k].cam.width + 7) / 8;
         it.npix = it.tiles_x * ((hi[k].cam.height + 7) / 8) * 64u;
@@ -1031,6 +1158,8 @@ int render_items(fl_ctx_t ctx, const HostItem *hi, uint32_t n_items, const fl_re
     }
     if (max_paths < a.total_pix) max_paths = a.total_pix;
     uint32_t spp_pass = std::max(1u, std::min(max_spp, max_paths / a.total_pix));
+    if ((uint64_t)a.total_pix * spp_pass * 2u >= (uint64_t)kVisClosest)   // shadow targets j * cap + id, tag bit 31
+        return set_error(ctx, FL_E_INVALID, "fl_render: too many paths per pass (at most 2^30)");
     a.cap = a.total_pix * spp_pass;
     a.depth_d = p->diffuse_depth;
     a.depth_r = p->reflection_depth;
@@ -1075,6 +1204,9 @@ int render_items(fl_ctx_t ctx, const HostItem *hi, uint32_t n_items, const fl_re
     a.sq_o = grow<float4>(B.p[12], B.cap[12], 2 * cap * 16, err);
     a.sq_d = grow<float4>(B.p[13], B.cap[13], 2 * cap * 16, err);
     a.sq_tgt = grow<uint32_t>(B.p[14], B.cap[14], 2 * cap * 4, err);
+    bool shadow_colors = false;
+    for (uint32_t k = 0; k < n_items; ++k) shadow_colors = shadow_colors || hi[k].scene->shadow_colors;
+    a.sq_occ = shadow_colors ? grow<float4>(B.p[25], B.cap[25], 2 * cap * 16, err) : nullptr;
     a.counts = grow<uint32_t>(B.p[15], B.cap[15], 64, err);
     a.stats = grow<unsigned long long>(B.p[16], B.cap[16], 256, err);
     a.film_sum = grow<float>(B.p[17], B.cap[17], (size_t)film_pix * 12, err);

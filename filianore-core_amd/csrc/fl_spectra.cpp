@@ -133,7 +133,7 @@ void build_spectral_tables(fl_scene_s *s, const fl_scene_desc *desc, int mode, s
     }
     g.stride = stride;
     const uint32_t nm = desc->n_materials, ne = desc->n_emission;
-    const size_t slots = 3 + 2 * (size_t)nm + ne;
+    const size_t slots = 3 + 2 * (size_t)nm + ne + 7;   // + the RGB->spectrum basis (normal AOV)
     T.host.assign(slots * stride, 0.f);
     float *tab = T.host.data();
     // colour matching functions
@@ -187,6 +187,12 @@ void build_spectral_tables(fl_scene_s *s, const fl_scene_desc *desc, int mode, s
     for (int i = 0; i < FL_SPD_N; ++i) spd_lam[i] = 360.f + (float)i;
     for (uint32_t e = 0; e < ne; ++e)
         project(mode, spd_lam, desc->emission_spd + (size_t)e * FL_SPD_N, FL_SPD_N, tab + (3 + 2 * (size_t)nm + e) * stride);
+    // the reflectance basis, white cyan magenta yellow red green blue, for
+    // from_reflectance_rgb on the device (NormalIntegrator)
+    s->rgb_basis = (uint32_t)(3 + 2 * (size_t)nm + ne);
+    const std::vector<float> *basis[7] = {&B.white, &B.cyan, &B.magenta, &B.yellow, &B.red, &B.green, &B.blue};
+    for (int j = 0; j < 7; ++j)
+        std::copy(basis[j]->begin(), basis[j]->begin() + stride, tab + (s->rgb_basis + (size_t)j) * stride);
 }
 
 } // namespace flx

@@ -46,6 +46,7 @@ struct ItemDev {
     const DevLight *lights;
     const float *spec;       // spectral tables of the render's mode
     uint32_t n_lights, nee_all;
+    uint32_t rgb_basis;      // table slot of the RGB->spectrum basis
     CameraDev cam;
     uint32_t npix, tiles_x;  // padded pixels per sample layer (8x8 tiles), tiles per row
     uint32_t spp_pass, sample_base;   // this pass: samples per pixel, index of the first
@@ -62,6 +63,7 @@ struct SceneRef {
     const DevLight *lights;
     const float *spec;
     uint32_t n_lights, nee_all;
+    uint32_t rgb_basis;
     uint64_t seed;
 };
 
@@ -76,6 +78,7 @@ struct RenderArgs {
     const DevLight *lights;
     uint32_t n_lights;
     uint32_t nee_all;        // 1: one NEE sample per light (uniform_sample_all_lights)
+    uint32_t rgb_basis;      // table slot of the RGB->spectrum basis (normal AOV)
     const float *spec;
     SpecGrid grid;
     uint32_t n_paths;        // this pass: path ids of all items
@@ -96,6 +99,7 @@ struct RenderArgs {
     float4 *hits;
     float4 *sq_o, *sq_d;
     uint32_t *sq_tgt;
+    float4 *sq_occ;          // per shadow ray: its NEE term when occluded (shadowColor); null if all black
     uint32_t *counts;        // [0],[1] ray queues, [2] shadow queue
     unsigned long long *stats; // [0] ext rays, [1] shadow rays, [2] nan samples, [3] nodes, [4] tris, [5] spills
     // films of all items, concatenated (pix_base of each item)
@@ -134,6 +138,7 @@ __device__ __forceinline__ SceneRef scene_of(const RenderArgs &a, uint32_t item)
         r.spec = it.spec;
         r.n_lights = it.n_lights;
         r.nee_all = it.nee_all;
+        r.rgb_basis = it.rgb_basis;
         r.seed = it.seed;
     } else {
         r.sc = a.sc;
@@ -142,6 +147,7 @@ __device__ __forceinline__ SceneRef scene_of(const RenderArgs &a, uint32_t item)
         r.spec = a.spec;
         r.n_lights = a.n_lights;
         r.nee_all = a.nee_all;
+        r.rgb_basis = a.rgb_basis;
         r.seed = a.seed;
     }
     return r;

@@ -18,7 +18,8 @@ FL_MAT_NONE, FL_MAT_LAMBERT, FL_MAT_CONDUCTOR, FL_MAT_DIELECTRIC = 0, 1, 2, 3
 FL_CLOSEST_HIT, FL_ANY_HIT, FL_SKIP_LIGHTS, FL_COUNTERS = 0, 1, 2, 4
 FL_SPECTRAL_HERO, FL_SPECTRAL_PARITY = 0, 1
 FL_RENDER_GUIDED, FL_RENDER_COUNTERS, FL_RENDER_VNDF_PDF, FL_RENDER_GRAPH = 1, 2, 4, 8
-FL_RENDER_TRAIN_GUIDE, FL_RENDER_NO_SORT, FL_RENDER_TIMING = 16, 32, 64
+FL_RENDER_TRAIN_GUIDE, FL_RENDER_NO_SORT, FL_RENDER_TIMING, FL_RENDER_NORMALS = 16, 32, 64, 128
+FL_LIGHT_POINT, FL_LIGHT_SPOT, FL_LIGHT_DIRECTIONAL = 1, 2, 3
 FL_NO_HIT = 0xFFFFFFFF
 FL_SPD_N = 471
 FL_GEN_CORNELL, FL_GEN_SPHERES, FL_GEN_BATCH, FL_GEN_GUIDED, FL_GEN_TERRAIN, FL_GEN_GUIDED_SKYLIGHT = 0, 1, 2, 3, 4, 5
@@ -35,7 +36,14 @@ class Material(C.Structure):
 
 
 class AreaLight(C.Structure):
-    _fields_ = [("tri", C.c_uint32), ("emission", C.c_uint32), ("intensity", C.c_float), ("pad", C.c_uint32)]
+    _fields_ = [("tri", C.c_uint32), ("emission", C.c_uint32), ("intensity", C.c_float),
+                ("shadow_emission", C.c_uint32)]
+
+
+class DeltaLight(C.Structure):
+    _fields_ = [("type", C.c_int32), ("position", f3), ("direction", f3), ("emission", C.c_uint32),
+                ("intensity", C.c_float), ("cos_cone", C.c_float), ("cos_penumbra", C.c_float),
+                ("shadow_emission", C.c_uint32)]
 
 
 class SceneDesc(C.Structure):
@@ -44,7 +52,8 @@ class SceneDesc(C.Structure):
                 ("tri_material", C.POINTER(C.c_uint32)),
                 ("materials", C.POINTER(Material)), ("n_materials", C.c_uint32),
                 ("lights", C.POINTER(AreaLight)), ("n_lights", C.c_uint32),
-                ("emission_spd", C.POINTER(C.c_float)), ("n_emission", C.c_uint32)]
+                ("emission_spd", C.POINTER(C.c_float)), ("n_emission", C.c_uint32),
+                ("delta_lights", C.POINTER(DeltaLight)), ("n_delta_lights", C.c_uint32)]
 
 
 class SceneInfo(C.Structure):
@@ -237,6 +246,8 @@ class GenSceneData:
             self.emission = np.ctypeslib.as_array(g.emission_spd, (g.n_emission * FL_SPD_N,)).copy() \
                 if g.n_emission else np.zeros(FL_SPD_N, np.float32)
             self.n_emission = g.n_emission
+            self.delta_lights = (DeltaLight * 1)()
+            self.n_delta_lights = 0
             self.camera = Camera()
             C.pointer(self.camera)[0] = g.camera
             self.spp = g.spp
@@ -245,18 +256,22 @@ class GenSceneData:
             L.fl_gen_free(C.byref(g))
 
     @classmethod
-    def from_arrays(cls, vertices, indices, tri_material, materials, lights, emission, camera, spp=16, depth=8):
+    def from_arrays(cls, vertices, indices, tri_material, materials, lights, emission, camera, spp=16, depth=8,
+                    delta_lights=()):
         """A scene from explicit buffers: materials is a list of Material,
-        lights a list of (tri, emission_index, intensity), emission an
-        (n, FL_SPD_N) array, camera a Camera."""
+        lights a list of (tri, emission_index, intensity[, shadow_emission]),
+        delta_lights a list of DeltaLight, emission an (n, FL_SPD_N) array,
+        camera a Camera."""
         self = cls.__new__(cls)
         self.vertices = np.ascontiguousarray(vertices, np.float32).reshape(-1, 3)
         self.indices = np.ascontiguousarray(indices, np.uint32).reshape(-1, 3)
         self.tri_material = np.ascontiguousarray(tri_material, np.uint32)
         self.materials = (Material * len(materials))(*materials)
-        ls = [AreaLight(t, e, i, 0) for (t, e, i) in lights]
+        ls = [AreaLight(*(tuple(l) + (0,) * (4 - len(l)))) for l in lights]
         self.lights = (AreaLight * max(1, len(ls)))(*ls)
         self.n_lights = len(ls)
+        self.delta_lights = (DeltaLight * max(1, len(delta_lights)))(*delta_lights)
+        self.n_delta_lights = len(delta_lights)
         em = np.ascontiguousarray(emission, np.float32).reshape(-1, FL_SPD_N)
         self.emission = em.ravel().copy()
         self.n_emission = em.shape[0]
@@ -285,6 +300,8 @@ class GenSceneData:
         d.n_lights = self.n_lights
         d.emission_spd = e.ctypes.data_as(C.POINTER(C.c_float))
         d.n_emission = self.n_emission
+        d.delta_lights = C.cast(self.delta_lights, C.POINTER(DeltaLight))
+        d.n_delta_lights = self.n_delta_lights
         return d
 
     def triangles(self):

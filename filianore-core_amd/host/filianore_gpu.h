@@ -113,6 +113,56 @@ struct DiffuseAreaIlluminant {
     size_t mesh = 0, triangle = 0;   // the emitting triangle of the geometry
     std::vector<float> spd;
     float intensity = 1.f;
+    std::vector<float> shadowColor;  // Illuminant::shadowColor (FL_SPD_N samples); empty = black
+};
+
+// Delta illuminants. The reference builds them from a lightToWorld Transform;
+// here the caller passes what the constructors derive from it: the light's
+// world position (lightToWorld applied to the origin) and its world
+// direction (vector_transform of dirIllum).
+// PointIlluminant (illuminants/point.cpp:6-10)
+struct PointIlluminant {
+    Vector3f position;
+    std::vector<float> color;        // FL_SPD_N samples on 360..830 nm
+    float intensity = 1.f;
+    std::vector<float> shadowColor;  // empty = black
+};
+
+// SpotIlluminant (illuminants/spot.cpp:6-21): coneAngle / penumbraAngle in
+// degrees unless angleInRadians, halved when usehalfAngles, as the reference
+// constructor does; light is 0 outside the cone, 1 inside the penumbra
+// angle, smoothstep between.
+struct SpotIlluminant {
+    Vector3f position, direction;
+    float coneAngle = 45.f, penumbraAngle = 40.f;
+    bool angleInRadians = false, usehalfAngles = false;
+    std::vector<float> color;
+    float intensity = 1.f;
+    std::vector<float> shadowColor;
+    float cos_cone() const { return std::cos(angle(coneAngle)); }
+    float cos_penumbra() const { return std::cos(angle(penumbraAngle)); }
+
+private:
+    float angle(float a) const {
+        float r = angleInRadians ? a : a * (3.14159265358979323846f / 180.f);
+        return usehalfAngles ? r / 2.f : r;
+    }
+};
+
+// DirectionalIlluminant (illuminants/directional.cpp:6-13): direction is the
+// way the light travels; the light sits at -direction, beyond the world.
+struct DirectionalIlluminant {
+    Vector3f direction;
+    std::vector<float> color;
+    float intensity = 1.f;
+    std::vector<float> shadowColor;
+};
+
+// Scene illuminants besides the area lights, sampled after them.
+struct DeltaIlluminants {
+    std::vector<PointIlluminant> points;
+    std::vector<SpotIlluminant> spots;
+    std::vector<DirectionalIlluminant> directionals;
 };
 
 // Planck spectrum at temperature T normalized to a peak of 1 over 360..830 nm.
@@ -172,7 +222,7 @@ private:
 class Scene {
 public:
     Scene(Device &dev, const SceneGeometry &geometry, const std::vector<Material> &materials,
-          const std::vector<DiffuseAreaIlluminant> &illuminants)
+          const std::vector<DiffuseAreaIlluminant> &illuminants, const DeltaIlluminants &deltas = DeltaIlluminants())
         : ctx_(dev.handle()) {
         if (!ctx_) {
             err_ = dev.error();
@@ -203,20 +253,66 @@ public:
         std::vector<fl_material> mats;
         for (const Material &m : materials) mats.push_back(m.m);
         std::vector<fl_area_light> lights;
+        std::vector<fl_delta_light> delta;
         std::vector<float> spd;
+        // appends an SPD to the emission table; returns its index, or -1 if
+        // it is not FL_SPD_N samples
+        auto add_spd = [&](const std::vector<float> &v) -> long {
+            if (v.size() != FL_SPD_N) return -1;
+            spd.insert(spd.end(), v.begin(), v.end());
+            return (long)(spd.size() / FL_SPD_N) - 1;
+        };
+        auto shadow_of = [&](const std::vector<float> &v, bool &bad) -> uint32_t {
+            if (v.empty()) return 0u;
+            const long e = add_spd(v);
+            bad = bad || e < 0;
+            return (uint32_t)(e + 1);
+        };
         for (size_t i = 0; i < illuminants.size(); ++i) {
             const DiffuseAreaIlluminant &il = illuminants[i];
-            if (il.mesh >= geometry.triangleMeshes.size() ||
-                il.triangle >= geometry.triangleMeshes[il.mesh]->triangles.size() || il.spd.size() != FL_SPD_N) {
-                err_ = "illuminant " + std::to_string(i) + ": no such triangle, or spd is not 471 samples";
+            bool bad = il.mesh >= geometry.triangleMeshes.size() ||
+                       il.triangle >= geometry.triangleMeshes[il.mesh]->triangles.size();
+            fl_area_light l{};
+            const long e = bad ? -1 : add_spd(il.spd);
+            l.shadow_emission = shadow_of(il.shadowColor, bad);
+            if (bad || e < 0) {
+                err_ = "illuminant " + std::to_string(i) + ": no such triangle, or an spd is not 471 samples";
                 return;
             }
-            fl_area_light l{};
             l.tri = (uint32_t)(first[il.mesh] + il.triangle);
-            l.emission = (uint32_t)i;
+            l.emission = (uint32_t)e;
             l.intensity = il.intensity;
             lights.push_back(l);
-            spd.insert(spd.end(), il.spd.begin(), il.spd.end());
+        }
+        auto add_delta = [&](int type, const Vector3f &pos, const Vector3f &dir, const std::vector<float> &color,
+                             float intensity, float cc, float cp, const std::vector<float> &shadow) {
+            bool bad = false;
+            fl_delta_light l{};
+            l.type = type;
+            l.position[0] = pos.x; l.position[1] = pos.y; l.position[2] = pos.z;
+            l.direction[0] = dir.x; l.direction[1] = dir.y; l.direction[2] = dir.z;
+            const long e = add_spd(color);
+            l.shadow_emission = shadow_of(shadow, bad);
+            if (bad || e < 0) return false;
+            l.emission = (uint32_t)e;
+            l.intensity = intensity;
+            l.cos_cone = cc;
+            l.cos_penumbra = cp;
+            delta.push_back(l);
+            return true;
+        };
+        bool ok = true;
+        for (const PointIlluminant &l : deltas.points)
+            ok = ok && add_delta(FL_LIGHT_POINT, l.position, Vector3f(), l.color, l.intensity, 0.f, 0.f, l.shadowColor);
+        for (const SpotIlluminant &l : deltas.spots)
+            ok = ok && add_delta(FL_LIGHT_SPOT, l.position, l.direction, l.color, l.intensity, l.cos_cone(),
+                                 l.cos_penumbra(), l.shadowColor);
+        for (const DirectionalIlluminant &l : deltas.directionals)
+            ok = ok && add_delta(FL_LIGHT_DIRECTIONAL, Vector3f(), l.direction, l.color, l.intensity, 0.f, 0.f,
+                                 l.shadowColor);
+        if (!ok) {
+            err_ = "delta illuminant: a color or shadowColor is not 471 samples";
+            return;
         }
         fl_scene_desc d{};
         d.vertices = verts.data();
@@ -229,7 +325,9 @@ public:
         d.lights = lights.data();
         d.n_lights = (uint32_t)lights.size();
         d.emission_spd = spd.empty() ? nullptr : spd.data();
-        d.n_emission = (uint32_t)illuminants.size();
+        d.n_emission = (uint32_t)(spd.size() / FL_SPD_N);
+        d.delta_lights = delta.empty() ? nullptr : delta.data();
+        d.n_delta_lights = (uint32_t)delta.size();
         if (fl_scene_create(ctx_, &d, &scene_) != FL_OK) {
             err_ = fl_last_error(ctx_);
             scene_ = nullptr;
@@ -399,7 +497,7 @@ public:
         p.rr_threshold = rrThreshold_;
         p.rr_start = 3;   // RR after bounce 3 (pathintegrator.cpp:77-83)
         p.spectral_mode = parity ? FL_SPECTRAL_PARITY : FL_SPECTRAL_HERO;
-        p.flags = FL_RENDER_GRAPH | (parity ? 0u : (uint32_t)FL_RENDER_VNDF_PDF);
+        p.flags = FL_RENDER_GRAPH | (parity ? 0u : (uint32_t)FL_RENDER_VNDF_PDF) | extra_flags_;
         p.seed = seed;
         fl_film f{};
         f.xyz_sum = film.xyz_sum.data();
@@ -418,10 +516,23 @@ public:
 
     RenderParams renderParams;
 
+protected:
+    uint32_t extra_flags_ = 0;   // FL_RENDER_* added to every render
+
 private:
     float rrThreshold_;
     fl_stats stats_{};
     std::string err_;
+};
+
+// ---- GpuNormalIntegrator (integrators/normalintegrator.h) --------------------
+// Li = from_reflectance_rgb(isect.n) of the first hit, black on a miss
+// (normalintegrator.cpp:11-26); renderParams.samples samples per pixel.
+class GpuNormalIntegrator : public GpuPathIntegrator {
+public:
+    explicit GpuNormalIntegrator(const RenderParams &renderParams) : GpuPathIntegrator(renderParams) {
+        extra_flags_ = FL_RENDER_NORMALS;
+    }
 };
 
 } // namespace gpu

@@ -36,7 +36,7 @@
 extern "C" {
 #endif
 
-#define FL_ABI_VERSION 1
+#define FL_ABI_VERSION 2
 
 /* ---- status codes ------------------------------------------------------ */
 enum {
@@ -84,8 +84,34 @@ typedef struct {
     uint32_t tri;        /* emitting triangle ID */
     uint32_t emission;   /* index into fl_scene_desc.emission_spd */
     float intensity;     /* DiffuseAreaIlluminant::intensity */
-    uint32_t pad;
+    uint32_t shadow_emission; /* Illuminant::shadowColor, the light's value at an occluded
+                                 sample (integrator.cpp:47-50): 0 = black, else
+                                 1 + index into emission_spd */
 } fl_area_light;
+
+/* Delta illuminants (illuminants/point.cpp, spot.cpp, directional.cpp). They
+ * are sampled by NEE only, with pdf 1 and no MIS (is_delta_illuminant,
+ * illuminant.cpp:6-8; integrator.cpp:53-54,62). Point and spot lights fall
+ * off with the squared distance: Illuminant::evaluate_decay_rate falls
+ * through every case to d^2 whatever the decay rate (illuminant.cpp:20-38). */
+enum {
+    FL_LIGHT_POINT = 1,        /* PointIlluminant */
+    FL_LIGHT_SPOT = 2,         /* SpotIlluminant */
+    FL_LIGHT_DIRECTIONAL = 3   /* DirectionalIlluminant */
+};
+
+typedef struct {
+    int32_t type;              /* FL_LIGHT_* */
+    float position[3];         /* point, spot: lightToWorld applied to the origin (posIllum) */
+    float direction[3];        /* spot: the cone axis, lightToWorld.vector_transform(dirIllum);
+                                  directional: lightToWorld.vector_transform(dirIllum), the way
+                                  the light travels (the light is at -direction) */
+    uint32_t emission;         /* color: index into fl_scene_desc.emission_spd */
+    float intensity;
+    float cos_cone;            /* spot: cos of the (half) cone angle; no light outside it */
+    float cos_penumbra;        /* spot: cos of the (half) penumbra angle; full light inside it */
+    uint32_t shadow_emission;  /* shadowColor: 0 = black, else 1 + index into emission_spd */
+} fl_delta_light;
 
 #define FL_SPD_N 471     /* emission SPDs are sampled on 360..830 nm at 1 nm */
 
@@ -101,6 +127,8 @@ typedef struct {
     uint32_t n_lights;
     const float *emission_spd;   /* n_emission * FL_SPD_N */
     uint32_t n_emission;
+    const fl_delta_light *delta_lights; /* sampled after the area lights, in this order */
+    uint32_t n_delta_lights;
 } fl_scene_desc;
 
 typedef struct {
@@ -142,7 +170,10 @@ enum {
     FL_RENDER_GRAPH = 8,         /* capture the bounce loop in a hipGraph */
     FL_RENDER_TRAIN_GUIDE = 16,  /* train the guide online on this render's samples */
     FL_RENDER_NO_SORT = 32,      /* disable shading sort */
-    FL_RENDER_TIMING = 64        /* per-kernel HIP-event timing (ms_* fields of fl_stats) */
+    FL_RENDER_TIMING = 64,       /* per-kernel HIP-event timing (ms_* fields of fl_stats) */
+    FL_RENDER_NORMALS = 128      /* NormalIntegrator (integrators/normalintegrator.cpp:11-26):
+                                    the first hit's normal through from_reflectance_rgb, black
+                                    on a miss; no light transport */
 };
 
 typedef struct {

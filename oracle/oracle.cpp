@@ -553,11 +553,15 @@ struct BNode {
 };
 
 struct Light {
+    int type = 0;               // 0: DiffuseAreaIlluminant over a triangle, else FL_LIGHT_*
     V3f p0, p1, p2;
-    float area;
+    float area = 0.f;
     S60 color;
-    float intensity;
-    uint32_t tri;
+    float intensity = 0.f;
+    uint32_t tri = 0;
+    V3f pos, dir;               // point, spot: posIllum; spot: dirIllum; directional: dirIllum
+    float cos_cone = 0.f, cos_penumbra = 0.f;
+    S60 shadow;                 // Illuminant::shadowColor
 };
 
 struct Material {
@@ -888,11 +892,57 @@ bool unoccluded(Ctx &c, const Isect &p0, const V3f &p1) {
     return true;
 }
 
-// estimate_direct (integrator.cpp:25-110) for one area light
+// Illuminant::evaluate_decay_rate (illuminant.cpp:20-38): every case falls
+// through to the default, d^2
+float decay(const V3f &d) {
+    float dL = d.length();
+    return dL * dL;
+}
+
+// SpotIlluminant::Falloff (spot.cpp:41-55)
+float spot_falloff(const Light &L, const V3f &w) {
+    float cosTheta = dot(normalize(w), normalize(L.dir));
+    if (cosTheta < L.cos_cone) return 0.f;
+    if (cosTheta > L.cos_penumbra) return 1.f;
+    float t = clampv((cosTheta - L.cos_cone) / (L.cos_penumbra - L.cos_cone), 0.f, 1.f);
+    return t * t * (3.f - 2.f * t);
+}
+
+// estimate_direct (integrator.cpp:25-62) for a delta light: sample_li of
+// point.cpp:12-20, spot.cpp:23-31, directional.cpp:15-24 (pdf 1), no MIS,
+// no BSDF-sampled half (is_delta_illuminant)
+S60 estimate_direct_delta(Ctx &c, const Isect &it, const Bsdf<float, NS> &bsdf, const Light &L) {
+    const uint32_t flags = BSDF_ALL & ~BSDF_SPECULAR;
+    S60 Ld(0.f);
+    V3f wi, target;
+    S60 Li;
+    if (L.type == FL_LIGHT_DIRECTIONAL) {
+        wi = L.dir;
+        target = it.p + L.dir * (2.f * 1000.f);          // pOutside, worldRadius 1000
+        Li = L.color * L.intensity;
+    } else {
+        wi = normalize(L.pos - it.p);
+        target = L.pos;
+        if (L.type == FL_LIGHT_SPOT) Li = (L.color * L.intensity * spot_falloff(L, V3f(-wi.x, -wi.y, -wi.z))) / decay(L.pos - it.p);
+        else Li = (L.color * L.intensity) / decay(L.pos - it.p);
+    }
+    const float illumPdf = 1.f;
+    if (!Li.is_black()) {
+        S60 f = bsdf.evaluate(it.wo, wi, flags) * abs_dot(wi, it.n);
+        if (!f.is_black()) {
+            if (!unoccluded(c, it, target)) Li = L.shadow;
+            if (!Li.is_black()) Ld += f * Li / illumPdf;
+        }
+    }
+    return Ld;
+}
+
+// estimate_direct (integrator.cpp:25-110) for one light
 S60 estimate_direct(Ctx &c, const Isect &it, const Bsdf<float, NS> &bsdf, float us0, float us1, uint32_t li,
                     float ul0, float ul1) {
     const or_scene_s *s = c.s;
     const Light &L = s->lights[li];
+    if (L.type != 0) return estimate_direct_delta(c, it, bsdf, L);
     const uint32_t flags = BSDF_ALL & ~BSDF_SPECULAR;
     S60 Ld(0.f);
     // --- light sample: Triangle::sample + Shape::sample(ref) (triangle.cpp:134-154, shape.cpp:28-41)
@@ -918,7 +968,7 @@ S60 estimate_direct(Ctx &c, const Isect &it, const Bsdf<float, NS> &bsdf, float 
         S60 f = bsdf.evaluate(it.wo, wi, flags) * abs_dot(wi, it.n);
         scatteringPdf = bsdf.pdf(it.wo, wi, flags);
         if (!f.is_black()) {
-            if (!unoccluded(c, it, ps)) Li = S60(0.f);              // shadowColor = 0
+            if (!unoccluded(c, it, ps)) Li = L.shadow;              // shadowColor
             if (!Li.is_black()) {
                 float w = (illumPdf * illumPdf) / (illumPdf * illumPdf + scatteringPdf * scatteringPdf);
                 Ld += f * Li * w / illumPdf;
@@ -957,6 +1007,15 @@ S60 estimate_direct(Ctx &c, const Isect &it, const Bsdf<float, NS> &bsdf, float 
         }
     }
     return Ld;
+}
+
+// NormalIntegrator::Li (normalintegrator.cpp:11-26)
+S60 Li_normal(Ctx &c, V3f ro, V3f rd, float tmin) {
+    Isect it;
+    c.rays++;
+    if (!scene_intersect(c.s, ro, rd, tmin, INFINITI, &it)) return S60(0.f);
+    const float rgb[3] = {it.n.x, it.n.y, it.n.z};
+    return from_reflectance_rgb(rgb);
 }
 
 // PathIntegrator::Li (pathintegrator.cpp:12-87)
@@ -1188,7 +1247,30 @@ int or_scene_create(const fl_scene_desc *d, or_scene_t *out) {
         L.color = from_spd(lam, d->emission_spd + (size_t)a.emission * FL_SPD_N, FL_SPD_N);
         L.intensity = a.intensity;
         L.tri = a.tri;
+        if (a.shadow_emission)
+            L.shadow = from_spd(lam, d->emission_spd + (size_t)(a.shadow_emission - 1) * FL_SPD_N, FL_SPD_N);
         s->light[a.tri] = l + 1;
+    }
+    // delta lights after the area lights (uniform_sample_all_lights visits them in this order)
+    for (uint32_t l = 0; l < d->n_delta_lights; ++l) {
+        const fl_delta_light &a = d->delta_lights[l];
+        Light L;
+        L.type = a.type;
+        L.color = from_spd(lam, d->emission_spd + (size_t)a.emission * FL_SPD_N, FL_SPD_N);
+        L.intensity = a.intensity;
+        L.tri = 0xffffffffu;
+        if (a.shadow_emission)
+            L.shadow = from_spd(lam, d->emission_spd + (size_t)(a.shadow_emission - 1) * FL_SPD_N, FL_SPD_N);
+        const V3f dir(a.direction[0], a.direction[1], a.direction[2]);
+        if (a.type == FL_LIGHT_DIRECTIONAL) {
+            L.dir = normalize(V3f(-dir.x, -dir.y, -dir.z));         // directional.cpp:11
+        } else {
+            L.pos = V3f(a.position[0], a.position[1], a.position[2]);
+            L.dir = dir;
+            L.cos_cone = a.cos_cone;
+            L.cos_penumbra = a.cos_penumbra;
+        }
+        s->lights.push_back(L);
     }
     build_bvh(s);
     *out = s;
@@ -1265,7 +1347,8 @@ int or_render(or_scene_t s, const fl_camera *cam, const or_render_params *p, flo
                         float su = 1.f - ((float)x + jx) / (float)W;
                         float tv = 1.f - ((float)y + jy) / (float)H;
                         V3f dir = k.llc + k.hor * su + k.ver * tv - k.o - V3f(0, 0, 0);
-                        S60 L = Li(c, k.o, normalize(dir), 0.f, smp);   // tMin = time = 0
+                        S60 L = p->integrator == 1 ? Li_normal(c, k.o, normalize(dir), 0.f)
+                                                   : Li(c, k.o, normalize(dir), 0.f, smp);   // tMin = time = 0
                         float xyz[3];
                         to_xyz(L, xyz);
                         if (!std::isfinite(xyz[0]) || !std::isfinite(xyz[1]) || !std::isfinite(xyz[2]))

@@ -60,6 +60,7 @@ typedef struct {
     int32_t threads;
     uint32_t y0, y1;
     uint32_t x0, x1;
+    int32_t integrator;    /* 0 PathIntegrator, 1 NormalIntegrator */
 } or_render_params;
 
 typedef struct {

@@ -20,7 +20,7 @@ class RenderParams(C.Structure):
     _fields_ = [("spp", C.c_uint32), ("diffuse_depth", C.c_int32), ("reflection_depth", C.c_int32),
                 ("transmission_depth", C.c_int32), ("rr_threshold", C.c_float), ("rr_start", C.c_uint32),
                 ("seed", C.c_uint64), ("threads", C.c_int32), ("y0", C.c_uint32), ("y1", C.c_uint32),
-                ("x0", C.c_uint32), ("x1", C.c_uint32)]
+                ("x0", C.c_uint32), ("x1", C.c_uint32), ("integrator", C.c_int32)]
 
 
 class RenderStats(C.Structure):
@@ -169,7 +169,7 @@ class Scene:
         lib().or_intersect2(self.h, rays.ctypes.data, t2.ctypes.data, id2.ctypes.data, n, threads)
         return t2, id2
 
-    def render(self, spp, depth, seed=1, threads=0, rows=None, cols=None, camera=None, rr_start=3):
+    def render(self, spp, depth, seed=1, threads=0, rows=None, cols=None, camera=None, rr_start=3, integrator=0):
         cam = camera if camera is not None else self.gen.camera
         W, H = cam.width, cam.height
         p = RenderParams()
@@ -179,6 +179,7 @@ class Scene:
         p.rr_start = rr_start
         p.seed = seed
         p.threads = threads
+        p.integrator = integrator
         if rows is not None:
             p.y0, p.y1 = rows
         if cols is not None:

@@ -56,3 +56,72 @@ def rect_form_factor_center(a_half, b_half, h):
     F = (X / np.sqrt(1 + X * X) * np.arctan(Y / np.sqrt(1 + X * X)) +
          Y / np.sqrt(1 + Y * Y) * np.arctan(X / np.sqrt(1 + Y * Y))) / (2 * np.pi)
     return np.pi * 4 * F
+
+
+def delta_light(kind, position=(0, 0, 0), direction=(0, 0, 0), emission=0, intensity=1.0, cos_cone=0.0,
+                cos_penumbra=0.0, shadow_emission=0):
+    d = flgpu.DeltaLight()
+    d.type = kind
+    d.position[:] = list(position)
+    d.direction[:] = list(direction)
+    d.emission, d.intensity = emission, intensity
+    d.cos_cone, d.cos_penumbra = cos_cone, cos_penumbra
+    d.shadow_emission = shadow_emission
+    return d
+
+
+FLOOR_CAM_HEIGHT = 3.0
+FLOOR_VIEW_HALF = 2.0   # the camera sees the floor square [-2, 2]^2
+
+
+def delta_floor(delta_lights, spds, albedo=(0.6, 0.5, 0.4), occluder=False, W=32, area_lights=()):
+    """A Lambert floor (y = 0, normal +y) lit by delta lights, seen by a W x W
+    camera straight above it at y = 3 whose view covers [-2, 2]^2 of the
+    floor. With occluder, a 0.5 x 4 Lambert slab at y = 1, x in [1.5, 2]
+    (outside the view at that height) casts shadows into the view. spds:
+    (n, FL_SPD_N) emission table (light colours and shadow colours)."""
+    floor = quad((0, 0, 0), (1, 0, 0), (0, 0, 1), 50, 50)
+    parts = [floor]
+    mats = [lambert(albedo)]
+    tri_mat = [0, 0]
+    if occluder:
+        parts.append(quad((1.75, 1.0, 0), (1, 0, 0), (0, 0, 1), 0.25, 2.0))   # faces +y
+        parts.append(quad((1.75, 1.0, 0), (1, 0, 0), (0, 0, -1), 0.25, 2.0))  # faces -y
+        tri_mat += [0, 0, 0, 0]
+    lights = []
+    if area_lights:
+        mats.append(none_material())
+    for (c, half, intensity, em) in area_lights:
+        parts.append(quad(c, (1, 0, 0), (0, 0, -1), half, half))          # faces -y
+        n = sum(len(p) for p in parts) // 3
+        tri_mat += [len(mats) - 1, len(mats) - 1]
+        lights += [(n - 2, em, intensity), (n - 1, em, intensity)]
+    verts = np.concatenate(parts)
+    idx = np.arange(len(verts), dtype=np.uint32).reshape(-1, 3)
+    cam = flgpu.Camera()
+    cam.position[:] = [0.0, FLOOR_CAM_HEIGHT, 0.0]
+    cam.look_at[:] = [0.0, 0.0, 0.0]
+    cam.up[:] = [0.0, 0.0, -1.0]
+    cam.vfov_deg = float(2 * np.degrees(np.arctan(FLOOR_VIEW_HALF / FLOOR_CAM_HEIGHT)))
+    cam.width = cam.height = W
+    return flgpu.GenSceneData.from_arrays(verts, idx, np.array(tri_mat, np.uint32), mats, lights,
+                                          np.asarray(spds, np.float32).reshape(-1, flgpu.FL_SPD_N), cam, spp=1,
+                                          depth=0, delta_lights=list(delta_lights))
+
+
+def floor_points(cam, sub=8):
+    """Floor points (y = 0) seen by each pixel of the straight-down camera of
+    delta_floor, on a sub x sub grid inside every pixel: (H, W, sub*sub, 3)."""
+    from raysets import camera_basis
+    pos, llc, hor, ver = camera_basis(cam)
+    W, H = cam.width, cam.height
+    j = (np.arange(sub) + 0.5) / sub
+    jx, jy = np.meshgrid(j, j, indexing="xy")
+    xs = (np.arange(W)[None, :, None] + jx.ravel()[None, None, :])
+    ys = (np.arange(H)[:, None, None] + jy.ravel()[None, None, :])
+    s = 1.0 - xs / W
+    t = 1.0 - ys / H
+    d = llc + hor * s[..., None] + ver * t[..., None] - pos
+    d = np.broadcast_to(d, (H, W, sub * sub, 3))
+    tt = pos[1] / -d[..., 1]
+    return pos + d * tt[..., None]

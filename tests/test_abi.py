@@ -35,7 +35,7 @@ def test_embree_subset_exported():
 
 def test_abi_version_and_no_device_behaviour():
     L = flgpu.lib()
-    assert L.fl_abi_version() == 1
+    assert L.fl_abi_version() == 2
     if L.fl_device_count() == 0:
         h = flgpu.C.c_void_p()
         assert L.fl_ctx_create(0, flgpu.C.byref(h)) == flgpu.FL_E_UNSUPPORTED
@@ -46,6 +46,7 @@ def test_struct_sizes_match_header():
     C = flgpu.C
     assert C.sizeof(flgpu.Material) == 64
     assert C.sizeof(flgpu.AreaLight) == 16
+    assert C.sizeof(flgpu.DeltaLight) == 48
     assert C.sizeof(flgpu.BsdfQuery) == 112
     assert C.sizeof(flgpu.BsdfResult) == 56
     assert C.sizeof(flgpu.Camera) == 48
