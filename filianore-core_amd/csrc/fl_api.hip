@@ -123,6 +123,7 @@ void fl_ctx_destroy(fl_ctx_t ctx) {
     (void)hipStreamSynchronize(ctx->stream);
     flx::release_render_buffers(ctx);
     if (ctx->scratch) (void)hipFree(ctx->scratch);
+    if (ctx->develop_buf) (void)hipFree(ctx->develop_buf);
     (void)hipStreamDestroy(ctx->stream);
     delete ctx;
 }

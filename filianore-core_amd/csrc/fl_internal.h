@@ -53,6 +53,8 @@ struct fl_ctx_s {
     // scratch reused across calls (grown on demand)
     void *scratch = nullptr;
     size_t scratch_bytes = 0;
+    void *develop_buf = nullptr;     // fl_develop / fl_develop_film staging (grown on demand)
+    size_t develop_bytes = 0;
     hipEvent_t ev[16] = {};
     // render working set (queues, path state, film, pinned counters, the
     // captured bounce-loop graph), grown on demand and kept between calls
@@ -140,6 +142,8 @@ struct HostItem {
     uint32_t spp;
 };
 int render_items(fl_ctx_t ctx, const HostItem *items, uint32_t n, const fl_render_params *p, fl_stats *stats);
+// The device film sums of item `item` of the context's most recent render.
+int last_film(fl_ctx_t ctx, uint32_t item, const float **d_sum, uint32_t *W, uint32_t *H, uint32_t *spp);
 int autotune_items(fl_ctx_t ctx, const HostItem *items, uint32_t n, const fl_render_params *p, fl_tune_result *out);
 
 // Online training of the guide on render records (device, 16 floats each:

@@ -986,6 +986,10 @@ struct RenderState {
     hipGraphExec_t gexec = nullptr;
     RenderArgs gkey;
     int gvariant = -1;
+    // films of the most recent render, resident in p[17] (sums): per item
+    // its first pixel, size and samples per pixel
+    struct Film { uint32_t pix_base, W, H, spp; };
+    std::vector<Film> films;
 };
 
 namespace flx {
@@ -1456,6 +1460,9 @@ int render_items(fl_ctx_t ctx, const HostItem *hi, uint32_t n_items, const fl_re
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     for (int k = 0; k < 4; ++k) (void)hipEventDestroy(tev[k]);
+    B.films.resize(n_items);
+    for (uint32_t k = 0; k < n_items; ++k)
+        B.films[k] = RenderState::Film{items[k].pix_base, hi[k].cam.width, hi[k].cam.height, hi[k].spp};
     for (uint32_t k = 0; k < n_items; ++k) {
         fl_film *film = hi[k].film;
         if (!film || !film->xyz_sum) continue;
@@ -1490,6 +1497,18 @@ int render_items(fl_ctx_t ctx, const HostItem *hi, uint32_t n_items, const fl_re
         stats->paths_per_pass = a.cap;
         stats->stack_depth = (uint32_t)tv->stack;
     }
+    return FL_OK;
+}
+
+int last_film(fl_ctx_t ctx, uint32_t item, const float **d_sum, uint32_t *W, uint32_t *H, uint32_t *spp) {
+    RenderState *r = ctx->render_state;
+    if (!r || item >= r->films.size() || !r->p[17])
+        return set_error(ctx, FL_E_INVALID, "fl_develop_film: no such item in the last render");
+    const RenderState::Film &f = r->films[item];
+    *d_sum = static_cast<const float *>(r->p[17]) + (size_t)f.pix_base * 3;
+    *W = f.W;
+    *H = f.H;
+    *spp = f.spp;
     return FL_OK;
 }
 

@@ -20,6 +20,9 @@ FL_SPECTRAL_HERO, FL_SPECTRAL_PARITY = 0, 1
 FL_RENDER_GUIDED, FL_RENDER_COUNTERS, FL_RENDER_VNDF_PDF, FL_RENDER_GRAPH = 1, 2, 4, 8
 FL_RENDER_TRAIN_GUIDE, FL_RENDER_NO_SORT, FL_RENDER_TIMING, FL_RENDER_NORMALS = 16, 32, 64, 128
 FL_LIGHT_POINT, FL_LIGHT_SPOT, FL_LIGHT_DIRECTIONAL = 1, 2, 3
+FL_COLOR_SRGB, FL_COLOR_ACES2065_1 = 0, 1
+FL_TONEMAP_NONE, FL_TONEMAP_ACES_BASIC, FL_TONEMAP_REINHARD, FL_TONEMAP_FILMIC = 1, 2, 3, 4
+FL_GAMMA_NONE, FL_GAMMA_REC_709, FL_GAMMA_1_8, FL_GAMMA_2_2, FL_GAMMA_4_0 = 1, 2, 3, 4, 5
 FL_NO_HIT = 0xFFFFFFFF
 FL_SPD_N = 471
 FL_GEN_CORNELL, FL_GEN_SPHERES, FL_GEN_BATCH, FL_GEN_GUIDED, FL_GEN_TERRAIN, FL_GEN_GUIDED_SKYLIGHT = 0, 1, 2, 3, 4, 5
@@ -131,6 +134,10 @@ class GuideConfig(C.Structure):
                 ("steps_per_pass", C.c_uint32)]
 
 
+class DevelopParams(C.Structure):
+    _fields_ = [("color_space", C.c_uint32), ("tone_map", C.c_uint32), ("gamma", C.c_uint32), ("pad", C.c_uint32)]
+
+
 class BsdfQuery(C.Structure):
     _fields_ = [("kind", C.c_int32), ("alpha_x", C.c_float), ("alpha_y", C.c_float), ("eta", f4), ("k", f4),
                 ("r", f4), ("t", f4), ("wo", f3), ("wi", f3), ("u", C.c_float * 2), ("flags", C.c_uint32)]
@@ -164,7 +171,7 @@ EXPORTS = [
     "fl_intersect", "fl_intersect_device", "fl_render", "fl_render_batch", "fl_autotune", "fl_autotune_batch",
     "fl_guide_create", "fl_guide_destroy", "fl_guide_train_step", "fl_guide_forward",
     "fl_guide_get_params", "fl_guide_set_params", "fl_guide_gradient", "fl_guide_stats",
-    "fl_bsdf_query_batch",
+    "fl_bsdf_query_batch", "fl_develop_film", "fl_develop",
     "fl_gen_default_options", "fl_gen_config", "fl_gen_free", "fl_gen_describe", "fl_gen_blackbody",
 ]
 
@@ -204,6 +211,8 @@ def lib():
         L.fl_guide_gradient.argtypes = [vp, vp, C.c_uint32, vp, C.c_uint32, C.POINTER(C.c_float)]
         L.fl_guide_stats.argtypes = [vp, C.POINTER(C.c_uint32), C.POINTER(C.c_float), C.POINTER(C.c_int)]
         L.fl_bsdf_query_batch.argtypes = [vp, C.POINTER(BsdfQuery), C.POINTER(BsdfResult), C.c_uint32]
+        L.fl_develop_film.argtypes = [vp, C.c_uint32, C.POINTER(DevelopParams), vp, vp]
+        L.fl_develop.argtypes = [vp, vp, C.c_uint32, C.POINTER(DevelopParams), vp, vp]
         L.fl_gen_default_options.argtypes = [C.POINTER(GenOptions)]
         L.fl_gen_config.argtypes = [C.c_int, C.POINTER(GenOptions), C.POINTER(GenScene)]
         L.fl_gen_free.argtypes = [C.POINTER(GenScene)]
@@ -386,6 +395,26 @@ class Context:
         g = C.c_double(0.0)
         _check(self.L.fl_probe_copy(self.h, nbytes, iters, C.byref(g)), self.h, "fl_probe_copy")
         return g.value
+
+    def develop(self, xyz, color_space=FL_COLOR_SRGB, tone_map=FL_TONEMAP_NONE, gamma=FL_GAMMA_NONE):
+        """fl_develop of n pixel means (n x 3 XYZ): (rgb n x 3 float32, rgba8 n x 4 uint8)."""
+        x = np.ascontiguousarray(xyz, np.float32).reshape(-1, 3)
+        rgb = np.zeros(x.shape, np.float32)
+        u8 = np.zeros((x.shape[0], 4), np.uint8)
+        p = DevelopParams(color_space, tone_map, gamma, 0)
+        _check(self.L.fl_develop(self.h, x.ctypes.data, x.shape[0], C.byref(p), rgb.ctypes.data, u8.ctypes.data),
+               self.h, "fl_develop")
+        return rgb, u8
+
+    def develop_film(self, item, width, height, color_space=FL_COLOR_SRGB, tone_map=FL_TONEMAP_NONE,
+                     gamma=FL_GAMMA_NONE):
+        """fl_develop_film of item `item` of the last render: (rgb H x W x 3, rgba8 H x W x 4)."""
+        rgb = np.zeros((height, width, 3), np.float32)
+        u8 = np.zeros((height, width, 4), np.uint8)
+        p = DevelopParams(color_space, tone_map, gamma, 0)
+        _check(self.L.fl_develop_film(self.h, item, C.byref(p), rgb.ctypes.data, u8.ctypes.data), self.h,
+               "fl_develop_film")
+        return rgb, u8
 
     def bsdf_queries(self, queries):
         n = len(queries)

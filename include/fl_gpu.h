@@ -337,6 +337,31 @@ typedef struct {
 
 int fl_bsdf_query_batch(fl_ctx_t ctx, const fl_bsdf_query *q, fl_bsdf_result *r, uint32_t n);
 
+/* ---- output path (color/spectrumoperations.cpp:58-76,209-286) -------------
+ * A pixel's mean XYZ (film sum / spp) goes to a colour space (xyz_to_rgb,
+ * :58-66, or xyz_to_aces2065_1, :68-76), through tone_map (:254-286), then
+ * gamma_correct (:215-252). The enum values are the reference's. */
+enum { FL_COLOR_SRGB = 0, FL_COLOR_ACES2065_1 = 1 };
+enum { FL_TONEMAP_NONE = 1, FL_TONEMAP_ACES_BASIC = 2, FL_TONEMAP_REINHARD = 3, FL_TONEMAP_FILMIC = 4 };
+enum { FL_GAMMA_NONE = 1, FL_GAMMA_REC_709 = 2, FL_GAMMA_1_8 = 3, FL_GAMMA_2_2 = 4, FL_GAMMA_4_0 = 5 };
+
+typedef struct {
+    uint32_t color_space;   /* FL_COLOR_* */
+    uint32_t tone_map;      /* FL_TONEMAP_* */
+    uint32_t gamma;         /* FL_GAMMA_* */
+    uint32_t pad;
+} fl_develop_params;
+
+/* Develops the film of item `item` of the context's most recent fl_render /
+ * fl_render_batch, still resident on the device: rgb receives width*height*3
+ * floats and/or rgba8 width*height*4 bytes (round(clamp(v, 0, 1) * 255),
+ * NaN -> 0, alpha 255); either may be NULL. Only the developed pixels cross
+ * PCIe (4 B per pixel for rgba8 against the film's 24). */
+int fl_develop_film(fl_ctx_t ctx, uint32_t item, const fl_develop_params *params, float *rgb, uint8_t *rgba8);
+/* The same for n host pixel means (n*3 floats of XYZ). */
+int fl_develop(fl_ctx_t ctx, const float *xyz, uint32_t n, const fl_develop_params *params, float *rgb,
+               uint8_t *rgba8);
+
 #ifdef __cplusplus
 }
 #endif

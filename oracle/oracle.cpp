@@ -1316,6 +1316,61 @@ int or_bsdf_batch(const fl_bsdf_query *q, fl_bsdf_result *r, uint32_t n, int pre
     return FL_OK;
 }
 
+// Output path: xyz_to_rgb / xyz_to_aces2065_1 (spectrumoperations.cpp:58-76),
+// tone_map (:254-286), gamma_correct (:215-252), as written there.
+static void or_develop_one(const float xyz[3], int space, int tm, int gm, float out[3]) {
+    float c[3];
+    if (space == FL_COLOR_ACES2065_1) {
+        c[0] = 1.0498110175f * xyz[0] + 0.0000000000f * xyz[1] - 0.0000974845f * xyz[2];
+        c[1] = -0.4959030231f * xyz[0] + 1.3733130458f * xyz[1] + 0.0982400361f * xyz[2];
+        c[2] = 0.0000000000f * xyz[0] + 0.0000000000f * xyz[1] + 0.9912520182f * xyz[2];
+    } else {
+        c[0] = 3.240479f * xyz[0] - 1.537150f * xyz[1] - 0.498535f * xyz[2];
+        c[1] = -0.969256f * xyz[0] + 1.875991f * xyz[1] + 0.041556f * xyz[2];
+        c[2] = 0.055648f * xyz[0] - 0.204043f * xyz[1] + 1.057311f * xyz[2];
+    }
+    // tone_map
+    if (tm == FL_TONEMAP_ACES_BASIC) {
+        const float a = 2.51f, b = 0.03f, cc = 2.43f, d = 0.59f, e = 0.14f;
+        for (int k = 0; k < 3; ++k) c[k] = clampv((c[k] * (a * c[k] + b)) / (c[k] * (cc * c[k] + d) + e), 0.f, 1.f);
+    } else if (tm == FL_TONEMAP_FILMIC) {
+        for (int k = 0; k < 3; ++k) {
+            float X = std::max(0.f, c[k] - 0.004f);
+            float result = (X * (6.2f * X + 0.5f)) / (X * (6.2f * X + 1.7f) + 0.06f);
+            c[k] = powf(result, 2.2f);
+        }
+    } else if (tm == FL_TONEMAP_REINHARD) {
+        const float L_white = 4.f;
+        for (int k = 0; k < 3; ++k) c[k] = (c[k] * (1.f + c[k] / (L_white * L_white))) / (1.f + c[k]);
+    }
+    // gamma_correct
+    for (int k = 0; k < 3; ++k) {
+        if (gm == FL_GAMMA_REC_709) {
+            if (c[k] >= 0.018f) c[k] = (1.099f * powf(c[k], 0.45f)) - 0.099f;
+            else c[k] *= ((1.099f * powf(0.018f, 0.45f)) - 0.099f) / 0.018f;
+        } else if (gm == FL_GAMMA_1_8) {
+            c[k] = powf(c[k], 1.f / 1.8f);
+        } else if (gm == FL_GAMMA_2_2) {
+            c[k] = powf(c[k], 1.f / 2.2f);
+        } else if (gm == FL_GAMMA_4_0) {
+            c[k] = powf(c[k], 1.f / 4.f);
+        }
+        out[k] = c[k];
+    }
+}
+
+void or_develop(const float *xyz, uint32_t n, int color_space, int tone_map, int gamma, float *rgb, uint8_t *rgba8) {
+    for (uint32_t i = 0; i < n; ++i) {
+        float c[3];
+        or_develop_one(xyz + 3 * (size_t)i, color_space, tone_map, gamma, c);
+        for (int k = 0; k < 3; ++k) {
+            if (rgb) rgb[3 * (size_t)i + k] = c[k];
+            if (rgba8) rgba8[4 * (size_t)i + k] = std::isnan(c[k]) ? 0 : (uint8_t)(clampv(c[k], 0.f, 1.f) * 255.f + 0.5f);
+        }
+        if (rgba8) rgba8[4 * (size_t)i + 3] = 255;
+    }
+}
+
 int or_render(or_scene_t s, const fl_camera *cam, const or_render_params *p, float *film_sum, float *film_sq,
               or_render_stats *st) {
     if (!s || !cam || !p) return FL_E_INVALID;

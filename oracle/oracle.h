@@ -72,6 +72,10 @@ typedef struct {
 int or_render(or_scene_t s, const fl_camera *cam, const or_render_params *p, float *film_sum, float *film_sq,
               or_render_stats *st);
 
+/* output path: n pixel means of XYZ to colour space, tone map, gamma
+ * (FL_COLOR_*, FL_TONEMAP_*, FL_GAMMA_*); rgba8 as fl_develop */
+void or_develop(const float *xyz, uint32_t n, int color_space, int tone_map, int gamma, float *rgb, uint8_t *rgba8);
+
 /* learned proposal (oracle_guide.cpp): parameters are 5 x (64 x 64 + 64)
  * doubles in the GPU's order; records are n x 68 floats (encoded input 64,
  * direction 3, weight 1); bf16 = 1 rounds where the GPU rounds to bf16 */

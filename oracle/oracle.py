@@ -53,6 +53,7 @@ def lib():
         L.or_intersect2.argtypes = [vp, vp, vp, vp, C.c_uint32, C.c_int]
         L.or_bsdf_batch.argtypes = [C.POINTER(flgpu.BsdfQuery), C.POINTER(flgpu.BsdfResult), C.c_uint32, C.c_int]
         L.or_render.argtypes = [vp, C.POINTER(flgpu.Camera), C.POINTER(RenderParams), fp, fp, C.POINTER(RenderStats)]
+        L.or_develop.argtypes = [fp, C.c_uint32, C.c_int, C.c_int, C.c_int, fp, C.POINTER(C.c_uint8)]
         dp = C.POINTER(C.c_double)
         L.or_bf16_round.argtypes = [C.c_double]
         L.or_bf16_round.restype = C.c_double
@@ -191,6 +192,16 @@ class Scene:
         if rc != 0:
             raise RuntimeError("or_render failed %d" % rc)
         return s, q, st
+
+
+def develop(xyz, color_space, tone_map, gamma):
+    """Output path of n pixel means (n x 3 XYZ): (rgb float n x 3, rgba8 n x 4)."""
+    x = np.ascontiguousarray(xyz, np.float32).reshape(-1, 3)
+    n = x.shape[0]
+    rgb = np.zeros((n, 3), np.float32)
+    u8 = np.zeros((n, 4), np.uint8)
+    lib().or_develop(_f(x), n, color_space, tone_map, gamma, _f(rgb), u8.ctypes.data_as(C.POINTER(C.c_uint8)))
+    return rgb, u8
 
 
 # ---- learned proposal (oracle_guide.cpp) ----------------------------------
