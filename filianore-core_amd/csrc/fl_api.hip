@@ -38,6 +38,7 @@ void free_scene(fl_scene_s *s) {
     if (!s) return;
     (void)hipFree(s->d_nodes);
     (void)hipFree(s->d_tris);
+    (void)hipFree(s->d_std);
     (void)hipFree(s->d_mats);
     (void)hipFree(s->d_lights);
     for (auto &t : s->spec) (void)hipFree(t.d_tab);
@@ -151,6 +152,18 @@ int fl_scene_create(fl_ctx_t ctx, const fl_scene_desc *desc, fl_scene_t *out) {
     for (uint32_t l = 0; l < desc->n_lights; ++l)
         if (desc->lights[l].shadow_emission > desc->n_emission)
             return set_error(ctx, FL_E_INVALID, "fl_scene_create: bad light shadow color");
+    if (desc->n_standard_surfaces && !desc->standard_surfaces)
+        return set_error(ctx, FL_E_INVALID, "fl_scene_create: missing standard surfaces");
+    for (uint32_t m = 0; m < desc->n_materials; ++m)
+        if (desc->materials[m].kind == FL_MAT_STANDARD && desc->materials[m].standard >= desc->n_standard_surfaces)
+            return set_error(ctx, FL_E_INVALID, "fl_scene_create: standard surface index out of range");
+    for (uint32_t j = 0; j < desc->n_standard_surfaces; ++j) {
+        const fl_standard_surface &ss = desc->standard_surfaces[j];
+        if (ss.fresnel_type < FL_FRESNEL_DIELECTRIC || ss.fresnel_type > FL_FRESNEL_WAVELENGTH_METALLIC ||
+            !(ss.ks_ior > 0.f) || (ss.coat_weight > 0.f && !(ss.coat_ior > 0.f)) ||
+            (ss.thin_film_thickness > 0.f && !(ss.thin_film_ior > 0.f)))
+            return set_error(ctx, FL_E_INVALID, "fl_scene_create: bad standard surface");
+    }
     for (uint32_t t = 0; t < 3ull * desc->n_triangles; ++t)
         if (desc->indices[t] >= desc->n_vertices) return set_error(ctx, FL_E_INVALID, "fl_scene_create: index out of range");
     if (desc->tri_material)
@@ -193,6 +206,9 @@ int fl_scene_create(fl_ctx_t ctx, const fl_scene_desc *desc, fl_scene_t *out) {
         d.em_slot = 0; // set after tables are sized
     }
     std::vector<DevMaterial> mats;
+    s->h_std.resize(desc->n_standard_surfaces);
+    for (uint32_t m = 0; m < desc->n_materials; ++m) s->has_standard = s->has_standard || desc->materials[m].kind == FL_MAT_STANDARD;
+    for (uint32_t j = 0; j < desc->n_standard_surfaces; ++j) s->h_std[j] = flx::standard_params(desc->standard_surfaces[j]);
     flx::build_spectral_tables(s, desc, FL_SPECTRAL_PARITY, mats);
     flx::build_spectral_tables(s, desc, FL_SPECTRAL_HERO, mats);
     s->h_mats = mats;
@@ -251,6 +267,7 @@ int fl_scene_create(fl_ctx_t ctx, const fl_scene_desc *desc, fl_scene_t *out) {
     if (rc) { free_scene(s); return rc; }
     if ((rc = upload(ctx, &s->d_mats, s->h_mats.data(), s->h_mats.size())) ||
         (rc = upload(ctx, &s->d_lights, s->h_lights.data(), s->h_lights.size())) ||
+        (rc = upload(ctx, &s->d_std, s->h_std.data(), s->h_std.size())) ||
         (rc = upload(ctx, &s->spec[0].d_tab, s->spec[0].host.data(), s->spec[0].host.size())) ||
         (rc = upload(ctx, &s->spec[1].d_tab, s->spec[1].host.data(), s->spec[1].host.size()))) {
         free_scene(s);
@@ -342,6 +359,30 @@ int fl_render(fl_scene_t s, const fl_camera *camera, const fl_render_params *par
     if (!s) return FL_E_INVALID;
     (void)hipSetDevice(s->ctx->device);
     return flx::render(s, camera, params, film, stats);
+}
+
+int fl_bsdf_query_standard(fl_ctx_t ctx, const fl_standard_surface *m, const fl_standard_query *q,
+                           fl_bsdf_result *r, uint32_t n) {
+    if (!ctx || !m || (n && (!q || !r))) return FL_E_INVALID;
+    if (m->fresnel_type < FL_FRESNEL_DIELECTRIC || m->fresnel_type > FL_FRESNEL_WAVELENGTH_METALLIC)
+        return set_error(ctx, FL_E_INVALID, "fl_bsdf_query_standard: bad fresnel type");
+    if (n == 0) return FL_OK;
+    (void)hipSetDevice(ctx->device);
+    fl_standard_query *dq = nullptr;
+    fl_bsdf_result *dr = nullptr;
+    int rc = FL_OK;
+    if (hipMalloc((void **)&dq, sizeof(*q) * (size_t)n) != hipSuccess ||
+        hipMalloc((void **)&dr, sizeof(*r) * (size_t)n) != hipSuccess)
+        rc = set_error(ctx, FL_E_OOM, "bsdf buffers");
+    if (!rc && hipMemcpy(dq, q, sizeof(*q) * (size_t)n, hipMemcpyHostToDevice) != hipSuccess)
+        rc = set_error(ctx, FL_E_HIP, "upload");
+    if (!rc) rc = flx::bsdf_queries_standard(ctx, m, dq, dr, n);
+    if (!rc && hipStreamSynchronize(ctx->stream) != hipSuccess) rc = set_error(ctx, FL_E_HIP, "bsdf kernel");
+    if (!rc && hipMemcpy(r, dr, sizeof(*r) * (size_t)n, hipMemcpyDeviceToHost) != hipSuccess)
+        rc = set_error(ctx, FL_E_HIP, "download");
+    (void)hipFree(dq);
+    (void)hipFree(dr);
+    return rc;
 }
 
 int fl_bsdf_query_batch(fl_ctx_t ctx, const fl_bsdf_query *q, fl_bsdf_result *r, uint32_t n) {

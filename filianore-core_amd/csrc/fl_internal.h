@@ -8,6 +8,7 @@
 #include "../../include/fl_gpu.h"
 #include "fl_bvh.h"
 #include "fl_shade.h"
+#include "fl_standard.h"
 
 struct DevMaterial {        // 32 bytes
     int32_t kind;
@@ -66,15 +67,18 @@ struct fl_scene_s {
     uint32_t n_tris = 0, n_verts = 0, n_nodes = 0, depth = 0, n_mats = 0, n_lights = 0;
     uint32_t rgb_basis = 0;         // table slot of the first of the 7 RGB->spectrum basis curves
     bool shadow_colors = false;     // some light has a non-black shadowColor
+    bool has_standard = false;      // some material is FL_MAT_STANDARD
     fl::Node8 *d_nodes = nullptr;
     fl::TriRec *d_tris = nullptr;
     DevMaterial *d_mats = nullptr;
     DevLight *d_lights = nullptr;
+    fl::StdParams *d_std = nullptr;   // StandardSurface lobe stacks (FL_MAT_STANDARD materials)
     SpecTables spec[2];     // [FL_SPECTRAL_HERO, FL_SPECTRAL_PARITY]
     double build_ms = 0.0;
     float bmin[3], bmax[3];
     std::vector<DevMaterial> h_mats;
     std::vector<DevLight> h_lights;
+    std::vector<fl::StdParams> h_std;
 };
 
 // Learned proposal: fp32 master parameters + Adam moments, and the bf16
@@ -123,9 +127,11 @@ int set_error(fl_ctx_t ctx, int code, const std::string &msg);
 int build_bvh(fl_ctx_t ctx, fl_scene_s *s, const float *d_verts, const uint32_t *d_idx,
               const uint32_t *d_mat, const uint32_t *d_light);
 
-// spectral preprocessing (fl_spectra.cpp)
+// spectral preprocessing (fl_spectra.cpp); s->h_std must be filled first
 void build_spectral_tables(fl_scene_s *s, const fl_scene_desc *desc, int mode,
                            std::vector<DevMaterial> &mats);
+// the lobe stack StandardSurfaceMaterial builds for these parameters
+fl::StdParams standard_params(const fl_standard_surface &ss);
 
 // traversal (fl_trace.hip)
 int trace_rays(fl_ctx_t ctx, fl_scene_s *s, const fl_ray *d_rays, fl_hit *d_hits, uint32_t n,
@@ -133,6 +139,8 @@ int trace_rays(fl_ctx_t ctx, fl_scene_s *s, const fl_ray *d_rays, fl_hit *d_hits
 
 // BSDF queries (fl_render.hip)
 int bsdf_queries(fl_ctx_t ctx, const fl_bsdf_query *d_q, fl_bsdf_result *d_r, uint32_t n);
+int bsdf_queries_standard(fl_ctx_t ctx, const fl_standard_surface *m, const fl_standard_query *d_q, fl_bsdf_result *d_r,
+                          uint32_t n);
 
 // One (scene, camera, film, spp) of a render; fl_render is a batch of one.
 struct HostItem {

@@ -404,6 +404,34 @@ __device__ __forceinline__ void setup_bsdf(const RenderArgs &a, const float *spe
     b.dispersive = m.cauchy_b != 0.f;
 }
 
+// A standard material's spectra at one vertex, read from the scene's tables
+// when a lobe needs them (they are not all kept in registers).
+struct TabSpec {
+    const float *spec;
+    const uint32_t *slot;
+    int gs;
+    Lanes ln;
+    bool parity;
+    __device__ float at(int which, int k) const { return spec_at(spec + (size_t)slot[which] * gs, ln, k); }
+    // ThinFilmInteference samples 60 wavelengths 400, 405, ... (thinfilm.cpp:38): in
+    // parity mode a lane sees its bin's; otherwise its own wavelength
+    __device__ float tf_lambda(int k) const { return parity ? 400.f + 5.f * (float)ln.idx[k] : ln.lam[k]; }
+};
+
+constexpr uint32_t kNonSpecular = (BSDF_REFLECTION | BSDF_TRANSMISSION | BSDF_DIFFUSE | BSDF_GLOSSY);
+constexpr uint32_t kAllLobes = kNonSpecular | BSDF_SPECULAR;
+
+// BSDF::evaluate / pdf with flags BSDF_ALL & ~BSDF_SPECULAR for either kind of material
+__device__ __forceinline__ void shade_eval_nonspec(const Bsdf &b, const StdParams *sd, const TabSpec &tsp, v3 wo, v3 wi,
+                                                   bool refl, float f[4]) {
+    if (sd) std_bsdf_eval(*sd, tsp, b.vndf_pdf, wo, wi, refl, kNonSpecular, f);
+    else bsdf_eval_nonspec(b, wo, wi, refl, f);
+}
+
+__device__ __forceinline__ float shade_pdf_nonspec(const Bsdf &b, const StdParams *sd, v3 wo, v3 wi) {
+    return sd ? std_bsdf_pdf(*sd, b.vndf_pdf, wo, wi, kNonSpecular) : bsdf_pdf_nonspec(b, wo, wi);
+}
+
 // The value of light Lj toward the shading point for delta lights
 // (sample_li of point.cpp:12-20, spot.cpp:23-31, directional.cpp:15-24):
 // the direction wi toward the light, the unnormalized segment to its
@@ -441,8 +469,8 @@ __device__ __forceinline__ bool delta_light_sample(const DevLight &Lj, v3 p, v3 
 // One NEE sample of light `li` (estimate_direct's light half, integrator.cpp:29-59):
 // returns true with the (unoccluded) contribution, the contribution if the
 // segment turns out occluded (Li = shadowColor) and the shadow segment.
-__device__ __forceinline__ bool nee_sample(const RenderArgs &a, const SceneRef &S, const Bsdf &b, const Lanes &ln,
-                                           uint32_t li, float sel,
+__device__ __forceinline__ bool nee_sample(const RenderArgs &a, const SceneRef &S, const Bsdf &b, const StdParams *sd,
+                                           const TabSpec &tsp, const Lanes &ln, uint32_t li, float sel,
                                            float ux, float uy, v3 p, v3 perr, v3 ng, v3 fs, v3 ft, v3 woW, v3 wo,
                                            const float beta[4], float contrib[4], float occ[4], float4 &sh_o,
                                            float4 &sh_d, const float *lb, float mix) {
@@ -457,7 +485,7 @@ __device__ __forceinline__ bool nee_sample(const RenderArgs &a, const SceneRef &
         v3 wil(dot(wi, fs), dot(wi, ft), dot(wi, ng));
         bool refl = dot(wi, ng) * dot(woW, ng) > 0.f;
         float f[4];
-        bsdf_eval_nonspec(b, wo, wil, refl, f);
+        shade_eval_nonspec(b, sd, tsp, wo, wil, refl, f);
         const float cw = absdot(wi, ng);
         bool black = true;
         for (int k = 0; k < 4; ++k) { f[k] = f[k] * cw; black = black && f[k] == 0.f; }
@@ -489,9 +517,9 @@ __device__ __forceinline__ bool nee_sample(const RenderArgs &a, const SceneRef &
     v3 wil(dot(wi, fs), dot(wi, ft), dot(wi, ng));
     bool refl = dot(wi, ng) * dot(woW, ng) > 0.f;
     float f[4];
-    bsdf_eval_nonspec(b, wo, wil, refl, f);
+    shade_eval_nonspec(b, sd, tsp, wo, wil, refl, f);
     float cw = absdot(wi, ng);
-    float pb = bsdf_pdf_nonspec(b, wo, wil);
+    float pb = shade_pdf_nonspec(b, sd, wo, wil);
     // guided vertices sample the mixture (1 - mix) BSDF + mix guide: that is
     // the density the light sample's MIS weight must compete with
     if (lb) pb = (1.f - mix) * pb + mix * guide_pdf_at(lb, wi);
@@ -553,7 +581,7 @@ __device__ __forceinline__ void normal_spectrum(const float *basis, int GS, v3 n
 // (counting sort in LDS) before shading them, so a wave runs one BSDF's code
 // instead of every kind present in its 64 entries. Per-path results do not
 // depend on the order; only the order of the next queue changes.
-constexpr uint32_t kSortKeys = 5;   // miss / no material, Lambert, conductor, dielectric, past the end
+constexpr uint32_t kSortKeys = 6;   // miss / no material, Lambert, conductor, dielectric, standard, past the end
 
 __device__ __forceinline__ uint32_t shade_key(const RenderArgs &a, int cur, uint32_t i, uint32_t n) {
     if (i >= n) return kSortKeys - 1;
@@ -565,7 +593,10 @@ __device__ __forceinline__ uint32_t shade_key(const RenderArgs &a, int cur, uint
     return (uint32_t)S.mats[mat].kind;   // MAT_LAMBERT 1, MAT_CONDUCTOR 2, MAT_DIELECTRIC 3
 }
 
-template <int WAVES>
+// STD: the render has standard-surface materials. Their lobe stack is only
+// compiled into that variant, so renders without them keep the smaller
+// kernel (registers and instruction cache).
+template <int WAVES, bool STD>
 __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WAVES, 8))) k_shade(RenderArgs a, int b) {
     __shared__ uint32_t s_cnt[kSortKeys];
     __shared__ uint16_t s_perm[kBlock];
@@ -685,9 +716,11 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WAV
         // -------- phase 2: NEE, one sample per light (uniform_sample_all_lights)
         Bsdf bs_;
         v3 fs, ft, woW, wo;
+        const StdParams *sd = nullptr;     // the lobe stack of a standard material
         if (mat_hit) {
             const DevMaterial m = S.mats[mkind];
             setup_bsdf(a, S.spec, m, ln, bs_);
+            if (STD && m.kind == MAT_STANDARD) sd = S.stds + m.spec_a;
             eta_bsdf = m.eta_bsdf;
             coordinate_system(ng, &fs, &ft);                       // BSDF frame, bsdf.cpp:6-10 (ns = ng)
             fs = normalize(fs);
@@ -696,7 +729,9 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WAV
             rb = rng_base(S.seed, a.st_key[sid].x, a.st_key[sid].y, bounce);
         }
         const float *lb = (a.guided && mat_hit && guidable(bs_.kind)) ? a.lobes + (size_t)i * kGOut : nullptr;
-        const bool do_nee = mat_hit && !is_specular_kind(bs_.kind) && S.n_lights > 0;
+        const TabSpec tsp{S.spec, sd ? sd->slot : nullptr, GS, ln, a.grid.linear == 0};
+        const bool do_nee = mat_hit && S.n_lights > 0 &&
+                            (sd ? std_num_components(*sd, kNonSpecular) > 0 : !is_specular_kind(bs_.kind));
 #pragma unroll
         for (int j = 0; j < 2; ++j) {
             float contrib[4] = {0.f, 0.f, 0.f, 0.f}, occ[4] = {0.f, 0.f, 0.f, 0.f};
@@ -710,7 +745,7 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WAV
                     li = min((uint32_t)(rng_u(rb, 7) * (float)S.n_lights), S.n_lights - 1u);
                     sel = 1.f / (float)S.n_lights;
                 }
-                ok = nee_sample(a, S, bs_, ln, li, sel, rng_u(rb, 3 + 2 * j), rng_u(rb, 4 + 2 * j), p, perr, ng, fs, ft,
+                ok = nee_sample(a, S, bs_, sd, tsp, ln, li, sel, rng_u(rb, 3 + 2 * j), rng_u(rb, 4 + 2 * j), p, perr, ng, fs, ft,
                                 woW, wo, beta, contrib, occ, sh_o, sh_d, lb, a.guide_mix);
                 nee_delta = S.lights[li].type != 0u;
             }
@@ -770,6 +805,17 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WAV
                 }
                 ok = bsm.pdf > 0.f;
                 if (ok) {
+                    bool black = true;
+                    for (int k = 0; k < 4; ++k) black = black && bsm.f[k] == 0.f;
+                    if (black) ok = false;
+                }
+            } else if (STD && sd) {
+                // BSDF::sample over the lobe stack (component choice and u remap inside)
+                bsm.wi = v3(0.f, 0.f, 0.f);
+                std_bsdf_sample(*sd, tsp, bs_.vndf_pdf, wo, rng_u(rb, 0), rng_u(rb, 1), kAllLobes, bsm);
+                ok = bsm.pdf != 0.f;
+                if (ok) {
+                    wiW = fs * bsm.wi.x + ft * bsm.wi.y + ng * bsm.wi.z;   // to_world
                     bool black = true;
                     for (int k = 0; k < 4; ++k) black = black && bsm.f[k] == 0.f;
                     if (black) ok = false;
@@ -956,6 +1002,53 @@ __global__ void k_bsdf_query(const fl_bsdf_query *__restrict__ q, fl_bsdf_result
     r[i] = R;
 }
 
+// Spectra of a standard-surface query: given per lane
+struct QuerySpec {
+    const fl_standard_query *q;
+    __device__ float at(int which, int k) const {
+        switch (which) {
+        case SS_KD: return q->kd[k];
+        case SS_KS: return q->ks[k];
+        case SS_KT: return q->kt[k];
+        case SS_SHEEN: return q->sheen[k];
+        case SS_COAT: return q->coat[k];
+        case SS_ETA: return q->eta[k];
+        default: return q->k[k];
+        }
+    }
+    __device__ float tf_lambda(int k) const { return q->lam[k]; }
+};
+
+// The StandardSurface BSDF for one material: evaluate and pdf with
+// BSDF_ALL & ~BSDF_SPECULAR (what next-event estimation asks), sample with
+// BSDF_ALL (what the path continuation asks), in the local frame (ng = z).
+__global__ void k_bsdf_query_standard(StdParams m, const fl_standard_query *__restrict__ q, fl_bsdf_result *__restrict__ r,
+                                      uint32_t n) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const fl_standard_query Q = q[i];
+    const QuerySpec sp{&Q};
+    const bool vndf = (Q.flags & FL_RENDER_VNDF_PDF) != 0;
+    v3 wo(Q.wo[0], Q.wo[1], Q.wo[2]), wi(Q.wi[0], Q.wi[1], Q.wi[2]);
+    fl_bsdf_result R;
+    float f[4];
+    std_bsdf_eval(m, sp, vndf, wo, wi, wi.z * wo.z > 0.f, kNonSpecular, f);
+    for (int k = 0; k < 4; ++k) R.f_eval[k] = f[k];
+    R.pdf_eval = std_bsdf_pdf(m, vndf, wo, wi, kNonSpecular);
+    BsdfSample s;
+    s.wi = v3(0.f, 0.f, 0.f);
+    std_bsdf_sample(m, sp, vndf, wo, Q.u[0], Q.u[1], kAllLobes, s);
+    if (s.pdf == 0.f) {
+        s.type = 0;
+        for (int k = 0; k < 4; ++k) s.f[k] = 0.f;
+    }
+    for (int k = 0; k < 4; ++k) R.f_sample[k] = s.f[k];
+    R.pdf_sample = s.pdf;
+    R.wi_sample[0] = s.wi.x; R.wi_sample[1] = s.wi.y; R.wi_sample[2] = s.wi.z;
+    R.sampled_type = s.type;
+    r[i] = R;
+}
+
 template <typename T>
 T *grow(void *&p, size_t &cap, size_t bytes, bool &err) {
     if (cap < bytes) {
@@ -1110,6 +1203,7 @@ int render_items(fl_ctx_t ctx, const HostItem *hi, uint32_t n_items, const fl_re
     a.n_lights = s->n_lights;
     a.nee_all = s->n_lights <= 2 ? 1u : 0u;
     a.rgb_basis = s->rgb_basis;
+    a.stds = s->d_std;
     a.spec = tab.d_tab;
     a.grid = tab.grid;
     a.seed = p->seed;
@@ -1128,6 +1222,7 @@ int render_items(fl_ctx_t ctx, const HostItem *hi, uint32_t n_items, const fl_re
         it.n_lights = sk->n_lights;
         it.nee_all = sk->n_lights <= 2 ? 1u : 0u;
         it.rgb_basis = sk->rgb_basis;
+        it.stds = sk->d_std;
         it.cam = make_camera(&hi[k].cam);
         it.tiles_x = (hi[k].cam.width + 7) / 8;
         it.npix = it.tiles_x * ((hi[k].cam.height + 7) / 8) * 64u;
@@ -1284,12 +1379,20 @@ int render_items(fl_ctx_t ctx, const HostItem *hi, uint32_t n_items, const fl_re
     // already runs, so the GPU does not wait for a host round trip. With
     // FL_RENDER_GRAPH a chunk is one captured graph, replayed.
     constexpr uint32_t kChunk = 4;   // even: a chunk starts and ends on queue parity 0
-    const int variant = (counters ? 1 : 0) | (p->shade_waves >= 4 ? 2 : 0) | ((int)(tv - kTraceVariants) << 2);
+    bool has_std = false;
+    for (uint32_t k = 0; k < n_items; ++k) has_std = has_std || hi[k].scene->has_standard;
+    const int variant = (counters ? 1 : 0) | (p->shade_waves >= 4 ? 2 : 0) | ((int)(tv - kTraceVariants) << 2) |
+                        (has_std ? 1 << 16 : 0);
     const uint32_t infer_grid = (uint32_t)ctx->n_cu * 16;   // one-wave blocks of 10.5 KB LDS
     auto launch_shade = [&](const RenderArgs &args, dim3 g, int j, bool with_infer = true) {
         if (with_infer && args.guided) hipLaunchKernelGGL(k_guide_infer, dim3(infer_grid), dim3(64), 0, st, args, j);
-        if (variant & 2) hipLaunchKernelGGL(k_shade<4>, g, dim3(kBlock), 0, st, args, j);
-        else hipLaunchKernelGGL(k_shade<2>, g, dim3(kBlock), 0, st, args, j);
+        if (has_std) {
+            if (variant & 2) hipLaunchKernelGGL((k_shade<4, true>), g, dim3(kBlock), 0, st, args, j);
+            else hipLaunchKernelGGL((k_shade<2, true>), g, dim3(kBlock), 0, st, args, j);
+        } else {
+            if (variant & 2) hipLaunchKernelGGL((k_shade<4, false>), g, dim3(kBlock), 0, st, args, j);
+            else hipLaunchKernelGGL((k_shade<2, false>), g, dim3(kBlock), 0, st, args, j);
+        }
     };
     auto launch_bounce = [&](const RenderArgs &args, uint32_t j) {
         tv->launch(counters, dim3(grid_pers), st, args, (int)j);
@@ -1591,6 +1694,15 @@ int autotune_items(fl_ctx_t ctx, const HostItem *hi, uint32_t n, const fl_render
     out->ms_best = ms_cur;
     out->ms_default = ms_default;
     out->n_tried = tried;
+    return FL_OK;
+}
+
+int bsdf_queries_standard(fl_ctx_t ctx, const fl_standard_surface *m, const fl_standard_query *d_q, fl_bsdf_result *d_r,
+                          uint32_t n) {
+    if (n == 0) return FL_OK;
+    const StdParams P = standard_params(*m);
+    hipLaunchKernelGGL(k_bsdf_query_standard, dim3((n + 255) / 256), dim3(256), 0, ctx->stream, P, d_q, d_r, n);
+    FL_HIP_CHECK(ctx, hipGetLastError());
     return FL_OK;
 }
 

@@ -11,6 +11,7 @@
 // wavelength sees the continuous spectrum.
 #include <algorithm>
 #include <cmath>
+#include <cstring>
 #include <vector>
 
 #include "../data/fl_tables.inc"
@@ -132,8 +133,9 @@ void build_spectral_tables(fl_scene_s *s, const fl_scene_desc *desc, int mode, s
         g.lo = 360.f; g.hi = 830.f; g.g0 = 360.f; g.ginv = 1.f; g.last = kHeroN - 2; g.linear = 1;
     }
     g.stride = stride;
-    const uint32_t nm = desc->n_materials, ne = desc->n_emission;
-    const size_t slots = 3 + 2 * (size_t)nm + ne + 7;   // + the RGB->spectrum basis (normal AOV)
+    const uint32_t nm = desc->n_materials, ne = desc->n_emission, nstd = desc->n_standard_surfaces;
+    // + the RGB->spectrum basis (normal AOV) + 7 spectra per standard surface
+    const size_t slots = 3 + 2 * (size_t)nm + ne + 7 + (size_t)fl::SS_N * nstd;
     T.host.assign(slots * stride, 0.f);
     float *tab = T.host.data();
     // colour matching functions
@@ -181,6 +183,9 @@ void build_spectral_tables(fl_scene_s *s, const fl_scene_desc *desc, int mode, s
             from_reflectance_rgb(B, fm.rgb_t, N, tb);
             d.eta_bsdf = fm.ior;
             d.cauchy_b = fm.cauchy_b;
+        } else if (fm.kind == FL_MAT_STANDARD) {
+            d.spec_a = fm.standard;                 // index of its lobe stack
+            d.eta_bsdf = s->h_std[fm.standard].eta_bsdf;
         }
     }
     float spd_lam[FL_SPD_N];
@@ -193,6 +198,69 @@ void build_spectral_tables(fl_scene_s *s, const fl_scene_desc *desc, int mode, s
     const std::vector<float> *basis[7] = {&B.white, &B.cyan, &B.magenta, &B.yellow, &B.red, &B.green, &B.blue};
     for (int j = 0; j < 7; ++j)
         std::copy(basis[j]->begin(), basis[j]->begin() + stride, tab + (s->rgb_basis + (size_t)j) * stride);
+    // standard surfaces: kd, ks, kt, sheen, coat (from_reflectance_rgb, then
+    // spectrum_clamp as compute_scattering_functions applies it), forge eta, k
+    const uint32_t std0 = s->rgb_basis + 7;
+    for (uint32_t j = 0; j < nstd; ++j) {
+        const fl_standard_surface &ss = desc->standard_surfaces[j];
+        fl::StdParams &P = s->h_std[j];
+        for (int w = 0; w < fl::SS_N; ++w) P.slot[w] = std0 + fl::SS_N * j + (uint32_t)w;
+        const float *rgbs[5] = {ss.kd, ss.ks, ss.kt, ss.sheen_color, ss.coat};
+        for (int w = 0; w < 5; ++w) from_reflectance_rgb(B, rgbs[w], N, tab + (size_t)P.slot[w] * stride);
+        int mt = (ss.metal >= 0 && ss.metal < FL_N_METALS) ? ss.metal : 0;
+        project(mode, FL_METAL_LAMBDA[mt], FL_METAL_ETA[mt], FL_METAL_COUNT[mt], tab + (size_t)P.slot[fl::SS_ETA] * stride);
+        project(mode, FL_METAL_LAMBDA[mt], FL_METAL_K[mt], FL_METAL_COUNT[mt], tab + (size_t)P.slot[fl::SS_K] * stride);
+    }
+}
+
+// StandardSurfaceMaterial::compute_scattering_functions
+// (shading/materials/standardsurface.cpp:72-159) with constant textures: the
+// lobes it adds, in order, and the constants their constructors derive.
+fl::StdParams standard_params(const fl_standard_surface &ss) {
+    fl::StdParams P;
+    std::memset(&P, 0, sizeof(P));
+    const bool metal_wl = ss.fresnel_type == FL_FRESNEL_WAVELENGTH_METALLIC;
+    P.eta_bsdf = (ss.ks_weight > 0.f && ss.kt_weight > 0.f) ? ss.ks_ior : 1.f;
+    P.kd_weight = ss.kd_weight;
+    if ((ss.kd_weight > 0.f && ss.ks_weight <= 0.f) || (ss.kd_weight > 0.f && ss.ks_weight > 0.f && metal_wl)) {
+        P.lobe[P.n++] = ss.thin_walled ? fl::SL_DIFFTRANS : fl::SL_OREN;
+        fl::oren_nayar_AB(ss.kd_roughness * 90.f, &P.on_A, &P.on_B);   // remap [0-1] roughness to [0-90] sigma
+    }
+    const float aspect = std::sqrt(1.f - ss.ks_anisotropic * 0.9f);
+    const float r2 = ss.ks_roughness * ss.ks_roughness;
+    P.ax = std::max(.001f, r2 / aspect);
+    P.ay = std::max(.001f, r2 * aspect);
+    P.ks_ior = ss.ks_ior;
+    P.ro = ((1.f - ss.ks_ior) / (1.f + ss.ks_ior)) * ((1.f - ss.ks_ior) / (1.f + ss.ks_ior));
+    P.ks_weight = ss.ks_weight;
+    P.fresnel_type = ss.fresnel_type;
+    if (ss.ks_weight > 0.f) {
+        if (ss.kt_weight > 0.f) {
+            P.lobe[P.n++] = fl::SL_FSPEC;
+        } else if (metal_wl) {
+            P.lobe[P.n++] = fl::SL_CONDUCTOR;
+        } else if (ss.fresnel_type == FL_FRESNEL_DIELECTRIC || ss.fresnel_type == FL_FRESNEL_METALLIC) {
+            P.lobe[P.n++] = fl::SL_FBLEND;
+            fl::oren_nayar_AB(ss.kd_roughness, &P.fb_A, &P.fb_B);       // the blend's OrenNayar takes kdroughness as is
+        }
+    }
+    if (ss.thin_film_thickness > 0.f) {
+        P.lobe[P.n++] = fl::SL_THINFILM;
+        P.tf_ior = ss.thin_film_ior;
+        P.tf_thickness = ss.thin_film_thickness;
+    }
+    if (ss.sheen_weight > 0.f) {
+        P.lobe[P.n++] = fl::SL_SHEEN;
+        P.sheen_weight = ss.sheen_weight;
+        P.sheen_rough = std::max(.001f, ss.sheen_roughness);
+    }
+    if (ss.coat_weight > 0.f) {
+        P.lobe[P.n++] = fl::SL_COAT;
+        P.coat_weight = ss.coat_weight;
+        P.coat_ior = ss.coat_ior;
+        P.coat_gloss = (1.f - ss.coat_gloss) * .1f + ss.coat_gloss * .001f;   // lerp(gloss, .1, .001)
+    }
+    return P;
 }
 
 } // namespace flx

@@ -45,6 +45,7 @@ struct ItemDev {
     const DevMaterial *mats;
     const DevLight *lights;
     const float *spec;       // spectral tables of the render's mode
+    const StdParams *stds;   // StandardSurface lobe stacks
     uint32_t n_lights, nee_all;
     uint32_t rgb_basis;      // table slot of the RGB->spectrum basis
     CameraDev cam;
@@ -62,6 +63,7 @@ struct SceneRef {
     const DevMaterial *mats;
     const DevLight *lights;
     const float *spec;
+    const StdParams *stds;
     uint32_t n_lights, nee_all;
     uint32_t rgb_basis;
     uint64_t seed;
@@ -79,6 +81,7 @@ struct RenderArgs {
     uint32_t n_lights;
     uint32_t nee_all;        // 1: one NEE sample per light (uniform_sample_all_lights)
     uint32_t rgb_basis;      // table slot of the RGB->spectrum basis (normal AOV)
+    const StdParams *stds;   // StandardSurface lobe stacks
     const float *spec;
     SpecGrid grid;
     uint32_t n_paths;        // this pass: path ids of all items
@@ -139,6 +142,7 @@ __device__ __forceinline__ SceneRef scene_of(const RenderArgs &a, uint32_t item)
         r.n_lights = it.n_lights;
         r.nee_all = it.nee_all;
         r.rgb_basis = it.rgb_basis;
+        r.stds = it.stds;
         r.seed = it.seed;
     } else {
         r.sc = a.sc;
@@ -148,6 +152,7 @@ __device__ __forceinline__ SceneRef scene_of(const RenderArgs &a, uint32_t item)
         r.n_lights = a.n_lights;
         r.nee_all = a.nee_all;
         r.rgb_basis = a.rgb_basis;
+        r.stds = a.stds;
         r.seed = a.seed;
     }
     return r;

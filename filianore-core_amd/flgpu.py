@@ -14,7 +14,8 @@ LIB_PATH = os.path.join(HERE, "libfl_gpu.so")
 
 FL_OK = 0
 FL_E_INVALID, FL_E_HIP, FL_E_OOM, FL_E_UNSUPPORTED, FL_E_OVERFLOW, FL_E_NUMERIC = -1, -2, -3, -4, -5, -6
-FL_MAT_NONE, FL_MAT_LAMBERT, FL_MAT_CONDUCTOR, FL_MAT_DIELECTRIC = 0, 1, 2, 3
+FL_MAT_NONE, FL_MAT_LAMBERT, FL_MAT_CONDUCTOR, FL_MAT_DIELECTRIC, FL_MAT_STANDARD = 0, 1, 2, 3, 4
+FL_FRESNEL_DIELECTRIC, FL_FRESNEL_METALLIC, FL_FRESNEL_WAVELENGTH_METALLIC = 0, 1, 2
 FL_CLOSEST_HIT, FL_ANY_HIT, FL_SKIP_LIGHTS, FL_COUNTERS = 0, 1, 2, 4
 FL_SPECTRAL_HERO, FL_SPECTRAL_PARITY = 0, 1
 FL_RENDER_GUIDED, FL_RENDER_COUNTERS, FL_RENDER_VNDF_PDF, FL_RENDER_GRAPH = 1, 2, 4, 8
@@ -35,7 +36,23 @@ f4 = C.c_float * 4
 class Material(C.Structure):
     _fields_ = [("kind", C.c_int32), ("rgb", f3), ("rgb_t", f3), ("roughness", C.c_float),
                 ("anisotropic", C.c_float), ("weight", C.c_float), ("metal", C.c_int32), ("ior", C.c_float),
-                ("cauchy_b", C.c_float), ("pad", C.c_uint32 * 3)]
+                ("cauchy_b", C.c_float), ("standard", C.c_uint32), ("pad", C.c_uint32 * 2)]
+
+
+class StandardSurface(C.Structure):
+    _fields_ = [("kd_weight", C.c_float), ("kd", f3), ("kd_roughness", C.c_float),
+                ("ks_weight", C.c_float), ("ks", f3), ("ks_roughness", C.c_float), ("ks_anisotropic", C.c_float),
+                ("fresnel_type", C.c_int32), ("ks_ior", C.c_float), ("metal", C.c_int32),
+                ("sheen_weight", C.c_float), ("sheen_color", f3), ("sheen_roughness", C.c_float),
+                ("kt_weight", C.c_float), ("kt", f3),
+                ("coat_weight", C.c_float), ("coat", f3), ("coat_ior", C.c_float), ("coat_gloss", C.c_float),
+                ("thin_film_thickness", C.c_float), ("thin_film_ior", C.c_float),
+                ("thin_walled", C.c_int32), ("pad", C.c_uint32)]
+
+
+class StandardQuery(C.Structure):
+    _fields_ = [("kd", f4), ("ks", f4), ("kt", f4), ("sheen", f4), ("coat", f4), ("eta", f4), ("k", f4), ("lam", f4),
+                ("wo", f3), ("wi", f3), ("u", C.c_float * 2), ("flags", C.c_uint32)]
 
 
 class AreaLight(C.Structure):
@@ -56,7 +73,8 @@ class SceneDesc(C.Structure):
                 ("materials", C.POINTER(Material)), ("n_materials", C.c_uint32),
                 ("lights", C.POINTER(AreaLight)), ("n_lights", C.c_uint32),
                 ("emission_spd", C.POINTER(C.c_float)), ("n_emission", C.c_uint32),
-                ("delta_lights", C.POINTER(DeltaLight)), ("n_delta_lights", C.c_uint32)]
+                ("delta_lights", C.POINTER(DeltaLight)), ("n_delta_lights", C.c_uint32),
+                ("standard_surfaces", C.POINTER(StandardSurface)), ("n_standard_surfaces", C.c_uint32)]
 
 
 class SceneInfo(C.Structure):
@@ -171,7 +189,7 @@ EXPORTS = [
     "fl_intersect", "fl_intersect_device", "fl_render", "fl_render_batch", "fl_autotune", "fl_autotune_batch",
     "fl_guide_create", "fl_guide_destroy", "fl_guide_train_step", "fl_guide_forward",
     "fl_guide_get_params", "fl_guide_set_params", "fl_guide_gradient", "fl_guide_stats",
-    "fl_bsdf_query_batch", "fl_develop_film", "fl_develop",
+    "fl_bsdf_query_batch", "fl_bsdf_query_standard", "fl_develop_film", "fl_develop",
     "fl_gen_default_options", "fl_gen_config", "fl_gen_free", "fl_gen_describe", "fl_gen_blackbody",
 ]
 
@@ -212,6 +230,8 @@ def lib():
         L.fl_guide_stats.argtypes = [vp, C.POINTER(C.c_uint32), C.POINTER(C.c_float), C.POINTER(C.c_int)]
         L.fl_bsdf_query_batch.argtypes = [vp, C.POINTER(BsdfQuery), C.POINTER(BsdfResult), C.c_uint32]
         L.fl_develop_film.argtypes = [vp, C.c_uint32, C.POINTER(DevelopParams), vp, vp]
+        L.fl_bsdf_query_standard.argtypes = [vp, C.POINTER(StandardSurface), C.POINTER(StandardQuery),
+                                             C.POINTER(BsdfResult), C.c_uint32]
         L.fl_develop.argtypes = [vp, vp, C.c_uint32, C.POINTER(DevelopParams), vp, vp]
         L.fl_gen_default_options.argtypes = [C.POINTER(GenOptions)]
         L.fl_gen_config.argtypes = [C.c_int, C.POINTER(GenOptions), C.POINTER(GenScene)]
@@ -257,6 +277,8 @@ class GenSceneData:
             self.n_emission = g.n_emission
             self.delta_lights = (DeltaLight * 1)()
             self.n_delta_lights = 0
+            self.standard_surfaces = (StandardSurface * 1)()
+            self.n_standard_surfaces = 0
             self.camera = Camera()
             C.pointer(self.camera)[0] = g.camera
             self.spp = g.spp
@@ -266,7 +288,7 @@ class GenSceneData:
 
     @classmethod
     def from_arrays(cls, vertices, indices, tri_material, materials, lights, emission, camera, spp=16, depth=8,
-                    delta_lights=()):
+                    delta_lights=(), standard_surfaces=()):
         """A scene from explicit buffers: materials is a list of Material,
         lights a list of (tri, emission_index, intensity[, shadow_emission]),
         delta_lights a list of DeltaLight, emission an (n, FL_SPD_N) array,
@@ -281,6 +303,8 @@ class GenSceneData:
         self.n_lights = len(ls)
         self.delta_lights = (DeltaLight * max(1, len(delta_lights)))(*delta_lights)
         self.n_delta_lights = len(delta_lights)
+        self.standard_surfaces = (StandardSurface * max(1, len(standard_surfaces)))(*standard_surfaces)
+        self.n_standard_surfaces = len(standard_surfaces)
         em = np.ascontiguousarray(emission, np.float32).reshape(-1, FL_SPD_N)
         self.emission = em.ravel().copy()
         self.n_emission = em.shape[0]
@@ -311,6 +335,8 @@ class GenSceneData:
         d.n_emission = self.n_emission
         d.delta_lights = C.cast(self.delta_lights, C.POINTER(DeltaLight))
         d.n_delta_lights = self.n_delta_lights
+        d.standard_surfaces = C.cast(self.standard_surfaces, C.POINTER(StandardSurface))
+        d.n_standard_surfaces = self.n_standard_surfaces
         return d
 
     def triangles(self):
@@ -415,6 +441,24 @@ class Context:
         _check(self.L.fl_develop_film(self.h, item, C.byref(p), rgb.ctypes.data, u8.ctypes.data), self.h,
                "fl_develop_film")
         return rgb, u8
+
+    def bsdf_queries_standard(self, material, queries):
+        """fl_bsdf_query_standard: the full StandardSurface BSDF of one
+        material, over a list of StandardQuery (returns a list of BsdfResult)
+        or a numpy array of dtype(StandardQuery) (returns an array of
+        dtype(BsdfResult))."""
+        if isinstance(queries, np.ndarray):
+            q = np.ascontiguousarray(queries, np.dtype(StandardQuery))
+            r = np.zeros(q.shape[0], np.dtype(BsdfResult))
+            _check(self.L.fl_bsdf_query_standard(self.h, C.byref(material), q.ctypes.data_as(C.POINTER(StandardQuery)),
+                                                 r.ctypes.data_as(C.POINTER(BsdfResult)), q.shape[0]),
+                   self.h, "fl_bsdf_query_standard")
+            return r
+        n = len(queries)
+        q = (StandardQuery * n)(*queries)
+        r = (BsdfResult * n)()
+        _check(self.L.fl_bsdf_query_standard(self.h, C.byref(material), q, r, n), self.h, "fl_bsdf_query_standard")
+        return list(r)
 
     def bsdf_queries(self, queries):
         n = len(queries)

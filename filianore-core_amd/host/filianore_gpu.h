@@ -98,6 +98,31 @@ struct Material {
         r.m.weight = 1.f;
         return r;
     }
+    // StandardSurfaceMaterial (shading/materials/standardsurface.h:14-37), its
+    // constructor's parameters in their order, every texture constant (colours
+    // as reflectance RGB); the bump map is not supported. fresnelType is
+    // FL_FRESNEL_* (FresnelType), metalType a forge metal.
+    static Material standard_surface(float kdweight, Vector3f kd, float kdroughness, float ksweight, Vector3f ks,
+                                     float ksroughness, float ksanisotropic, int fresnelType, float ksIOR,
+                                     int metalType, float sheenweight, Vector3f sheenColor, float sheenroughness,
+                                     float ktweight, Vector3f kt, float krcoatweight, Vector3f krcoat,
+                                     float krcoatior, float krcoatgloss, float thinFilmThickness,
+                                     float thinFilmIOR, bool thinWalled) {
+        Material r;
+        r.m.kind = FL_MAT_STANDARD;
+        r.has_standard = true;
+        fl_standard_surface &p = r.standard;
+        auto rgb = [](float *o, const Vector3f &v) { o[0] = v.x; o[1] = v.y; o[2] = v.z; };
+        p.kd_weight = kdweight; rgb(p.kd, kd); p.kd_roughness = kdroughness;
+        p.ks_weight = ksweight; rgb(p.ks, ks); p.ks_roughness = ksroughness; p.ks_anisotropic = ksanisotropic;
+        p.fresnel_type = fresnelType; p.ks_ior = ksIOR; p.metal = metalType;
+        p.sheen_weight = sheenweight; rgb(p.sheen_color, sheenColor); p.sheen_roughness = sheenroughness;
+        p.kt_weight = ktweight; rgb(p.kt, kt);
+        p.coat_weight = krcoatweight; rgb(p.coat, krcoat); p.coat_ior = krcoatior; p.coat_gloss = krcoatgloss;
+        p.thin_film_thickness = thinFilmThickness; p.thin_film_ior = thinFilmIOR;
+        p.thin_walled = thinWalled ? 1 : 0;
+        return r;
+    }
     // a primitive without material: paths pass through it
     // (integrators/pathintegrator.cpp:33-38)
     static Material none() {
@@ -105,6 +130,8 @@ struct Material {
         r.m.kind = FL_MAT_NONE;
         return r;
     }
+    bool has_standard = false;
+    fl_standard_surface standard{};   // FL_MAT_STANDARD: the material's parameters
 };
 
 // DiffuseAreaIlluminant over one triangle (illuminants/diffusearea.cpp:7-39).
@@ -251,7 +278,14 @@ public:
             }
         }
         std::vector<fl_material> mats;
-        for (const Material &m : materials) mats.push_back(m.m);
+        std::vector<fl_standard_surface> standard;
+        for (const Material &m : materials) {
+            mats.push_back(m.m);
+            if (m.has_standard) {
+                mats.back().standard = (uint32_t)standard.size();
+                standard.push_back(m.standard);
+            }
+        }
         std::vector<fl_area_light> lights;
         std::vector<fl_delta_light> delta;
         std::vector<float> spd;
@@ -328,6 +362,8 @@ public:
         d.n_emission = (uint32_t)(spd.size() / FL_SPD_N);
         d.delta_lights = delta.empty() ? nullptr : delta.data();
         d.n_delta_lights = (uint32_t)delta.size();
+        d.standard_surfaces = standard.empty() ? nullptr : standard.data();
+        d.n_standard_surfaces = (uint32_t)standard.size();
         if (fl_scene_create(ctx_, &d, &scene_) != FL_OK) {
             err_ = fl_last_error(ctx_);
             scene_ = nullptr;

@@ -3,11 +3,12 @@
 // triangles, a Scene over it, intersect rays one at a time and in batches,
 // and render a film with GpuPathIntegrator.
 //
-//   fl_render_example [film.bin] [width height spp]
+//   fl_render_example [film.bin] [width height spp [standard]]
 //
 // The scene is the procedural Cornell box (include/fl_scenegen.h), rebuilt
 // here as one TriangleMesh per run of equal material so that triangle IDs
-// keep the generator's order. Prints one JSON line; writes the film sums
+// keep the generator's order. With "standard", the walls and blocks become
+// StandardSurfaceMaterials (Material::standard_surface). Prints one JSON line; writes the film sums
 // (float32, height x width x 3) to film.bin when given. Exit status: 0 ok,
 // 2 no GPU, 1 any other failure.
 #include <chrono>
@@ -59,6 +60,24 @@ int main(int argc, char **argv) {
         Material mm;
         mm.m = g.materials[m];
         mats.push_back(mm);
+    }
+    if (argc > 5 && strcmp(argv[5], "standard") == 0 && mats.size() >= 4) {
+        // white walls: blended diffuse/specular, thin film, sheen, coat; red wall:
+        // Oren-Nayar under a wavelength-dependent metal, thin film, coat; green
+        // wall: Schlick-metallic blend; blocks: thin-walled translucent
+        const Vector3f ks(0.9f, 0.85f, 0.8f), kt(0.8f, 0.9f, 0.95f), sheen(0.9f, 0.6f, 0.4f), coat(0.95f, 0.95f, 1.f);
+        mats[0] = Material::standard_surface(0.6f, Vector3f(0.75f, 0.72f, 0.68f), 0.5f, 0.6f, ks, 0.35f, 0.3f,
+                                             FL_FRESNEL_DIELECTRIC, 1.5f, 0, 0.5f, sheen, 0.6f, 0.f, kt, 0.5f, coat,
+                                             1.4f, 0.5f, 410.f, 1.45f, false);
+        mats[1] = Material::standard_surface(0.5f, Vector3f(0.65f, 0.12f, 0.10f), 0.3f, 0.7f, ks, 0.5f, 0.f,
+                                             FL_FRESNEL_WAVELENGTH_METALLIC, 1.5f, 2, 0.f, sheen, 0.f, 0.f, kt, 0.4f,
+                                             coat, 1.5f, 0.2f, 250.f, 1.33f, false);
+        mats[2] = Material::standard_surface(0.5f, Vector3f(0.12f, 0.45f, 0.15f), 0.2f, 0.9f, ks, 0.55f, 0.6f,
+                                             FL_FRESNEL_METALLIC, 1.5f, 4, 0.f, sheen, 0.f, 0.f, kt, 0.f, coat, 1.5f,
+                                             0.f, 0.f, 1.33f, false);
+        mats[3] = Material::standard_surface(0.5f, Vector3f(0.7f, 0.7f, 0.6f), 0.6f, 0.8f, ks, 0.f, 0.f,
+                                             FL_FRESNEL_WAVELENGTH_METALLIC, 1.45f, 6, 0.f, sheen, 0.f, 0.9f, kt, 0.3f,
+                                             coat, 1.5f, 0.4f, 0.f, 1.33f, true);
     }
     std::vector<DiffuseAreaIlluminant> lights;
     for (uint32_t l = 0; l < g.n_lights; ++l) {

@@ -62,9 +62,31 @@ enum {
     FL_MAT_LAMBERT = 1,    /* LambertMaterial, shading/materials/lambert.cpp:9-17 */
     FL_MAT_CONDUCTOR = 2,  /* StandardSurface WavelengthDependentMetallic branch,
                               standardsurface.cpp:101-125 (kdweight = 0) */
-    FL_MAT_DIELECTRIC = 3  /* StandardSurface ks>0, kt>0 branch: FresnelSpecularBXDF,
+    FL_MAT_DIELECTRIC = 3, /* StandardSurface ks>0, kt>0 branch: FresnelSpecularBXDF,
                               standardsurface.cpp:77-81,113-117 */
+    FL_MAT_STANDARD = 4    /* the whole StandardSurfaceMaterial lobe stack
+                              (standardsurface.cpp:72-159), parameters in
+                              fl_scene_desc.standard_surfaces[fl_material.standard] */
 };
+
+/* StandardSurfaceMaterial's parameters (shading/materials/standardsurface.h:
+ * constant textures; colours are reflectance RGB, turned into spectra by
+ * from_reflectance_rgb and clamped, as an RGB texture would be). */
+enum { FL_FRESNEL_DIELECTRIC = 0, FL_FRESNEL_METALLIC = 1, FL_FRESNEL_WAVELENGTH_METALLIC = 2 };
+
+typedef struct {
+    float kd_weight, kd[3], kd_roughness;                  /* diffuse */
+    float ks_weight, ks[3], ks_roughness, ks_anisotropic;  /* specular */
+    int32_t fresnel_type;                                  /* FL_FRESNEL_* */
+    float ks_ior;
+    int32_t metal;                                         /* forge metal for the wavelength-dependent metallic */
+    float sheen_weight, sheen_color[3], sheen_roughness;   /* sheen */
+    float kt_weight, kt[3];                                /* transmission */
+    float coat_weight, coat[3], coat_ior, coat_gloss;      /* clearcoat */
+    float thin_film_thickness, thin_film_ior;              /* thin film (nm) */
+    int32_t thin_walled;
+    uint32_t pad;
+} fl_standard_surface;
 
 typedef struct {
     int32_t kind;        /* FL_MAT_* */
@@ -76,7 +98,8 @@ typedef struct {
     int32_t metal;       /* forge metal 0..8 (metallurgy.cpp:42-75) */
     float ior;           /* dielectric ksIOR; with cauchy_b != 0: Cauchy A */
     float cauchy_b;      /* Cauchy B [um^2]; 0 = scalar eta as in the reference */
-    uint32_t pad[3];
+    uint32_t standard;   /* FL_MAT_STANDARD: index into fl_scene_desc.standard_surfaces */
+    uint32_t pad[2];
 } fl_material;
 
 /* One DiffuseAreaIlluminant over one triangle (illuminants/diffusearea.cpp). */
@@ -129,6 +152,8 @@ typedef struct {
     uint32_t n_emission;
     const fl_delta_light *delta_lights; /* sampled after the area lights, in this order */
     uint32_t n_delta_lights;
+    const fl_standard_surface *standard_surfaces;
+    uint32_t n_standard_surfaces;
 } fl_scene_desc;
 
 typedef struct {
@@ -336,6 +361,21 @@ typedef struct {
 } fl_bsdf_result;
 
 int fl_bsdf_query_batch(fl_ctx_t ctx, const fl_bsdf_query *q, fl_bsdf_result *r, uint32_t n);
+
+/* The full StandardSurfaceMaterial BSDF (BSDF::evaluate / sample / pdf over
+ * its lobe stack, shading/bsdf.cpp:6-151, with flags BSDF_ALL) for n queries
+ * of one material. Per query the spectra are given per lane: kd, ks, kt,
+ * sheen, coat (from_reflectance_rgb of the material's colours), the forge
+ * metal's eta and k, and the lane wavelengths (nm; the thin film's phase). */
+typedef struct {
+    float kd[4], ks[4], kt[4], sheen[4], coat[4], eta[4], k[4], lam[4];
+    float wo[3], wi[3];           /* local shading frame */
+    float u[2];
+    uint32_t flags;               /* FL_RENDER_VNDF_PDF */
+} fl_standard_query;
+
+int fl_bsdf_query_standard(fl_ctx_t ctx, const fl_standard_surface *m, const fl_standard_query *q,
+                           fl_bsdf_result *r, uint32_t n);
 
 /* ---- output path (color/spectrumoperations.cpp:58-76,209-286) -------------
  * A pixel's mean XYZ (film sum / spp) goes to a colour space (xyz_to_rgb,

@@ -336,61 +336,239 @@ T fresnel_dielectric(T cosThetaI, T etaI, T etaT) {     // fresneldielectric.cpp
     return (Rparl * Rparl + Rperp * Rperp) * T(0.5);
 }
 
-// One-lobe BSDF (every supported material has one lobe).
+// Lobes of the StandardSurfaceMaterial stack (kinds beyond FL_MAT_*)
+enum : int { OL_OREN = 11, OL_DIFFTRANS = 12, OL_FBLEND = 13, OL_THINFILM = 14, OL_SHEEN = 15, OL_COAT = 16 };
+
+// OrenNayarBRDF::OrenNayarBRDF (orennayar.cpp:6-13), sigma in degrees
+template <class T> void oren_nayar_AB(T sigma_deg, T *A, T *B) {
+    T sigma = sigma_deg * (T(PI) / T(180));
+    T sigma2 = sigma * sigma;
+    *A = T(1) - (sigma2 / (T(2) * (sigma2 + T(0.33f))));
+    *B = T(0.45f) * sigma2 / (sigma2 + T(0.09f));
+}
+
+// ThinFilmInteference::evaluate (thinfilm.cpp:8-56, amplitude coefficients
+// fresnel.h:9-23) at one wavelength; sin^2 formed as written (no index ratio)
+template <class T> T thin_film(T cosThetaI, T tfIOR, T thickness, T mediumIOR, T lambda) {
+    const T extIOR = 1;
+    T sinSqrd1 = ((extIOR / tfIOR) / (extIOR / tfIOR)) * (1 - cosThetaI * cosThetaI);
+    T sinSqrd2 = ((extIOR / mediumIOR) / (extIOR / mediumIOR)) * (1 - cosThetaI * cosThetaI);
+    if ((sinSqrd1 > 1) || (sinSqrd2 > 1)) return T(1);
+    T cos1 = std::sqrt(1 - sinSqrd1), cos2 = std::sqrt(1 - sinSqrd2);
+    auto sp_r = [](T n1, T n2, T cI, T cT) { return (n1 * cI - n2 * cT) / (n1 * cI + n2 * cT); };
+    auto sp_t = [](T n1, T n2, T cI, T cT) { return (T(2) * n1 * cI) / (n1 * cI + n2 * cT); };
+    auto pp_r = [](T n1, T n2, T cI, T cT) { return (n2 * cI - n1 * cT) / (n1 * cT + n2 * cI); };
+    auto pp_t = [](T n1, T n2, T cI, T cT) { return (T(2) * n1 * cI) / (n1 * cT + n2 * cI); };
+    T alphaSP = sp_r(tfIOR, extIOR, cos1, cosThetaI) * sp_r(mediumIOR, tfIOR, cos1, cos2);
+    T alphaPP = pp_r(tfIOR, extIOR, cos1, cosThetaI) * pp_r(mediumIOR, tfIOR, cos1, cos2);
+    T betaSP = sp_t(extIOR, tfIOR, cosThetaI, cos1) * sp_t(tfIOR, mediumIOR, cos1, cos2);
+    T betaPP = pp_t(extIOR, tfIOR, cosThetaI, cos1) * pp_t(tfIOR, mediumIOR, cos1, cos2);
+    T delta = (tfIOR < extIOR ? T(PI) : T(0)) + (tfIOR < mediumIOR ? T(PI) : T(0));
+    T phi = (T(2) * T(PI) / lambda) * (T(2) * tfIOR * thickness * cos1) + delta;
+    T ts = (betaSP * betaSP) / ((alphaSP * alphaSP) - (T(2) * alphaSP * std::cos(phi)) + T(1));
+    T tp = (betaPP * betaPP) / ((alphaPP * alphaPP) - (T(2) * alphaPP * std::cos(phi)) + T(1));
+    T beamRatio = (mediumIOR * cos2) / (extIOR * cosThetaI);
+    return T(1) - (ts + tp) * T(0.5) * beamRatio;
+}
+
+// EstevezSheenDistribution (estevez.cpp:8-42)
+template <class T> T estevez_D(T r, const V3<T> &wh) {
+    T invR = T(1) / r;
+    return (T(2) + invR) * std::pow(sint(wh), invR) / (T(2) * T(PI));
+}
+template <class T> T estevez_lambda(T r, const V3<T> &w) {
+    auto pl = [r](T p0, T p1) { return ((1 - r) * (1 - r)) * p0 + (1 - ((1 - r) * (1 - r))) * p1; };
+    T a = pl(T(25.3245f), T(21.5473f)), b = pl(T(3.32435f), T(3.82987f)), c = pl(T(0.16801f), T(0.19823f)),
+      d = pl(T(-1.27393f), T(-1.97760f)), e = pl(T(-4.85967f), T(-4.32054f));
+    auto L = [=](T v) { return a / (T(1) + b * std::pow(v, c)) + (d * v) + e; };
+    T ac = std::abs(w.z);
+    if (ac < T(0.5)) return std::exp(L(ac));
+    return std::exp(T(2) * L(T(0.5)) - L(T(1) - ac));
+}
+
+// Clearcoat helpers (clearcoat.cpp:7-20)
+template <class T> T gtr1(T cosTheta, T alpha) {
+    T alpha2 = alpha * alpha;
+    return (alpha2 - 1) / (T(PI) * std::log(alpha2) * (1 + (alpha2 - 1) * cosTheta * cosTheta));
+}
+template <class T> T smith_g_ggx(T cosTheta, T alpha) {
+    T alpha2 = alpha * alpha, cosTheta2 = cosTheta * cosTheta;
+    return T(1) / (cosTheta + std::sqrt(alpha2 + cosTheta2 - alpha2 * cosTheta2));
+}
+
+// One lobe: the single-lobe materials (FL_MAT_*) or a StandardSurface lobe (OL_*).
 template <class T, int N>
 struct Bxdf {
-    int kind = 0;               // FL_MAT_*
+    int kind = 0;
     GGXd<T> ggx;
     T weight = 1;
     Spec<T, N> R, Tt, eta, k;
     T ior = 1;
     bool vndf = false;
+    // StandardSurface lobes
+    T on_A = 1, on_B = 0;             // OrenNayar / DiffuseTransmission (R, weight)
+    Spec<T, N> kd, ks;                // FresnelBlended
+    T kdw = 0, fb_A = 1, fb_B = 0, ksw = 0, ro = 0;
+    bool metallic = false;            // SchlickMetallic(ks) instead of SchlickDielectric(ro)
+    T tf_ior = 1, tf_th = 0;          // thin film (medium: ior)
+    Spec<T, N> lam;                   // wavelength per lane / bin (thin film phase)
+    T sheen_r = 1;                    // Estevez roughness (R, weight)
+    T gloss = 0.1;                    // clearcoat (R, weight, ior)
     uint32_t type() const {
         switch (kind) {
-        case FL_MAT_LAMBERT: return BSDF_REFLECTION | BSDF_DIFFUSE;
-        case FL_MAT_CONDUCTOR: return BSDF_REFLECTION | BSDF_GLOSSY;
+        case FL_MAT_LAMBERT: case OL_OREN: return BSDF_REFLECTION | BSDF_DIFFUSE;
+        case OL_DIFFTRANS: return BSDF_TRANSMISSION | BSDF_DIFFUSE;
+        case FL_MAT_CONDUCTOR: case OL_FBLEND: case OL_THINFILM: case OL_SHEEN: case OL_COAT:
+            return BSDF_REFLECTION | BSDF_GLOSSY;
         case FL_MAT_DIELECTRIC: return BSDF_REFLECTION | BSDF_TRANSMISSION | BSDF_SPECULAR;
         }
         return 0;
     }
     bool matches(uint32_t flags) const { return (type() & flags) == type(); }
+    T oren_shape(T A, T B, const V3<T> &wo, const V3<T> &wi) const {   // orennayar.cpp:15-41 (without R, weight)
+        T sinThetaI = sint(wi), sinThetaO = sint(wo);
+        T maxCos = 0;
+        if (sinThetaI > T(1e-4) && sinThetaO > T(1e-4)) {
+            T dCos = cosphi(wi) * cosphi(wo) + sinphi(wi) * sinphi(wo);
+            maxCos = std::max(T(0), dCos);
+        }
+        T sinAlpha, tanBeta;
+        if (std::abs(wi.z) > std::abs(wo.z)) { sinAlpha = sinThetaO; tanBeta = sinThetaI / std::abs(wi.z); }
+        else { sinAlpha = sinThetaI; tanBeta = sinThetaO / std::abs(wo.z); }
+        return A + B * maxCos * sinAlpha * tanBeta;
+    }
+    Spec<T, N> microfacet_f(const V3<T> &wo, const V3<T> &wi) const {   // microfacetreflection.cpp:13-32
+        T cO = std::abs(wo.z), cI = std::abs(wi.z);
+        V3<T> wh = wi + wo;
+        if (cI == 0 || cO == 0) return Spec<T, N>(0);
+        if (wh.x == 0 && wh.y == 0 && wh.z == 0) return Spec<T, N>(0);
+        wh = normalize(wh);
+        T cosF = dot(wi, face_forward(wh, V3<T>(0, 0, 1)));
+        if (kind == FL_MAT_CONDUCTOR) {
+            Spec<T, N> F = fresnel_conductor<T, N>(cosF, eta, k);
+            return Spec<T, N>(1) * weight * ggx.D(wh) * ggx.G(wo, wi) * F / (T(4) * cO * cI);
+        }
+        if (kind == OL_THINFILM) {
+            Spec<T, N> F;
+            for (int i = 0; i < N; ++i) F.c[i] = thin_film<T>(cosF, tf_ior, tf_th, ior, lam.c[i]);
+            return Spec<T, N>(1) * weight * ggx.D(wh) * ggx.G(wo, wi) * F / (T(4) * cO * cI);
+        }
+        // sheen: FresnelNull
+        T G = T(1) / (T(1) + estevez_lambda(sheen_r, wo) + estevez_lambda(sheen_r, wi));
+        return R * weight * estevez_D(sheen_r, wh) * G * Spec<T, N>(1) / (T(4) * cO * cI);
+    }
     Spec<T, N> f(const V3<T> &wo, const V3<T> &wi) const {
         if (kind == FL_MAT_LAMBERT) return R * T(INV_PI) * T(1);             // lambert.cpp:6-8
-        if (kind == FL_MAT_CONDUCTOR) {                                      // microfacetreflection.cpp:13-32
-            T cO = std::abs(wo.z), cI = std::abs(wi.z);
+        if (kind == FL_MAT_CONDUCTOR || kind == OL_THINFILM || kind == OL_SHEEN) return microfacet_f(wo, wi);
+        if (kind == OL_OREN || kind == OL_DIFFTRANS) return R * T(INV_PI) * oren_shape(on_A, on_B, wo, wi) * weight;
+        if (kind == OL_FBLEND) {                                             // fresnelblendeddiffspec.cpp:22-43
+            T r1 = T(1) - T(.5) * std::abs(wi.z);
+            T r1pow5 = (r1 * r1) * (r1 * r1) * r1;
+            T r2 = T(1) - T(.5) * std::abs(wo.z);
+            T r2pow5 = (r2 * r2) * (r2 * r2) * r2;
+            Spec<T, N> diffuse = (T(28) / T(23)) * (kd * T(INV_PI) * oren_shape(fb_A, fb_B, wo, wi) * kdw) * (T(1) - ro) *
+                                 (T(1) - r1pow5) * (T(1) - r2pow5);
             V3<T> wh = wi + wo;
-            if (cI == 0 || cO == 0) return Spec<T, N>(0);
             if (wh.x == 0 && wh.y == 0 && wh.z == 0) return Spec<T, N>(0);
             wh = normalize(wh);
-            Spec<T, N> F = fresnel_conductor<T, N>(dot(wi, face_forward(wh, V3<T>(0, 0, 1))), eta, k);
-            return Spec<T, N>(1) * weight * ggx.D(wh) * ggx.G(wo, wi) * F / (T(4) * cO * cI);
+            T cwh = dot(wi, wh);
+            T R1 = T(1) - cwh;
+            Spec<T, N> F;
+            if (metallic) F = ks + (Spec<T, N>(1) - ks) * (R1 * R1 * R1 * R1 * R1);     // schlickmetallic.cpp:7-10
+            else F = Spec<T, N>(ro + ((R1 * R1) * (R1 * R1) * R1) * (T(1) - ro));       // schlickdielectric.cpp:7-11
+            Spec<T, N> specular = ggx.D(wh) * F / (T(4) * std::abs(cwh) * std::max(std::abs(wi.z), std::abs(wo.z)));
+            specular = ks * specular * ksw;
+            return diffuse + specular;
+        }
+        if (kind == OL_COAT) {                                               // clearcoat.cpp:28-42
+            V3<T> wh = wi + wo;
+            if (wh.x == 0 && wh.y == 0 && wh.z == 0) return Spec<T, N>(0);
+            wh = normalize(wh);
+            T Dr = gtr1(std::abs(wh.z), gloss);
+            T Fr = fresnel_dielectric<T>(dot(wo, wh), T(1), ior);
+            T Gr = smith_g_ggx(std::abs(wo.z), T(.25)) * smith_g_ggx(std::abs(wi.z), T(.25));
+            return R * Fr * weight * Gr * Dr / T(4);
         }
         return Spec<T, N>(0);                                               // fresnelspecular.cpp:13-15
     }
     T pdf(const V3<T> &wo, const V3<T> &wi) const {
-        if (kind == FL_MAT_LAMBERT) return same_hemisphere(wo, wi) ? std::abs(wi.z) * T(INV_PI) : T(0);
-        if (kind == FL_MAT_CONDUCTOR) {                                      // microfacetreflection.cpp:55-61
-            if (!same_hemisphere(wo, wi)) return 0.f;
+        switch (kind) {
+        case FL_MAT_LAMBERT: case OL_OREN: return same_hemisphere(wo, wi) ? std::abs(wi.z) * T(INV_PI) : T(0);
+        case OL_DIFFTRANS: return !same_hemisphere(wo, wi) ? std::abs(wi.z) * T(INV_PI) : T(0);
+        case FL_MAT_CONDUCTOR: case OL_THINFILM: {                           // microfacetreflection.cpp:55-61
+            if (!same_hemisphere(wo, wi)) return T(0);
             V3<T> wh = normalize(wo + wi);
             return ggx.pdf(wo, wh, vndf) / (T(4) * dot(wo, wh));
         }
-        return 0.f;
+        case OL_SHEEN: {
+            if (!same_hemisphere(wo, wi)) return T(0);
+            V3<T> wh = normalize(wo + wi);
+            return estevez_D(sheen_r, wh) * std::abs(wh.z) / (T(4) * dot(wo, wh));
+        }
+        case OL_FBLEND: {                                                    // fresnelblendeddiffspec.cpp:62-67
+            if (!same_hemisphere(wo, wi)) return T(0);
+            V3<T> wh = normalize(wo + wi);
+            T pdf_wh = ggx.pdf(wo, wh, vndf);
+            return T(.5) * (std::abs(wi.z) * T(INV_PI) + pdf_wh / (T(4) * dot(wo, wh)));
+        }
+        case OL_COAT: {                                                      // clearcoat.cpp:66-79
+            if (!same_hemisphere(wo, wi)) return T(0);
+            V3<T> wh = wi + wo;
+            if (wh.x == 0 && wh.y == 0 && wh.z == 0) return T(0);
+            wh = normalize(wh);
+            return gtr1(std::abs(wh.z), gloss) * std::abs(wh.z) / (T(4) * dot(wo, wh));
+        }
+        }
+        return T(0);
     }
     Spec<T, N> sample(const V3<T> &wo, V3<T> *wi, T u0, T u1, T *pdf_, uint32_t *st) const {
-        if (kind == FL_MAT_LAMBERT) {                                        // lambert.cpp:10-20
+        if (kind == FL_MAT_LAMBERT || kind == OL_OREN || kind == OL_DIFFTRANS) {   // lambert.cpp:10-20, orennayar.cpp:43-55
             *wi = cosine_hemisphere_sample(u0, u1);
-            if (wo.z < 0) wi->z *= -1.f;
+            if (kind == OL_DIFFTRANS ? wo.z > 0 : wo.z < 0) wi->z *= -1.f;
             *pdf_ = pdf(wo, *wi);
             *st = type();
             return f(wo, *wi);
         }
-        if (kind == FL_MAT_CONDUCTOR) {                                      // microfacetreflection.cpp:34-53
+        if (kind == FL_MAT_CONDUCTOR || kind == OL_THINFILM || kind == OL_SHEEN) {   // microfacetreflection.cpp:34-53
             if (wo.z == 0) return Spec<T, N>(0);
-            V3<T> wh = ggx.sample_wh(wo, u0, u1);
+            V3<T> wh;
+            if (kind == OL_SHEEN) {                                          // estevez.cpp:35-41
+                wh = cosine_hemisphere_sample(u0, u1);
+                if (wo.z < 0) wh.z *= T(-1);
+            } else {
+                wh = ggx.sample_wh(wo, u0, u1);
+            }
             if (dot(wo, wh) < 0) return Spec<T, N>(0);
             *wi = reflect(wo, wh);
             if (!same_hemisphere(wo, *wi)) return Spec<T, N>(0);
-            *pdf_ = ggx.pdf(wo, wh, vndf) / (T(4) * dot(wo, wh));
+            *pdf_ = (kind == OL_SHEEN ? estevez_D(sheen_r, wh) * std::abs(wh.z) : ggx.pdf(wo, wh, vndf)) / (T(4) * dot(wo, wh));
+            return f(wo, *wi);
+        }
+        if (kind == OL_FBLEND) {                                             // fresnelblendeddiffspec.cpp:45-60
+            if (u0 < T(.5)) {
+                T ua = std::min(T(2) * u0, T(1.f - SHADOW_EPSILON));
+                *wi = cosine_hemisphere_sample(ua, u1);
+                if (wo.z < 0) wi->z *= T(-1);
+            } else {
+                T ua = std::min(T(2) * (u0 - T(.5)), T(1.f - SHADOW_EPSILON));
+                V3<T> wh = ggx.sample_wh(wo, ua, u1);
+                *wi = reflect(wo, wh);
+                if (!same_hemisphere(wo, *wi)) return Spec<T, N>(0);
+            }
+            *pdf_ = pdf(wo, *wi);
+            return f(wo, *wi);
+        }
+        if (kind == OL_COAT) {                                               // clearcoat.cpp:44-64
+            if (wo.z == 0) return Spec<T, N>(0);
+            T alpha2 = gloss * gloss;
+            T cosTheta = std::sqrt(std::max(T(0), (T(1) - std::pow(alpha2, T(1) - u0)) / (T(1) - alpha2)));
+            T sinTheta = std::sqrt(std::max(T(0), T(1) - cosTheta * cosTheta));
+            T phi = T(2) * T(PI) * u1;
+            V3<T> wh(sinTheta * std::cos(phi), sinTheta * std::sin(phi), cosTheta);
+            if (!same_hemisphere(wo, wh)) wh = -wh;
+            *wi = reflect(wo, wh);
+            if (!same_hemisphere(wo, *wi)) return Spec<T, N>(0);
+            *pdf_ = pdf(wo, *wi);
             return f(wo, *wi);
         }
         if (kind == FL_MAT_DIELECTRIC) {                                     // fresnelspecular.cpp:21-48
@@ -415,13 +593,15 @@ struct Bxdf {
     }
 };
 
-// BSDF (bsdf.cpp) over one lobe with frame (ng, ns, s, t)
+// BSDF (bsdf.cpp) over up to 8 lobes with frame (ng, ns, s, t)
 template <class T, int N>
 struct Bsdf {
-    Bxdf<T, N> lobe;
+    Bxdf<T, N> lobes[8];
+    int n = 0;
     bool has = false;
     T eta = 1;
     V3<T> ng, ns, s, t;
+    Bxdf<T, N> &add() { has = true; return lobes[n++]; }
     void frame_identity() {                             // local frame == world frame (BxDF queries)
         ng = ns = V3<T>(0, 0, 1);
         s = V3<T>(1, 0, 0);
@@ -434,44 +614,60 @@ struct Bsdf {
     }
     V3<T> to_local(const V3<T> &v) const { return V3<T>(dot(v, s), dot(v, t), dot(v, ns)); }
     V3<T> to_world(const V3<T> &v) const { return (s * v.x + t * v.y + ns * v.z); }
-    int num_components(uint32_t flags) const { return (has && lobe.matches(flags)) ? 1 : 0; }
+    int num_components(uint32_t flags) const {          // bsdf.cpp:12-20
+        int m = 0;
+        for (int i = 0; i < n; ++i) m += lobes[i].matches(flags) ? 1 : 0;
+        return m;
+    }
     Spec<T, N> evaluate(const V3<T> &woW, const V3<T> &wiW, uint32_t flags) const {   // bsdf.cpp:30-50
         V3<T> wi = to_local(wiW), wo = to_local(woW);
         if (wo.z == 0) return Spec<T, N>(0);
         bool refl = dot(wiW, ng) * dot(woW, ng) > 0;
         Spec<T, N> total(0);
-        if (has && lobe.matches(flags) && ((refl && (lobe.type() & BSDF_REFLECTION)) ||
-                                           (!refl && (lobe.type() & BSDF_TRANSMISSION))))
-            total += lobe.f(wo, wi);
+        for (int i = 0; i < n; ++i)
+            if (lobes[i].matches(flags) && ((refl && (lobes[i].type() & BSDF_REFLECTION)) ||
+                                            (!refl && (lobes[i].type() & BSDF_TRANSMISSION))))
+                total += lobes[i].f(wo, wi);
         return total;
     }
     Spec<T, N> sample(const V3<T> &woW, V3<T> *wiW, T u0, T u1, T *pdf, uint32_t flags, uint32_t *st) const {
         int match = num_components(flags);                                   // bsdf.cpp:52-125
         if (match == 0) { *pdf = 0; *st = 0; return Spec<T, N>(0); }
         int comp = std::min((int)std::floor(u0 * match), match - 1);
+        const Bxdf<T, N> *bx = nullptr;
+        int count = comp;
+        for (int i = 0; i < n; ++i)
+            if (lobes[i].matches(flags) && count-- == 0) { bx = &lobes[i]; break; }
         T ur = std::min(u0 * match - comp, T(1.f - SHADOW_EPSILON));   // float constant, bsdf.cpp:75
         V3<T> wi, wo = to_local(woW);
         if (wo.z == 0) { *pdf = 0; *st = 0; return Spec<T, N>(0); }
         *pdf = 0;
-        *st = lobe.type();
-        Spec<T, N> f = lobe.sample(wo, &wi, ur, u1, pdf, st);
+        *st = bx->type();
+        Spec<T, N> f = bx->sample(wo, &wi, ur, u1, pdf, st);
         if (*pdf == 0) { *st = 0; return Spec<T, N>(0); }
         *wiW = to_world(wi);
-        if (!(lobe.type() & BSDF_SPECULAR)) {
+        if (!(bx->type() & BSDF_SPECULAR) && match > 1)
+            for (int i = 0; i < n; ++i)
+                if (&lobes[i] != bx && lobes[i].matches(flags)) *pdf += lobes[i].pdf(wo, wi);
+        if (match > 1) *pdf /= match;
+        if (!(bx->type() & BSDF_SPECULAR)) {
             bool refl = dot(*wiW, ng) * dot(woW, ng) > 0;
             f = Spec<T, N>(0);
-            if ((refl && (lobe.type() & BSDF_REFLECTION)) || (!refl && (lobe.type() & BSDF_TRANSMISSION)))
-                f += lobe.f(wo, wi);
+            for (int i = 0; i < n; ++i)
+                if (lobes[i].matches(flags) && ((refl && (lobes[i].type() & BSDF_REFLECTION)) ||
+                                                (!refl && (lobes[i].type() & BSDF_TRANSMISSION))))
+                    f += lobes[i].f(wo, wi);
         }
         return f;
     }
     T pdf(const V3<T> &woW, const V3<T> &wiW, uint32_t flags) const {       // bsdf.cpp:127-151
-        if (!has) return 0.f;
+        if (n == 0) return 0.f;
         V3<T> wi = to_local(wiW), wo = to_local(woW);
         if (wo.z == 0) return 0.f;
         T p = 0;
         int m = 0;
-        if (lobe.matches(flags)) { ++m; p += lobe.pdf(wo, wi); }
+        for (int i = 0; i < n; ++i)
+            if (lobes[i].matches(flags)) { ++m; p += lobes[i].pdf(wo, wi); }
         return m > 0 ? p / m : T(0);
     }
 };
@@ -569,6 +765,9 @@ struct Material {
     S60 R, T;
     S60 eta, k;
     float ax, ay, weight, ior;
+    // FL_MAT_STANDARD: parameters and the 60-bin spectra of its colours
+    fl_standard_surface ss;
+    S60 kd, ks, kt, sheen, coat;
 };
 
 } // namespace
@@ -858,22 +1057,94 @@ struct Ctx {
     uint64_t rays = 0;
 };
 
+// StandardSurfaceMaterial::compute_scattering_functions (standardsurface.cpp:72-159)
+// with constant textures: the lobes, in the order it adds them. Colours are
+// already spectra (from_reflectance_rgb, clamped); eta and k are the forge
+// metal's; lam is each bin's (or lane's) wavelength for the thin film.
+template <class T, int N>
+void add_standard_lobes(Bsdf<T, N> &b, const fl_standard_surface &ss, const Spec<T, N> &kd, const Spec<T, N> &ks,
+                        const Spec<T, N> &kt, const Spec<T, N> &sheen, const Spec<T, N> &coat, const Spec<T, N> &eta,
+                        const Spec<T, N> &k, const Spec<T, N> &lam, bool vndf) {
+    b.has = true;
+    b.eta = (ss.ks_weight > 0 && ss.kt_weight > 0) ? T(ss.ks_ior) : T(1);
+    const bool metal_wl = ss.fresnel_type == FL_FRESNEL_WAVELENGTH_METALLIC;
+    if ((ss.kd_weight > 0 && ss.ks_weight <= 0) || (ss.kd_weight > 0 && ss.ks_weight > 0 && metal_wl)) {
+        Bxdf<T, N> &l = b.add();
+        l.kind = ss.thin_walled ? OL_DIFFTRANS : OL_OREN;
+        l.R = kd;
+        l.weight = T(ss.kd_weight);
+        float A, B;
+        oren_nayar_AB<float>(ss.kd_roughness * 90.f, &A, &B);            // [0-1] roughness to [0-90] sigma
+        l.on_A = T(A); l.on_B = T(B);
+    }
+    float aspect = std::sqrt(1.f - ss.ks_anisotropic * 0.9f);
+    GGXd<T> g;
+    g.ax = T(std::max(.001f, (ss.ks_roughness * ss.ks_roughness) / aspect));
+    g.ay = T(std::max(.001f, (ss.ks_roughness * ss.ks_roughness) * aspect));
+    float ro = ((1.f - ss.ks_ior) / (1.f + ss.ks_ior)) * ((1.f - ss.ks_ior) / (1.f + ss.ks_ior));
+    if (ss.ks_weight > 0) {
+        if (ss.kt_weight > 0) {
+            Bxdf<T, N> &l = b.add();                                          // FresnelSpecularBXDF(ks, kt, 1, ksIOR)
+            l.kind = FL_MAT_DIELECTRIC;
+            l.R = ks; l.Tt = kt; l.ior = T(ss.ks_ior);
+        } else if (metal_wl) {
+            Bxdf<T, N> &l = b.add();                                          // MicrofacetReflection(GGX, FresnelConductor, 1, ksweight)
+            l.kind = FL_MAT_CONDUCTOR;
+            l.ggx = g; l.weight = T(ss.ks_weight); l.eta = eta; l.k = k; l.vndf = vndf;
+        } else if (ss.fresnel_type == FL_FRESNEL_DIELECTRIC || ss.fresnel_type == FL_FRESNEL_METALLIC) {
+            Bxdf<T, N> &l = b.add();                                          // FresnelBlendedDiffuseSpecularBRDF
+            l.kind = OL_FBLEND;
+            l.ggx = g; l.vndf = vndf;
+            l.kd = kd; l.kdw = T(ss.kd_weight);
+            float A, B;
+            oren_nayar_AB<float>(ss.kd_roughness, &A, &B);                   // kdroughness passed as sigma
+            l.fb_A = T(A); l.fb_B = T(B);
+            l.ks = ks; l.ksw = T(ss.ks_weight); l.ro = T(ro);
+            l.metallic = ss.fresnel_type == FL_FRESNEL_METALLIC;
+        }
+    }
+    if (ss.thin_film_thickness > 0) {
+        Bxdf<T, N> &l = b.add();                                              // MicrofacetReflection(GGX, ThinFilm, 1, 1)
+        l.kind = OL_THINFILM;
+        l.ggx = g; l.vndf = vndf; l.weight = T(1);
+        l.tf_ior = T(ss.thin_film_ior); l.tf_th = T(ss.thin_film_thickness); l.ior = T(ss.ks_ior); l.lam = lam;
+    }
+    if (ss.sheen_weight > 0) {
+        Bxdf<T, N> &l = b.add();                                              // MicrofacetReflection(Estevez, FresnelNull, sheen, w)
+        l.kind = OL_SHEEN;
+        l.R = sheen; l.weight = T(ss.sheen_weight);
+        l.sheen_r = T(std::max(.001f, ss.sheen_roughness));
+    }
+    if (ss.coat_weight > 0) {
+        Bxdf<T, N> &l = b.add();                                              // ClearcoatReflectionBRDF
+        l.kind = OL_COAT;
+        l.R = coat; l.weight = T(ss.coat_weight); l.ior = T(ss.coat_ior);
+        l.gloss = T((1.f - ss.coat_gloss) * .1f + ss.coat_gloss * .001f);    // lerp(gloss, .1, .001)
+    }
+}
+
 Bsdf<float, NS> make_bsdf(const or_scene_s *s, const Isect &it) {
     Bsdf<float, NS> b;
     uint32_t m = s->mat[it.prim];
     if (m == 0xffffu || s->mats[m].kind == FL_MAT_NONE) return b;
     const Material &M = s->mats[m];
-    b.has = true;
     b.frame(it.n, it.n);                 // ns = Shading.n = n (repair R3)
-    b.lobe.kind = M.kind;
-    b.lobe.ggx.ax = M.ax;
-    b.lobe.ggx.ay = M.ay;
-    b.lobe.weight = M.weight;
-    b.lobe.R = M.R;
-    b.lobe.Tt = M.T;
-    b.lobe.eta = M.eta;
-    b.lobe.k = M.k;
-    b.lobe.ior = M.ior;
+    if (M.kind == FL_MAT_STANDARD) {
+        S60 lam;
+        for (int i = 0; i < NS; ++i) lam.c[i] = (float)(400 + 5 * i);     // thinfilm.cpp:38
+        add_standard_lobes<float, NS>(b, M.ss, M.kd, M.ks, M.kt, M.sheen, M.coat, M.eta, M.k, lam, false);
+        return b;
+    }
+    Bxdf<float, NS> &l = b.add();
+    l.kind = M.kind;
+    l.ggx.ax = M.ax;
+    l.ggx.ay = M.ay;
+    l.weight = M.weight;
+    l.R = M.R;
+    l.Tt = M.T;
+    l.eta = M.eta;
+    l.k = M.k;
+    l.ior = M.ior;
     b.eta = (M.kind == FL_MAT_DIELECTRIC) ? M.ior : 1.f;
     return b;
 }
@@ -1128,17 +1399,17 @@ static void parallel_for(uint32_t n, int threads, F fn) {
 template <class T>
 static void bsdf_query(const fl_bsdf_query &Q, fl_bsdf_result &R) {
     Bsdf<T, 4> b;
-    b.has = true;
     b.frame_identity();
-    b.lobe.kind = Q.kind;
-    b.lobe.ggx.ax = Q.alpha_x;
-    b.lobe.ggx.ay = Q.alpha_y;
-    b.lobe.weight = 1;
-    b.lobe.vndf = (Q.flags & FL_RENDER_VNDF_PDF) != 0;
+    Bxdf<T, 4> &l = b.add();
+    l.kind = Q.kind;
+    l.ggx.ax = Q.alpha_x;
+    l.ggx.ay = Q.alpha_y;
+    l.weight = 1;
+    l.vndf = (Q.flags & FL_RENDER_VNDF_PDF) != 0;
     for (int k = 0; k < 4; ++k) {
-        b.lobe.R.c[k] = Q.r[k]; b.lobe.Tt.c[k] = Q.t[k]; b.lobe.eta.c[k] = Q.eta[k]; b.lobe.k.c[k] = Q.k[k];
+        l.R.c[k] = Q.r[k]; l.Tt.c[k] = Q.t[k]; l.eta.c[k] = Q.eta[k]; l.k.c[k] = Q.k[k];
     }
-    b.lobe.ior = Q.eta[0];
+    l.ior = Q.eta[0];
     V3<T> wo(Q.wo[0], Q.wo[1], Q.wo[2]), wi(Q.wi[0], Q.wi[1], Q.wi[2]);
     const uint32_t ns = BSDF_ALL & ~BSDF_SPECULAR;
     Spec<T, 4> fe = b.evaluate(wo, wi, ns);
@@ -1154,9 +1425,45 @@ static void bsdf_query(const fl_bsdf_query &Q, fl_bsdf_result &R) {
     R.sampled_type = st;
 }
 
+template <class T>
+static void bsdf_query_standard(const fl_standard_surface &m, const fl_standard_query &Q, fl_bsdf_result &R) {
+    Bsdf<T, 4> b;
+    b.frame_identity();
+    Spec<T, 4> kd, ks, kt, sh, co, eta, kk, lam;
+    for (int k = 0; k < 4; ++k) {
+        kd.c[k] = Q.kd[k]; ks.c[k] = Q.ks[k]; kt.c[k] = Q.kt[k]; sh.c[k] = Q.sheen[k]; co.c[k] = Q.coat[k];
+        eta.c[k] = Q.eta[k]; kk.c[k] = Q.k[k]; lam.c[k] = Q.lam[k];
+    }
+    add_standard_lobes<T, 4>(b, m, kd, ks, kt, sh, co, eta, kk, lam, (Q.flags & FL_RENDER_VNDF_PDF) != 0);
+    V3<T> wo(Q.wo[0], Q.wo[1], Q.wo[2]), wi(Q.wi[0], Q.wi[1], Q.wi[2]);
+    const uint32_t ns = BSDF_ALL & ~BSDF_SPECULAR;
+    Spec<T, 4> fe = b.evaluate(wo, wi, ns);
+    for (int k = 0; k < 4; ++k) R.f_eval[k] = (float)fe.c[k];
+    R.pdf_eval = (float)b.pdf(wo, wi, ns);
+    V3<T> ws(0, 0, 0);
+    T pdf = 0;
+    uint32_t st = 0;
+    Spec<T, 4> fs = b.sample(wo, &ws, (T)Q.u[0], (T)Q.u[1], &pdf, BSDF_ALL, &st);
+    for (int k = 0; k < 4; ++k) R.f_sample[k] = (float)fs.c[k];
+    R.pdf_sample = (float)pdf;
+    R.wi_sample[0] = (float)ws.x; R.wi_sample[1] = (float)ws.y; R.wi_sample[2] = (float)ws.z;
+    R.sampled_type = st;
+}
+
 } // namespace
 
 extern "C" {
+
+int or_bsdf_standard(const fl_standard_surface *m, const fl_standard_query *q, fl_bsdf_result *r, uint32_t n,
+                     int precision) {
+    if (!m) return FL_E_INVALID;
+    for (uint32_t i = 0; i < n; ++i) {
+        memset(&r[i], 0, sizeof(r[i]));
+        if (precision == 64) bsdf_query_standard<double>(*m, q[i], r[i]);
+        else bsdf_query_standard<float>(*m, q[i], r[i]);
+    }
+    return FL_OK;
+}
 
 int or_spectral_bins(void) { return NS; }
 
@@ -1234,6 +1541,17 @@ int or_scene_create(const fl_scene_desc *d, or_scene_t *out) {
             M.R = from_reflectance_rgb(fm.rgb).clamp0();
             M.T = from_reflectance_rgb(fm.rgb_t).clamp0();
             M.ior = fm.ior;
+        } else if (fm.kind == FL_MAT_STANDARD) {
+            if (fm.standard >= d->n_standard_surfaces) { delete s; return FL_E_INVALID; }
+            M.ss = d->standard_surfaces[fm.standard];
+            M.kd = from_reflectance_rgb(M.ss.kd).clamp0();                  // Texture::evaluate().spectrum_clamp()
+            M.ks = from_reflectance_rgb(M.ss.ks).clamp0();
+            M.kt = from_reflectance_rgb(M.ss.kt).clamp0();
+            M.sheen = from_reflectance_rgb(M.ss.sheen_color).clamp0();
+            M.coat = from_reflectance_rgb(M.ss.coat).clamp0();
+            int mt = (M.ss.metal >= 0 && M.ss.metal < FL_N_METALS) ? M.ss.metal : 0;
+            M.eta = T.eta[mt];
+            M.k = T.k[mt];
         }
     }
     float lam[FL_SPD_N];

@@ -47,6 +47,9 @@ int or_intersect2(or_scene_t s, const fl_ray *rays, float *t2, uint32_t *id2, ui
 
 /* BxDF queries: precision 32 or 64 */
 int or_bsdf_batch(const fl_bsdf_query *q, fl_bsdf_result *r, uint32_t n, int precision);
+/* the StandardSurface lobe stack of one material (as fl_bsdf_query_standard) */
+int or_bsdf_standard(const fl_standard_surface *m, const fl_standard_query *q, fl_bsdf_result *r, uint32_t n,
+                     int precision);
 
 /* Render with the restated PathIntegrator (60-bin spectra, fp32).
  * film_sum/film_sq: W*H*3 floats (CIE XYZ per sample summed); pixels

@@ -54,6 +54,8 @@ def lib():
         L.or_bsdf_batch.argtypes = [C.POINTER(flgpu.BsdfQuery), C.POINTER(flgpu.BsdfResult), C.c_uint32, C.c_int]
         L.or_render.argtypes = [vp, C.POINTER(flgpu.Camera), C.POINTER(RenderParams), fp, fp, C.POINTER(RenderStats)]
         L.or_develop.argtypes = [fp, C.c_uint32, C.c_int, C.c_int, C.c_int, fp, C.POINTER(C.c_uint8)]
+        L.or_bsdf_standard.argtypes = [C.POINTER(flgpu.StandardSurface), C.POINTER(flgpu.StandardQuery),
+                                       C.POINTER(flgpu.BsdfResult), C.c_uint32, C.c_int]
         dp = C.POINTER(C.c_double)
         L.or_bf16_round.argtypes = [C.c_double]
         L.or_bf16_round.restype = C.c_double
@@ -127,6 +129,23 @@ def tri_intersect(ray, v0, v1, v2):
     o = np.zeros(3, np.float32)
     hit = lib().or_tri_intersect(_f(r), _f(a), _f(b), _f(c), _f(o))
     return bool(hit), o
+
+
+def bsdf_standard(material, queries, precision=64):
+    """or_bsdf_standard over a list of StandardQuery (returns a list of
+    BsdfResult) or a numpy array of dtype(StandardQuery) (returns an array of
+    dtype(BsdfResult))."""
+    if isinstance(queries, np.ndarray):
+        q = np.ascontiguousarray(queries, np.dtype(flgpu.StandardQuery))
+        r = np.zeros(q.shape[0], np.dtype(flgpu.BsdfResult))
+        lib().or_bsdf_standard(C.byref(material), q.ctypes.data_as(C.POINTER(flgpu.StandardQuery)),
+                               r.ctypes.data_as(C.POINTER(flgpu.BsdfResult)), q.shape[0], precision)
+        return r
+    n = len(queries)
+    q = (flgpu.StandardQuery * n)(*queries)
+    r = (flgpu.BsdfResult * n)()
+    lib().or_bsdf_standard(C.byref(material), q, r, n, precision)
+    return list(r)
 
 
 def bsdf_batch(queries, precision=64):

@@ -47,6 +47,29 @@ def test_cpp_layer_matches_c_abi_render(tmp_path):
     assert np.array_equal(got.view(np.uint32), ref.view(np.uint32))
 
 
+@pytest.mark.gpu
+def test_cpp_layer_standard_surface_matches_c_abi(tmp_path):
+    """Material::standard_surface in the C++ layer: the Cornell box with
+    StandardSurfaceMaterials built through it renders the same film, bit for
+    bit, as the same materials given to the C ABI as fl_standard_surface."""
+    from test_gpu_standard import standard_box
+
+    film = str(tmp_path / "film.bin")
+    rc, out = run_example(film, "48", "40", "16", "standard")
+    assert rc == 0 and out["ok"], out
+    got = np.fromfile(film, np.float32).reshape(40, 48, 3)
+    gen = standard_box(48, 40)
+    with flgpu.Context(0) as ctx, ctx.scene(gen) as sc:
+        p = flgpu.default_params(spp=16, depth=gen.depth, mode=flgpu.FL_SPECTRAL_HERO, seed=7,
+                                 flags=flgpu.FL_RENDER_GRAPH | flgpu.FL_RENDER_VNDF_PDF)
+        ref, _, _ = sc.render(p)
+    assert np.array_equal(got.view(np.uint32), ref.view(np.uint32))
+    plain = flgpu.GenSceneData(flgpu.FL_GEN_CORNELL, width=48, height=40)
+    with flgpu.Context(0) as ctx, ctx.scene(plain) as sc:
+        other, _, _ = sc.render(p)
+    assert not np.array_equal(other, ref)
+
+
 def test_rtc_example_built_and_reports_missing_gpu():
     assert os.access(RTC_EXE, os.X_OK), "build() did not produce the Embree-API example"
     if flgpu.lib().fl_device_count() > 0:
