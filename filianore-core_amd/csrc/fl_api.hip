@@ -39,6 +39,10 @@ void free_scene(fl_scene_s *s) {
     (void)hipFree(s->d_nodes);
     (void)hipFree(s->d_tris);
     (void)hipFree(s->d_std);
+    (void)hipFree(s->d_tex_info);
+    (void)hipFree(s->d_tex_data);
+    (void)hipFree(s->d_tri_uv);
+    (void)hipFree(s->d_texscene);
     (void)hipFree(s->d_mats);
     (void)hipFree(s->d_lights);
     for (auto &t : s->spec) (void)hipFree(t.d_tab);
@@ -164,6 +168,23 @@ int fl_scene_create(fl_ctx_t ctx, const fl_scene_desc *desc, fl_scene_t *out) {
             (ss.thin_film_thickness > 0.f && !(ss.thin_film_ior > 0.f)))
             return set_error(ctx, FL_E_INVALID, "fl_scene_create: bad standard surface");
     }
+    if (desc->n_textures && !desc->textures) return set_error(ctx, FL_E_INVALID, "fl_scene_create: missing textures");
+    uint64_t tex_bytes = 0;
+    for (uint32_t j = 0; j < desc->n_textures; ++j) {
+        const fl_texture &tx = desc->textures[j];
+        if (tx.rgb && tx.width && tx.height) tex_bytes += (3ull * tx.width * tx.height + 3) & ~3ull;
+    }
+    if (tex_bytes > 0xffffffffull) return set_error(ctx, FL_E_INVALID, "fl_scene_create: textures exceed 4 GiB");
+    for (uint32_t m = 0; m < desc->n_materials; ++m)
+        if (desc->materials[m].kind == FL_MAT_LAMBERT && desc->materials[m].texture > desc->n_textures)
+            return set_error(ctx, FL_E_INVALID, "fl_scene_create: texture index out of range");
+    for (uint32_t j = 0; j < desc->n_standard_surfaces; ++j) {
+        const fl_standard_surface &ss = desc->standard_surfaces[j];
+        const uint32_t t[5] = {ss.kd_texture, ss.ks_texture, ss.sheen_texture, ss.kt_texture, ss.coat_texture};
+        for (uint32_t k = 0; k < 5; ++k)
+            if (t[k] > desc->n_textures)
+                return set_error(ctx, FL_E_INVALID, "fl_scene_create: texture index out of range");
+    }
     for (uint32_t t = 0; t < 3ull * desc->n_triangles; ++t)
         if (desc->indices[t] >= desc->n_vertices) return set_error(ctx, FL_E_INVALID, "fl_scene_create: index out of range");
     if (desc->tri_material)
@@ -209,6 +230,13 @@ int fl_scene_create(fl_ctx_t ctx, const fl_scene_desc *desc, fl_scene_t *out) {
     s->h_std.resize(desc->n_standard_surfaces);
     for (uint32_t m = 0; m < desc->n_materials; ++m) s->has_standard = s->has_standard || desc->materials[m].kind == FL_MAT_STANDARD;
     for (uint32_t j = 0; j < desc->n_standard_surfaces; ++j) s->h_std[j] = flx::standard_params(desc->standard_surfaces[j]);
+    for (uint32_t m = 0; m < desc->n_materials; ++m) {
+        const fl_material &fm = desc->materials[m];
+        if (fm.kind == FL_MAT_LAMBERT && fm.texture) s->has_textures = true;
+        if (fm.kind == FL_MAT_STANDARD)
+            for (uint32_t k = 0; k < (uint32_t)fl::SS_ETA; ++k)
+                if (s->h_std[fm.standard].tex[k]) s->has_textures = true;
+    }
     flx::build_spectral_tables(s, desc, FL_SPECTRAL_PARITY, mats);
     flx::build_spectral_tables(s, desc, FL_SPECTRAL_HERO, mats);
     s->h_mats = mats;
@@ -265,6 +293,43 @@ int fl_scene_create(fl_ctx_t ctx, const fl_scene_desc *desc, fl_scene_t *out) {
     rc = flx::build_bvh(ctx, s, d_verts, d_idx, d_mat, d_light);
     (void)hipFree(d_verts); (void)hipFree(d_idx); (void)hipFree(d_mat); (void)hipFree(d_light);
     if (rc) { free_scene(s); return rc; }
+    if (s->has_textures) {
+        // texels of every texture, each padded to 4 bytes; per-triangle uvs
+        std::vector<fl::DevTexture> info(desc->n_textures);
+        std::vector<uint8_t> data((size_t)tex_bytes);
+        uint64_t off = 0;
+        for (uint32_t j = 0; j < desc->n_textures; ++j) {
+            const fl_texture &tx = desc->textures[j];
+            info[j] = fl::DevTexture{(uint32_t)off, 0u, 0u, 0u};
+            if (!(tx.rgb && tx.width && tx.height)) continue;   // black
+            const size_t nb = 3ull * tx.width * tx.height;
+            memcpy(data.data() + off, tx.rgb, nb);
+            info[j].width = tx.width;
+            info[j].height = tx.height;
+            off += (nb + 3) & ~(size_t)3;
+        }
+        std::vector<float2> tri_uv;
+        if (desc->uvs) {
+            tri_uv.resize(3ull * desc->n_triangles);
+            for (size_t t = 0; t < desc->n_triangles; ++t)
+                for (int k = 0; k < 3; ++k) {
+                    const size_t vtx = desc->indices[3 * t + k];
+                    tri_uv[3 * t + k] = make_float2(desc->uvs[2 * vtx], desc->uvs[2 * vtx + 1]);
+                }
+        }
+        if (data.empty()) data.resize(4);
+        if ((rc = upload(ctx, &s->d_tex_info, info.data(), info.size())) ||
+            (rc = upload(ctx, &s->d_tex_data, data.data(), data.size())) ||
+            (!tri_uv.empty() && (rc = upload(ctx, &s->d_tri_uv, tri_uv.data(), tri_uv.size())))) {
+            free_scene(s);
+            return rc;
+        }
+        fl::TexScene ts{s->d_tex_info, s->d_tex_data, s->d_tri_uv};
+        if ((rc = upload(ctx, &s->d_texscene, &ts, 1))) {
+            free_scene(s);
+            return rc;
+        }
+    }
     if ((rc = upload(ctx, &s->d_mats, s->h_mats.data(), s->h_mats.size())) ||
         (rc = upload(ctx, &s->d_lights, s->h_lights.data(), s->h_lights.size())) ||
         (rc = upload(ctx, &s->d_std, s->h_std.data(), s->h_std.size())) ||

@@ -9,8 +9,9 @@
 #include "fl_bvh.h"
 #include "fl_shade.h"
 #include "fl_standard.h"
+#include "fl_texture.h"
 
-struct DevMaterial {        // 32 bytes
+struct DevMaterial {        // 48 bytes
     int32_t kind;
     float ax, ay;
     float weight;
@@ -18,6 +19,8 @@ struct DevMaterial {        // 32 bytes
     uint32_t spec_a;        // table slot: Lambert R / conductor eta / dielectric R
     uint32_t spec_b;        // table slot: conductor k / dielectric T
     float cauchy_b;         // dielectric Cauchy B (um^2), 0 = scalar eta
+    uint32_t tex;           // Lambert: 1 + texture of R, 0 = the table's constant
+    uint32_t pad[3];
 };
 
 // One illuminant (80 bytes). Area lights (type 0) come first, in
@@ -68,11 +71,16 @@ struct fl_scene_s {
     uint32_t rgb_basis = 0;         // table slot of the first of the 7 RGB->spectrum basis curves
     bool shadow_colors = false;     // some light has a non-black shadowColor
     bool has_standard = false;      // some material is FL_MAT_STANDARD
+    bool has_textures = false;      // some material reads an image texture
     fl::Node8 *d_nodes = nullptr;
     fl::TriRec *d_tris = nullptr;
     DevMaterial *d_mats = nullptr;
     DevLight *d_lights = nullptr;
     fl::StdParams *d_std = nullptr;   // StandardSurface lobe stacks (FL_MAT_STANDARD materials)
+    fl::DevTexture *d_tex_info = nullptr;
+    uint8_t *d_tex_data = nullptr;
+    float2 *d_tri_uv = nullptr;       // 3 per triangle when the scene has uvs
+    fl::TexScene *d_texscene = nullptr;   // the three above, for the kernels (null: no textures)
     SpecTables spec[2];     // [FL_SPECTRAL_HERO, FL_SPECTRAL_PARITY]
     double build_ms = 0.0;
     float bmin[3], bmax[3];

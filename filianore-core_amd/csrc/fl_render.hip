@@ -412,7 +412,12 @@ struct TabSpec {
     int gs;
     Lanes ln;
     bool parity;
-    __device__ float at(int which, int k) const { return spec_at(spec + (size_t)slot[which] * gs, ln, k); }
+    const uint32_t *tex;     // the material's texture per spectrum (StdParams::tex); null: none
+    const float *tv;         // this thread's textured lane values in LDS: [which * 4 + k] at stride kBlock
+    __device__ float at(int which, int k) const {
+        if (tex && which < SS_ETA && tex[which]) return tv[(which * 4 + k) * kBlock];
+        return spec_at(spec + (size_t)slot[which] * gs, ln, k);
+    }
     // ThinFilmInteference samples 60 wavelengths 400, 405, ... (thinfilm.cpp:38): in
     // parity mode a lane sees its bin's; otherwise its own wavelength
     __device__ float tf_lambda(int k) const { return parity ? 400.f + 5.f * (float)ln.idx[k] : ln.lam[k]; }
@@ -539,40 +544,9 @@ __device__ __forceinline__ bool nee_sample(const RenderArgs &a, const SceneRef &
 
 // NormalIntegrator::Li (normalintegrator.cpp:11-26): from_reflectance_rgb of
 // the hit's normal (spectrumoperations.cpp:113-147) at the path's four
-// wavelengths. basis: the 7 basis curves (white, cyan, magenta, yellow, red,
-// green, blue), GS floats apart. The combination is formed, scaled by .94
-// and clamped at the two table entries around each wavelength, as the host
-// forms a material's spectrum on the table grid, then interpolated.
+// wavelengths. basis: the 7 basis curves, GS floats apart.
 __device__ __forceinline__ void normal_spectrum(const float *basis, int GS, v3 n, const Lanes &ln, float L[4]) {
-    const float r = n.x, g = n.y, bl = n.z;
-    int b1, b2;
-    float c0, c1, c2;
-    if (r <= g && r <= bl) {
-        c0 = r;
-        if (g <= bl) { b1 = 1; c1 = g - r; b2 = 6; c2 = bl - g; }
-        else { b1 = 1; c1 = bl - r; b2 = 5; c2 = g - bl; }
-    } else if (g <= r && g <= bl) {
-        c0 = g;
-        if (r <= bl) { b1 = 2; c1 = r - g; b2 = 6; c2 = bl - r; }
-        else { b1 = 2; c1 = bl - g; b2 = 4; c2 = r - bl; }
-    } else {
-        c0 = bl;
-        if (r <= g) { b1 = 3; c1 = r - bl; b2 = 5; c2 = g - r; }
-        else { b1 = 3; c1 = g - bl; b2 = 4; c2 = r - g; }
-    }
-    const float *w = basis, *s1 = basis + (size_t)b1 * GS, *s2 = basis + (size_t)b2 * GS;
-    for (int k = 0; k < 4; ++k) {
-        float v[2];
-        for (int e = 0; e < 2; ++e) {
-            const int i = ln.idx[k] + e;
-            float x = 0.f + w[i] * c0;
-            x = x + s1[i] * c1;
-            x = x + s2[i] * c2;
-            x = x * 0.94f;
-            v[e] = x < 0.f ? 0.f : x;   // spectrum_clamp
-        }
-        L[k] = v[0] + ln.frac[k] * (v[1] - v[0]);
-    }
+    for (int k = 0; k < 4; ++k) L[k] = rgb_spectrum_lane(basis, GS, n.x, n.y, n.z, ln, k);
 }
 
 // WAVES: minimum waves per SIMD requested from the register allocator (2 lets
@@ -593,13 +567,17 @@ __device__ __forceinline__ uint32_t shade_key(const RenderArgs &a, int cur, uint
     return (uint32_t)S.mats[mat].kind;   // MAT_LAMBERT 1, MAT_CONDUCTOR 2, MAT_DIELECTRIC 3
 }
 
-// STD: the render has standard-surface materials. Their lobe stack is only
-// compiled into that variant, so renders without them keep the smaller
-// kernel (registers and instruction cache).
+// STD: the render has standard-surface materials or image textures. Their
+// code is only compiled into that variant, so renders without them keep the
+// smaller kernel (registers and instruction cache).
 template <int WAVES, bool STD>
 __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WAVES, 8))) k_shade(RenderArgs a, int b) {
     __shared__ uint32_t s_cnt[kSortKeys];
     __shared__ uint16_t s_perm[kBlock];
+    // a textured standard material's kd, ks, kt, sheen and coat at the vertex's
+    // four wavelengths, one column per thread (20 KB: registers would hold
+    // them across the whole shading of every vertex)
+    __shared__ float s_tv[STD ? SS_ETA * 4 * kBlock : 1];
     const uint32_t stride = gridDim.x * kBlock;
     const int cur = b & 1, nxt = cur ^ 1;
     const uint32_t n = a.counts[C_EXT + cur];
@@ -631,6 +609,7 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WAV
         float beta[4] = {0.f, 0.f, 0.f, 0.f}, L[4] = {0.f, 0.f, 0.f, 0.f};
         float lam0 = 0.f, eta_scale = 1.f, prev_pdf = 0.f, eta_bsdf = 1.f;
         v3 p, ng, d, no, nd, perr;
+        float2 uv = make_float2(0.f, 0.f);   // texture coordinates of the hit (textured scenes)
         Lanes ln;
         if (active) {
             float4 ro = a.rq_o[cur][i], rd = a.rq_d[cur][i];
@@ -666,6 +645,7 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WAV
                 const uint32_t mat = q1.w, light = q2.w;
                 p = v3(ro.x, ro.y, ro.z) + d * t;                  // ray.point_at_t(t)
                 ng = normalize(cross(p1 - p0, p2 - p0));
+                if (STD && S.tex) uv = hit_uv(*S.tex, q0.w, hh.z, hh.w);
                 ln = make_lanes(a.grid, lam0);
                 // emission: L += beta * le(-dir) at bounce 0 / after specular; MIS otherwise
                 if (a.flags & FL_RENDER_NORMALS) {
@@ -721,6 +701,25 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WAV
             const DevMaterial m = S.mats[mkind];
             setup_bsdf(a, S.spec, m, ln, bs_);
             if (STD && m.kind == MAT_STANDARD) sd = S.stds + m.spec_a;
+            if (STD && S.tex) {
+                // ImageMapTexture::evaluate for each textured spectrum of the
+                // material (lambert.cpp:12-13, standardsurface.cpp:84-85,115,146,155)
+                const float *basis = S.spec + (size_t)S.rgb_basis * GS;
+                if (m.tex) {
+                    float rgb[3];
+                    texel_rgb(*S.tex, m.tex - 1u, uv, rgb);
+                    for (int k = 0; k < 4; ++k) bs_.s1[k] = rgb_spectrum_lane(basis, GS, rgb[0], rgb[1], rgb[2], ln, k);
+                }
+                if (sd)
+                    for (int w = 0; w < SS_ETA; ++w)
+                        if (sd->tex[w]) {
+                            float rgb[3];
+                            texel_rgb(*S.tex, sd->tex[w] - 1u, uv, rgb);
+                            for (int k = 0; k < 4; ++k)
+                                s_tv[(w * 4 + k) * kBlock + threadIdx.x] =
+                                    rgb_spectrum_lane(basis, GS, rgb[0], rgb[1], rgb[2], ln, k);
+                        }
+            }
             eta_bsdf = m.eta_bsdf;
             coordinate_system(ng, &fs, &ft);                       // BSDF frame, bsdf.cpp:6-10 (ns = ng)
             fs = normalize(fs);
@@ -729,7 +728,8 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WAV
             rb = rng_base(S.seed, a.st_key[sid].x, a.st_key[sid].y, bounce);
         }
         const float *lb = (a.guided && mat_hit && guidable(bs_.kind)) ? a.lobes + (size_t)i * kGOut : nullptr;
-        const TabSpec tsp{S.spec, sd ? sd->slot : nullptr, GS, ln, a.grid.linear == 0};
+        const TabSpec tsp{S.spec, sd ? sd->slot : nullptr, GS, ln, a.grid.linear == 0,
+                          (STD && sd && S.tex) ? sd->tex : nullptr, s_tv + (STD ? threadIdx.x : 0u)};
         const bool do_nee = mat_hit && S.n_lights > 0 &&
                             (sd ? std_num_components(*sd, kNonSpecular) > 0 : !is_specular_kind(bs_.kind));
 #pragma unroll
@@ -1204,6 +1204,7 @@ int render_items(fl_ctx_t ctx, const HostItem *hi, uint32_t n_items, const fl_re
     a.nee_all = s->n_lights <= 2 ? 1u : 0u;
     a.rgb_basis = s->rgb_basis;
     a.stds = s->d_std;
+    a.tex = s->d_texscene;
     a.spec = tab.d_tab;
     a.grid = tab.grid;
     a.seed = p->seed;
@@ -1223,6 +1224,7 @@ int render_items(fl_ctx_t ctx, const HostItem *hi, uint32_t n_items, const fl_re
         it.nee_all = sk->n_lights <= 2 ? 1u : 0u;
         it.rgb_basis = sk->rgb_basis;
         it.stds = sk->d_std;
+        it.tex = sk->d_texscene;
         it.cam = make_camera(&hi[k].cam);
         it.tiles_x = (hi[k].cam.width + 7) / 8;
         it.npix = it.tiles_x * ((hi[k].cam.height + 7) / 8) * 64u;
@@ -1379,8 +1381,10 @@ int render_items(fl_ctx_t ctx, const HostItem *hi, uint32_t n_items, const fl_re
     // already runs, so the GPU does not wait for a host round trip. With
     // FL_RENDER_GRAPH a chunk is one captured graph, replayed.
     constexpr uint32_t kChunk = 4;   // even: a chunk starts and ends on queue parity 0
+    // standard surfaces and image textures run in the second shade kernel
     bool has_std = false;
-    for (uint32_t k = 0; k < n_items; ++k) has_std = has_std || hi[k].scene->has_standard;
+    for (uint32_t k = 0; k < n_items; ++k)
+        has_std = has_std || hi[k].scene->has_standard || hi[k].scene->has_textures;
     const int variant = (counters ? 1 : 0) | (p->shade_waves >= 4 ? 2 : 0) | ((int)(tv - kTraceVariants) << 2) |
                         (has_std ? 1 << 16 : 0);
     const uint32_t infer_grid = (uint32_t)ctx->n_cu * 16;   // one-wave blocks of 10.5 KB LDS

@@ -165,9 +165,11 @@ void build_spectral_tables(fl_scene_s *s, const fl_scene_desc *desc, int mode, s
         d.spec_a = (uint32_t)(3 + 2 * m);
         d.spec_b = (uint32_t)(4 + 2 * m);
         d.cauchy_b = 0.f;
+        d.tex = 0;
         float *ta = tab + (size_t)d.spec_a * stride, *tb = tab + (size_t)d.spec_b * stride;
         if (fm.kind == FL_MAT_LAMBERT) {
             from_reflectance_rgb(B, fm.rgb, N, ta);
+            d.tex = fm.texture;   // LambertMaterial's kd texture (lambert.cpp:12-13)
         } else if (fm.kind == FL_MAT_CONDUCTOR) {
             // standardsurface.cpp:104-106, GGXDistribution ctor ggx.cpp:7-11
             float aspect = std::sqrt(1.f - fm.anisotropic * 0.9f);
@@ -222,6 +224,11 @@ fl::StdParams standard_params(const fl_standard_surface &ss) {
     const bool metal_wl = ss.fresnel_type == FL_FRESNEL_WAVELENGTH_METALLIC;
     P.eta_bsdf = (ss.ks_weight > 0.f && ss.kt_weight > 0.f) ? ss.ks_ior : 1.f;
     P.kd_weight = ss.kd_weight;
+    P.tex[fl::SS_KD] = ss.kd_texture;
+    P.tex[fl::SS_KS] = ss.ks_texture;
+    P.tex[fl::SS_KT] = ss.kt_texture;
+    P.tex[fl::SS_SHEEN] = ss.sheen_texture;
+    P.tex[fl::SS_COAT] = ss.coat_texture;
     if ((ss.kd_weight > 0.f && ss.ks_weight <= 0.f) || (ss.kd_weight > 0.f && ss.ks_weight > 0.f && metal_wl)) {
         P.lobe[P.n++] = ss.thin_walled ? fl::SL_DIFFTRANS : fl::SL_OREN;
         fl::oren_nayar_AB(ss.kd_roughness * 90.f, &P.on_A, &P.on_B);   // remap [0-1] roughness to [0-90] sigma

@@ -52,6 +52,7 @@ struct StdParams {
     float coat_weight, coat_ior, coat_gloss;   // gloss after lerp(gloss, .1, .001)
     float eta_bsdf;
     uint32_t slot[SS_N];           // spectral table slots (render path)
+    uint32_t tex[SS_ETA];          // 1 + image texture of kd, ks, kt, sheen, coat; 0 = the slot's constant
     uint32_t pad[3];
 };
 

@@ -46,6 +46,7 @@ struct ItemDev {
     const DevLight *lights;
     const float *spec;       // spectral tables of the render's mode
     const StdParams *stds;   // StandardSurface lobe stacks
+    const TexScene *tex;     // image textures and uvs (null: none)
     uint32_t n_lights, nee_all;
     uint32_t rgb_basis;      // table slot of the RGB->spectrum basis
     CameraDev cam;
@@ -64,6 +65,7 @@ struct SceneRef {
     const DevLight *lights;
     const float *spec;
     const StdParams *stds;
+    const TexScene *tex;
     uint32_t n_lights, nee_all;
     uint32_t rgb_basis;
     uint64_t seed;
@@ -82,6 +84,7 @@ struct RenderArgs {
     uint32_t nee_all;        // 1: one NEE sample per light (uniform_sample_all_lights)
     uint32_t rgb_basis;      // table slot of the RGB->spectrum basis (normal AOV)
     const StdParams *stds;   // StandardSurface lobe stacks
+    const TexScene *tex;     // image textures and uvs (null: none)
     const float *spec;
     SpecGrid grid;
     uint32_t n_paths;        // this pass: path ids of all items
@@ -143,6 +146,7 @@ __device__ __forceinline__ SceneRef scene_of(const RenderArgs &a, uint32_t item)
         r.nee_all = it.nee_all;
         r.rgb_basis = it.rgb_basis;
         r.stds = it.stds;
+        r.tex = it.tex;
         r.seed = it.seed;
     } else {
         r.sc = a.sc;
@@ -153,6 +157,7 @@ __device__ __forceinline__ SceneRef scene_of(const RenderArgs &a, uint32_t item)
         r.nee_all = a.nee_all;
         r.rgb_basis = a.rgb_basis;
         r.stds = a.stds;
+        r.tex = a.tex;
         r.seed = a.seed;
     }
     return r;

@@ -36,7 +36,7 @@ f4 = C.c_float * 4
 class Material(C.Structure):
     _fields_ = [("kind", C.c_int32), ("rgb", f3), ("rgb_t", f3), ("roughness", C.c_float),
                 ("anisotropic", C.c_float), ("weight", C.c_float), ("metal", C.c_int32), ("ior", C.c_float),
-                ("cauchy_b", C.c_float), ("standard", C.c_uint32), ("pad", C.c_uint32 * 2)]
+                ("cauchy_b", C.c_float), ("standard", C.c_uint32), ("texture", C.c_uint32), ("pad", C.c_uint32)]
 
 
 class StandardSurface(C.Structure):
@@ -47,7 +47,12 @@ class StandardSurface(C.Structure):
                 ("kt_weight", C.c_float), ("kt", f3),
                 ("coat_weight", C.c_float), ("coat", f3), ("coat_ior", C.c_float), ("coat_gloss", C.c_float),
                 ("thin_film_thickness", C.c_float), ("thin_film_ior", C.c_float),
-                ("thin_walled", C.c_int32), ("pad", C.c_uint32)]
+                ("thin_walled", C.c_int32), ("kd_texture", C.c_uint32), ("ks_texture", C.c_uint32),
+                ("sheen_texture", C.c_uint32), ("kt_texture", C.c_uint32), ("coat_texture", C.c_uint32)]
+
+
+class Texture(C.Structure):
+    _fields_ = [("width", C.c_uint32), ("height", C.c_uint32), ("rgb", C.POINTER(C.c_uint8))]
 
 
 class StandardQuery(C.Structure):
@@ -74,7 +79,8 @@ class SceneDesc(C.Structure):
                 ("lights", C.POINTER(AreaLight)), ("n_lights", C.c_uint32),
                 ("emission_spd", C.POINTER(C.c_float)), ("n_emission", C.c_uint32),
                 ("delta_lights", C.POINTER(DeltaLight)), ("n_delta_lights", C.c_uint32),
-                ("standard_surfaces", C.POINTER(StandardSurface)), ("n_standard_surfaces", C.c_uint32)]
+                ("standard_surfaces", C.POINTER(StandardSurface)), ("n_standard_surfaces", C.c_uint32),
+                ("textures", C.POINTER(Texture)), ("n_textures", C.c_uint32), ("uvs", C.POINTER(C.c_float))]
 
 
 class SceneInfo(C.Structure):
@@ -279,6 +285,8 @@ class GenSceneData:
             self.n_delta_lights = 0
             self.standard_surfaces = (StandardSurface * 1)()
             self.n_standard_surfaces = 0
+            self.textures = []
+            self.uvs = None
             self.camera = Camera()
             C.pointer(self.camera)[0] = g.camera
             self.spp = g.spp
@@ -288,11 +296,12 @@ class GenSceneData:
 
     @classmethod
     def from_arrays(cls, vertices, indices, tri_material, materials, lights, emission, camera, spp=16, depth=8,
-                    delta_lights=(), standard_surfaces=()):
+                    delta_lights=(), standard_surfaces=(), textures=(), uvs=None):
         """A scene from explicit buffers: materials is a list of Material,
         lights a list of (tri, emission_index, intensity[, shadow_emission]),
         delta_lights a list of DeltaLight, emission an (n, FL_SPD_N) array,
-        camera a Camera."""
+        camera a Camera, textures a list of (height, width, 3) uint8 images
+        (rows top to bottom), uvs an (n_vertices, 2) array or None."""
         self = cls.__new__(cls)
         self.vertices = np.ascontiguousarray(vertices, np.float32).reshape(-1, 3)
         self.indices = np.ascontiguousarray(indices, np.uint32).reshape(-1, 3)
@@ -305,6 +314,8 @@ class GenSceneData:
         self.n_delta_lights = len(delta_lights)
         self.standard_surfaces = (StandardSurface * max(1, len(standard_surfaces)))(*standard_surfaces)
         self.n_standard_surfaces = len(standard_surfaces)
+        self.textures = list(textures)
+        self.uvs = uvs
         em = np.ascontiguousarray(emission, np.float32).reshape(-1, FL_SPD_N)
         self.emission = em.ravel().copy()
         self.n_emission = em.shape[0]
@@ -337,6 +348,21 @@ class GenSceneData:
         d.n_delta_lights = self.n_delta_lights
         d.standard_surfaces = C.cast(self.standard_surfaces, C.POINTER(StandardSurface))
         d.n_standard_surfaces = self.n_standard_surfaces
+        texs = [np.ascontiguousarray(t, np.uint8) for t in getattr(self, "textures", [])]
+        for t in texs:
+            if t.ndim != 3 or t.shape[2] != 3:
+                raise ValueError("a texture is a (height, width, 3) uint8 image")
+        self._keep_tex = texs
+        self._tex = (Texture * max(1, len(texs)))(
+            *[Texture(t.shape[1], t.shape[0], t.ctypes.data_as(C.POINTER(C.c_uint8))) for t in texs])
+        d.textures = C.cast(self._tex, C.POINTER(Texture))
+        d.n_textures = len(texs)
+        uvs = getattr(self, "uvs", None)
+        if uvs is not None:
+            self._keep_uv = np.ascontiguousarray(uvs, np.float32).reshape(-1, 2)
+            if self._keep_uv.shape[0] != v.shape[0]:
+                raise ValueError("uvs: one (u, v) per vertex")
+            d.uvs = self._keep_uv.ctypes.data_as(C.POINTER(C.c_float))
         return d
 
     def triangles(self):

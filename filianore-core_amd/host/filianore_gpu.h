@@ -49,9 +49,20 @@ struct Vector3f {
     Vector3f(float a, float b, float c) : x(a), y(b), z(c) {}
 };
 
+struct Vector2f {
+    float x = 0.f, y = 0.f;
+    Vector2f() = default;
+    Vector2f(float a, float b) : x(a), y(b) {}
+};
+
 // ---- scene input (bvh/bvh.h:9-19) ----------------------------------------
+// uv1..uv3: the vertices' texture coordinates (the Vertex::uv of
+// shapes/triangle.h); a scene where every triangle leaves them at (0, 0)
+// has no uvs, and every hit reads a texture at uv (0, 0) as the reference's
+// Embree path does.
 struct BasicTriangle {
     Vector3f v1, v2, v3;
+    Vector2f uv1, uv2, uv3;
 };
 
 struct TriangleMesh {
@@ -64,8 +75,25 @@ struct SceneGeometry {
 };
 
 // ---- materials the render path implements --------------------------------
+// ImageMapTexture (shading/textures/imagemap.h:8-23): the reference loads the
+// file with stb_image; here the caller passes the decoded 8-bit RGB image,
+// rows top to bottom. An empty image is black, as one that failed to load.
+struct ImageMapTexture {
+    uint32_t width = 0, height = 0;
+    std::vector<uint8_t> rgb;   // width * height * 3
+};
+
+// which spectrum of a material an image texture replaces
+enum class TextureSlot { Kd = 0, Ks = 1, SheenColor = 2, Kt = 3, KrCoat = 4 };
+
 struct Material {
     fl_material m{};
+    // image textures in place of constant colours (Lambert: Kd only)
+    std::shared_ptr<const ImageMapTexture> textures[5];
+    Material &with_texture(TextureSlot which, std::shared_ptr<const ImageMapTexture> t) {
+        textures[(int)which] = std::move(t);
+        return *this;
+    }
     // LambertMaterial(kd), shading/materials/lambert.cpp:9-17
     static Material lambert(Vector3f kd) {
         Material r;
@@ -260,18 +288,23 @@ public:
         for (size_t m = 0; m < geometry.triangleMeshes.size(); ++m)
             first[m + 1] = first[m] + geometry.triangleMeshes[m]->triangles.size();
         const size_t n = first.back();
-        std::vector<float> verts(n * 9);
+        std::vector<float> verts(n * 9), uvs(n * 6);
         std::vector<uint32_t> idx(n * 3), tmat(n);
+        bool has_uv = false;
         for (size_t m = 0; m < geometry.triangleMeshes.size(); ++m) {
             const TriangleMesh &mesh = *geometry.triangleMeshes[m];
             for (size_t t = 0; t < mesh.triangles.size(); ++t) {
                 const size_t id = first[m] + t;
                 const BasicTriangle &tr = *mesh.triangles[t];
                 const Vector3f v[3] = {tr.v1, tr.v2, tr.v3};
+                const Vector2f uv[3] = {tr.uv1, tr.uv2, tr.uv3};
                 for (int k = 0; k < 3; ++k) {
                     verts[id * 9 + 3 * k + 0] = v[k].x;
                     verts[id * 9 + 3 * k + 1] = v[k].y;
                     verts[id * 9 + 3 * k + 2] = v[k].z;
+                    uvs[id * 6 + 2 * k + 0] = uv[k].x;
+                    uvs[id * 6 + 2 * k + 1] = uv[k].y;
+                    has_uv = has_uv || uv[k].x != 0.f || uv[k].y != 0.f;
                     idx[id * 3 + k] = (uint32_t)(id * 3 + k);
                 }
                 tmat[id] = mesh.material;
@@ -279,11 +312,35 @@ public:
         }
         std::vector<fl_material> mats;
         std::vector<fl_standard_surface> standard;
+        std::vector<fl_texture> textures;
+        std::vector<const ImageMapTexture *> tex_seen;
+        // 1 + index of a texture in `textures` (each image once)
+        auto tex_ref = [&](const std::shared_ptr<const ImageMapTexture> &t) -> uint32_t {
+            if (!t) return 0u;
+            for (size_t j = 0; j < tex_seen.size(); ++j)
+                if (tex_seen[j] == t.get()) return (uint32_t)j + 1u;
+            fl_texture ft{};
+            if (t->rgb.size() >= 3ull * t->width * t->height) {
+                ft.width = t->width;
+                ft.height = t->height;
+                ft.rgb = t->rgb.data();
+            }
+            tex_seen.push_back(t.get());
+            textures.push_back(ft);
+            return (uint32_t)textures.size();
+        };
         for (const Material &m : materials) {
             mats.push_back(m.m);
+            if (m.m.kind == FL_MAT_LAMBERT) mats.back().texture = tex_ref(m.textures[(int)TextureSlot::Kd]);
             if (m.has_standard) {
                 mats.back().standard = (uint32_t)standard.size();
                 standard.push_back(m.standard);
+                fl_standard_surface &ss = standard.back();
+                ss.kd_texture = tex_ref(m.textures[(int)TextureSlot::Kd]);
+                ss.ks_texture = tex_ref(m.textures[(int)TextureSlot::Ks]);
+                ss.sheen_texture = tex_ref(m.textures[(int)TextureSlot::SheenColor]);
+                ss.kt_texture = tex_ref(m.textures[(int)TextureSlot::Kt]);
+                ss.coat_texture = tex_ref(m.textures[(int)TextureSlot::KrCoat]);
             }
         }
         std::vector<fl_area_light> lights;
@@ -364,6 +421,9 @@ public:
         d.n_delta_lights = (uint32_t)delta.size();
         d.standard_surfaces = standard.empty() ? nullptr : standard.data();
         d.n_standard_surfaces = (uint32_t)standard.size();
+        d.textures = textures.empty() ? nullptr : textures.data();
+        d.n_textures = (uint32_t)textures.size();
+        d.uvs = has_uv ? uvs.data() : nullptr;
         if (fl_scene_create(ctx_, &d, &scene_) != FL_OK) {
             err_ = fl_last_error(ctx_);
             scene_ = nullptr;

@@ -3,12 +3,15 @@
 // triangles, a Scene over it, intersect rays one at a time and in batches,
 // and render a film with GpuPathIntegrator.
 //
-//   fl_render_example [film.bin] [width height spp [standard]]
+//   fl_render_example [film.bin] [width height spp [standard [textures.bin]]]
 //
 // The scene is the procedural Cornell box (include/fl_scenegen.h), rebuilt
 // here as one TriangleMesh per run of equal material so that triangle IDs
 // keep the generator's order. With "standard", the walls and blocks become
-// StandardSurfaceMaterials (Material::standard_surface). Prints one JSON line; writes the film sums
+// StandardSurfaceMaterials (Material::standard_surface); with a textures file
+// (5 images, each uint32 width, uint32 height, then width * height RGB
+// bytes) they also read ImageMapTextures at uv = ((x + z + 2) / 4,
+// (y + 1) / 2). Prints one JSON line; writes the film sums
 // (float32, height x width x 3) to film.bin when given. Exit status: 0 ok,
 // 2 no GPU, 1 any other failure.
 #include <chrono>
@@ -51,6 +54,12 @@ int main(int argc, char **argv) {
         tri->v1 = Vector3f(v[3 * ix[0]], v[3 * ix[0] + 1], v[3 * ix[0] + 2]);
         tri->v2 = Vector3f(v[3 * ix[1]], v[3 * ix[1] + 1], v[3 * ix[1] + 2]);
         tri->v3 = Vector3f(v[3 * ix[2]], v[3 * ix[2] + 1], v[3 * ix[2] + 2]);
+        if (argc > 6) {
+            auto uv_of = [](const Vector3f &q) { return Vector2f((q.x + q.z + 2.f) / 4.f, (q.y + 1.f) / 2.f); };
+            tri->uv1 = uv_of(tri->v1);
+            tri->uv2 = uv_of(tri->v2);
+            tri->uv3 = uv_of(tri->v3);
+        }
         mesh_of[t] = geo.triangleMeshes.size() - 1;
         index_in_mesh[t] = geo.triangleMeshes.back()->triangles.size();
         geo.triangleMeshes.back()->triangles.push_back(tri);
@@ -78,6 +87,31 @@ int main(int argc, char **argv) {
         mats[3] = Material::standard_surface(0.5f, Vector3f(0.7f, 0.7f, 0.6f), 0.6f, 0.8f, ks, 0.f, 0.f,
                                              FL_FRESNEL_WAVELENGTH_METALLIC, 1.45f, 6, 0.f, sheen, 0.f, 0.9f, kt, 0.3f,
                                              coat, 1.5f, 0.4f, 0.f, 1.33f, true);
+        if (argc > 6) {
+            std::vector<std::shared_ptr<const ImageMapTexture>> tex;
+            FILE *f = fopen(argv[6], "rb");
+            for (int k = 0; f && k < 5; ++k) {
+                auto t = std::make_shared<ImageMapTexture>();
+                uint32_t wh[2] = {0, 0};
+                if (fread(wh, 4, 2, f) != 2) break;
+                t->width = wh[0];
+                t->height = wh[1];
+                t->rgb.resize(3ull * wh[0] * wh[1]);
+                if (fread(t->rgb.data(), 1, t->rgb.size(), f) != t->rgb.size()) break;
+                tex.push_back(t);
+            }
+            if (f) fclose(f);
+            if (tex.size() != 5) {
+                printf("{\"ok\": false, \"error\": \"cannot read %s\"}\n", argv[6]);
+                fl_gen_free(&g);
+                return 1;
+            }
+            mats[0].with_texture(TextureSlot::Kd, tex[0]).with_texture(TextureSlot::SheenColor, tex[1])
+                .with_texture(TextureSlot::KrCoat, tex[2]);
+            mats[1].with_texture(TextureSlot::Kd, tex[3]);
+            mats[2].with_texture(TextureSlot::Ks, tex[4]);
+            mats[3].with_texture(TextureSlot::Kd, tex[1]).with_texture(TextureSlot::Kt, tex[2]);
+        }
     }
     std::vector<DiffuseAreaIlluminant> lights;
     for (uint32_t l = 0; l < g.n_lights; ++l) {

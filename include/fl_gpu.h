@@ -36,7 +36,7 @@
 extern "C" {
 #endif
 
-#define FL_ABI_VERSION 2
+#define FL_ABI_VERSION 3
 
 /* ---- status codes ------------------------------------------------------ */
 enum {
@@ -85,7 +85,10 @@ typedef struct {
     float coat_weight, coat[3], coat_ior, coat_gloss;      /* clearcoat */
     float thin_film_thickness, thin_film_ior;              /* thin film (nm) */
     int32_t thin_walled;
-    uint32_t pad;
+    /* image textures (ImageMapTexture) in place of the constant colours kd,
+     * ks, sheen_color, kt, coat: 0 = the constant, else 1 + an index into
+     * fl_scene_desc.textures */
+    uint32_t kd_texture, ks_texture, sheen_texture, kt_texture, coat_texture;
 } fl_standard_surface;
 
 typedef struct {
@@ -99,7 +102,9 @@ typedef struct {
     float ior;           /* dielectric ksIOR; with cauchy_b != 0: Cauchy A */
     float cauchy_b;      /* Cauchy B [um^2]; 0 = scalar eta as in the reference */
     uint32_t standard;   /* FL_MAT_STANDARD: index into fl_scene_desc.standard_surfaces */
-    uint32_t pad[2];
+    uint32_t texture;    /* Lambert: an ImageMapTexture for kd, 1 + index into
+                            fl_scene_desc.textures (0 = the constant rgb) */
+    uint32_t pad;
 } fl_material;
 
 /* One DiffuseAreaIlluminant over one triangle (illuminants/diffusearea.cpp). */
@@ -138,6 +143,18 @@ typedef struct {
 
 #define FL_SPD_N 471     /* emission SPDs are sampled on 360..830 nm at 1 nm */
 
+/* An ImageMapTexture (shading/textures/imagemap.cpp): 8-bit RGB, 3 bytes per
+ * texel, rows top to bottom (as stbi_load returns them). evaluate looks up
+ * the texel at i = int(u W), j = int((1 - v) H - 0.001), both clamped to the
+ * image (the reference clamps j into i, imagemap.cpp:54-55; here j is
+ * clamped), and returns from_reflectance_rgb(rgb / 255), clamped at 0 as the
+ * materials clamp their spectra. A texture with no data (rgb NULL or a zero
+ * size) is black, as an image that failed to load is. */
+typedef struct {
+    uint32_t width, height;
+    const uint8_t *rgb;          /* width * height * 3 bytes */
+} fl_texture;
+
 typedef struct {
     const float *vertices;       /* n_vertices * 3 floats */
     uint32_t n_vertices;
@@ -154,6 +171,13 @@ typedef struct {
     uint32_t n_delta_lights;
     const fl_standard_surface *standard_surfaces;
     uint32_t n_standard_surfaces;
+    const fl_texture *textures;
+    uint32_t n_textures;
+    /* per-vertex texture coordinates (n_vertices * 2 floats), interpolated
+     * with the hit's barycentric weights as Triangle::intersect does
+     * (shapes/triangle.cpp:66-73); NULL: every hit has uv = (0, 0), as on the
+     * reference's Embree path (bvh/bvh.cpp:69-71) */
+    const float *uvs;
 } fl_scene_desc;
 
 typedef struct {

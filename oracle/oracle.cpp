@@ -768,6 +768,13 @@ struct Material {
     // FL_MAT_STANDARD: parameters and the 60-bin spectra of its colours
     fl_standard_surface ss;
     S60 kd, ks, kt, sheen, coat;
+    uint32_t tex;   // Lambert: 1 + image texture of kd, 0 = R
+};
+
+// ImageMapTexture (shading/textures/imagemap.cpp): RGB8, rows top to bottom
+struct Texture {
+    uint32_t w = 0, h = 0;
+    std::vector<uint8_t> rgb;
 };
 
 } // namespace
@@ -775,6 +782,8 @@ struct Material {
 struct or_scene_s {
     std::vector<V3f> v0, v1, v2;      // per triangle (flattened ID order)
     std::vector<uint32_t> mat, light; // per triangle: material, light+1
+    std::vector<float> uv;            // per triangle: 3 vertex uvs (empty: none)
+    std::vector<Texture> tex;
     std::vector<Material> mats;
     std::vector<Light> lights;
     std::vector<BNode> nodes;
@@ -1016,6 +1025,7 @@ struct Isect {
     float t;
     V3f p, pError, n, wo;
     uint32_t prim;
+    float bu, bv;   // barycentric weights of the triangle's second and third vertices
 };
 
 // BVH::intersect with repairs R1-R4 (bvh/bvh.cpp:49-84)
@@ -1029,6 +1039,8 @@ bool scene_intersect(const or_scene_s *s, const V3f &o, const V3f &d, float tmin
     it->n = normalize(cross(s->v1[p] - s->v0[p], s->v2[p] - s->v0[p]));  // repair R3: Ng = (v1-v0)x(v2-v0)
     it->wo = normalize(-d);
     it->prim = p;                                                  // repair R2
+    it->bu = h.u;
+    it->bv = h.v;
     return true;
 }
 
@@ -1123,16 +1135,49 @@ void add_standard_lobes(Bsdf<T, N> &b, const fl_standard_surface &ss, const Spec
     }
 }
 
+// The hit's texture coordinates: the vertex uvs weighted by the hit's
+// barycentric coordinates, which Triangle::intersect forms as the area
+// ratios a1, a2, a3 (shapes/triangle.cpp:61-73); (0, 0) without vertex uvs,
+// as the Embree path leaves SurfaceInteraction::uv (bvh/bvh.cpp:69-71).
+void hit_uv(const or_scene_s *s, const Isect &it, float uv[2]) {
+    uv[0] = uv[1] = 0.f;
+    if (s->uv.empty()) return;
+    const float *q = s->uv.data() + 6ull * it.prim;
+    const float w = 1.f - it.bu - it.bv;
+    for (int k = 0; k < 2; ++k) uv[k] = q[k] * w + q[2 + k] * it.bu + q[4 + k] * it.bv;
+}
+
+// ImageMapTexture::evaluate (imagemap.cpp:38-62), then the material's
+// spectrum_clamp. Repair R14: j is clamped (the reference clamps i there).
+S60 texture_eval(const Texture &T, const float uv[2]) {
+    if (T.w == 0 || T.h == 0) return S60(0.f);
+    int i = int(uv[0] * (float)T.w);
+    int j = int((1.f - uv[1]) * (float)T.h - 0.001f);
+    if (i < 0) i = 0;
+    if (j < 0) j = 0;
+    if (i > (int)T.w - 1) i = (int)T.w - 1;
+    if (j > (int)T.h - 1) j = (int)T.h - 1;
+    const uint8_t *px = T.rgb.data() + 3ull * ((size_t)i + (size_t)T.w * (size_t)j);
+    const float rgb[3] = {int(px[0]) / 255.f, int(px[1]) / 255.f, int(px[2]) / 255.f};
+    return from_reflectance_rgb(rgb).clamp0();
+}
+
 Bsdf<float, NS> make_bsdf(const or_scene_s *s, const Isect &it) {
     Bsdf<float, NS> b;
     uint32_t m = s->mat[it.prim];
     if (m == 0xffffu || s->mats[m].kind == FL_MAT_NONE) return b;
     const Material &M = s->mats[m];
     b.frame(it.n, it.n);                 // ns = Shading.n = n (repair R3)
+    float uv[2];
+    hit_uv(s, it, uv);
     if (M.kind == FL_MAT_STANDARD) {
         S60 lam;
         for (int i = 0; i < NS; ++i) lam.c[i] = (float)(400 + 5 * i);     // thinfilm.cpp:38
-        add_standard_lobes<float, NS>(b, M.ss, M.kd, M.ks, M.kt, M.sheen, M.coat, M.eta, M.k, lam, false);
+        // kd, ks, kt, sheenColor, krcoat: a texture's value in place of the constant
+        auto tx = [&](uint32_t t, const S60 &c) { return t ? texture_eval(s->tex[t - 1], uv) : c; };
+        add_standard_lobes<float, NS>(b, M.ss, tx(M.ss.kd_texture, M.kd), tx(M.ss.ks_texture, M.ks),
+                                      tx(M.ss.kt_texture, M.kt), tx(M.ss.sheen_texture, M.sheen),
+                                      tx(M.ss.coat_texture, M.coat), M.eta, M.k, lam, false);
         return b;
     }
     Bxdf<float, NS> &l = b.add();
@@ -1140,7 +1185,7 @@ Bsdf<float, NS> make_bsdf(const or_scene_s *s, const Isect &it) {
     l.ggx.ax = M.ax;
     l.ggx.ay = M.ay;
     l.weight = M.weight;
-    l.R = M.R;
+    l.R = (M.kind == FL_MAT_LAMBERT && M.tex) ? texture_eval(s->tex[M.tex - 1], uv) : M.R;   // lambert.cpp:12-13
     l.Tt = M.T;
     l.eta = M.eta;
     l.k = M.k;
@@ -1519,6 +1564,23 @@ int or_scene_create(const fl_scene_desc *d, or_scene_t *out) {
         s->v0[t] = V(ix[0]); s->v1[t] = V(ix[1]); s->v2[t] = V(ix[2]);
         s->mat[t] = d->tri_material ? d->tri_material[t] : 0u;
     }
+    if (d->uvs) {
+        s->uv.resize(6ull * n);
+        for (uint32_t t = 0; t < n; ++t)
+            for (int k = 0; k < 3; ++k) {
+                const uint32_t v = d->indices[3ull * t + k];
+                s->uv[6ull * t + 2 * k] = d->uvs[2ull * v];
+                s->uv[6ull * t + 2 * k + 1] = d->uvs[2ull * v + 1];
+            }
+    }
+    s->tex.resize(d->n_textures);
+    for (uint32_t j = 0; j < d->n_textures; ++j) {
+        const fl_texture &tx = d->textures[j];
+        if (!tx.rgb || !tx.width || !tx.height) continue;            // no image: black
+        s->tex[j].w = tx.width;
+        s->tex[j].h = tx.height;
+        s->tex[j].rgb.assign(tx.rgb, tx.rgb + 3ull * tx.width * tx.height);
+    }
     const Tables &T = tables();
     s->mats.resize(d->n_materials);
     for (uint32_t m = 0; m < d->n_materials; ++m) {
@@ -1526,8 +1588,11 @@ int or_scene_create(const fl_scene_desc *d, or_scene_t *out) {
         Material &M = s->mats[m];
         M.kind = fm.kind;
         M.ax = M.ay = 1.f; M.weight = 1.f; M.ior = 1.f;
+        M.tex = 0;
         if (fm.kind == FL_MAT_LAMBERT) {
             M.R = from_reflectance_rgb(fm.rgb).clamp0();                  // LambertMaterial: kd.spectrum_clamp()
+            if (fm.texture > d->n_textures) { delete s; return FL_E_INVALID; }
+            M.tex = fm.texture;
         } else if (fm.kind == FL_MAT_CONDUCTOR) {                           // standardsurface.cpp:101-125
             float aspect = std::sqrt(1.f - fm.anisotropic * 0.9f);
             float r2 = fm.roughness * fm.roughness;
@@ -1544,6 +1609,8 @@ int or_scene_create(const fl_scene_desc *d, or_scene_t *out) {
         } else if (fm.kind == FL_MAT_STANDARD) {
             if (fm.standard >= d->n_standard_surfaces) { delete s; return FL_E_INVALID; }
             M.ss = d->standard_surfaces[fm.standard];
+            for (uint32_t t : {M.ss.kd_texture, M.ss.ks_texture, M.ss.kt_texture, M.ss.sheen_texture, M.ss.coat_texture})
+                if (t > d->n_textures) { delete s; return FL_E_INVALID; }
             M.kd = from_reflectance_rgb(M.ss.kd).clamp0();                  // Texture::evaluate().spectrum_clamp()
             M.ks = from_reflectance_rgb(M.ss.ks).clamp0();
             M.kt = from_reflectance_rgb(M.ss.kt).clamp0();

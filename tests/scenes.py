@@ -125,3 +125,45 @@ def floor_points(cam, sub=8):
     d = np.broadcast_to(d, (H, W, sub * sub, 3))
     tt = pos[1] / -d[..., 1]
     return pos + d * tt[..., None]
+
+
+def checker_texture(w=5, h=4, seed=0):
+    """A w x h RGB8 image of distinct random colours (rows top to bottom)."""
+    rng = np.random.default_rng(seed)
+    return rng.integers(20, 250, size=(h, w, 3), dtype=np.uint8)
+
+
+def textured_floor(texture, spds, with_uvs=True, standard=False, W=32, intensity=2.0):
+    """delta_floor's floor lit straight from above by a directional light,
+    its albedo an image texture: uv = ((x + 2) / 4, (z + 2) / 4), so the
+    camera's view [-2, 2]^2 spans the image once (outside it the lookup
+    clamps). standard: a StandardSurface with only an Oren-Nayar lobe of
+    roughness 0 (Lambert) reads the texture as kd instead of a Lambert
+    material."""
+    light = delta_light(flgpu.FL_LIGHT_DIRECTIONAL, direction=(0.0, -1.0, 0.0), intensity=intensity)
+    gen = delta_floor([light], spds, W=W)
+    if standard:
+        s = flgpu.StandardSurface()
+        s.kd_weight, s.kd_roughness, s.ks_ior = 1.0, 0.0, 1.5
+        s.kd[:] = (0.3, 0.3, 0.3)
+        s.kd_texture = 1
+        gen.materials[0].kind = flgpu.FL_MAT_STANDARD
+        gen.materials[0].standard = 0
+        gen.standard_surfaces = (flgpu.StandardSurface * 1)(s)
+        gen.n_standard_surfaces = 1
+    else:
+        gen.materials[0].texture = 1
+    gen.textures = [texture]
+    if with_uvs:
+        v = gen.vertices
+        gen.uvs = np.stack([(v[:, 0] + 2.0) / 4.0, (v[:, 2] + 2.0) / 4.0], -1).astype(np.float32)
+    return gen
+
+
+def texel_of(texture, u, v):
+    """ImageMapTexture::evaluate's texel index (imagemap.cpp:38-55, with j
+    clamped) for uv arrays, as float32 arithmetic gives it."""
+    h, w = texture.shape[:2]
+    i = np.trunc(np.float32(u) * np.float32(w)).astype(np.int64)
+    j = np.trunc((np.float32(1.0) - np.float32(v)) * np.float32(h) - np.float32(0.001)).astype(np.int64)
+    return np.clip(i, 0, w - 1), np.clip(j, 0, h - 1)

@@ -2,6 +2,10 @@
 No compute call is made here (this runs without a GPU)."""
 import os
 import re
+import shutil
+import subprocess
+
+import pytest
 
 import flgpu
 
@@ -35,7 +39,7 @@ def test_embree_subset_exported():
 
 def test_abi_version_and_no_device_behaviour():
     L = flgpu.lib()
-    assert L.fl_abi_version() == 2
+    assert L.fl_abi_version() == 3
     if L.fl_device_count() == 0:
         h = flgpu.C.c_void_p()
         assert L.fl_ctx_create(0, flgpu.C.byref(h)) == flgpu.FL_E_UNSUPPORTED
@@ -52,3 +56,34 @@ def test_struct_sizes_match_header():
     assert C.sizeof(flgpu.Camera) == 48
     assert C.sizeof(flgpu.Stats) == 176
     assert C.sizeof(flgpu.RenderParams) == 88
+
+
+@pytest.mark.skipif(shutil.which("gcc") is None, reason="needs a C compiler")
+def test_struct_layouts_match_the_c_compiler(tmp_path):
+    """sizeof and field offsets of the ctypes mirrors equal the C header's,
+    as gcc lays the structs out."""
+    C = flgpu.C
+    structs = {"fl_material": flgpu.Material, "fl_standard_surface": flgpu.StandardSurface,
+               "fl_texture": flgpu.Texture, "fl_area_light": flgpu.AreaLight, "fl_delta_light": flgpu.DeltaLight,
+               "fl_scene_desc": flgpu.SceneDesc, "fl_standard_query": flgpu.StandardQuery,
+               "fl_bsdf_query": flgpu.BsdfQuery, "fl_bsdf_result": flgpu.BsdfResult,
+               "fl_scene_info": flgpu.SceneInfo, "fl_camera": flgpu.Camera, "fl_render_params": flgpu.RenderParams,
+               "fl_film": flgpu.Film, "fl_stats": flgpu.Stats, "fl_batch_item": flgpu.BatchItem,
+               "fl_tune_result": flgpu.TuneResult, "fl_guide_config": flgpu.GuideConfig,
+               "fl_develop_params": flgpu.DevelopParams}
+    lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "fl_gpu.h"', "int main(void) {"]
+    for cname, py in structs.items():
+        lines.append('printf("%%s %%zu\\n", "%s", sizeof(%s));' % (cname, cname))
+        for f in py._fields_:
+            lines.append('printf("%%s.%%s %%zu\\n", "%s", "%s", offsetof(%s, %s));' % (cname, f[0], cname, f[0]))
+    lines.append("return 0; }")
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)], check=True)
+    got = dict(l.rsplit(" ", 1) for l in subprocess.run([str(exe)], capture_output=True, text=True,
+                                                        check=True).stdout.split("\n") if l)
+    for cname, py in structs.items():
+        assert int(got[cname]) == C.sizeof(py), cname
+        for f in py._fields_:
+            assert int(got[cname + "." + f[0]]) == getattr(py, f[0]).offset, (cname, f[0])

@@ -70,6 +70,29 @@ def test_cpp_layer_standard_surface_matches_c_abi(tmp_path):
     assert not np.array_equal(other, ref)
 
 
+@pytest.mark.gpu
+def test_cpp_layer_textures_match_c_abi(tmp_path):
+    """ImageMapTexture, Material::with_texture and BasicTriangle uvs in the
+    C++ layer: the textured standard box renders the same film, bit for bit,
+    as the same textures and uvs given to the C ABI."""
+    from test_gpu_textures import textured_box
+
+    gen = textured_box(48, 40)
+    blob = b"".join(np.array([t.shape[1], t.shape[0]], np.uint32).tobytes() + np.ascontiguousarray(t).tobytes()
+                    for t in gen.textures)
+    texf = tmp_path / "textures.bin"
+    texf.write_bytes(blob)
+    film = str(tmp_path / "film.bin")
+    rc, out = run_example(film, "48", "40", "16", "standard", str(texf))
+    assert rc == 0 and out["ok"], out
+    got = np.fromfile(film, np.float32).reshape(40, 48, 3)
+    with flgpu.Context(0) as ctx, ctx.scene(gen) as sc:
+        p = flgpu.default_params(spp=16, depth=gen.depth, mode=flgpu.FL_SPECTRAL_HERO, seed=7,
+                                 flags=flgpu.FL_RENDER_GRAPH | flgpu.FL_RENDER_VNDF_PDF)
+        ref, _, _ = sc.render(p)
+    assert np.array_equal(got.view(np.uint32), ref.view(np.uint32))
+
+
 def test_rtc_example_built_and_reports_missing_gpu():
     assert os.access(RTC_EXE, os.X_OK), "build() did not produce the Embree-API example"
     if flgpu.lib().fl_device_count() > 0:
