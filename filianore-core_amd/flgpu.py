@@ -188,7 +188,7 @@ class GenScene(C.Structure):
                 ("camera", Camera), ("spp", C.c_uint32), ("depth", C.c_int32)]
 
 
-# exported symbols of include/fl_gpu.h and include/fl_scenegen.h
+# exported symbols of include/fl_gpu.h, include/fl_scenegen.h and include/fl_scenefile.h
 EXPORTS = [
     "fl_abi_version", "fl_device_count", "fl_probe_copy", "fl_ctx_create", "fl_ctx_destroy", "fl_last_error",
     "fl_scene_create", "fl_scene_destroy", "fl_scene_get_info", "fl_scene_download_bvh",
@@ -197,6 +197,8 @@ EXPORTS = [
     "fl_guide_get_params", "fl_guide_set_params", "fl_guide_gradient", "fl_guide_stats",
     "fl_bsdf_query_batch", "fl_bsdf_query_standard", "fl_develop_film", "fl_develop",
     "fl_gen_default_options", "fl_gen_config", "fl_gen_free", "fl_gen_describe", "fl_gen_blackbody",
+    "fl_scene_file_write", "fl_scene_file_read", "fl_scene_file_error", "fl_scene_file_desc",
+    "fl_scene_file_cameras", "fl_scene_file_free",
 ]
 
 _lib = None
@@ -244,6 +246,14 @@ def lib():
         L.fl_gen_free.argtypes = [C.POINTER(GenScene)]
         L.fl_gen_describe.argtypes = [C.POINTER(GenScene), C.POINTER(SceneDesc)]
         L.fl_gen_blackbody.argtypes = [C.c_float, C.POINTER(C.c_float)]
+        L.fl_scene_file_write.argtypes = [C.c_char_p, C.POINTER(SceneDesc), C.POINTER(Camera), C.c_uint32]
+        L.fl_scene_file_read.argtypes = [C.c_char_p, C.POINTER(vp)]
+        L.fl_scene_file_error.restype = C.c_char_p
+        L.fl_scene_file_desc.argtypes = [vp]
+        L.fl_scene_file_desc.restype = C.POINTER(SceneDesc)
+        L.fl_scene_file_cameras.argtypes = [vp, C.POINTER(C.POINTER(Camera))]
+        L.fl_scene_file_cameras.restype = C.c_uint32
+        L.fl_scene_file_free.argtypes = [vp]
         _lib = L
     return _lib
 
@@ -492,6 +502,71 @@ class Context:
         r = (BsdfResult * n)()
         _check(self.L.fl_bsdf_query_batch(self.h, q, r, n), self.h, "fl_bsdf_query_batch")
         return r
+
+
+def save_scene_file(path, gen, cameras=None):
+    """fl_scene_file_write: gen's scene description and its cameras (default:
+    gen.camera) to path."""
+    cams = [gen.camera] if cameras is None else list(cameras)
+    arr = (Camera * max(1, len(cams)))(*cams)
+    d = gen.desc()
+    if lib().fl_scene_file_write(os.fsencode(path), C.byref(d), arr, len(cams)) != FL_OK:
+        raise FlError("fl_scene_file_write: " + lib().fl_scene_file_error().decode())
+
+
+def load_scene_file(path, spp=16, depth=8):
+    """fl_scene_file_read: the file's scene as a GenSceneData (owned copies),
+    its first camera as .camera and all of them as .cameras."""
+    L = lib()
+    h = C.c_void_p()
+    if L.fl_scene_file_read(os.fsencode(path), C.byref(h)) != FL_OK:
+        raise FlError("fl_scene_file_read: " + L.fl_scene_file_error().decode())
+    try:
+        d = L.fl_scene_file_desc(h).contents
+        cp = C.POINTER(Camera)()
+        nc = L.fl_scene_file_cameras(h, C.byref(cp))
+        cams = []
+        for k in range(nc):
+            c = Camera()
+            C.pointer(c)[0] = cp[k]
+            cams.append(c)
+
+        def arr(ptr, n, dtype, shape):
+            return np.ctypeslib.as_array(ptr, (n,)).copy().astype(dtype).reshape(shape) if n else \
+                np.zeros(shape, dtype)
+
+        nv, nt = d.n_vertices, d.n_triangles
+        verts = arr(d.vertices, 3 * nv, np.float32, (nv, 3))
+        idx = arr(d.indices, 3 * nt, np.uint32, (nt, 3))
+        tmat = arr(d.tri_material, nt, np.uint32, (nt,)) if d.tri_material else np.zeros(nt, np.uint32)
+        mats = [d.materials[i] for i in range(d.n_materials)]
+        lights = [(d.lights[i].tri, d.lights[i].emission, d.lights[i].intensity, d.lights[i].shadow_emission)
+                  for i in range(d.n_lights)]
+        em = arr(d.emission_spd, d.n_emission * FL_SPD_N, np.float32, (d.n_emission, FL_SPD_N))
+        textures = []
+        for j in range(d.n_textures):
+            t = d.textures[j]
+            if t.rgb and t.width and t.height:
+                textures.append(np.ctypeslib.as_array(t.rgb, (t.height * t.width * 3,)).copy()
+                                .reshape(t.height, t.width, 3))
+            else:
+                textures.append(np.zeros((0, 0, 3), np.uint8))
+        uvs = arr(d.uvs, 2 * nv, np.float32, (nv, 2)) if d.uvs else None
+        gen = GenSceneData.from_arrays(verts, idx, tmat, mats, lights, em if d.n_emission else
+                                       np.zeros((1, FL_SPD_N), np.float32),
+                                       cams[0] if cams else Camera(), spp=spp, depth=depth,
+                                       delta_lights=[d.delta_lights[i] for i in range(d.n_delta_lights)],
+                                       standard_surfaces=[d.standard_surfaces[i]
+                                                          for i in range(d.n_standard_surfaces)],
+                                       textures=textures, uvs=uvs)
+        if not d.n_emission:
+            gen.n_emission = 0
+        if not d.tri_material:
+            gen.tri_material = np.zeros(nt, np.uint32)
+        gen.cameras = cams
+        return gen
+    finally:
+        L.fl_scene_file_free(h)
 
 
 class Scene:

@@ -20,7 +20,7 @@ def declared(header):
 
 def test_all_declared_symbols_exported():
     L = flgpu.lib()
-    names = declared("fl_gpu.h") + declared("fl_scenegen.h")
+    names = declared("fl_gpu.h") + declared("fl_scenegen.h") + declared("fl_scenefile.h")
     assert len(names) >= 25
     missing = [n for n in names if not hasattr(L, n)]
     assert not missing, missing
