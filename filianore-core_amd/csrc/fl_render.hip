@@ -1386,7 +1386,7 @@ int render_items(fl_ctx_t ctx, const HostItem *hi, uint32_t n_items, const fl_re
     for (uint32_t k = 0; k < n_items; ++k)
         has_std = has_std || hi[k].scene->has_standard || hi[k].scene->has_textures;
     const int variant = (counters ? 1 : 0) | (p->shade_waves >= 4 ? 2 : 0) | ((int)(tv - kTraceVariants) << 2) |
-                        (has_std ? 1 << 16 : 0);
+                        (has_std ? 1 << 16 : 0) | (p->shade_waves == 3 ? 1 << 17 : 0);
     const uint32_t infer_grid = (uint32_t)ctx->n_cu * 16;   // one-wave blocks of 10.5 KB LDS
     auto launch_shade = [&](const RenderArgs &args, dim3 g, int j, bool with_infer = true) {
         if (with_infer && args.guided) hipLaunchKernelGGL(k_guide_infer, dim3(infer_grid), dim3(64), 0, st, args, j);
@@ -1395,6 +1395,7 @@ int render_items(fl_ctx_t ctx, const HostItem *hi, uint32_t n_items, const fl_re
             else hipLaunchKernelGGL((k_shade<2, true>), g, dim3(kBlock), 0, st, args, j);
         } else {
             if (variant & 2) hipLaunchKernelGGL((k_shade<4, false>), g, dim3(kBlock), 0, st, args, j);
+            else if (variant & (1 << 17)) hipLaunchKernelGGL((k_shade<3, false>), g, dim3(kBlock), 0, st, args, j);
             else hipLaunchKernelGGL((k_shade<2, false>), g, dim3(kBlock), 0, st, args, j);
         }
     };
@@ -1685,7 +1686,7 @@ int autotune_items(fl_ctx_t ctx, const HostItem *hi, uint32_t n, const fl_render
     if (rc == FL_OK)
         rc = try_values([](fl_render_params &q, uint32_t v) { q.flags = v ? (q.flags | FL_RENDER_NO_SORT) : q.flags; },
                         {1u});
-    if (rc == FL_OK) rc = try_values([](fl_render_params &q, uint32_t v) { q.shade_waves = v; }, {4u});
+    if (rc == FL_OK) rc = try_values([](fl_render_params &q, uint32_t v) { q.shade_waves = v; }, {3u, 4u});
     if (rc != FL_OK) return rc;
     memset(out, 0, sizeof(*out));
     out->max_paths = cur.max_paths;

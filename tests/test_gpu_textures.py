@@ -65,3 +65,25 @@ def test_bad_texture_index_is_rejected(ctx):
     gen.materials[0].texture = 2          # only one texture
     with pytest.raises(flgpu.FlError):
         ctx.scene(gen)
+
+
+def test_batch_with_textured_and_plain_scenes(ctx):
+    """A batch mixing a textured scene, a standard-surface scene and a plain
+    Lambert scene: each item's film equals rendering it alone with its seed
+    (the plain one alone runs the smaller shade kernel)."""
+    gens = [textured_box(24, 20), flgpu.GenSceneData(flgpu.FL_GEN_CORNELL, width=20, height=16), standard_box(16, 16)]
+    seed = 77
+    flags = flgpu.FL_RENDER_GRAPH
+    scenes = [ctx.scene(g) for g in gens]
+    try:
+        p = flgpu.default_params(spp=8, depth=8, seed=seed, flags=flags)
+        films, st = ctx.render_batch([(sc, None, 8) for sc in scenes], p)
+        assert st.nan_samples == 0
+        for k, sc in enumerate(scenes):
+            ps = flgpu.default_params(spp=8, depth=8, seed=(seed + k * flgpu.ITEM_SEED_STRIDE) & ((1 << 64) - 1),
+                                      flags=flags)
+            single, _, _ = sc.render(ps)
+            assert np.array_equal(films[k][0].view(np.uint32), single.view(np.uint32)), k
+    finally:
+        for sc in scenes:
+            sc.close()
