@@ -38,6 +38,10 @@ void free_scene(fl_scene_s *s) {
     if (!s) return;
     (void)hipFree(s->d_nodes);
     (void)hipFree(s->d_tris);
+    (void)hipFree(s->d_verts);
+    (void)hipFree(s->d_idx);
+    (void)hipFree(s->d_mat);
+    (void)hipFree(s->d_light);
     (void)hipFree(s->d_std);
     (void)hipFree(s->d_tex_info);
     (void)hipFree(s->d_tex_data);
@@ -168,6 +172,8 @@ int fl_scene_create(fl_ctx_t ctx, const fl_scene_desc *desc, fl_scene_t *out) {
             (ss.thin_film_thickness > 0.f && !(ss.thin_film_ior > 0.f)))
             return set_error(ctx, FL_E_INVALID, "fl_scene_create: bad standard surface");
     }
+    if (desc->bvh_width == 1 || desc->bvh_width > 8)
+        return set_error(ctx, FL_E_INVALID, "fl_scene_create: bvh_width must be 0 (8) or 2..8");
     if (desc->n_textures && !desc->textures) return set_error(ctx, FL_E_INVALID, "fl_scene_create: missing textures");
     uint64_t tex_bytes = 0;
     for (uint32_t j = 0; j < desc->n_textures; ++j) {
@@ -269,13 +275,12 @@ int fl_scene_create(fl_ctx_t ctx, const fl_scene_desc *desc, fl_scene_t *out) {
     for (const DevLight &d : s->h_lights) s->shadow_colors = s->shadow_colors || d.shadow_slot != 0u;
 
     int rc;
-    float *d_verts = nullptr;
-    uint32_t *d_idx = nullptr, *d_mat = nullptr, *d_light = nullptr;
     std::vector<uint32_t> mat_h;
-    if ((rc = upload(ctx, &d_verts, desc->vertices, 3ull * desc->n_vertices)) ||
-        (rc = upload(ctx, &d_idx, desc->indices, 3ull * desc->n_triangles)) ||
-        (rc = upload(ctx, &d_light, tri_light.data(), desc->n_triangles))) {
-        (void)hipFree(d_verts); (void)hipFree(d_idx); (void)hipFree(d_light);
+    // the build inputs stay on the device for rebuilds at another node width
+    s->bvh_width = desc->bvh_width ? desc->bvh_width : 8u;
+    if ((rc = upload(ctx, &s->d_verts, desc->vertices, 3ull * desc->n_vertices)) ||
+        (rc = upload(ctx, &s->d_idx, desc->indices, 3ull * desc->n_triangles)) ||
+        (rc = upload(ctx, &s->d_light, tri_light.data(), desc->n_triangles))) {
         free_scene(s);
         return rc;
     }
@@ -284,14 +289,12 @@ int fl_scene_create(fl_ctx_t ctx, const fl_scene_desc *desc, fl_scene_t *out) {
         mat_h.assign(desc->tri_material, desc->tri_material + desc->n_triangles);
         for (auto &m : mat_h)
             if (m != fl::kNoMaterial && desc->materials[m].kind == FL_MAT_NONE) m = fl::kNoMaterial;
-        if ((rc = upload(ctx, &d_mat, mat_h.data(), desc->n_triangles))) {
-            (void)hipFree(d_verts); (void)hipFree(d_idx); (void)hipFree(d_light);
+        if ((rc = upload(ctx, &s->d_mat, mat_h.data(), desc->n_triangles))) {
             free_scene(s);
             return rc;
         }
     }
-    rc = flx::build_bvh(ctx, s, d_verts, d_idx, d_mat, d_light);
-    (void)hipFree(d_verts); (void)hipFree(d_idx); (void)hipFree(d_mat); (void)hipFree(d_light);
+    rc = flx::build_bvh(ctx, s, s->d_verts, s->d_idx, s->d_mat, s->d_light);
     if (rc) { free_scene(s); return rc; }
     if (s->has_textures) {
         // texels of every texture, each padded to 4 bytes; per-triangle uvs

@@ -196,6 +196,7 @@ struct BuildView {
     fl::Node8 *nodes;
     fl::TriRec *tris;
     uint32_t *counters;     // [0] nodes, [1] tris, [2] next work count
+    int width;              // children per wide node, 2..8 (the node format holds 8)
 };
 
 __device__ void child_box(const BuildView &v, uint32_t c, float lo[3], float hi[3]) {
@@ -243,7 +244,7 @@ __global__ void k_collapse(BuildView v, const uint32_t *__restrict__ work_in, ui
     } else {
         ch[cnt++] = v.left[bnode];
         ch[cnt++] = v.right[bnode];
-        while (cnt < 8) {
+        while (cnt < v.width) {
             int best = -1;
             float bestA = -1.f;
             for (int c = 0; c < cnt; ++c) {
@@ -492,6 +493,7 @@ int build_bvh(fl_ctx_t ctx, fl_scene_s *s, const float *d_verts, const uint32_t 
     v.verts = d_verts; v.idx = d_idx; v.mat = d_mat; v.light = d_light;
     v.vals = vals_b; v.tbox = tbox; v.left = left; v.right = right;
     v.nbox = nbox; v.ncount = ncount; v.nodes = nodes; v.tris = tris; v.counters = counters;
+    v.width = (int)s->bvh_width;
 
     uint32_t root = (n >= 2) ? 0u : (0u | kLeafBit);
     hipLaunchKernelGGL(k_set_root_work, dim3(1), dim3(1), 0, st, work_a, root, counters);
@@ -538,6 +540,20 @@ int build_bvh(fl_ctx_t ctx, fl_scene_s *s, const float *d_verts, const uint32_t 
     s->build_ms = ms;
 #undef ALLOC
     return FL_OK;
+}
+
+// Rebuilds the scene's BVH with `width` children per node from the build
+// inputs the scene keeps on the device.
+int rebuild_bvh(fl_ctx_t ctx, fl_scene_s *s, uint32_t width) {
+    if (width < 2 || width > 8) return set_error(ctx, FL_E_INVALID, "BVH width must be 2..8");
+    if (width == s->bvh_width) return FL_OK;
+    FL_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));   // no render still reads the old nodes
+    (void)hipFree(s->d_nodes);
+    (void)hipFree(s->d_tris);
+    s->d_nodes = nullptr;
+    s->d_tris = nullptr;
+    s->bvh_width = width;
+    return build_bvh(ctx, s, s->d_verts, s->d_idx, s->d_mat, s->d_light);
 }
 
 } // namespace flx

@@ -74,6 +74,10 @@ struct fl_scene_s {
     bool has_textures = false;      // some material reads an image texture
     fl::Node8 *d_nodes = nullptr;
     fl::TriRec *d_tris = nullptr;
+    uint32_t bvh_width = 8;           // children per wide node (autotuner knob)
+    // build inputs kept for rebuilds at another width (12 B per vertex, 20 B per triangle)
+    float *d_verts = nullptr;
+    uint32_t *d_idx = nullptr, *d_mat = nullptr, *d_light = nullptr;
     DevMaterial *d_mats = nullptr;
     DevLight *d_lights = nullptr;
     fl::StdParams *d_std = nullptr;   // StandardSurface lobe stacks (FL_MAT_STANDARD materials)
@@ -134,6 +138,8 @@ int set_error(fl_ctx_t ctx, int code, const std::string &msg);
 // BVH build (fl_build.hip)
 int build_bvh(fl_ctx_t ctx, fl_scene_s *s, const float *d_verts, const uint32_t *d_idx,
               const uint32_t *d_mat, const uint32_t *d_light);
+// the same build from the scene's kept inputs with `width` children per node
+int rebuild_bvh(fl_ctx_t ctx, fl_scene_s *s, uint32_t width);
 
 // spectral preprocessing (fl_spectra.cpp); s->h_std must be filled first
 void build_spectral_tables(fl_scene_s *s, const fl_scene_desc *desc, int mode,

@@ -1658,6 +1658,30 @@ int autotune_items(fl_ctx_t ctx, const HostItem *hi, uint32_t n, const fl_render
     if (rc != FL_OK) return rc;
     fl_render_params cur = base;
     double ms_cur = ms_default;
+    // BVH node width: every scene of the job rebuilt at each width, then left
+    // at the fastest
+    std::vector<fl_scene_s *> scenes;
+    for (const HostItem &h : items)
+        if (std::find(scenes.begin(), scenes.end(), h.scene) == scenes.end()) scenes.push_back(h.scene);
+    const uint32_t width0 = scenes[0]->bvh_width;
+    uint32_t width_best = width0;
+    auto rebuild_all = [&](uint32_t w) -> int {
+        for (fl_scene_s *sc : scenes) {
+            const int r = rebuild_bvh(ctx, sc, w);
+            if (r != FL_OK) return r;
+        }
+        return FL_OK;
+    };
+    for (uint32_t w : {8u, 6u, 4u}) {
+        bool same = true;
+        for (fl_scene_s *sc : scenes) same = same && sc->bvh_width == w;
+        if (same && w == width0) continue;   // the default probe measured it
+        if ((rc = rebuild_all(w)) != FL_OK) return rc;
+        double ms = 0.0;
+        if ((rc = probe(cur, &ms)) != FL_OK) return rc;
+        if (ms < ms_cur) { ms_cur = ms; width_best = w; }
+    }
+    if ((rc = rebuild_all(width_best)) != FL_OK) return rc;
     auto try_values = [&](auto set, const std::vector<uint32_t> &vals) -> int {
         fl_render_params keep = cur;
         for (uint32_t v : vals) {
@@ -1699,6 +1723,7 @@ int autotune_items(fl_ctx_t ctx, const HostItem *hi, uint32_t n, const fl_render
     out->ms_best = ms_cur;
     out->ms_default = ms_default;
     out->n_tried = tried;
+    out->node_width = width_best;
     return FL_OK;
 }
 

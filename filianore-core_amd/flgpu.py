@@ -80,7 +80,8 @@ class SceneDesc(C.Structure):
                 ("emission_spd", C.POINTER(C.c_float)), ("n_emission", C.c_uint32),
                 ("delta_lights", C.POINTER(DeltaLight)), ("n_delta_lights", C.c_uint32),
                 ("standard_surfaces", C.POINTER(StandardSurface)), ("n_standard_surfaces", C.c_uint32),
-                ("textures", C.POINTER(Texture)), ("n_textures", C.c_uint32), ("uvs", C.POINTER(C.c_float))]
+                ("textures", C.POINTER(Texture)), ("n_textures", C.c_uint32), ("uvs", C.POINTER(C.c_float)),
+                ("bvh_width", C.c_uint32)]
 
 
 class SceneInfo(C.Structure):
@@ -137,7 +138,7 @@ class TuneResult(C.Structure):
     _fields_ = [("max_paths", C.c_uint32), ("stack_depth", C.c_uint32), ("trav_waves", C.c_uint32),
                 ("trav_budget", C.c_uint32), ("shade_waves", C.c_uint32), ("sort_shading", C.c_uint32),
                 ("ms_best", C.c_double), ("ms_default", C.c_double), ("n_tried", C.c_uint32),
-                ("refill_lanes", C.c_uint32)]
+                ("refill_lanes", C.c_uint32), ("node_width", C.c_uint32)]
 
     def apply(self, params):
         """Copies the tuned values into fl_render_params."""
@@ -367,6 +368,7 @@ class GenSceneData:
             *[Texture(t.shape[1], t.shape[0], t.ctypes.data_as(C.POINTER(C.c_uint8))) for t in texs])
         d.textures = C.cast(self._tex, C.POINTER(Texture))
         d.n_textures = len(texs)
+        d.bvh_width = getattr(self, "bvh_width", 0)
         uvs = getattr(self, "uvs", None)
         if uvs is not None:
             self._keep_uv = np.ascontiguousarray(uvs, np.float32).reshape(-1, 2)

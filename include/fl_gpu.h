@@ -178,6 +178,10 @@ typedef struct {
      * (shapes/triangle.cpp:66-73); NULL: every hit has uv = (0, 0), as on the
      * reference's Embree path (bvh/bvh.cpp:69-71) */
     const float *uvs;
+    /* children per BVH node, 2..8 (0: 8). Nodes always use the 8-slot,
+     * 128-byte format; a smaller width leaves slots empty and makes the tree
+     * deeper. fl_autotune may rebuild the scene at another width. */
+    uint32_t bvh_width;
 } fl_scene_desc;
 
 typedef struct {
@@ -318,18 +322,18 @@ int fl_render_batch(fl_ctx_t ctx, const fl_batch_item *items, uint32_t n,
 
 /* ---- autotuner -------------------------------------------------------------- */
 /* Probe renders of the same job (the given spp, at most 64) choose, one knob
- * after the other, the values of fl_render_params that minimise device time:
- * paths in flight per pass (queue chunk), traversal variant (LDS stack depth x
- * minimum waves per SIMD), traversal steps between lane refills, refill
- * threshold, shading sort on/off, shading-kernel occupancy. Node width is not a knob: nodes are
- * 8-wide by construction (one 128-byte line). No counterpart in the
- * reference. */
+ * after the other, the values that minimise device time: BVH node width (the
+ * scenes are rebuilt, and left at the chosen width), paths in flight per pass
+ * (queue chunk), traversal variant (LDS stack depth x minimum waves per
+ * SIMD), traversal steps between lane refills, refill threshold, shading sort
+ * on/off, shading-kernel occupancy. No counterpart in the reference. */
 typedef struct {
     uint32_t max_paths, stack_depth, trav_waves, trav_budget, shade_waves;
     uint32_t sort_shading;        /* 1: per-block material sort on (FL_RENDER_NO_SORT clear) */
     double ms_best, ms_default;   /* device ms of one render with the chosen and the default values */
     uint32_t n_tried;             /* probe configurations measured */
     uint32_t refill_lanes;        /* traversal refill threshold */
+    uint32_t node_width;          /* BVH children per node the scenes now have */
 } fl_tune_result;
 
 int fl_autotune(fl_scene_t scene, const fl_camera *camera, const fl_render_params *params,

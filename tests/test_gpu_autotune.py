@@ -1,7 +1,8 @@
-"""The autotuner (fl_autotune): probe renders choose traversal, queue and
-shading knobs. It must return a valid configuration no slower than the
-default, and the knobs it sets other than the pass size must not change the
-image (they only reorder work)."""
+"""The autotuner (fl_autotune): probe renders choose BVH node width,
+traversal, queue and shading knobs. It must return a valid configuration no
+slower than the default, and the knobs it sets other than the pass size must
+not change the image (they only reorder work): every node width gives the
+same hits and the same film, bit for bit."""
 import numpy as np
 import pytest
 
@@ -20,6 +21,11 @@ def test_autotune_valid_and_image_preserving(ctx):
             r.ms_default, r.n_tried))
         assert r.n_tried >= 8
         assert 0 < r.ms_best <= r.ms_default
+        assert r.node_width in (4, 6, 8)
+        fresh = flgpu.GenSceneData(flgpu.FL_GEN_SPHERES, subdiv=5, width=192, height=128)
+        fresh.bvh_width = r.node_width
+        with ctx.scene(fresh) as sc2:
+            assert sc2.info().n_nodes == sc.info().n_nodes   # the scene was left at the chosen width
         assert r.stack_depth in (6, 8, 12, 16) and r.trav_budget in (8, 16, 32)
         base, _, _ = sc.render(p)
         q = flgpu.default_params(spp=16, depth=gen.depth, seed=4, flags=p.flags)
@@ -28,3 +34,37 @@ def test_autotune_valid_and_image_preserving(ctx):
         tuned, _, st = sc.render(q)
         assert st.nan_samples == 0
     assert np.array_equal(base.view(np.uint32), tuned.view(np.uint32))
+
+
+def test_node_widths_give_identical_hits_and_films(ctx):
+    from raysets import primary_rays, secondary_rays
+
+    films, hits, nodes = [], [], []
+    rays = None
+    for w in (8, 6, 4, 2):
+        gen = flgpu.GenSceneData(flgpu.FL_GEN_SPHERES, subdiv=3, width=64, height=48)
+        gen.bvh_width = w
+        with ctx.scene(gen) as sc:
+            nodes.append(sc.info().n_nodes)
+            if rays is None:   # primary rays and cosine bounces off their hits
+                prim = primary_rays(gen.camera, 128, 96)
+                t0, tri0, _ = sc.intersect(prim)
+                rays = np.concatenate([prim, secondary_rays(gen.triangles(), prim, t0, tri0, 20000)])
+            t, tri, uv = sc.intersect(rays)
+            hits.append((t.view(np.uint32).copy(), tri.copy(), uv.view(np.uint32).copy()))
+            f, _, st = sc.render(flgpu.default_params(spp=8, depth=gen.depth, seed=2, flags=flgpu.FL_RENDER_GRAPH))
+            assert st.nan_samples == 0
+            films.append(f)
+    assert nodes[0] < nodes[1] < nodes[2] < nodes[3]          # narrower nodes, more of them
+    assert (hits[0][1] != flgpu.FL_NO_HIT).mean() > 0.3
+    for k in range(1, 4):
+        for a, b in zip(hits[0], hits[k]):
+            assert np.array_equal(a, b), k
+        assert np.array_equal(films[0].view(np.uint32), films[k].view(np.uint32)), k
+
+
+def test_bad_node_width_is_rejected(ctx):
+    gen = flgpu.GenSceneData(flgpu.FL_GEN_CORNELL, width=8, height=8)
+    gen.bvh_width = 9
+    with pytest.raises(flgpu.FlError):
+        ctx.scene(gen)
