@@ -110,7 +110,7 @@ def batch_section(flgpu, ctx, n_scenes, tune):
             r.apply(p)
             tuned = {"max_paths": r.max_paths, "stack_depth": r.stack_depth, "trav_waves": r.trav_waves,
                      "trav_budget": r.trav_budget, "refill_lanes": r.refill_lanes, "shade_waves": r.shade_waves,
-                     "sort_shading": r.sort_shading, "node_width": r.node_width,
+                     "sort_shading": r.sort_shading, "node_width": r.node_width, "trav_prefetch": r.trav_prefetch,
                      "ms_default": round(r.ms_default, 2), "ms_tuned": round(r.ms_best, 2), "probes": r.n_tried}
         ctx.render_batch(entries, p, want_film=False)
         ts = time.perf_counter()

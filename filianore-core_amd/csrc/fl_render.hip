@@ -190,7 +190,7 @@ struct TraceSrc {
 
 // STACK: LDS short-stack entries per lane (deeper entries spill to global
 // memory); WAVES: minimum waves per SIMD asked of the register allocator.
-template <bool COUNT, int STACK, int WAVES>
+template <bool COUNT, int STACK, int WAVES, bool PREFETCH>
 __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WAVES))) k_trace(RenderArgs a, int b) {
     __shared__ unsigned long long stk[STACK * kBlock];
     const int cur = b & 1;
@@ -212,7 +212,7 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WAV
     TravCounters c;
     TraceSrc src{a.sq_o,    a.sq_d, a.sq_tgt, a.pend,    a.rq_o[cur], a.rq_d[cur], a.rq_sid[cur],
                  a.hits,    ns,     a.cap,    a.items,   a.st_item,   a.n_items,   a.sc, a.sq_occ};
-    trace_persistent<COUNT, true>(src, ns + ne, &a.counts[C_FETCH + cur], ls, a.trav_budget, a.refill_lanes, c);
+    trace_persistent<COUNT, true, PREFETCH>(src, ns + ne, &a.counts[C_FETCH + cur], ls, a.trav_budget, a.refill_lanes, c);
     if (COUNT) {
         atomicAdd(&a.stats[3], (unsigned long long)c.nodes);
         atomicAdd(&a.stats[4], (unsigned long long)c.tris);
@@ -222,18 +222,26 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WAV
 
 template <int S, int WV>
 void launch_trace_t(bool count, dim3 g, hipStream_t st, const RenderArgs &a, int j) {
-    if (count) hipLaunchKernelGGL((k_trace<true, S, WV>), g, dim3(kBlock), 0, st, a, j);
-    else hipLaunchKernelGGL((k_trace<false, S, WV>), g, dim3(kBlock), 0, st, a, j);
+    if (a.trav_prefetch) {
+        if (count) hipLaunchKernelGGL((k_trace<true, S, WV, true>), g, dim3(kBlock), 0, st, a, j);
+        else hipLaunchKernelGGL((k_trace<false, S, WV, true>), g, dim3(kBlock), 0, st, a, j);
+    } else {
+        if (count) hipLaunchKernelGGL((k_trace<true, S, WV, false>), g, dim3(kBlock), 0, st, a, j);
+        else hipLaunchKernelGGL((k_trace<false, S, WV, false>), g, dim3(kBlock), 0, st, a, j);
+    }
 }
 template <int S, int WV>
-const void *trace_fn() { return reinterpret_cast<const void *>(k_trace<false, S, WV>); }
+const void *trace_fn(bool prefetch) {
+    return prefetch ? reinterpret_cast<const void *>(k_trace<false, S, WV, true>)
+                    : reinterpret_cast<const void *>(k_trace<false, S, WV, false>);
+}
 
 // traversal variants (LDS stack entries, minimum waves per SIMD); the first
 // is the default, the rest are tuning choices
 struct TraceVariant {
     int stack, waves;
     void (*launch)(bool, dim3, hipStream_t, const RenderArgs &, int);
-    const void *(*fn)();
+    const void *(*fn)(bool);
 };
 const TraceVariant kTraceVariants[] = {
     {12, 1, launch_trace_t<12, 1>, trace_fn<12, 1>}, {8, 1, launch_trace_t<8, 1>, trace_fn<8, 1>},
@@ -1283,8 +1291,17 @@ int render_items(fl_ctx_t ctx, const HostItem *hi, uint32_t n_items, const fl_re
     for (const TraceVariant &v : kTraceVariants)
         if (v.stack == want_stack && v.waves == want_waves) tv = &v;
     if (!tv) return set_error(ctx, FL_E_INVALID, "fl_render: no traversal variant with this stack_depth/trav_waves");
-    int occ = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, tv->fn(), kBlock, 0) != hipSuccess || occ <= 0) occ = 4;
+    if (p->trav_prefetch == 1 || p->trav_prefetch == 2) {
+        a.trav_prefetch = p->trav_prefetch == 1 ? 1 : 0;
+    } else {   // auto: prefetch when the largest scene's rays are long (its BVH is deep)
+        uint32_t max_tris = 0;
+        for (uint32_t k = 0; k < n_items; ++k) max_tris = std::max(max_tris, hi[k].scene->n_tris);
+        a.trav_prefetch = max_tris >= kPrefetchTris ? 1 : 0;
+    }
+    int occ = 0;   // of the instantiation the launches use
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, tv->fn(a.trav_prefetch != 0), kBlock, 0) != hipSuccess ||
+        occ <= 0)
+        occ = 4;
     const uint32_t grid_pers = (uint32_t)(ctx->n_cu * occ);
     a.trav_budget = p->trav_budget ? (int)p->trav_budget : 16;
     a.refill_lanes = p->refill_lanes ? (int)std::min(64u, p->refill_lanes) : 16;
@@ -1707,6 +1724,7 @@ int autotune_items(fl_ctx_t ctx, const HostItem *hi, uint32_t n, const fl_render
         }
     if (rc == FL_OK) rc = try_values([](fl_render_params &q, uint32_t v) { q.trav_budget = v; }, {8u, 32u});
     if (rc == FL_OK) rc = try_values([](fl_render_params &q, uint32_t v) { q.refill_lanes = v; }, {1u, 32u});
+    if (rc == FL_OK) rc = try_values([](fl_render_params &q, uint32_t v) { q.trav_prefetch = v; }, {1u, 2u});
     if (rc == FL_OK)
         rc = try_values([](fl_render_params &q, uint32_t v) { q.flags = v ? (q.flags | FL_RENDER_NO_SORT) : q.flags; },
                         {1u});
@@ -1718,6 +1736,7 @@ int autotune_items(fl_ctx_t ctx, const HostItem *hi, uint32_t n, const fl_render
     out->trav_waves = cur.trav_waves;
     out->trav_budget = cur.trav_budget ? cur.trav_budget : 16u;
     out->refill_lanes = cur.refill_lanes ? cur.refill_lanes : 16u;
+    out->trav_prefetch = cur.trav_prefetch;
     out->shade_waves = cur.shade_waves;
     out->sort_shading = (cur.flags & FL_RENDER_NO_SORT) ? 0u : 1u;
     out->ms_best = ms_cur;

@@ -83,8 +83,10 @@ __device__ __forceinline__ void trav_init(TravState &s, v3 o, v3 d, float tmin, 
     s.sp = 0;
 }
 
-// Runs at most `budget` loop iterations; returns true when the ray is done
-// (state holds the closest hit, or with `any` the first accepted hit).
+// Runs at most `budget` loop iterations, and returns early (false) for all
+// lanes once `leave` lanes of the wave (0: none) have finished, so that those
+// start their next rays; returns true when the ray is done (state holds the
+// closest hit, or with `any` the first accepted hit).
 // `any` and `skip_lights` are per lane, so closest-hit and any-hit rays in one
 // wave run the same instruction stream.
 //
@@ -98,9 +100,12 @@ __device__ __forceinline__ void trav_init(TravState &s, v3 o, v3 d, float tmin, 
 // node tests (together 23% of the traversal time on the spheres workload).
 template <bool COUNT>
 __device__ __forceinline__ bool trav_step(const TraceScene &sc, TravState &s, const LaneStack &ls, int budget,
-                                          TravCounters &cnt, bool any, bool skip_lights) {
+                                          TravCounters &cnt, bool any, bool skip_lights, int leave = 0) {
     const float kLo = 1.0f - 0x1p-20f, kHi = 1.0f + 0x1p-20f;
+    // (signed: the lane counts are compared with a floor that may be negative)
+    const int floor_active = leave > 0 ? (int)__popcll(__ballot(1)) - leave : -1;
     for (int it = 0; it < budget; ++it) {
+        if (leave > 0 && it > 0 && (int)__popcll(__ballot(1)) <= floor_active) return false;   // enough lanes done
         if (s.t_hits) {
             uint32_t b = __ffs(s.t_hits) - 1;
             s.t_hits &= s.t_hits - 1u;
@@ -233,26 +238,91 @@ __device__ __forceinline__ bool traverse(const TraceScene &sc, v3 o, v3 d, float
     return s.best_id != kNoHit;
 }
 
-// Persistent traversal over n rays with per-lane refill.  Src provides
+// Persistent traversal over n rays with per-lane refill (the two modes are
+// separate instantiations, PREFETCH, so each gets its own register budget).
+// Src provides
 //   void load(uint32_t i, v3 &o, v3 &d, float &tmin, float &tmax, bool &any,
 //             TraceScene &sc) const     (sc: the BVH this ray traverses)
 //   void store(uint32_t i, const TravState &s) const
 // Lane g first takes ray g (no atomic); later rays come from `fetch` (which
-// must be 0 at launch) offset by the grid size, one atomic per wave refill.
-// A lane runs `budget` traversal steps between refill checks; idle lanes are
-// refilled together once `refill_lanes` of them (or all) are idle.
+// must be 0 at launch) offset by the grid size. Two refill modes:
+//  - prefetch: each lane holds its next ray already loaded, so a finished
+//    lane starts it at once; the traversal yields to do that once
+//    `refill_lanes` lanes of the wave have finished, and the next rays' loads
+//    overlap the traversal that follows. Indices come 64 at a time per wave
+//    (one atomic per 64 rays: a refill per finished lane would serialise
+//    every wave of the grid on that one counter). Best for long rays (large
+//    scenes), where the yields are rare against the traversal.
+//  - refill at the end of `budget` iterations, once `refill_lanes` lanes are
+//    idle (or all are), the refilled lanes waiting for their rays' loads.
+//    Best for short rays, where the yields would cost more than idle lanes.
 // All lanes of the wave must call this (it uses wave-wide ballots).
 // `any` is per ray: the first hit ends it (emitters skipped when ANY_SKIP).
-template <bool COUNT, bool ANY_SKIP, class Src>
+// scenes from this many triangles on traverse in prefetch mode by default
+constexpr uint32_t kPrefetchTris = 65536;
+
+template <bool COUNT, bool ANY_SKIP, bool PREFETCH, class Src>
 __device__ __forceinline__ void trace_persistent(const Src &src, uint32_t n, uint32_t *fetch, const LaneStack &ls,
                                                  int budget, int refill_lanes, TravCounters &cnt) {
     TraceScene sc{};
     TravState s;
     const uint32_t lane = __lane_id();
     const uint32_t nthreads = gridDim.x * blockDim.x;
-    uint32_t first = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t first = blockIdx.x * blockDim.x + threadIdx.x;
     int ray = -1;
     bool any = false;
+    if constexpr (!PREFETCH) {
+        // refill mode
+        bool exhausted = false;
+        if (first < n) {
+            ray = (int)first;
+            v3 o, d;
+            float tmin, tmax;
+            src.load(first, o, d, tmin, tmax, any, sc);
+            trav_init(s, o, d, tmin, tmax);
+        } else {
+            exhausted = true;
+        }
+        while (true) {
+            if (__ballot(ray >= 0) == 0ull && __ballot(!exhausted) == 0ull) break;
+            if (ray >= 0) {
+                const bool done = trav_step<COUNT>(sc, s, ls, budget, cnt, any, any && ANY_SKIP);
+                if (done) {
+                    src.store((uint32_t)ray, s);
+                    ray = -1;
+                }
+            }
+            // refill once at least refill_lanes lanes are idle (or all are): one
+            // atomic and one round of ray loads serve many lanes
+            const bool need = (ray < 0) && !exhausted;
+            const unsigned long long m = __ballot(need);
+            if (m && (__popcll(m) >= (unsigned)refill_lanes || __ballot(ray >= 0) == 0ull)) {
+                const int leader = __ffsll(m) - 1;
+                uint32_t base = 0;
+                if (lane == (uint32_t)leader) base = atomicAdd(fetch, (uint32_t)__popcll(m));
+                base = __shfl(base, leader);
+                if (need) {
+                    const uint32_t idx = nthreads + base + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
+                    if (idx >= n) {
+                        exhausted = true;
+                    } else {
+                        ray = (int)idx;
+                        v3 o, d;
+                        float tmin, tmax;
+                        src.load(idx, o, d, tmin, tmax, any, sc);
+                        trav_init(s, o, d, tmin, tmax);
+                    }
+                }
+            }
+        }
+        return;
+    }
+    // prefetch mode: the lane's next ray, loaded while the current one runs
+    int nxt = -1;
+    v3 no(0.f, 0.f, 0.f), nd(0.f, 0.f, 0.f);
+    float ntmin = 0.f, ntmax = 0.f;
+    bool nany = false;
+    TraceScene nsc{};
     bool exhausted = false;
     if (first < n) {
         ray = (int)first;
@@ -263,35 +333,51 @@ __device__ __forceinline__ void trace_persistent(const Src &src, uint32_t n, uin
     } else {
         exhausted = true;
     }
+    // the wave's pool of ray indices [pool_next, pool_end)
+    uint32_t pool_next = 0, pool_end = 0;
     while (true) {
-        if (__ballot(ray >= 0) == 0ull && __ballot(!exhausted) == 0ull) break;
-        if (ray >= 0) {
-            bool done = trav_step<COUNT>(sc, s, ls, budget, cnt, any, any && ANY_SKIP);
-            if (done) {
-                src.store((uint32_t)ray, s);
-                ray = -1;
+        const bool need = nxt < 0 && !exhausted;
+        const unsigned long long m = __ballot(need);
+        if (m) {
+            const uint32_t want = (uint32_t)__popcll(m);
+            const uint32_t avail = pool_end - pool_next;
+            const uint32_t chunk = 64u;   // >= want - avail (want <= 64)
+            uint32_t nbase = 0;
+            if (avail < want) {
+                const int leader = __ffsll(m) - 1;
+                if (lane == (uint32_t)leader) nbase = atomicAdd(fetch, chunk);
+                nbase = nthreads + __shfl(nbase, leader);
             }
-        }
-        // refill once at least refill_lanes lanes are idle (or all are): one
-        // atomic and one round of ray loads serve many lanes
-        bool need = (ray < 0) && !exhausted;
-        unsigned long long m = __ballot(need);
-        if (m && (__popcll(m) >= (unsigned)refill_lanes || __ballot(ray >= 0) == 0ull)) {
-            int leader = __ffsll(m) - 1;
-            uint32_t base = 0;
-            if (lane == (uint32_t)leader) base = atomicAdd(fetch, (uint32_t)__popcll(m));
-            base = __shfl(base, leader);
             if (need) {
-                uint32_t idx = nthreads + base + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
+                const uint32_t rank = (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
+                const uint32_t idx = rank < avail ? pool_next + rank : nbase + (rank - avail);
                 if (idx >= n) {
                     exhausted = true;
                 } else {
-                    ray = (int)idx;
-                    v3 o, d;
-                    float tmin, tmax;
-                    src.load(idx, o, d, tmin, tmax, any, sc);
-                    trav_init(s, o, d, tmin, tmax);
+                    nxt = (int)idx;
+                    src.load(idx, no, nd, ntmin, ntmax, nany, nsc);
                 }
+            }
+            if (avail < want) {
+                pool_next = nbase + (want - avail);
+                pool_end = nbase + chunk;
+            } else {
+                pool_next += want;
+            }
+        }
+        if (ray < 0 && nxt >= 0) {
+            ray = nxt;
+            nxt = -1;
+            any = nany;
+            sc = nsc;
+            trav_init(s, no, nd, ntmin, ntmax);
+        }
+        if (__ballot(ray >= 0) == 0ull) break;   // no ray left to start
+        if (ray >= 0) {
+            const bool done = trav_step<COUNT>(sc, s, ls, budget, cnt, any, any && ANY_SKIP, refill_lanes);
+            if (done) {
+                src.store((uint32_t)ray, s);
+                ray = -1;
             }
         }
     }

@@ -42,7 +42,7 @@ struct ApiSrc {
 
 __global__ void k_zero_fetch(uint32_t *p) { *p = 0u; }
 
-template <bool ANY, bool SKIP, bool COUNT>
+template <bool ANY, bool SKIP, bool COUNT, bool PREFETCH>
 __global__ void __launch_bounds__(kBlock) k_trace_api(fl::TraceScene sc, ApiSrc src, uint32_t n, uint2 *spill,
                                                       uint32_t *fetch, unsigned long long *counters) {
     __shared__ unsigned long long stk[kStack * kBlock];
@@ -54,7 +54,7 @@ __global__ void __launch_bounds__(kBlock) k_trace_api(fl::TraceScene sc, ApiSrc 
     ls.spill = spill + gtid;
     ls.spill_stride = gridDim.x * kBlock;
     fl::TravCounters c;
-    fl::trace_persistent<COUNT, SKIP>(src, n, fetch, ls, kBudget, 1, c);
+    fl::trace_persistent<COUNT, SKIP, PREFETCH>(src, n, fetch, ls, kBudget, 16, c);
     if (COUNT) {
         atomicAdd(&counters[0], (unsigned long long)c.nodes);
         atomicAdd(&counters[1], (unsigned long long)c.tris);
@@ -62,11 +62,12 @@ __global__ void __launch_bounds__(kBlock) k_trace_api(fl::TraceScene sc, ApiSrc 
     }
 }
 
-template <bool ANY, bool SKIP, bool COUNT>
+template <bool ANY, bool SKIP, bool COUNT, bool PREFETCH>
 int launch(fl_ctx_t ctx, int depth, fl::TraceScene sc, ApiSrc src, uint32_t n, unsigned long long *c) {
     int occ = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, reinterpret_cast<const void *>(k_trace_api<ANY, SKIP, COUNT>),
-                                                     kBlock, 0) != hipSuccess || occ <= 0)
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(
+            &occ, reinterpret_cast<const void *>(k_trace_api<ANY, SKIP, COUNT, PREFETCH>), kBlock, 0) != hipSuccess ||
+        occ <= 0)
         occ = 4;
     uint32_t grid = (uint32_t)(ctx->n_cu * occ);
     uint32_t need = (n + kBlock - 1) / kBlock;
@@ -83,8 +84,8 @@ int launch(fl_ctx_t ctx, int depth, fl::TraceScene sc, ApiSrc src, uint32_t n, u
     uint32_t *fetch = reinterpret_cast<uint32_t *>(ctx->scratch);
     uint2 *spill = reinterpret_cast<uint2 *>(reinterpret_cast<char *>(ctx->scratch) + 256);
     hipLaunchKernelGGL(k_zero_fetch, dim3(1), dim3(1), 0, ctx->stream, fetch);
-    hipLaunchKernelGGL((k_trace_api<ANY, SKIP, COUNT>), dim3(grid), dim3(kBlock), 0, ctx->stream, sc, src, n, spill,
-                       fetch, c);
+    hipLaunchKernelGGL((k_trace_api<ANY, SKIP, COUNT, PREFETCH>), dim3(grid), dim3(kBlock), 0, ctx->stream, sc, src, n,
+                       spill, fetch, c);
     return FL_OK;
 }
 
@@ -99,13 +100,22 @@ int trace_rays(fl_ctx_t ctx, fl_scene_s *s, const fl_ray *d_rays, fl_hit *d_hits
     ApiSrc src{d_rays, d_hits, (flags & FL_ANY_HIT) != 0, sc};
     const int depth = (int)s->depth;
     bool any = flags & FL_ANY_HIT, skip = flags & FL_SKIP_LIGHTS, cnt = (flags & FL_COUNTERS) && d_counters;
+    // the render path's default refill mode; counting runs use the refill mode
+    const bool pf = s->n_tris >= fl::kPrefetchTris;
     int rc;
-    if (!any) rc = cnt ? launch<false, false, true>(ctx, depth, sc, src, n, d_counters)
-                       : launch<false, false, false>(ctx, depth, sc, src, n, d_counters);
-    else if (skip) rc = cnt ? launch<true, true, true>(ctx, depth, sc, src, n, d_counters)
-                            : launch<true, true, false>(ctx, depth, sc, src, n, d_counters);
-    else rc = cnt ? launch<true, false, true>(ctx, depth, sc, src, n, d_counters)
-                  : launch<true, false, false>(ctx, depth, sc, src, n, d_counters);
+    if (cnt) {
+        if (!any) rc = launch<false, false, true, false>(ctx, depth, sc, src, n, d_counters);
+        else if (skip) rc = launch<true, true, true, false>(ctx, depth, sc, src, n, d_counters);
+        else rc = launch<true, false, true, false>(ctx, depth, sc, src, n, d_counters);
+    } else if (pf) {
+        if (!any) rc = launch<false, false, false, true>(ctx, depth, sc, src, n, d_counters);
+        else if (skip) rc = launch<true, true, false, true>(ctx, depth, sc, src, n, d_counters);
+        else rc = launch<true, false, false, true>(ctx, depth, sc, src, n, d_counters);
+    } else {
+        if (!any) rc = launch<false, false, false, false>(ctx, depth, sc, src, n, d_counters);
+        else if (skip) rc = launch<true, true, false, false>(ctx, depth, sc, src, n, d_counters);
+        else rc = launch<true, false, false, false>(ctx, depth, sc, src, n, d_counters);
+    }
     if (rc) return rc;
     FL_HIP_CHECK(ctx, hipGetLastError());
     return FL_OK;

@@ -11,6 +11,8 @@ import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libfl_gpu.so")
+# the struct layouts below are those of this version of include/fl_gpu.h
+ABI_VERSION = 4
 
 FL_OK = 0
 FL_E_INVALID, FL_E_HIP, FL_E_OOM, FL_E_UNSUPPORTED, FL_E_OVERFLOW, FL_E_NUMERIC = -1, -2, -3, -4, -5, -6
@@ -103,7 +105,7 @@ class RenderParams(C.Structure):
                 ("trav_budget", C.c_uint32), ("shade_waves", C.c_uint32), ("trav_waves", C.c_uint32),
                 ("guide", C.c_void_p),
                 ("guide_mix", C.c_float),
-                ("guide_train_fraction", C.c_float), ("refill_lanes", C.c_uint32), ("pad1", C.c_uint32)]
+                ("guide_train_fraction", C.c_float), ("refill_lanes", C.c_uint32), ("trav_prefetch", C.c_uint32)]
 
 
 class Film(C.Structure):
@@ -138,13 +140,14 @@ class TuneResult(C.Structure):
     _fields_ = [("max_paths", C.c_uint32), ("stack_depth", C.c_uint32), ("trav_waves", C.c_uint32),
                 ("trav_budget", C.c_uint32), ("shade_waves", C.c_uint32), ("sort_shading", C.c_uint32),
                 ("ms_best", C.c_double), ("ms_default", C.c_double), ("n_tried", C.c_uint32),
-                ("refill_lanes", C.c_uint32), ("node_width", C.c_uint32)]
+                ("refill_lanes", C.c_uint32), ("node_width", C.c_uint32), ("trav_prefetch", C.c_uint32)]
 
     def apply(self, params):
         """Copies the tuned values into fl_render_params."""
         params.max_paths, params.stack_depth, params.trav_waves = self.max_paths, self.stack_depth, self.trav_waves
         params.trav_budget, params.shade_waves = self.trav_budget, self.shade_waves
         params.refill_lanes = self.refill_lanes
+        params.trav_prefetch = self.trav_prefetch
         if self.sort_shading:
             params.flags &= ~FL_RENDER_NO_SORT
         else:
@@ -255,6 +258,9 @@ def lib():
         L.fl_scene_file_cameras.argtypes = [vp, C.POINTER(C.POINTER(Camera))]
         L.fl_scene_file_cameras.restype = C.c_uint32
         L.fl_scene_file_free.argtypes = [vp]
+        if L.fl_abi_version() != ABI_VERSION:
+            raise RuntimeError("libfl_gpu.so has ABI %d, these bindings need %d; rebuild it"
+                               % (L.fl_abi_version(), ABI_VERSION))
         _lib = L
     return _lib
 

@@ -36,7 +36,7 @@
 extern "C" {
 #endif
 
-#define FL_ABI_VERSION 3
+#define FL_ABI_VERSION 4
 
 /* ---- status codes ------------------------------------------------------ */
 enum {
@@ -248,7 +248,10 @@ typedef struct {
     float guide_mix;             /* proposal probability of the guide (0.5) */
     float guide_train_fraction;  /* with TRAIN_GUIDE: fraction of spp used to train */
     uint32_t refill_lanes;       /* traversal: idle lanes of a wave that trigger a refill (0: auto) */
-    uint32_t pad1;
+    uint32_t trav_prefetch;      /* traversal refill mode: 1 = each lane's next ray loaded ahead, the
+                                    wave yields once refill_lanes lanes are done; 2 = refill after
+                                    trav_budget iterations; 0 = auto (1 for scenes from 65,536
+                                    triangles on, whose rays are long) */
 } fl_render_params;
 
 /* Film: per-pixel sums of linear CIE XYZ (3 floats per pixel) and of its
@@ -334,6 +337,7 @@ typedef struct {
     uint32_t n_tried;             /* probe configurations measured */
     uint32_t refill_lanes;        /* traversal refill threshold */
     uint32_t node_width;          /* BVH children per node the scenes now have */
+    uint32_t trav_prefetch;       /* traversal refill mode (fl_render_params.trav_prefetch) */
 } fl_tune_result;
 
 int fl_autotune(fl_scene_t scene, const fl_camera *camera, const fl_render_params *params,
