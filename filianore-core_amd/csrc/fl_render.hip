@@ -40,15 +40,26 @@ constexpr int C_EXT = 0;    // [0],[1]: extension queue sizes by bounce parity
 constexpr int C_SH = 2;     // [2],[3]: shadow queue sizes by the parity of the bounce that wrote them
 constexpr int C_FETCH = 4;  // [4],[5]: trace refill cursors by bounce parity
 
-__device__ __forceinline__ uint32_t wave_append(bool want, uint32_t *counter) {
-    unsigned long long mask = __ballot(want);
-    if (mask == 0ull) return 0u;
-    uint32_t lane = __lane_id();
-    int leader = __ffsll(mask) - 1;
-    uint32_t base = 0;
-    if (lane == (uint32_t)leader) base = atomicAdd(counter, (uint32_t)__popcll(mask));
-    base = __shfl(base, leader);
-    return base + (uint32_t)__popcll(mask & ((1ull << lane) - 1ull));
+// Reserves `wave_count` queue slots for each wave of the block with one
+// atomic per block and returns the calling wave's first slot. Every thread of
+// the block must call it; `s` (kBlock/64 + 1 words of LDS) may be reused by
+// the next call only after another barrier of the block.
+__device__ __forceinline__ uint32_t block_reserve(uint32_t wave_count, uint32_t *counter, uint32_t *s) {
+    constexpr uint32_t kWaves = kBlock / 64;
+    const uint32_t w = threadIdx.x >> 6;
+    if (__lane_id() == 0) s[w] = wave_count;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t tot = 0;
+        for (uint32_t k = 0; k < kWaves; ++k) {
+            const uint32_t c = s[k];
+            s[k] = tot;
+            tot += c;
+        }
+        s[kWaves] = tot ? atomicAdd(counter, tot) : 0u;
+    }
+    __syncthreads();
+    return s[kWaves] + s[w];
 }
 
 __device__ __forceinline__ v3 ld3(const float *p) { return v3(p[0], p[1], p[2]); }
@@ -305,6 +316,7 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3, 8)))
     const int q = threadIdx.x;
     const uint32_t n = a.counts[C_EXT + (b & 1)];
     const uint32_t nchunks = (n + 63) / 64;
+    unsigned long long n_queries = 0;   // this block's count, one atomic at the end
     for (uint32_t ch = blockIdx.x; ch < nchunks; ch += gridDim.x) {
         const uint32_t i = ch * 64u + q;
         float pos01[3] = {0.f, 0.f, 0.f}, lam01 = 0.f, rough = 0.f;
@@ -333,8 +345,7 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3, 8)))
                 }
             }
         }
-        const unsigned long long wm = __ballot(want);
-        if (q == 0 && wm) atomicAdd(&a.stats[6], (unsigned long long)__popcll(wm));
+        n_queries += (unsigned long long)__popcll(__ballot(want));
         encode_row_bf16(sT, q, want, pos01, n_in, lam01, rough);
         __syncthreads();
         for (int k = 0; k < 4; ++k) {
@@ -360,6 +371,7 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3, 8)))
         }
         __syncthreads();
     }
+    if (q == 0 && n_queries) atomicAdd(&a.stats[6], n_queries);
 }
 
 // Online training runs in kTrainRounds passes of doubling size over the
@@ -582,6 +594,9 @@ template <int WAVES, bool STD>
 __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WAVES, 8))) k_shade(RenderArgs a, int b) {
     __shared__ uint32_t s_cnt[kSortKeys];
     __shared__ uint16_t s_perm[kBlock];
+    // per-block queue reservations: [0] shadow rays, [1] continuations,
+    // [2] training records
+    __shared__ uint32_t s_app[3][kBlock / 64 + 1];
     // a textured standard material's kd, ks, kt, sheen and coat at the vertex's
     // four wavelengths, one column per thread (20 KB: registers would hold
     // them across the whole shading of every vertex)
@@ -740,10 +755,13 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WAV
                           (STD && sd && S.tex) ? sd->tex : nullptr, s_tv + (STD ? threadIdx.x : 0u)};
         const bool do_nee = mat_hit && S.n_lights > 0 &&
                             (sd ? std_num_components(*sd, kNonSpecular) > 0 : !is_specular_kind(bs_.kind));
+        // both samples' shadow rays, appended together after the second
+        float4 sh_ov[2], sh_dv[2], occv[2];
+        bool okv[2], deltav[2];
 #pragma unroll
         for (int j = 0; j < 2; ++j) {
             float contrib[4] = {0.f, 0.f, 0.f, 0.f}, occ[4] = {0.f, 0.f, 0.f, 0.f};
-            float4 sh_o, sh_d;
+            float4 sh_o = make_float4(0.f, 0.f, 0.f, 0.f), sh_d = sh_o;
             bool ok = false, nee_delta = false;
             const int nslots = S.nee_all ? (int)S.n_lights : 1;
             if (do_nee && j < nslots) {
@@ -761,12 +779,26 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WAV
                 if (flags & PF_SECTERM) contrib[1] = contrib[2] = contrib[3] = occ[1] = occ[2] = occ[3] = 0.f;
                 a.pend[(size_t)j * a.cap + sid] = make_float4(contrib[0], contrib[1], contrib[2], contrib[3]);
             }
-            uint32_t ss = wave_append(ok, &a.counts[C_SH + cur]);
-            if (ok) {
-                a.sq_o[ss] = sh_o;
-                a.sq_d[ss] = sh_d;
-                a.sq_tgt[ss] = ((uint32_t)j * a.cap + sid) | (nee_delta ? kVisClosest : 0u);
-                if (a.sq_occ) a.sq_occ[ss] = make_float4(occ[0], occ[1], occ[2], occ[3]);
+            sh_ov[j] = sh_o;
+            sh_dv[j] = sh_d;
+            occv[j] = make_float4(occ[0], occ[1], occ[2], occ[3]);
+            okv[j] = ok;
+            deltav[j] = nee_delta;
+        }
+        {
+            const unsigned long long m0 = __ballot(okv[0]), m1 = __ballot(okv[1]);
+            const unsigned long long lt = (1ull << __lane_id()) - 1ull;
+            const uint32_t c0 = (uint32_t)__popcll(m0);
+            const uint32_t base = block_reserve(c0 + (uint32_t)__popcll(m1), &a.counts[C_SH + cur], s_app[0]);
+#pragma unroll
+            for (int j = 0; j < 2; ++j) {
+                if (okv[j]) {
+                    const uint32_t ss = base + (j ? c0 + (uint32_t)__popcll(m1 & lt) : (uint32_t)__popcll(m0 & lt));
+                    a.sq_o[ss] = sh_ov[j];
+                    a.sq_d[ss] = sh_dv[j];
+                    a.sq_tgt[ss] = ((uint32_t)j * a.cap + sid) | (deltav[j] ? kVisClosest : 0u);
+                    if (a.sq_occ) a.sq_occ[ss] = occv[j];
+                }
             }
         }
         // -------- phase 3: BSDF sample (continuation and MIS BSDF ray), RR
@@ -892,7 +924,9 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WAV
         }
         // -------- training record of this vertex (training passes only)
         if (a.recs) {
-            const uint32_t ridx = wave_append(rec_want, a.rec_count);
+            const unsigned long long mr = __ballot(rec_want);
+            const uint32_t ridx = block_reserve((uint32_t)__popcll(mr), a.rec_count, s_app[2]) +
+                                  (uint32_t)__popcll(mr & ((1ull << __lane_id()) - 1ull));
             if (rec_want && ridx < a.rec_cap) {
                 float *R = a.recs + (size_t)ridx * 16;
                 float pos01[3], lam01, rough;
@@ -909,8 +943,10 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WAV
                 a.st_rec[sid] = ridx + 1u;
             }
         }
-        // -------- compaction of the continuation: one atomic per wave
-        uint32_t slot = wave_append(cont, &a.counts[C_EXT + nxt]);
+        // -------- compaction of the continuation: one atomic per block
+        const unsigned long long mc = __ballot(cont);
+        const uint32_t slot = block_reserve((uint32_t)__popcll(mc), &a.counts[C_EXT + nxt], s_app[1]) +
+                              (uint32_t)__popcll(mc & ((1ull << __lane_id()) - 1ull));
         if (cont) {
             a.rq_o[nxt][slot] = make_float4(no.x, no.y, no.z, kShadowEps);
             a.rq_d[nxt][slot] = make_float4(nd.x, nd.y, nd.z, kInf);
@@ -1191,6 +1227,7 @@ int render_items(fl_ctx_t ctx, const HostItem *hi, uint32_t n_items, const fl_re
         if ((uint64_t)hi[k].cam.width * hi[k].cam.height > (1u << 26))
             return set_error(ctx, FL_E_INVALID, "fl_render: image larger than 2^26 pixels");
     }
+    if (p->trav_prefetch > 2) return set_error(ctx, FL_E_INVALID, "fl_render: trav_prefetch must be 0, 1 or 2");
     fl_guide_s *guide = (p->flags & FL_RENDER_GUIDED) ? p->guide : nullptr;
     if ((p->flags & FL_RENDER_GUIDED) && !guide) return set_error(ctx, FL_E_INVALID, "FL_RENDER_GUIDED needs params.guide");
     if (guide && guide->ctx != ctx) return set_error(ctx, FL_E_INVALID, "guide belongs to another context");
@@ -1402,13 +1439,17 @@ int render_items(fl_ctx_t ctx, const HostItem *hi, uint32_t n_items, const fl_re
     bool has_std = false;
     for (uint32_t k = 0; k < n_items; ++k)
         has_std = has_std || hi[k].scene->has_standard || hi[k].scene->has_textures;
-    const int variant = (counters ? 1 : 0) | (p->shade_waves >= 4 ? 2 : 0) | ((int)(tv - kTraceVariants) << 2) |
-                        (has_std ? 1 << 16 : 0) | (p->shade_waves == 3 ? 1 << 17 : 0);
+    // shading waves per SIMD, 0 = automatic: 3 for the plain kernel (168
+    // VGPRs, 2 spilled), 2 for the standard-surface kernel (346 spilled at 3)
+    const uint32_t shade_waves = p->shade_waves ? p->shade_waves : (has_std ? 2u : 3u);
+    const int variant = (counters ? 1 : 0) | (shade_waves >= 4 ? 2 : 0) | ((int)(tv - kTraceVariants) << 2) |
+                        (has_std ? 1 << 16 : 0) | (shade_waves == 3 ? 1 << 17 : 0);
     const uint32_t infer_grid = (uint32_t)ctx->n_cu * 16;   // one-wave blocks of 10.5 KB LDS
     auto launch_shade = [&](const RenderArgs &args, dim3 g, int j, bool with_infer = true) {
         if (with_infer && args.guided) hipLaunchKernelGGL(k_guide_infer, dim3(infer_grid), dim3(64), 0, st, args, j);
         if (has_std) {
             if (variant & 2) hipLaunchKernelGGL((k_shade<4, true>), g, dim3(kBlock), 0, st, args, j);
+            else if (variant & (1 << 17)) hipLaunchKernelGGL((k_shade<3, true>), g, dim3(kBlock), 0, st, args, j);
             else hipLaunchKernelGGL((k_shade<2, true>), g, dim3(kBlock), 0, st, args, j);
         } else {
             if (variant & 2) hipLaunchKernelGGL((k_shade<4, false>), g, dim3(kBlock), 0, st, args, j);
@@ -1728,7 +1769,7 @@ int autotune_items(fl_ctx_t ctx, const HostItem *hi, uint32_t n, const fl_render
     if (rc == FL_OK)
         rc = try_values([](fl_render_params &q, uint32_t v) { q.flags = v ? (q.flags | FL_RENDER_NO_SORT) : q.flags; },
                         {1u});
-    if (rc == FL_OK) rc = try_values([](fl_render_params &q, uint32_t v) { q.shade_waves = v; }, {3u, 4u});
+    if (rc == FL_OK) rc = try_values([](fl_render_params &q, uint32_t v) { q.shade_waves = v; }, {2u, 3u, 4u});
     if (rc != FL_OK) return rc;
     memset(out, 0, sizeof(*out));
     out->max_paths = cur.max_paths;

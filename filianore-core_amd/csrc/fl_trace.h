@@ -238,21 +238,49 @@ __device__ __forceinline__ bool traverse(const TraceScene &sc, v3 o, v3 d, float
     return s.best_id != kNoHit;
 }
 
+// The next ray index for each lane with `need`, from the wave's pool of
+// indices [pool_next, pool_end), which is refilled kPoolChunk at a time from
+// `fetch` (offset by nthreads: lane g of the grid starts with ray g). One
+// atomic per chunk: every wave of the grid contends for that one counter, so
+// an atomic per refill of a few lanes serialises the grid on it. Call with
+// the whole wave; the pool bounds are wave-uniform.
+constexpr uint32_t kPoolChunk = 64;   // >= 64: one refill serves the whole wave
+__device__ __forceinline__ uint32_t pool_take(bool need, uint32_t &pool_next, uint32_t &pool_end, uint32_t *fetch,
+                                              uint32_t nthreads) {
+    const unsigned long long m = __ballot(need);
+    const uint32_t lane = __lane_id();
+    const uint32_t want = (uint32_t)__popcll(m);
+    const uint32_t avail = pool_end - pool_next;
+    uint32_t nbase = 0;
+    if (avail < want) {
+        const int leader = __ffsll(m) - 1;
+        if (lane == (uint32_t)leader) nbase = atomicAdd(fetch, kPoolChunk);
+        nbase = nthreads + __shfl(nbase, leader);
+    }
+    const uint32_t rank = (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
+    const uint32_t idx = rank < avail ? pool_next + rank : nbase + (rank - avail);
+    if (avail < want) {
+        pool_next = nbase + (want - avail);
+        pool_end = nbase + kPoolChunk;
+    } else {
+        pool_next += want;
+    }
+    return idx;
+}
+
 // Persistent traversal over n rays with per-lane refill (the two modes are
 // separate instantiations, PREFETCH, so each gets its own register budget).
 // Src provides
 //   void load(uint32_t i, v3 &o, v3 &d, float &tmin, float &tmax, bool &any,
 //             TraceScene &sc) const     (sc: the BVH this ray traverses)
 //   void store(uint32_t i, const TravState &s) const
-// Lane g first takes ray g (no atomic); later rays come from `fetch` (which
-// must be 0 at launch) offset by the grid size. Two refill modes:
+// Lane g first takes ray g (no atomic); later rays come from the wave's pool
+// (pool_take; `fetch` must be 0 at launch). Two refill modes:
 //  - prefetch: each lane holds its next ray already loaded, so a finished
 //    lane starts it at once; the traversal yields to do that once
 //    `refill_lanes` lanes of the wave have finished, and the next rays' loads
-//    overlap the traversal that follows. Indices come 64 at a time per wave
-//    (one atomic per 64 rays: a refill per finished lane would serialise
-//    every wave of the grid on that one counter). Best for long rays (large
-//    scenes), where the yields are rare against the traversal.
+//    overlap the traversal that follows. Best for long rays (large scenes),
+//    where the yields are rare against the traversal.
 //  - refill at the end of `budget` iterations, once `refill_lanes` lanes are
 //    idle (or all are), the refilled lanes waiting for their rays' loads.
 //    Best for short rays, where the yields would cost more than idle lanes.
@@ -260,53 +288,50 @@ __device__ __forceinline__ bool traverse(const TraceScene &sc, v3 o, v3 d, float
 // `any` is per ray: the first hit ends it (emitters skipped when ANY_SKIP).
 // scenes from this many triangles on traverse in prefetch mode by default
 constexpr uint32_t kPrefetchTris = 65536;
+constexpr uint32_t kNoRay = ~0u;
 
 template <bool COUNT, bool ANY_SKIP, bool PREFETCH, class Src>
 __device__ __forceinline__ void trace_persistent(const Src &src, uint32_t n, uint32_t *fetch, const LaneStack &ls,
                                                  int budget, int refill_lanes, TravCounters &cnt) {
     TraceScene sc{};
     TravState s;
-    const uint32_t lane = __lane_id();
     const uint32_t nthreads = gridDim.x * blockDim.x;
     const uint32_t first = blockIdx.x * blockDim.x + threadIdx.x;
-    int ray = -1;
+    uint32_t ray = kNoRay;
     bool any = false;
+    bool exhausted = false;
+    if (first < n) {
+        ray = first;
+        v3 o, d;
+        float tmin, tmax;
+        src.load(first, o, d, tmin, tmax, any, sc);
+        trav_init(s, o, d, tmin, tmax);
+    } else {
+        exhausted = true;
+    }
+    uint32_t pool_next = 0, pool_end = 0;   // the wave's pool of ray indices
     if constexpr (!PREFETCH) {
         // refill mode
-        bool exhausted = false;
-        if (first < n) {
-            ray = (int)first;
-            v3 o, d;
-            float tmin, tmax;
-            src.load(first, o, d, tmin, tmax, any, sc);
-            trav_init(s, o, d, tmin, tmax);
-        } else {
-            exhausted = true;
-        }
         while (true) {
-            if (__ballot(ray >= 0) == 0ull && __ballot(!exhausted) == 0ull) break;
-            if (ray >= 0) {
+            if (__ballot(ray != kNoRay) == 0ull && __ballot(!exhausted) == 0ull) break;
+            if (ray != kNoRay) {
                 const bool done = trav_step<COUNT>(sc, s, ls, budget, cnt, any, any && ANY_SKIP);
                 if (done) {
-                    src.store((uint32_t)ray, s);
-                    ray = -1;
+                    src.store(ray, s);
+                    ray = kNoRay;
                 }
             }
             // refill once at least refill_lanes lanes are idle (or all are): one
-            // atomic and one round of ray loads serve many lanes
-            const bool need = (ray < 0) && !exhausted;
+            // round of ray loads serves many lanes
+            const bool need = (ray == kNoRay) && !exhausted;
             const unsigned long long m = __ballot(need);
-            if (m && (__popcll(m) >= (unsigned)refill_lanes || __ballot(ray >= 0) == 0ull)) {
-                const int leader = __ffsll(m) - 1;
-                uint32_t base = 0;
-                if (lane == (uint32_t)leader) base = atomicAdd(fetch, (uint32_t)__popcll(m));
-                base = __shfl(base, leader);
+            if (m && (__popcll(m) >= (unsigned)refill_lanes || __ballot(ray != kNoRay) == 0ull)) {
+                const uint32_t idx = pool_take(need, pool_next, pool_end, fetch, nthreads);
                 if (need) {
-                    const uint32_t idx = nthreads + base + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
                     if (idx >= n) {
                         exhausted = true;
                     } else {
-                        ray = (int)idx;
+                        ray = idx;
                         v3 o, d;
                         float tmin, tmax;
                         src.load(idx, o, d, tmin, tmax, any, sc);
@@ -318,66 +343,37 @@ __device__ __forceinline__ void trace_persistent(const Src &src, uint32_t n, uin
         return;
     }
     // prefetch mode: the lane's next ray, loaded while the current one runs
-    int nxt = -1;
+    uint32_t nxt = kNoRay;
     v3 no(0.f, 0.f, 0.f), nd(0.f, 0.f, 0.f);
     float ntmin = 0.f, ntmax = 0.f;
     bool nany = false;
     TraceScene nsc{};
-    bool exhausted = false;
-    if (first < n) {
-        ray = (int)first;
-        v3 o, d;
-        float tmin, tmax;
-        src.load(first, o, d, tmin, tmax, any, sc);
-        trav_init(s, o, d, tmin, tmax);
-    } else {
-        exhausted = true;
-    }
-    // the wave's pool of ray indices [pool_next, pool_end)
-    uint32_t pool_next = 0, pool_end = 0;
     while (true) {
-        const bool need = nxt < 0 && !exhausted;
-        const unsigned long long m = __ballot(need);
-        if (m) {
-            const uint32_t want = (uint32_t)__popcll(m);
-            const uint32_t avail = pool_end - pool_next;
-            const uint32_t chunk = 64u;   // >= want - avail (want <= 64)
-            uint32_t nbase = 0;
-            if (avail < want) {
-                const int leader = __ffsll(m) - 1;
-                if (lane == (uint32_t)leader) nbase = atomicAdd(fetch, chunk);
-                nbase = nthreads + __shfl(nbase, leader);
-            }
+        const bool need = nxt == kNoRay && !exhausted;
+        if (__ballot(need)) {
+            const uint32_t idx = pool_take(need, pool_next, pool_end, fetch, nthreads);
             if (need) {
-                const uint32_t rank = (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
-                const uint32_t idx = rank < avail ? pool_next + rank : nbase + (rank - avail);
                 if (idx >= n) {
                     exhausted = true;
                 } else {
-                    nxt = (int)idx;
+                    nxt = idx;
                     src.load(idx, no, nd, ntmin, ntmax, nany, nsc);
                 }
             }
-            if (avail < want) {
-                pool_next = nbase + (want - avail);
-                pool_end = nbase + chunk;
-            } else {
-                pool_next += want;
-            }
         }
-        if (ray < 0 && nxt >= 0) {
+        if (ray == kNoRay && nxt != kNoRay) {
             ray = nxt;
-            nxt = -1;
+            nxt = kNoRay;
             any = nany;
             sc = nsc;
             trav_init(s, no, nd, ntmin, ntmax);
         }
-        if (__ballot(ray >= 0) == 0ull) break;   // no ray left to start
-        if (ray >= 0) {
+        if (__ballot(ray != kNoRay) == 0ull) break;   // no ray left to start
+        if (ray != kNoRay) {
             const bool done = trav_step<COUNT>(sc, s, ls, budget, cnt, any, any && ANY_SKIP, refill_lanes);
             if (done) {
-                src.store((uint32_t)ray, s);
-                ray = -1;
+                src.store(ray, s);
+                ray = kNoRay;
             }
         }
     }

@@ -241,7 +241,8 @@ typedef struct {
     uint32_t max_paths;          /* paths in flight per pass (0: auto) */
     uint32_t stack_depth;        /* LDS short-stack entries per lane (0: auto = 12) */
     uint32_t trav_budget;        /* traversal iterations between lane refills (0: auto) */
-    uint32_t shade_waves;        /* shading kernel variant: min waves/SIMD (2, 3 or 4; 0: 2) */
+    uint32_t shade_waves;        /* shading kernel variant: min waves/SIMD (2, 3 or 4; 0: 3, or 2 with
+                                    standard surfaces or textures) */
     uint32_t trav_waves;         /* traversal variant: min waves/SIMD (0: auto); valid
                                     (stack_depth, trav_waves): (12,0) (8,0) (8,6) (8,8) (6,8) (16,0) */
     fl_guide_t guide;            /* guiding network when FL_RENDER_GUIDED */
