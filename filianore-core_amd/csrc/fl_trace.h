@@ -238,35 +238,69 @@ __device__ __forceinline__ bool traverse(const TraceScene &sc, v3 o, v3 d, float
     return s.best_id != kNoHit;
 }
 
-// The next ray index for each lane with `need`, from the wave's pool of
-// indices [pool_next, pool_end), which is refilled kPoolChunk at a time from
-// `fetch` (offset by nthreads: lane g of the grid starts with ray g). One
-// atomic per chunk: every wave of the grid contends for that one counter, so
-// an atomic per refill of a few lanes serialises the grid on it. Call with
-// the whole wave; the pool bounds are wave-uniform.
-constexpr uint32_t kPoolChunk = 64;   // >= 64: one refill serves the whole wave
-__device__ __forceinline__ uint32_t pool_take(bool need, uint32_t &pool_next, uint32_t &pool_end, uint32_t *fetch,
-                                              uint32_t nthreads) {
-    const unsigned long long m = __ballot(need);
-    const uint32_t lane = __lane_id();
-    const uint32_t want = (uint32_t)__popcll(m);
-    const uint32_t avail = pool_end - pool_next;
-    uint32_t nbase = 0;
-    if (avail < want) {
-        const int leader = __ffsll(m) - 1;
-        if (lane == (uint32_t)leader) nbase = atomicAdd(fetch, kPoolChunk);
-        nbase = nthreads + __shfl(nbase, leader);
+// Ray indices of a persistent traversal, handed out through a per-wave pool
+// [pool_next, pool_end) from which the wave's lanes take consecutive indices
+// as they need them (neighbouring queue entries are coherent rays). Wave w of
+// the grid (W waves) first takes the 64 rays of chunk w, then, for a static
+// share of the rays (static_share256/256), the chunks w + W, w + 2W, ... with
+// no atomic; the rest, [dyn_base, n), comes from `fetch` (0 at launch)
+// kChunk rays at a time. Every wave of the grid contends for that one
+// counter, whose atomics serialise at the device's coherence point: at one
+// atomic per 64 rays they took more time than the traversal's own memory
+// traffic. Drawing from the shared counter keeps all waves on one advancing
+// window of the queue, which large scenes need for their BVH to stay in the
+// caches; static chunks let waves drift apart but cost no atomics, which
+// serves small, cache-resident scenes.
+constexpr uint32_t kChunk = 256;
+struct RayCursor {
+    uint32_t n, nwaves, static_chunks, dyn_base;
+    uint32_t next_static;           // the wave's next static chunk
+    uint32_t pool_next, pool_end;   // wave-uniform
+    uint32_t *fetch;
+    __device__ __forceinline__ RayCursor(uint32_t n_, uint32_t *fetch_, uint32_t static_share256)
+        : n(n_), fetch(fetch_) {
+        const uint32_t nthreads = gridDim.x * blockDim.x;
+        nwaves = nthreads / 64u;
+        const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) / 64u;
+        const uint32_t rounds = (uint32_t)((uint64_t)n * static_share256 / 256u / nthreads);
+        static_chunks = rounds * nwaves;
+        dyn_base = rounds ? rounds * nthreads : nthreads;
+        pool_next = wave * 64u;             // chunk w first, in every case
+        pool_end = pool_next + 64u;
+        next_static = wave + nwaves;
     }
-    const uint32_t rank = (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
-    const uint32_t idx = rank < avail ? pool_next + rank : nbase + (rank - avail);
-    if (avail < want) {
-        pool_next = nbase + (want - avail);
-        pool_end = nbase + kPoolChunk;
-    } else {
-        pool_next += want;
+    // the next ray index of each lane with `need` (>= n: none left); call
+    // with the whole wave
+    __device__ __forceinline__ uint32_t take(bool need) {
+        const unsigned long long m = __ballot(need);
+        if (m == 0ull) return 0u;
+        const uint32_t lane = __lane_id();
+        const uint32_t want = (uint32_t)__popcll(m);
+        const uint32_t avail = pool_end - pool_next;
+        uint32_t nbase = 0, chunk = 0;
+        if (avail < want) {
+            if (next_static < static_chunks) {
+                nbase = next_static * 64u;
+                chunk = 64u;
+                next_static += nwaves;
+            } else {
+                const int leader = __ffsll(m) - 1;
+                if (lane == (uint32_t)leader) nbase = atomicAdd(fetch, kChunk);
+                nbase = dyn_base + __shfl(nbase, leader);
+                chunk = kChunk;
+            }
+        }
+        const uint32_t rank = (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
+        const uint32_t idx = rank < avail ? pool_next + rank : nbase + (rank - avail);
+        if (avail < want) {
+            pool_next = nbase + (want - avail);   // want - avail <= 64 <= chunk
+            pool_end = nbase + chunk;
+        } else {
+            pool_next += want;
+        }
+        return idx;
     }
-    return idx;
-}
+};
 
 // Persistent traversal over n rays with per-lane refill (the two modes are
 // separate instantiations, PREFETCH, so each gets its own register budget).
@@ -274,8 +308,7 @@ __device__ __forceinline__ uint32_t pool_take(bool need, uint32_t &pool_next, ui
 //   void load(uint32_t i, v3 &o, v3 &d, float &tmin, float &tmax, bool &any,
 //             TraceScene &sc) const     (sc: the BVH this ray traverses)
 //   void store(uint32_t i, const TravState &s) const
-// Lane g first takes ray g (no atomic); later rays come from the wave's pool
-// (pool_take; `fetch` must be 0 at launch). Two refill modes:
+// Rays come from a RayCursor (`fetch` must be 0 at launch). Two refill modes:
 //  - prefetch: each lane holds its next ray already loaded, so a finished
 //    lane starts it at once; the traversal yields to do that once
 //    `refill_lanes` lanes of the wave have finished, and the next rays' loads
@@ -295,8 +328,8 @@ __device__ __forceinline__ void trace_persistent(const Src &src, uint32_t n, uin
                                                  int budget, int refill_lanes, TravCounters &cnt) {
     TraceScene sc{};
     TravState s;
-    const uint32_t nthreads = gridDim.x * blockDim.x;
-    const uint32_t first = blockIdx.x * blockDim.x + threadIdx.x;
+    RayCursor cur(n, fetch, PREFETCH ? 0u : 192u);   // static share: small scenes only
+    const uint32_t first = cur.take(true);   // lane g of the grid: ray g
     uint32_t ray = kNoRay;
     bool any = false;
     bool exhausted = false;
@@ -309,7 +342,6 @@ __device__ __forceinline__ void trace_persistent(const Src &src, uint32_t n, uin
     } else {
         exhausted = true;
     }
-    uint32_t pool_next = 0, pool_end = 0;   // the wave's pool of ray indices
     if constexpr (!PREFETCH) {
         // refill mode
         while (true) {
@@ -326,7 +358,7 @@ __device__ __forceinline__ void trace_persistent(const Src &src, uint32_t n, uin
             const bool need = (ray == kNoRay) && !exhausted;
             const unsigned long long m = __ballot(need);
             if (m && (__popcll(m) >= (unsigned)refill_lanes || __ballot(ray != kNoRay) == 0ull)) {
-                const uint32_t idx = pool_take(need, pool_next, pool_end, fetch, nthreads);
+                const uint32_t idx = cur.take(need);
                 if (need) {
                     if (idx >= n) {
                         exhausted = true;
@@ -351,7 +383,7 @@ __device__ __forceinline__ void trace_persistent(const Src &src, uint32_t n, uin
     while (true) {
         const bool need = nxt == kNoRay && !exhausted;
         if (__ballot(need)) {
-            const uint32_t idx = pool_take(need, pool_next, pool_end, fetch, nthreads);
+            const uint32_t idx = cur.take(need);
             if (need) {
                 if (idx >= n) {
                     exhausted = true;
