@@ -1426,7 +1426,6 @@ int render_items(fl_ctx_t ctx, const HostItem *hi, uint32_t n_items, const fl_re
     uint32_t done_spp = 0, passes = 0, max_iter_seen = 0, n_trace = 0, n_shade = 0;
     double ms_trace = 0.0, ms_shade = 0.0, ms_infer = 0.0;
     const uint32_t max_iter = (uint32_t)std::max({p->diffuse_depth, p->reflection_depth, p->transmission_depth}) * 2 + 8;
-    const uint32_t shade_grid_max = (uint32_t)ctx->n_cu * 8;
     uint32_t host_counts[16];
 
     // Pipelined bounce loop: bounces are issued kChunk at a time with the
@@ -1444,7 +1443,27 @@ int render_items(fl_ctx_t ctx, const HostItem *hi, uint32_t n_items, const fl_re
     const uint32_t shade_waves = p->shade_waves ? p->shade_waves : (has_std ? 2u : 3u);
     const int variant = (counters ? 1 : 0) | (shade_waves >= 4 ? 2 : 0) | ((int)(tv - kTraceVariants) << 2) |
                         (has_std ? 1 << 16 : 0) | (shade_waves == 3 ? 1 << 17 : 0);
-    const uint32_t infer_grid = (uint32_t)ctx->n_cu * 16;   // one-wave blocks of 10.5 KB LDS
+    // Shading and inference run grid-stride loops in which every block takes
+    // the same share of the queue, so their grids are exactly the blocks that
+    // fit on the device at once: a grid of more blocks runs its remainder as
+    // a second round at partial occupancy.
+    auto resident = [&](const void *fn, int block, uint32_t fallback) {
+        int occ = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, fn, block, 0) != hipSuccess || occ <= 0) {
+            (void)hipGetLastError();
+            return fallback;
+        }
+        return (uint32_t)ctx->n_cu * (uint32_t)occ;
+    };
+    const void *shade_fn =
+        has_std ? ((variant & 2) ? reinterpret_cast<const void *>(&k_shade<4, true>)
+                                 : (variant & (1 << 17)) ? reinterpret_cast<const void *>(&k_shade<3, true>)
+                                                         : reinterpret_cast<const void *>(&k_shade<2, true>))
+                : ((variant & 2) ? reinterpret_cast<const void *>(&k_shade<4, false>)
+                                 : (variant & (1 << 17)) ? reinterpret_cast<const void *>(&k_shade<3, false>)
+                                                         : reinterpret_cast<const void *>(&k_shade<2, false>));
+    const uint32_t shade_grid_max = resident(shade_fn, kBlock, (uint32_t)ctx->n_cu * 8);
+    const uint32_t infer_grid = resident(reinterpret_cast<const void *>(&k_guide_infer), 64, (uint32_t)ctx->n_cu * 16);
     auto launch_shade = [&](const RenderArgs &args, dim3 g, int j, bool with_infer = true) {
         if (with_infer && args.guided) hipLaunchKernelGGL(k_guide_infer, dim3(infer_grid), dim3(64), 0, st, args, j);
         if (has_std) {
