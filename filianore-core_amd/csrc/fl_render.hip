@@ -261,30 +261,49 @@ const TraceVariant kTraceVariants[] = {
 };
 
 // ---------------------------------------------------------------- guide ----
-// Decoded lobes of one queue slot: 8 x (mu xyz, kappa, weight), world space.
-__device__ __forceinline__ float guide_pdf_at(const float *lb, v3 w) {
-    float p = 0.f;
-    for (int i = 0; i < kGLobes; ++i) {
-        const float *l = lb + 5 * i;
-        const float k = l[3];
-        p += l[4] * vmf_norm(k) * expf(k * (l[0] * w.x + l[1] * w.y + l[2] * w.z - 1.f));
-    }
-    return p;
+// Decoded lobes of one queue slot (PackedLobes, fp16). Sampling and density
+// read them the same way: the mean direction is renormalised after the fp16
+// rounding, and the weights are divided by their stored sum, so the density
+// is that of the sampling to fp32 rounding.
+static_assert(sizeof(PackedLobes) == 80, "80 bytes of lobes per slot");
+
+__device__ __forceinline__ VmfLobe lobe_at(const PackedLobes &P, int i) {
+    const v3 m((float)P.h[5 * i], (float)P.h[5 * i + 1], (float)P.h[5 * i + 2]);
+    VmfLobe L;
+    L.mu = m * (1.f / sqrtf(dot(m, m)));
+    L.kappa = (float)P.h[5 * i + 3];
+    L.w = (float)P.h[5 * i + 4];
+    return L;
 }
 
-__device__ __forceinline__ v3 guide_sample_at(const float *lb, float u_lobe, float u1, float u2) {
+__device__ __forceinline__ float lobe_wsum(const PackedLobes &P) {
+    float s = 0.f;
+#pragma unroll
+    for (int i = 0; i < kGLobes; ++i) s += (float)P.h[5 * i + 4];
+    return s;
+}
+
+__device__ __forceinline__ float guide_pdf_at(const PackedLobes *lb, v3 w) {
+    const PackedLobes &P = *lb;
+    float p = 0.f;
+#pragma unroll
+    for (int i = 0; i < kGLobes; ++i) {
+        const VmfLobe L = lobe_at(P, i);
+        p += L.w * vmf_norm(L.kappa) * expf(L.kappa * (dot(L.mu, w) - 1.f));
+    }
+    return p / lobe_wsum(P);
+}
+
+__device__ __forceinline__ v3 guide_sample_at(const PackedLobes *lb, float u_lobe, float u1, float u2) {
+    const PackedLobes &P = *lb;
+    const float target = u_lobe * lobe_wsum(P);
     int sel = kGLobes - 1;
     float c = 0.f;
     for (int i = 0; i < kGLobes - 1; ++i) {
-        c += lb[5 * i + 4];
-        if (u_lobe < c) { sel = i; break; }
+        c += (float)P.h[5 * i + 4];
+        if (target < c) { sel = i; break; }
     }
-    const float *l = lb + 5 * sel;
-    VmfLobe L;
-    L.mu = v3(l[0], l[1], l[2]);
-    L.kappa = l[3];
-    L.w = l[4];
-    return vmf_sample(L, u1, u2);
+    return vmf_sample(lobe_at(P, sel), u1, u2);
 }
 
 // Materials the guide serves: the non-specular ones.
@@ -357,17 +376,16 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3, 8)))
         if (want) {
             VmfLobe L[kGLobes];
             guide_decode(sB + q * kInferRaw, L);
-            float4 *lb = reinterpret_cast<float4 *>(a.lobes + (size_t)i * kGOut);
-            float o[kGOut];
-            for (int l = 0; l < kGLobes; ++l) {
-                o[5 * l + 0] = L[l].mu.x;
-                o[5 * l + 1] = L[l].mu.y;
-                o[5 * l + 2] = L[l].mu.z;
-                o[5 * l + 3] = L[l].kappa;
-                o[5 * l + 4] = L[l].w;
-            }
+            PackedLobes pk;
 #pragma unroll
-            for (int j = 0; j < kGOut / 4; ++j) lb[j] = make_float4(o[4 * j], o[4 * j + 1], o[4 * j + 2], o[4 * j + 3]);
+            for (int l = 0; l < kGLobes; ++l) {
+                pk.h[5 * l + 0] = (_Float16)L[l].mu.x;
+                pk.h[5 * l + 1] = (_Float16)L[l].mu.y;
+                pk.h[5 * l + 2] = (_Float16)L[l].mu.z;
+                pk.h[5 * l + 3] = (_Float16)L[l].kappa;   // exp of [-3, 7]: within fp16
+                pk.h[5 * l + 4] = (_Float16)L[l].w;
+            }
+            a.lobes[i] = pk;
         }
         __syncthreads();
     }
@@ -498,7 +516,7 @@ __device__ __forceinline__ bool nee_sample(const RenderArgs &a, const SceneRef &
                                            const TabSpec &tsp, const Lanes &ln, uint32_t li, float sel,
                                            float ux, float uy, v3 p, v3 perr, v3 ng, v3 fs, v3 ft, v3 woW, v3 wo,
                                            const float beta[4], float contrib[4], float occ[4], float4 &sh_o,
-                                           float4 &sh_d, const float *lb, float mix) {
+                                           float4 &sh_d, const PackedLobes *lb, float mix) {
     const DevLight &Lj = S.lights[li];
     const float *te = S.spec + (size_t)Lj.em_slot * a.grid.stride;
     const float *ts = Lj.shadow_slot ? S.spec + (size_t)Lj.shadow_slot * a.grid.stride : nullptr;
@@ -587,10 +605,12 @@ __device__ __forceinline__ uint32_t shade_key(const RenderArgs &a, int cur, uint
     return (uint32_t)S.mats[mat].kind;   // MAT_LAMBERT 1, MAT_CONDUCTOR 2, MAT_DIELECTRIC 3
 }
 
+// GUIDED: the render has a guide (mixture sampling at guided vertices and
+// training records); other renders run a kernel without that code.
 // STD: the render has standard-surface materials or image textures. Their
 // code is only compiled into that variant, so renders without them keep the
 // smaller kernel (registers and instruction cache).
-template <int WAVES, bool STD>
+template <int WAVES, bool STD, bool GUIDED>
 __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WAVES, 8))) k_shade(RenderArgs a, int b) {
     __shared__ uint32_t s_cnt[kSortKeys];
     __shared__ uint16_t s_perm[kBlock];
@@ -750,7 +770,7 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WAV
             wo = v3(dot(woW, fs), dot(woW, ft), dot(woW, ng));
             rb = rng_base(S.seed, a.st_key[sid].x, a.st_key[sid].y, bounce);
         }
-        const float *lb = (a.guided && mat_hit && guidable(bs_.kind)) ? a.lobes + (size_t)i * kGOut : nullptr;
+        const PackedLobes *lb = (GUIDED && a.guided && mat_hit && guidable(bs_.kind)) ? a.lobes + i : nullptr;
         const TabSpec tsp{S.spec, sd ? sd->slot : nullptr, GS, ln, a.grid.linear == 0,
                           (STD && sd && S.tex) ? sd->tex : nullptr, s_tv + (STD ? threadIdx.x : 0u)};
         const bool do_nee = mat_hit && S.n_lights > 0 &&
@@ -923,7 +943,7 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WAV
             if (cont) a.st_beta[sid] = make_float4(beta[0], beta[1], beta[2], beta[3]);
         }
         // -------- training record of this vertex (training passes only)
-        if (a.recs) {
+        if (GUIDED && a.recs) {
             const unsigned long long mr = __ballot(rec_want);
             const uint32_t ridx = block_reserve((uint32_t)__popcll(mr), a.rec_count, s_app[2]) +
                                   (uint32_t)__popcll(mr & ((1ull << __lane_id()) - 1ull));
@@ -953,6 +973,19 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WAV
             a.rq_sid[nxt][slot] = sid;
         }
     }
+}
+
+// The shading kernel for a render: standard surfaces / textures (STD), a
+// guide (GUIDED), and the waves per SIMD (2, 3 or 4).
+template <bool STD, bool GUIDED>
+const void *shade_kernel_t(uint32_t waves) {
+    if (waves >= 4) return reinterpret_cast<const void *>(&k_shade<4, STD, GUIDED>);
+    if (waves == 3) return reinterpret_cast<const void *>(&k_shade<3, STD, GUIDED>);
+    return reinterpret_cast<const void *>(&k_shade<2, STD, GUIDED>);
+}
+const void *shade_kernel(bool std_, bool guided, uint32_t waves) {
+    if (std_) return guided ? shade_kernel_t<true, true>(waves) : shade_kernel_t<true, false>(waves);
+    return guided ? shade_kernel_t<false, true>(waves) : shade_kernel_t<false, false>(waves);
 }
 
 // -------------------------------------------------------------- resolve ----
@@ -1293,8 +1326,8 @@ int render_items(fl_ctx_t ctx, const HostItem *hi, uint32_t n_items, const fl_re
     // samples per pixel per pass.
     uint32_t max_paths = p->max_paths;
     if (!max_paths) {
-        // guided passes also hold 160 B of lobes per path (2^26 paths: 10.7 GB)
-        const size_t per_path = kBytesPerPath + (guide ? (size_t)kGOut * sizeof(float) : 0u);
+        // guided passes also hold 80 B of lobes per path (2^26 paths: 5.4 GB)
+        const size_t per_path = kBytesPerPath + (guide ? sizeof(PackedLobes) : 0u);
         size_t free_b = 0, total_b = 0;
         max_paths = 1u << 26;
         if (hipMemGetInfo(&free_b, &total_b) == hipSuccess) {
@@ -1396,7 +1429,7 @@ int render_items(fl_ctx_t ctx, const HostItem *hi, uint32_t n_items, const fl_re
             a.box_lo[k] = s->bmin[k] - pad;
             a.box_inv[k] = 1.f / (s->bmax[k] - s->bmin[k] + 2.f * pad);
         }
-        a.lobes = grow<float>(B.p[20], B.cap[20], cap * kGOut * sizeof(float), err);
+        a.lobes = grow<PackedLobes>(B.p[20], B.cap[20], cap * sizeof(PackedLobes), err);
         if (train) {
             const float fr = p->guide_train_fraction > 0.f && p->guide_train_fraction <= 1.f ? p->guide_train_fraction
                                                                                               : 0.25f;
@@ -1442,7 +1475,7 @@ int render_items(fl_ctx_t ctx, const HostItem *hi, uint32_t n_items, const fl_re
     // VGPRs, 2 spilled), 2 for the standard-surface kernel (346 spilled at 3)
     const uint32_t shade_waves = p->shade_waves ? p->shade_waves : (has_std ? 2u : 3u);
     const int variant = (counters ? 1 : 0) | (shade_waves >= 4 ? 2 : 0) | ((int)(tv - kTraceVariants) << 2) |
-                        (has_std ? 1 << 16 : 0) | (shade_waves == 3 ? 1 << 17 : 0);
+                        (has_std ? 1 << 16 : 0) | (shade_waves == 3 ? 1 << 17 : 0) | (guide ? 1 << 18 : 0);
     // Shading and inference run grid-stride loops in which every block takes
     // the same share of the queue, so their grids are exactly the blocks that
     // fit on the device at once: a grid of more blocks runs its remainder as
@@ -1455,26 +1488,15 @@ int render_items(fl_ctx_t ctx, const HostItem *hi, uint32_t n_items, const fl_re
         }
         return (uint32_t)ctx->n_cu * (uint32_t)occ;
     };
-    const void *shade_fn =
-        has_std ? ((variant & 2) ? reinterpret_cast<const void *>(&k_shade<4, true>)
-                                 : (variant & (1 << 17)) ? reinterpret_cast<const void *>(&k_shade<3, true>)
-                                                         : reinterpret_cast<const void *>(&k_shade<2, true>))
-                : ((variant & 2) ? reinterpret_cast<const void *>(&k_shade<4, false>)
-                                 : (variant & (1 << 17)) ? reinterpret_cast<const void *>(&k_shade<3, false>)
-                                                         : reinterpret_cast<const void *>(&k_shade<2, false>));
+    const void *shade_fn = shade_kernel(has_std, guide != nullptr, shade_waves);
     const uint32_t shade_grid_max = resident(shade_fn, kBlock, (uint32_t)ctx->n_cu * 8);
     const uint32_t infer_grid = resident(reinterpret_cast<const void *>(&k_guide_infer), 64, (uint32_t)ctx->n_cu * 16);
     auto launch_shade = [&](const RenderArgs &args, dim3 g, int j, bool with_infer = true) {
         if (with_infer && args.guided) hipLaunchKernelGGL(k_guide_infer, dim3(infer_grid), dim3(64), 0, st, args, j);
-        if (has_std) {
-            if (variant & 2) hipLaunchKernelGGL((k_shade<4, true>), g, dim3(kBlock), 0, st, args, j);
-            else if (variant & (1 << 17)) hipLaunchKernelGGL((k_shade<3, true>), g, dim3(kBlock), 0, st, args, j);
-            else hipLaunchKernelGGL((k_shade<2, true>), g, dim3(kBlock), 0, st, args, j);
-        } else {
-            if (variant & 2) hipLaunchKernelGGL((k_shade<4, false>), g, dim3(kBlock), 0, st, args, j);
-            else if (variant & (1 << 17)) hipLaunchKernelGGL((k_shade<3, false>), g, dim3(kBlock), 0, st, args, j);
-            else hipLaunchKernelGGL((k_shade<2, false>), g, dim3(kBlock), 0, st, args, j);
-        }
+        RenderArgs ka = args;
+        int kj = j;
+        void *kargs[] = {&ka, &kj};
+        (void)hipLaunchKernel(shade_fn, g, dim3(kBlock), kargs, 0, st);
     };
     auto launch_bounce = [&](const RenderArgs &args, uint32_t j) {
         tv->launch(counters, dim3(grid_pers), st, args, (int)j);

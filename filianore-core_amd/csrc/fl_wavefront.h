@@ -1,9 +1,17 @@
 // fl_wavefront.h -- data shared by the wavefront render kernels.
 #pragma once
+#include "fl_guide.h"
 #include "fl_internal.h"
 #include "fl_trace.h"
 
 namespace fl {
+
+// The guide's decoded lobes of one extension-queue slot in fp16: 8 x (mu
+// xyz, kappa, weight), 80 bytes, written by the inference kernel and read by
+// shading (fl_render.hip: lobe_at, guide_pdf_at, guide_sample_at).
+struct alignas(16) PackedLobes {
+    _Float16 h[kGOut];
+};
 
 // path flags (high bits of the packed misc word; bounce in the low 8 bits)
 enum : uint32_t {
@@ -120,7 +128,7 @@ struct RenderArgs {
     float guide_mix;         // probability of sampling the guide
     const __bf16 *gW;        // bf16 weights [5][64][64]
     const float *gP;         // fp32 parameters (biases)
-    float *lobes;            // [cap][40]: decoded lobes per extension-queue slot
+    PackedLobes *lobes;      // [cap]: decoded lobes per extension-queue slot
     float box_lo[3], box_inv[3];   // scene box -> unit cube
     // training records (training passes): 16 floats each
     float *recs;
