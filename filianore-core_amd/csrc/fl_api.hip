@@ -178,7 +178,11 @@ int fl_scene_create(fl_ctx_t ctx, const fl_scene_desc *desc, fl_scene_t *out) {
     uint64_t tex_bytes = 0;
     for (uint32_t j = 0; j < desc->n_textures; ++j) {
         const fl_texture &tx = desc->textures[j];
-        if (tx.rgb && tx.width && tx.height) tex_bytes += (3ull * tx.width * tx.height + 3) & ~3ull;
+        if (!(tx.rgb && tx.width && tx.height)) continue;
+        if (tx.width > 0xffffffffull / 3 / tx.height)   // 3 w h alone beyond 4 GiB (and no wrap below)
+            return set_error(ctx, FL_E_INVALID, "fl_scene_create: textures exceed 4 GiB");
+        tex_bytes += (3ull * tx.width * tx.height + 3) & ~3ull;
+        if (tex_bytes > 0xffffffffull) break;
     }
     if (tex_bytes > 0xffffffffull) return set_error(ctx, FL_E_INVALID, "fl_scene_create: textures exceed 4 GiB");
     for (uint32_t m = 0; m < desc->n_materials; ++m)
@@ -191,7 +195,7 @@ int fl_scene_create(fl_ctx_t ctx, const fl_scene_desc *desc, fl_scene_t *out) {
             if (t[k] > desc->n_textures)
                 return set_error(ctx, FL_E_INVALID, "fl_scene_create: texture index out of range");
     }
-    for (uint32_t t = 0; t < 3ull * desc->n_triangles; ++t)
+    for (uint64_t t = 0; t < 3ull * desc->n_triangles; ++t)
         if (desc->indices[t] >= desc->n_vertices) return set_error(ctx, FL_E_INVALID, "fl_scene_create: index out of range");
     if (desc->tri_material)
         for (uint32_t t = 0; t < desc->n_triangles; ++t)

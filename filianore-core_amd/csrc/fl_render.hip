@@ -918,6 +918,16 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WAV
                     no = offset_point_origin(p, perr, ng, wiW);          // kindle_ray
                     nd = wiW;
                     prev_pdf = (ty & BSDF_SPECULAR) ? 0.f : bsm.pdf;
+                    // A stack with specular lobes draws this non-specular
+                    // direction with density S/(m+k) (k specular lobes among
+                    // m+k), while the light sample's MIS weight competes with
+                    // estimate_direct's BSDF density S/m (flags without
+                    // BSDF_SPECULAR, integrator.cpp:63-66). The emission weight
+                    // at the next hit takes the same S/m, so the two weights
+                    // sum to one; beta already divides by the true density.
+                    if (STD && sd && !(ty & BSDF_SPECULAR) &&
+                        std_num_components(*sd, kAllLobes) != std_num_components(*sd, kNonSpecular))
+                        prev_pdf = std_bsdf_pdf(*sd, bs_.vndf_pdf, wo, bsm.wi, kNonSpecular);
                     cont = true;
                     if (!brk) {
                         // Russian roulette, pathintegrator.cpp:77-83

@@ -1,4 +1,5 @@
 // fl_scenefile.cpp -- the scene wire format of include/fl_scenefile.h (host code).
+#include <algorithm>
 #include <cstdio>
 #include <cstring>
 #include <string>
@@ -128,6 +129,10 @@ int fl_scene_file_read(const char *path, fl_scene_file_t *out) {
     if (version != FL_SCENEFILE_VERSION) return bad("unsupported version " + std::to_string(version));
     fl_scene_desc &d = s->desc;
     size_t off = 16;
+    // every tag but TEXR appears at most once; the per-triangle and
+    // per-vertex sections must match the INDX and VERT counts
+    std::vector<uint32_t> seen;
+    uint32_t n_tmat = 0, n_uvs = 0;
     for (uint32_t k = 0; k < n_sections; ++k) {
         SectionHeader h;
         if (B.size() - off < sizeof(h)) return bad("truncated section header");
@@ -135,13 +140,17 @@ int fl_scene_file_read(const char *path, fl_scene_file_t *out) {
         off += sizeof(h);
         if (h.bytes > B.size() - off) return bad("truncated section");
         const uint8_t *p = B.data() + off;
+        if (h.tag != T_TEXR) {
+            if (std::find(seen.begin(), seen.end(), h.tag) != seen.end()) return bad("duplicate section");
+            seen.push_back(h.tag);
+        }
         auto expect = [&](size_t elem) { return (uint64_t)h.count * elem == h.bytes; };
         // payloads start 8-byte aligned (header 16, sections padded to 8), so
         // the casts below are aligned for every struct of fl_gpu.h
         bool size_ok = true;
         if (h.tag == T_VERT) { size_ok = expect(12); d.vertices = (const float *)p; d.n_vertices = h.count; }
         else if (h.tag == T_INDX) { size_ok = expect(12); d.indices = (const uint32_t *)p; d.n_triangles = h.count; }
-        else if (h.tag == T_TMAT) { size_ok = expect(4); d.tri_material = (const uint32_t *)p; }
+        else if (h.tag == T_TMAT) { size_ok = expect(4); d.tri_material = (const uint32_t *)p; n_tmat = h.count; }
         else if (h.tag == T_MATL) { size_ok = expect(sizeof(fl_material)); d.materials = (const fl_material *)p; d.n_materials = h.count; }
         else if (h.tag == T_STDS) {
             size_ok = expect(sizeof(fl_standard_surface));
@@ -151,14 +160,16 @@ int fl_scene_file_read(const char *path, fl_scene_file_t *out) {
         else if (h.tag == T_ALIT) { size_ok = expect(sizeof(fl_area_light)); d.lights = (const fl_area_light *)p; d.n_lights = h.count; }
         else if (h.tag == T_DLIT) { size_ok = expect(sizeof(fl_delta_light)); d.delta_lights = (const fl_delta_light *)p; d.n_delta_lights = h.count; }
         else if (h.tag == T_EMIS) { size_ok = expect(FL_SPD_N * 4); d.emission_spd = (const float *)p; d.n_emission = h.count; }
-        else if (h.tag == T_UVS) { size_ok = expect(8); d.uvs = (const float *)p; }
+        else if (h.tag == T_UVS) { size_ok = expect(8); d.uvs = (const float *)p; n_uvs = h.count; }
         else if (h.tag == T_CAMR) { size_ok = expect(sizeof(fl_camera)); s->cameras = (const fl_camera *)p; s->n_cameras = h.count; }
         else if (h.tag == T_TEXR) {
             uint32_t wh[2] = {0, 0};
             size_ok = h.count == 1 && h.bytes >= 8;
             if (size_ok) {
                 memcpy(wh, p, 8);
-                size_ok = h.bytes == 8 + 3ull * wh[0] * wh[1];
+                // 3 w h bytes, computed only where it cannot wrap
+                size_ok = wh[1] == 0 || wh[0] <= (UINT64_MAX - 8) / 3 / wh[1];
+                size_ok = size_ok && h.bytes == 8 + 3ull * wh[0] * wh[1];
             }
             fl_texture t{};
             t.width = wh[0];
@@ -175,6 +186,8 @@ int fl_scene_file_read(const char *path, fl_scene_file_t *out) {
     d.n_textures = (uint32_t)s->textures.size();
     // every reference in range (fl_scene_create checks them again)
     if (d.n_triangles && (!d.vertices || !d.indices)) return bad("triangles without vertices");
+    if (d.tri_material && n_tmat != d.n_triangles) return bad("material section does not match the triangle count");
+    if (d.uvs && n_uvs != d.n_vertices) return bad("uv section does not match the vertex count");
     if (!d.materials || d.n_materials == 0) return bad("no materials");
     for (uint64_t i = 0; i < 3ull * d.n_triangles; ++i)
         if (d.indices[i] >= d.n_vertices) return bad("vertex index out of range");

@@ -76,7 +76,34 @@ def test_generated_scene_round_trip(tmp_path):
     assert os.path.getsize(path) >= 16 + gen.vertices.nbytes + gen.indices.nbytes
 
 
-@pytest.mark.parametrize("damage", ["magic", "version", "truncate", "size", "index", "texture_ref"])
+def split_sections(raw):
+    """(header, [(tag, count, payload)]) of a scene file: 16-byte header,
+    then sections of a 16-byte header (tag, count, bytes) and a payload
+    padded to 8 bytes."""
+    head, n = bytes(raw[:16]), int(np.frombuffer(raw[12:16], np.uint32)[0])
+    off, out = 16, []
+    for _ in range(n):
+        tag, count = np.frombuffer(raw[off:off + 8], np.uint32)
+        nb = int(np.frombuffer(raw[off + 8:off + 16], np.uint64)[0])
+        out.append((int(tag), int(count), bytes(raw[off + 16:off + 16 + nb])))
+        off += 16 + nb + (8 - nb % 8) % 8
+    return head, out
+
+
+def join_sections(head, secs):
+    raw = bytearray(head[:12]) + np.uint32(len(secs)).tobytes()
+    for tag, count, payload in secs:
+        raw += np.array([tag, count], np.uint32).tobytes() + np.uint64(len(payload)).tobytes() + payload
+        raw += bytes((8 - len(payload) % 8) % 8)
+    return raw
+
+
+def tag_of(name):
+    return int(np.frombuffer(name.encode(), np.uint32)[0])
+
+
+@pytest.mark.parametrize("damage", ["magic", "version", "truncate", "size", "index", "texture_ref", "short_tmat",
+                                    "short_uvs", "duplicate", "texture_wrap"])
 def test_damaged_files_are_rejected(tmp_path, damage):
     gen = rich_scene()
     path = tmp_path / "box.flscene"
@@ -100,6 +127,25 @@ def test_damaged_files_are_rejected(tmp_path, damage):
         gen.materials[0].texture = 99
         flgpu.save_scene_file(path, gen)
         raw = bytearray(path.read_bytes())
+    elif damage in ("short_tmat", "short_uvs"):
+        # a per-triangle / per-vertex section one entry short, its own size
+        # consistent with its count (the reader would run past it)
+        head, secs = split_sections(raw)
+        t, elem = (tag_of("TMAT"), 4) if damage == "short_tmat" else (tag_of("UVS_"), 8)
+        secs = [(tg, c - 1, p[:-elem]) if tg == t else (tg, c, p) for tg, c, p in secs]
+        raw = join_sections(head, secs)
+    elif damage == "duplicate":
+        head, secs = split_sections(raw)
+        secs.append(next(x for x in secs if x[0] == tag_of("MATL")))
+        raw = join_sections(head, secs)
+    elif damage == "texture_wrap":
+        # 3 w h = 2 * 2^64 + 1399: computed in 64 bits it wraps to 1399, so a
+        # size check without an overflow test would accept 1399 texel bytes
+        head, secs = split_sections(raw)
+        w, h = 4294571377, 2863575501
+        assert (3 * w * h) % 2 ** 64 == 1399
+        secs.append((tag_of("TEXR"), 1, np.array([w, h], np.uint32).tobytes() + bytes(1399)))
+        raw = join_sections(head, secs)
     path.write_bytes(bytes(raw))
     with pytest.raises(flgpu.FlError) as e:
         flgpu.load_scene_file(path)
