@@ -186,6 +186,9 @@ int fl_scene_create(fl_ctx_t ctx, const fl_scene_desc *desc, fl_scene_t *out) {
     }
     if (tex_bytes > 0xffffffffull) return set_error(ctx, FL_E_INVALID, "fl_scene_create: textures exceed 4 GiB");
     for (uint32_t m = 0; m < desc->n_materials; ++m)
+        if (desc->materials[m].spd_r > desc->n_emission || desc->materials[m].spd_t > desc->n_emission)
+            return set_error(ctx, FL_E_INVALID, "fl_scene_create: material spectrum index out of range");
+    for (uint32_t m = 0; m < desc->n_materials; ++m)
         if (desc->materials[m].kind == FL_MAT_LAMBERT && desc->materials[m].texture > desc->n_textures)
             return set_error(ctx, FL_E_INVALID, "fl_scene_create: texture index out of range");
     for (uint32_t j = 0; j < desc->n_standard_surfaces; ++j) {
@@ -475,6 +478,27 @@ int fl_bsdf_query_batch(fl_ctx_t ctx, const fl_bsdf_query *q, fl_bsdf_result *r,
         rc = set_error(ctx, FL_E_HIP, "download");
     (void)hipFree(dq);
     (void)hipFree(dr);
+    return rc;
+}
+
+int fl_sampler_draw(fl_ctx_t ctx, uint64_t seed, const uint32_t *keys, uint32_t n, uint32_t n_dims, float *out) {
+    if (!ctx) return FL_E_INVALID;
+    if ((n && (!keys || !out)) || n_dims == 0 || n_dims > 64)
+        return set_error(ctx, FL_E_INVALID, "fl_sampler_draw: null arrays or n_dims outside 1..64");
+    if (n == 0) return FL_OK;
+    (void)hipSetDevice(ctx->device);
+    uint32_t *dk = nullptr;
+    float *dout = nullptr;
+    const size_t nk = 3ull * n, no = (size_t)n * n_dims;
+    int rc = FL_OK;
+    if (hipMalloc((void **)&dk, nk * 4) != hipSuccess || hipMalloc((void **)&dout, no * 4) != hipSuccess)
+        rc = set_error(ctx, FL_E_OOM, "sampler buffers");
+    if (!rc && hipMemcpy(dk, keys, nk * 4, hipMemcpyHostToDevice) != hipSuccess) rc = set_error(ctx, FL_E_HIP, "upload");
+    if (!rc) rc = flx::sampler_draw(ctx, seed, dk, n, n_dims, dout);
+    if (!rc && hipStreamSynchronize(ctx->stream) != hipSuccess) rc = set_error(ctx, FL_E_HIP, "sampler kernel");
+    if (!rc && hipMemcpy(out, dout, no * 4, hipMemcpyDeviceToHost) != hipSuccess) rc = set_error(ctx, FL_E_HIP, "download");
+    (void)hipFree(dk);
+    (void)hipFree(dout);
     return rc;
 }
 

@@ -333,8 +333,21 @@ __global__ void k_collapse(BuildView v, const uint32_t *__restrict__ work_in, ui
         for (int k = 0; k < 3; ++k) {
             double step = ldexp(1.0, ex[k]);
             double org = (double)node.org[k];
-            double ql = floor(((double)clo[c][k] - org) / step) - 1.0;
-            double qh = ceil(((double)chi[c][k] - org) / step) + 1.0;
+            double blo = (double)clo[c][k], bhi = (double)chi[c][k];
+            if (leafcnt[c] > (uint32_t)kMaxLeafTris) {
+                // an internal child's node quantizes its own slots on its own
+                // grid (org_c, step_c, from the same box): they lie in
+                // [org_c, org_c + min(65535, ceil((hi - org_c) / step_c) + 1) step_c].
+                // Covering that range keeps every dequantized child box inside
+                // this slot's box, not only the child's triangles.
+                const int ec = quant_exponent(clo[c][k], chi[c][k]);
+                const double sc = ldexp(1.0, ec);
+                const double oc = floor(blo / sc) * sc;
+                blo = oc;
+                bhi = oc + fmin(ceil((bhi - oc) / sc) + 1.0, 65535.0) * sc;
+            }
+            double ql = floor((blo - org) / step) - 1.0;
+            double qh = ceil((bhi - org) / step) + 1.0;
             ql = ql < 0.0 ? 0.0 : (ql > 65535.0 ? 65535.0 : ql);
             qh = qh < 0.0 ? 0.0 : (qh > 65535.0 ? 65535.0 : qh);
             node.qlo[k][s] = (uint16_t)ql;

@@ -153,6 +153,7 @@ int trace_rays(fl_ctx_t ctx, fl_scene_s *s, const fl_ray *d_rays, fl_hit *d_hits
 
 // BSDF queries (fl_render.hip)
 int bsdf_queries(fl_ctx_t ctx, const fl_bsdf_query *d_q, fl_bsdf_result *d_r, uint32_t n);
+int sampler_draw(fl_ctx_t ctx, uint64_t seed, const uint32_t *d_keys, uint32_t n, uint32_t n_dims, float *d_out);
 int bsdf_queries_standard(fl_ctx_t ctx, const fl_standard_surface *m, const fl_standard_query *d_q, fl_bsdf_result *d_r,
                           uint32_t n);
 

@@ -1136,6 +1136,16 @@ __global__ void k_bsdf_query_standard(StdParams m, const fl_standard_query *__re
     r[i] = R;
 }
 
+// The sampler of k_generate / k_shade for (pixel, sample, bounce) keys.
+__global__ void k_sampler_draw(uint64_t seed, const uint32_t *__restrict__ keys, uint32_t n, uint32_t n_dims,
+                               float *__restrict__ out) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (size_t)n * n_dims) return;
+    const size_t k = i / n_dims;
+    const uint32_t d = (uint32_t)(i - k * n_dims);
+    out[i] = rng_u(rng_base(seed, keys[3 * k], keys[3 * k + 1], keys[3 * k + 2]), d);
+}
+
 template <typename T>
 T *grow(void *&p, size_t &cap, size_t bytes, bool &err) {
     if (cap < bytes) {
@@ -1843,6 +1853,14 @@ int bsdf_queries_standard(fl_ctx_t ctx, const fl_standard_surface *m, const fl_s
     if (n == 0) return FL_OK;
     const StdParams P = standard_params(*m);
     hipLaunchKernelGGL(k_bsdf_query_standard, dim3((n + 255) / 256), dim3(256), 0, ctx->stream, P, d_q, d_r, n);
+    FL_HIP_CHECK(ctx, hipGetLastError());
+    return FL_OK;
+}
+
+int sampler_draw(fl_ctx_t ctx, uint64_t seed, const uint32_t *d_keys, uint32_t n, uint32_t n_dims, float *d_out) {
+    const size_t total = (size_t)n * n_dims;
+    hipLaunchKernelGGL(k_sampler_draw, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, ctx->stream, seed, d_keys, n,
+                       n_dims, d_out);
     FL_HIP_CHECK(ctx, hipGetLastError());
     return FL_OK;
 }

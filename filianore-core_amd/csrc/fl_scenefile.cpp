@@ -198,6 +198,7 @@ int fl_scene_file_read(const char *path, fl_scene_file_t *out) {
         const fl_material &fm = d.materials[m];
         if (fm.kind == FL_MAT_STANDARD && fm.standard >= d.n_standard_surfaces) return bad("standard surface index out of range");
         if (fm.kind == FL_MAT_LAMBERT && fm.texture > d.n_textures) return bad("texture index out of range");
+        if (fm.spd_r > d.n_emission || fm.spd_t > d.n_emission) return bad("material spectrum index out of range");
     }
     for (uint32_t j = 0; j < d.n_standard_surfaces; ++j) {
         const fl_standard_surface &ss = d.standard_surfaces[j];

@@ -154,6 +154,8 @@ void build_spectral_tables(fl_scene_s *s, const fl_scene_desc *desc, int mode, s
         for (int c = 0; c < 3; ++c) tab[c * stride + kHeroN] = tab[c * stride + kHeroN - 1];
     }
     Basis B = refl_basis(mode, N);
+    float spd_lam[FL_SPD_N];
+    for (int i = 0; i < FL_SPD_N; ++i) spd_lam[i] = 360.f + (float)i;
     mats.resize(nm);
     for (uint32_t m = 0; m < nm; ++m) {
         const fl_material &fm = desc->materials[m];
@@ -167,8 +169,14 @@ void build_spectral_tables(fl_scene_s *s, const fl_scene_desc *desc, int mode, s
         d.cauchy_b = 0.f;
         d.tex = 0;
         float *ta = tab + (size_t)d.spec_a * stride, *tb = tab + (size_t)d.spec_b * stride;
+        // a reflectance given as a spectrum: a row of the SPD table, clamped
+        auto spd_refl = [&](uint32_t row1, float *out) {
+            project(mode, spd_lam, desc->emission_spd + (size_t)(row1 - 1) * FL_SPD_N, FL_SPD_N, out);
+            for (int i = 0; i < stride; ++i) out[i] = std::max(0.f, out[i]);
+        };
         if (fm.kind == FL_MAT_LAMBERT) {
-            from_reflectance_rgb(B, fm.rgb, N, ta);
+            if (fm.spd_r) spd_refl(fm.spd_r, ta);
+            else from_reflectance_rgb(B, fm.rgb, N, ta);
             d.tex = fm.texture;   // LambertMaterial's kd texture (lambert.cpp:12-13)
         } else if (fm.kind == FL_MAT_CONDUCTOR) {
             // standardsurface.cpp:104-106, GGXDistribution ctor ggx.cpp:7-11
@@ -181,8 +189,10 @@ void build_spectral_tables(fl_scene_s *s, const fl_scene_desc *desc, int mode, s
             project(mode, FL_METAL_LAMBDA[mt], FL_METAL_ETA[mt], FL_METAL_COUNT[mt], ta);
             project(mode, FL_METAL_LAMBDA[mt], FL_METAL_K[mt], FL_METAL_COUNT[mt], tb);
         } else if (fm.kind == FL_MAT_DIELECTRIC) {
-            from_reflectance_rgb(B, fm.rgb, N, ta);
-            from_reflectance_rgb(B, fm.rgb_t, N, tb);
+            if (fm.spd_r) spd_refl(fm.spd_r, ta);
+            else from_reflectance_rgb(B, fm.rgb, N, ta);
+            if (fm.spd_t) spd_refl(fm.spd_t, tb);
+            else from_reflectance_rgb(B, fm.rgb_t, N, tb);
             d.eta_bsdf = fm.ior;
             d.cauchy_b = fm.cauchy_b;
         } else if (fm.kind == FL_MAT_STANDARD) {
@@ -190,8 +200,6 @@ void build_spectral_tables(fl_scene_s *s, const fl_scene_desc *desc, int mode, s
             d.eta_bsdf = s->h_std[fm.standard].eta_bsdf;
         }
     }
-    float spd_lam[FL_SPD_N];
-    for (int i = 0; i < FL_SPD_N; ++i) spd_lam[i] = 360.f + (float)i;
     for (uint32_t e = 0; e < ne; ++e)
         project(mode, spd_lam, desc->emission_spd + (size_t)e * FL_SPD_N, FL_SPD_N, tab + (3 + 2 * (size_t)nm + e) * stride);
     // the reflectance basis, white cyan magenta yellow red green blue, for

@@ -12,7 +12,7 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libfl_gpu.so")
 # the struct layouts below are those of this version of include/fl_gpu.h
-ABI_VERSION = 4
+ABI_VERSION = 5
 
 FL_OK = 0
 FL_E_INVALID, FL_E_HIP, FL_E_OOM, FL_E_UNSUPPORTED, FL_E_OVERFLOW, FL_E_NUMERIC = -1, -2, -3, -4, -5, -6
@@ -38,7 +38,8 @@ f4 = C.c_float * 4
 class Material(C.Structure):
     _fields_ = [("kind", C.c_int32), ("rgb", f3), ("rgb_t", f3), ("roughness", C.c_float),
                 ("anisotropic", C.c_float), ("weight", C.c_float), ("metal", C.c_int32), ("ior", C.c_float),
-                ("cauchy_b", C.c_float), ("standard", C.c_uint32), ("texture", C.c_uint32), ("pad", C.c_uint32)]
+                ("cauchy_b", C.c_float), ("standard", C.c_uint32), ("texture", C.c_uint32), ("spd_r", C.c_uint16),
+                ("spd_t", C.c_uint16)]
 
 
 class StandardSurface(C.Structure):
@@ -199,7 +200,7 @@ EXPORTS = [
     "fl_intersect", "fl_intersect_device", "fl_render", "fl_render_batch", "fl_autotune", "fl_autotune_batch",
     "fl_guide_create", "fl_guide_destroy", "fl_guide_train_step", "fl_guide_forward",
     "fl_guide_get_params", "fl_guide_set_params", "fl_guide_gradient", "fl_guide_stats",
-    "fl_bsdf_query_batch", "fl_bsdf_query_standard", "fl_develop_film", "fl_develop",
+    "fl_bsdf_query_batch", "fl_bsdf_query_standard", "fl_sampler_draw", "fl_develop_film", "fl_develop",
     "fl_gen_default_options", "fl_gen_config", "fl_gen_free", "fl_gen_describe", "fl_gen_blackbody",
     "fl_scene_file_write", "fl_scene_file_read", "fl_scene_file_error", "fl_scene_file_desc",
     "fl_scene_file_cameras", "fl_scene_file_free",
@@ -245,6 +246,7 @@ def lib():
         L.fl_bsdf_query_standard.argtypes = [vp, C.POINTER(StandardSurface), C.POINTER(StandardQuery),
                                              C.POINTER(BsdfResult), C.c_uint32]
         L.fl_develop.argtypes = [vp, vp, C.c_uint32, C.POINTER(DevelopParams), vp, vp]
+        L.fl_sampler_draw.argtypes = [vp, C.c_uint64, vp, C.c_uint32, C.c_uint32, vp]
         L.fl_gen_default_options.argtypes = [C.POINTER(GenOptions)]
         L.fl_gen_config.argtypes = [C.c_int, C.POINTER(GenOptions), C.POINTER(GenScene)]
         L.fl_gen_free.argtypes = [C.POINTER(GenScene)]
@@ -503,6 +505,14 @@ class Context:
         r = (BsdfResult * n)()
         _check(self.L.fl_bsdf_query_standard(self.h, C.byref(material), q, r, n), self.h, "fl_bsdf_query_standard")
         return list(r)
+
+    def sampler_draw(self, seed, keys, n_dims):
+        """fl_sampler_draw: keys (n, 3) uint32 (pixel, sample, bounce) -> (n, n_dims) float32."""
+        k = np.ascontiguousarray(keys, np.uint32).reshape(-1, 3)
+        out = np.zeros((k.shape[0], n_dims), np.float32)
+        _check(self.L.fl_sampler_draw(self.h, C.c_uint64(seed), k.ctypes.data, k.shape[0], n_dims, out.ctypes.data),
+               self.h, "fl_sampler_draw")
+        return out
 
     def bsdf_queries(self, queries):
         n = len(queries)

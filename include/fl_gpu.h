@@ -36,7 +36,7 @@
 extern "C" {
 #endif
 
-#define FL_ABI_VERSION 4
+#define FL_ABI_VERSION 5
 
 /* ---- status codes ------------------------------------------------------ */
 enum {
@@ -104,7 +104,13 @@ typedef struct {
     uint32_t standard;   /* FL_MAT_STANDARD: index into fl_scene_desc.standard_surfaces */
     uint32_t texture;    /* Lambert: an ImageMapTexture for kd, 1 + index into
                             fl_scene_desc.textures (0 = the constant rgb) */
-    uint32_t pad;
+    /* A reflectance given as a spectrum instead of reflectance RGB, as a
+     * ConstantTexture<PrincipalSpectrum> built by from_spd would hold it
+     * (color/spectrumoperations.cpp:88-103; shading/textures/constant.h:9-20):
+     * 0 = the rgb above, else 1 + a row of fl_scene_desc.emission_spd,
+     * clamped at 0 like the materials clamp their spectra. spd_r stands for
+     * the Lambert kd or the dielectric ks, spd_t for the dielectric kt. */
+    uint16_t spd_r, spd_t;
 } fl_material;
 
 /* One DiffuseAreaIlluminant over one triangle (illuminants/diffusearea.cpp). */
@@ -394,6 +400,15 @@ typedef struct {
 } fl_bsdf_result;
 
 int fl_bsdf_query_batch(fl_ctx_t ctx, const fl_bsdf_query *q, fl_bsdf_result *r, uint32_t n);
+
+/* ---- sampler (Sampler::get_1d / get_2d, sampling/sampler.h:14-18) ---------
+ * The render path's counter-based sampler, evaluated on the device: for key
+ * i = (pixel, sample, bounce) = keys[3i..3i+2] and dimension d < n_dims,
+ * out[i * n_dims + d] is the uniform [0, 1) number a render with `seed` draws
+ * at that vertex and dimension (a hash of seed, pixel, sample, bounce and
+ * dimension; 24-bit mantissa). The reference draws from a seeded mt19937
+ * instead (samplers/whitenoise.h:11-29); the two are compared statistically. */
+int fl_sampler_draw(fl_ctx_t ctx, uint64_t seed, const uint32_t *keys, uint32_t n, uint32_t n_dims, float *out);
 
 /* The full StandardSurfaceMaterial BSDF (BSDF::evaluate / sample / pdf over
  * its lobe stack, shading/bsdf.cpp:6-151, with flags BSDF_ALL) for n queries

@@ -1066,6 +1066,7 @@ struct Ctx {
     int dd, rd, td;
     float rr_thr;
     uint32_t rr_start;
+    int mut = OR_MUT_NONE;
     uint64_t rays = 0;
 };
 
@@ -1344,7 +1345,9 @@ S60 Li(Ctx &c, V3f ro, V3f rd, float tmin, Sampler &sampler) {
         Isect it;
         c.rays++;
         bool hit = scene_intersect(s, ro, rd, tmin, INFINITI, &it);
-        if (bounces == 0 || specularBounce) {
+        const bool emit = c.mut == OR_MUT_EMIT_ALWAYS ? true
+                          : c.mut == OR_MUT_NO_SPECULAR_EMIT ? bounces == 0 : (bounces == 0 || specularBounce);
+        if (emit) {
             if (hit && s->light[it.prim]) L += beta * light_L(s->lights[s->light[it.prim] - 1], it.n, -rd);
         }
         if (!hit) break;
@@ -1372,12 +1375,12 @@ S60 Li(Ctx &c, V3f ro, V3f rd, float tmin, Sampler &sampler) {
         sampler.get_2d(&u0, &u1);
         S60 f = bsdf.sample(wo, &wi, u0, u1, &pdf, BSDF_ALL, &flags);
         if (f.is_black() || pdf == 0) break;
-        if (c.dd == bounces && (flags & BSDF_DIFFUSE)) break;
+        if ((c.mut == OR_MUT_DEPTH_LATE ? c.dd + 1 : c.dd) == bounces && (flags & BSDF_DIFFUSE)) break;
         if ((c.rd == bounces && (flags & BSDF_SPECULAR)) || (c.rd == bounces && (flags & BSDF_GLOSSY))) break;
         if (c.td == bounces && (flags & BSDF_TRANSMISSION)) break;
         beta *= f * abs_dot(wi, it.n) / pdf;
         specularBounce = (flags & BSDF_SPECULAR) != 0;
-        if ((flags & BSDF_SPECULAR) && (flags & BSDF_TRANSMISSION)) {
+        if ((flags & BSDF_SPECULAR) && (flags & BSDF_TRANSMISSION) && c.mut != OR_MUT_NO_ETASCALE) {
             float eta = bsdf.eta;
             etaScale *= (dot(wo, it.n) > 0) ? (eta * eta) : 1.f / (eta * eta);
         }
@@ -1388,7 +1391,7 @@ S60 Li(Ctx &c, V3f ro, V3f rd, float tmin, Sampler &sampler) {
         if (rrBeta.max_component() < c.rr_thr && bounces > (int)c.rr_start) {
             float q = std::max(0.05f, 1.f - rrBeta.max_component());
             if (sampler.get_1d() < q) break;
-            beta /= 1.f - q;
+            beta /= c.mut == OR_MUT_RR_DIV_Q ? q : 1.f - q;
         }
     }
     return L;
@@ -1583,14 +1586,23 @@ int or_scene_create(const fl_scene_desc *d, or_scene_t *out) {
     }
     const Tables &T = tables();
     s->mats.resize(d->n_materials);
+    float spd_lam[FL_SPD_N];
+    for (int i = 0; i < FL_SPD_N; ++i) spd_lam[i] = 360.f + (float)i;
+    // a reflectance given as a spectrum: ConstantTexture<PrincipalSpectrum>
+    // of from_spd (constant.h:9-20, spectrumoperations.cpp:88-103), clamped
+    auto refl = [&](uint32_t row1, const float rgb[3]) {
+        if (!row1) return from_reflectance_rgb(rgb).clamp0();
+        return from_spd(spd_lam, d->emission_spd + (size_t)(row1 - 1) * FL_SPD_N, FL_SPD_N).clamp0();
+    };
     for (uint32_t m = 0; m < d->n_materials; ++m) {
         const fl_material &fm = d->materials[m];
         Material &M = s->mats[m];
+        if (fm.spd_r > d->n_emission || fm.spd_t > d->n_emission) { delete s; return FL_E_INVALID; }
         M.kind = fm.kind;
         M.ax = M.ay = 1.f; M.weight = 1.f; M.ior = 1.f;
         M.tex = 0;
         if (fm.kind == FL_MAT_LAMBERT) {
-            M.R = from_reflectance_rgb(fm.rgb).clamp0();                  // LambertMaterial: kd.spectrum_clamp()
+            M.R = refl(fm.spd_r, fm.rgb);                                   // LambertMaterial: kd.spectrum_clamp()
             if (fm.texture > d->n_textures) { delete s; return FL_E_INVALID; }
             M.tex = fm.texture;
         } else if (fm.kind == FL_MAT_CONDUCTOR) {                           // standardsurface.cpp:101-125
@@ -1603,8 +1615,8 @@ int or_scene_create(const fl_scene_desc *d, or_scene_t *out) {
             M.eta = T.eta[mt];
             M.k = T.k[mt];
         } else if (fm.kind == FL_MAT_DIELECTRIC) {                          // standardsurface.cpp:113-117
-            M.R = from_reflectance_rgb(fm.rgb).clamp0();
-            M.T = from_reflectance_rgb(fm.rgb_t).clamp0();
+            M.R = refl(fm.spd_r, fm.rgb);
+            M.T = refl(fm.spd_t, fm.rgb_t);
             M.ior = fm.ior;
         } else if (fm.kind == FL_MAT_STANDARD) {
             if (fm.standard >= d->n_standard_surfaces) { delete s; return FL_E_INVALID; }
@@ -1772,7 +1784,7 @@ int or_render(or_scene_t s, const fl_camera *cam, const or_render_params *p, flo
     auto worker = [&]() {
         Ctx c;
         c.s = s; c.dd = p->diffuse_depth; c.rd = p->reflection_depth; c.td = p->transmission_depth;
-        c.rr_thr = p->rr_threshold; c.rr_start = p->rr_start;
+        c.rr_thr = p->rr_threshold; c.rr_start = p->rr_start; c.mut = p->mutation;
         while (true) {
             uint32_t tile = next.fetch_add(1);
             if (tile >= tx * ty) break;

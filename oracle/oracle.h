@@ -64,7 +64,19 @@ typedef struct {
     uint32_t y0, y1;
     uint32_t x0, x1;
     int32_t integrator;    /* 0 PathIntegrator, 1 NormalIntegrator */
+    int32_t mutation;      /* OR_MUT_*: a planted mistake in Li, so that tests can show
+                              their checks reject it; 0 = the restatement as written */
 } or_render_params;
+
+/* planted mistakes of PathIntegrator::Li (test infrastructure) */
+enum {
+    OR_MUT_NONE = 0,
+    OR_MUT_RR_DIV_Q = 1,           /* roulette survivors divided by q instead of 1 - q */
+    OR_MUT_EMIT_ALWAYS = 2,        /* emission added at every hit, not only at bounce 0 / after specular */
+    OR_MUT_DEPTH_LATE = 3,         /* the diffuse depth break one bounce late */
+    OR_MUT_NO_ETASCALE = 4,        /* etaScale never updated */
+    OR_MUT_NO_SPECULAR_EMIT = 5    /* emission only at bounce 0, not after a specular bounce */
+};
 
 typedef struct {
     double seconds;

@@ -20,7 +20,11 @@ class RenderParams(C.Structure):
     _fields_ = [("spp", C.c_uint32), ("diffuse_depth", C.c_int32), ("reflection_depth", C.c_int32),
                 ("transmission_depth", C.c_int32), ("rr_threshold", C.c_float), ("rr_start", C.c_uint32),
                 ("seed", C.c_uint64), ("threads", C.c_int32), ("y0", C.c_uint32), ("y1", C.c_uint32),
-                ("x0", C.c_uint32), ("x1", C.c_uint32), ("integrator", C.c_int32)]
+                ("x0", C.c_uint32), ("x1", C.c_uint32), ("integrator", C.c_int32), ("mutation", C.c_int32)]
+
+
+# planted mistakes of Li (OR_MUT_* in oracle.h)
+MUT_NONE, MUT_RR_DIV_Q, MUT_EMIT_ALWAYS, MUT_DEPTH_LATE, MUT_NO_ETASCALE, MUT_NO_SPECULAR_EMIT = range(6)
 
 
 class RenderStats(C.Structure):
@@ -189,13 +193,16 @@ class Scene:
         lib().or_intersect2(self.h, rays.ctypes.data, t2.ctypes.data, id2.ctypes.data, n, threads)
         return t2, id2
 
-    def render(self, spp, depth, seed=1, threads=0, rows=None, cols=None, camera=None, rr_start=3, integrator=0):
+    def render(self, spp, depth, seed=1, threads=0, rows=None, cols=None, camera=None, rr_start=3, integrator=0,
+               depths=None, rr_threshold=1.0, mutation=MUT_NONE):
+        """depths: (diffuse, reflection, transmission) in place of depth for all three."""
         cam = camera if camera is not None else self.gen.camera
         W, H = cam.width, cam.height
         p = RenderParams()
         p.spp = spp
-        p.diffuse_depth = p.reflection_depth = p.transmission_depth = depth
-        p.rr_threshold = 1.0
+        p.diffuse_depth, p.reflection_depth, p.transmission_depth = depths if depths is not None else (depth,) * 3
+        p.rr_threshold = rr_threshold
+        p.mutation = mutation
         p.rr_start = rr_start
         p.seed = seed
         p.threads = threads

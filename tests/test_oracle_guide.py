@@ -151,3 +151,57 @@ def test_bf16_emulation_changes_little():
     b = oracle.guide_forward(P, rec[:, :64], bf16=True)
     rel = np.linalg.norm(a - b) / np.linalg.norm(a)
     assert 1e-5 < rel < 5e-2
+
+
+def numpy_forward(P, x):
+    """The network written with numpy matrix products: four hidden layers of
+    64 with ReLU, then the 40-output head; each of the 5 parameter blocks is
+    a row-major 64 x 64 weight (out, in) followed by 64 biases."""
+    h = x.astype(np.float64)
+    for k in range(5):
+        blk = P[k * 4160:(k + 1) * 4160]
+        W, b = blk[:4096].reshape(64, 64), blk[4096:]
+        z = h @ W.T + b
+        h = np.maximum(z, 0.0) if k < 4 else z
+    return h[:, :40]
+
+
+def numpy_mixture_pdf(raw, d):
+    """vMF mixture density of one head output: lobe i is raw[5i:5i+5] =
+    (mu direction unnormalised, log kappa clamped to [-3, 7], logit of the
+    weight); weights are the softmax of the logits; a lobe's density is
+    kappa / (2 pi (1 - e^{-2 kappa})) e^{kappa (mu.d - 1)}."""
+    r = raw.reshape(8, 5)
+    mu = r[:, :3] / np.linalg.norm(r[:, :3], axis=1, keepdims=True)
+    kappa = np.exp(np.clip(r[:, 3], -3.0, 7.0))
+    w = np.exp(r[:, 4] - r[:, 4].max())
+    w /= w.sum()
+    lobe = kappa / (2 * np.pi * (1 - np.exp(-2 * kappa))) * np.exp(kappa * (mu @ d - 1.0))
+    return float(np.dot(w, lobe))
+
+
+def test_forward_matches_numpy_matmul():
+    """oracle.guide_forward (FP64, no bf16) against numpy matrix products at
+    random parameters and inputs; a transposed weight or a missing bias
+    would move the outputs by O(1)."""
+    rng = np.random.default_rng(5)
+    P = rng.normal(0.0, 0.25, oracle.GUIDE_PARAMS)
+    x = rng.uniform(-1.0, 1.0, (64, 64)).astype(np.float32)
+    got = oracle.guide_forward(P, x, bf16=False)
+    want = numpy_forward(P, x)
+    assert np.allclose(got, want, rtol=1e-12, atol=1e-12 * np.abs(want).max())
+    # the comparison has power: a transposed layer differs
+    Pt = P.copy()
+    Pt[4160:4160 + 4096] = P[4160:4160 + 4096].reshape(64, 64).T.ravel()
+    assert np.abs(numpy_forward(Pt, x) - want).max() > 1e-3 * np.abs(want).max()
+
+
+def test_decoded_mixture_matches_numpy():
+    rng = np.random.default_rng(6)
+    for _ in range(20):
+        raw = rng.normal(0.0, 1.5, 40)
+        dirs = rng.normal(size=(8, 3))
+        dirs /= np.linalg.norm(dirs, axis=1, keepdims=True)
+        got = oracle.vmf_mixture_pdf(raw, dirs)
+        want = np.array([numpy_mixture_pdf(raw, d) for d in dirs])
+        assert np.allclose(got, want, rtol=1e-12)
