@@ -148,7 +148,7 @@ def test_checker_rejects_broken_trees(ctx, scenes):
     with ctx.scene(gen) as sc:
         nodes_raw, tris_raw = sc.bvh()
     check(nodes_raw, tris_raw, gen, 8)
-    # a leaf slot of the root's first internal child, shrunk to nothing
+    # the first leaf slot of the tree (in node order), shrunk to nothing
     nodes = nodes_raw.copy().view(NODE).ravel()
     n = None
     for i in range(len(nodes)):
@@ -161,7 +161,7 @@ def test_checker_rejects_broken_trees(ctx, scenes):
         check(nodes.view(np.uint8).reshape(-1, 128), tris_raw, gen, 8)
     # a leaf slot pointing at its neighbour's triangles: one record twice
     nodes = nodes_raw.copy().view(NODE).ravel()
-    m = nodes["meta"][n][s]
-    nodes["meta"][n][s] = (m & ~31) | (((m & 31) + 1) & 31)
+    m = int(nodes["meta"][n][s])
+    nodes["meta"][n][s] = (m & 0xE0) | (((m & 31) + 1) & 31)
     with pytest.raises(BvhError):
         check(nodes.view(np.uint8).reshape(-1, 128), tris_raw, gen, 8)
