@@ -122,6 +122,11 @@ int fl_ctx_create(int device, fl_ctx_t *out) {
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, device) == hipSuccess) c->n_cu = prop.multiProcessorCount;
     if (c->n_cu <= 0) c->n_cu = 256;
+    if (flx::noalloc_scratch_init(c) != FL_OK) {
+        (void)hipStreamDestroy(c->stream);
+        delete c;
+        return FL_E_HIP;
+    }
     *out = c;
     return FL_OK;
 }
@@ -394,6 +399,15 @@ int fl_intersect_device(fl_scene_t s, const fl_ray *d_rays, fl_hit *d_hits, uint
     if (!s || (n && (!d_rays || !d_hits))) return FL_E_INVALID;
     (void)hipSetDevice(s->ctx->device);
     return flx::trace_rays(s->ctx, s, d_rays, d_hits, n, flags & ~FL_COUNTERS, nullptr);
+}
+
+int fl_intersect_device_noalloc(fl_scene_t s, const fl_ray *d_rays, fl_hit *d_hits, uint32_t n, uint32_t flags,
+                                void *stream) {
+    if (!s || (n && (!d_rays || !d_hits))) return FL_E_INVALID;
+    if (flags & FL_COUNTERS)
+        return set_error(s->ctx, FL_E_INVALID, "fl_intersect_device_noalloc: FL_COUNTERS needs fl_intersect");
+    (void)hipSetDevice(s->ctx->device);
+    return flx::trace_rays(s->ctx, s, d_rays, d_hits, n, flags, nullptr, true, static_cast<hipStream_t>(stream));
 }
 
 int fl_intersect(fl_scene_t s, const fl_ray *rays, fl_hit *hits, uint32_t n, uint32_t flags, uint64_t *counters) {

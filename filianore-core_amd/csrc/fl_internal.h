@@ -57,6 +57,7 @@ struct fl_ctx_s {
     // scratch reused across calls (grown on demand)
     void *scratch = nullptr;
     size_t scratch_bytes = 0;
+    void *noalloc_scratch = nullptr;  // the device's fixed traversal scratch (fl_intersect_device_noalloc)
     void *develop_buf = nullptr;     // fl_develop / fl_develop_film staging (grown on demand)
     size_t develop_bytes = 0;
     hipEvent_t ev[16] = {};
@@ -148,8 +149,13 @@ void build_spectral_tables(fl_scene_s *s, const fl_scene_desc *desc, int mode,
 fl::StdParams standard_params(const fl_standard_surface &ss);
 
 // traversal (fl_trace.hip)
-int trace_rays(fl_ctx_t ctx, fl_scene_s *s, const fl_ray *d_rays, fl_hit *d_hits, uint32_t n,
-               uint32_t flags, unsigned long long *d_counters);
+// fixed_scratch: the refill cursor and spill area in the library's fixed
+// device array (no allocation); stream: nullptr = the context's stream
+int trace_rays(fl_ctx_t ctx, fl_scene_s *s, const fl_ray *d_rays, fl_hit *d_hits, uint32_t n, uint32_t flags,
+               unsigned long long *d_counters, bool fixed_scratch = false, hipStream_t stream = nullptr);
+// the device address of the fixed traversal scratch, looked up once per device
+int noalloc_scratch_init(fl_ctx_t ctx);
+size_t noalloc_scratch_bytes();
 
 // BSDF queries (fl_render.hip)
 int bsdf_queries(fl_ctx_t ctx, const fl_bsdf_query *d_q, fl_bsdf_result *d_r, uint32_t n);

@@ -4,8 +4,17 @@
 // Persistent grid sized by occupancy; lanes refill with new rays as their ray
 // finishes, so the global spill area of the short stack is bounded by the
 // grid, not by n.
+#include <mutex>
+#include <vector>
+
 #include "fl_internal.h"
 #include "fl_trace.h"
+
+// The fixed scratch of fl_intersect_device_noalloc (refill cursor, then the
+// spill area of the traversal stacks): every call on a device shares that
+// device's copy, so calls that overlap on one device race.
+constexpr size_t kNoAllocScratchBytes = size_t(16) << 20;
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
 
 namespace {
 
@@ -62,8 +71,17 @@ __global__ void __launch_bounds__(kBlock) k_trace_api(fl::TraceScene sc, ApiSrc 
     }
 }
 
+// Where a launch keeps its refill cursor (256 bytes) and its spill area:
+// the context's grown buffer, or a fixed area whose size bounds the grid.
+struct TraceScratch {
+    void *base;          // nullptr: grow ctx->scratch as needed
+    size_t bytes;        // capacity of a fixed area
+    hipStream_t stream;
+};
+
 template <bool ANY, bool SKIP, bool COUNT, bool PREFETCH>
-int launch(fl_ctx_t ctx, int depth, fl::TraceScene sc, ApiSrc src, uint32_t n, unsigned long long *c) {
+int launch(fl_ctx_t ctx, int depth, fl::TraceScene sc, ApiSrc src, uint32_t n, unsigned long long *c,
+           const TraceScratch &ts) {
     int occ = 0;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(
             &occ, reinterpret_cast<const void *>(k_trace_api<ANY, SKIP, COUNT, PREFETCH>), kBlock, 0) != hipSuccess ||
@@ -72,30 +90,70 @@ int launch(fl_ctx_t ctx, int depth, fl::TraceScene sc, ApiSrc src, uint32_t n, u
     uint32_t grid = (uint32_t)(ctx->n_cu * occ);
     uint32_t need = (n + kBlock - 1) / kBlock;
     if (grid > need) grid = need;
-    size_t spill_entries = (size_t)grid * kBlock * (size_t)(depth > kStack ? depth - kStack + 1 : 1);
-    size_t bytes = spill_entries * sizeof(uint2) + 256;
-    if (ctx->scratch_bytes < bytes) {
-        if (ctx->scratch) (void)hipFree(ctx->scratch);
-        ctx->scratch = nullptr;
-        ctx->scratch_bytes = 0;
-        if (hipMalloc(&ctx->scratch, bytes) != hipSuccess) return flx::set_error(ctx, FL_E_OOM, "spill stack");
-        ctx->scratch_bytes = bytes;
+    const size_t per_lane = sizeof(uint2) * (size_t)(depth > kStack ? depth - kStack + 1 : 1);
+    void *base = ts.base;
+    if (base) {
+        // a fixed area: fewer persistent blocks when the BVH is deep (each
+        // lane still takes rays until none are left)
+        const size_t fit = ts.bytes > 256 ? (ts.bytes - 256) / (per_lane * kBlock) : 0;
+        if (fit == 0)
+            return flx::set_error(ctx, FL_E_OVERFLOW, "BVH too deep for the fixed traversal scratch");
+        if (grid > fit) grid = (uint32_t)fit;
+    } else {
+        const size_t bytes = (size_t)grid * kBlock * per_lane + 256;
+        if (ctx->scratch_bytes < bytes) {
+            if (ctx->scratch) (void)hipFree(ctx->scratch);
+            ctx->scratch = nullptr;
+            ctx->scratch_bytes = 0;
+            if (hipMalloc(&ctx->scratch, bytes) != hipSuccess) return flx::set_error(ctx, FL_E_OOM, "spill stack");
+            ctx->scratch_bytes = bytes;
+        }
+        base = ctx->scratch;
     }
-    uint32_t *fetch = reinterpret_cast<uint32_t *>(ctx->scratch);
-    uint2 *spill = reinterpret_cast<uint2 *>(reinterpret_cast<char *>(ctx->scratch) + 256);
-    hipLaunchKernelGGL(k_zero_fetch, dim3(1), dim3(1), 0, ctx->stream, fetch);
-    hipLaunchKernelGGL((k_trace_api<ANY, SKIP, COUNT, PREFETCH>), dim3(grid), dim3(kBlock), 0, ctx->stream, sc, src, n,
+    uint32_t *fetch = reinterpret_cast<uint32_t *>(base);
+    uint2 *spill = reinterpret_cast<uint2 *>(reinterpret_cast<char *>(base) + 256);
+    // the cursor starts at zero: a kernel, not a memset, so that the call can
+    // be captured into a graph and replayed
+    hipLaunchKernelGGL(k_zero_fetch, dim3(1), dim3(1), 0, ts.stream, fetch);
+    hipLaunchKernelGGL((k_trace_api<ANY, SKIP, COUNT, PREFETCH>), dim3(grid), dim3(kBlock), 0, ts.stream, sc, src, n,
                        spill, fetch, c);
     return FL_OK;
 }
+
+std::mutex g_scratch_mu;
+std::vector<void *> g_scratch_ptr;   // per device: the device address of `scratch`
 
 } // namespace
 
 namespace flx {
 
+int noalloc_scratch_init(fl_ctx_t ctx) {
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (g_scratch_ptr.size() <= (size_t)ctx->device) g_scratch_ptr.resize((size_t)ctx->device + 1, nullptr);
+    if (!g_scratch_ptr[ctx->device]) {
+        // the device current (fl_ctx_create set it): each device holds its own
+        // copy of the array at its own address; host code uses only that
+        // address, never the array's name
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) return set_error(ctx, FL_E_HIP, "traversal scratch: hipGetSymbolAddress failed");
+        g_scratch_ptr[ctx->device] = p;
+    }
+    ctx->noalloc_scratch = g_scratch_ptr[ctx->device];
+    return FL_OK;
+}
+
+size_t noalloc_scratch_bytes() { return kNoAllocScratchBytes; }
+
 int trace_rays(fl_ctx_t ctx, fl_scene_s *s, const fl_ray *d_rays, fl_hit *d_hits, uint32_t n, uint32_t flags,
-               unsigned long long *d_counters) {
+               unsigned long long *d_counters, bool fixed_scratch, hipStream_t stream) {
     if (n == 0) return FL_OK;
+    TraceScratch ts{nullptr, 0, stream ? stream : ctx->stream};
+    if (fixed_scratch) {
+        if (!ctx->noalloc_scratch) return set_error(ctx, FL_E_INVALID, "traversal scratch not set up");
+        ts.base = ctx->noalloc_scratch;
+        ts.bytes = kNoAllocScratchBytes;
+    }
     fl::TraceScene sc{s->d_nodes, s->d_tris};
     ApiSrc src{d_rays, d_hits, (flags & FL_ANY_HIT) != 0, sc};
     const int depth = (int)s->depth;
@@ -104,17 +162,17 @@ int trace_rays(fl_ctx_t ctx, fl_scene_s *s, const fl_ray *d_rays, fl_hit *d_hits
     const bool pf = s->n_tris >= fl::kPrefetchTris;
     int rc;
     if (cnt) {
-        if (!any) rc = launch<false, false, true, false>(ctx, depth, sc, src, n, d_counters);
-        else if (skip) rc = launch<true, true, true, false>(ctx, depth, sc, src, n, d_counters);
-        else rc = launch<true, false, true, false>(ctx, depth, sc, src, n, d_counters);
+        if (!any) rc = launch<false, false, true, false>(ctx, depth, sc, src, n, d_counters, ts);
+        else if (skip) rc = launch<true, true, true, false>(ctx, depth, sc, src, n, d_counters, ts);
+        else rc = launch<true, false, true, false>(ctx, depth, sc, src, n, d_counters, ts);
     } else if (pf) {
-        if (!any) rc = launch<false, false, false, true>(ctx, depth, sc, src, n, d_counters);
-        else if (skip) rc = launch<true, true, false, true>(ctx, depth, sc, src, n, d_counters);
-        else rc = launch<true, false, false, true>(ctx, depth, sc, src, n, d_counters);
+        if (!any) rc = launch<false, false, false, true>(ctx, depth, sc, src, n, d_counters, ts);
+        else if (skip) rc = launch<true, true, false, true>(ctx, depth, sc, src, n, d_counters, ts);
+        else rc = launch<true, false, false, true>(ctx, depth, sc, src, n, d_counters, ts);
     } else {
-        if (!any) rc = launch<false, false, false, false>(ctx, depth, sc, src, n, d_counters);
-        else if (skip) rc = launch<true, true, false, false>(ctx, depth, sc, src, n, d_counters);
-        else rc = launch<true, false, false, false>(ctx, depth, sc, src, n, d_counters);
+        if (!any) rc = launch<false, false, false, false>(ctx, depth, sc, src, n, d_counters, ts);
+        else if (skip) rc = launch<true, true, false, false>(ctx, depth, sc, src, n, d_counters, ts);
+        else rc = launch<true, false, false, false>(ctx, depth, sc, src, n, d_counters, ts);
     }
     if (rc) return rc;
     FL_HIP_CHECK(ctx, hipGetLastError());

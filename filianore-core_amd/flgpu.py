@@ -197,7 +197,7 @@ class GenScene(C.Structure):
 EXPORTS = [
     "fl_abi_version", "fl_device_count", "fl_probe_copy", "fl_ctx_create", "fl_ctx_destroy", "fl_last_error",
     "fl_scene_create", "fl_scene_destroy", "fl_scene_get_info", "fl_scene_download_bvh",
-    "fl_intersect", "fl_intersect_device", "fl_render", "fl_render_batch", "fl_autotune", "fl_autotune_batch",
+    "fl_intersect", "fl_intersect_device", "fl_intersect_device_noalloc", "fl_render", "fl_render_batch", "fl_autotune", "fl_autotune_batch",
     "fl_guide_create", "fl_guide_destroy", "fl_guide_train_step", "fl_guide_forward",
     "fl_guide_get_params", "fl_guide_set_params", "fl_guide_gradient", "fl_guide_stats",
     "fl_bsdf_query_batch", "fl_bsdf_query_standard", "fl_sampler_draw", "fl_develop_film", "fl_develop",
@@ -227,6 +227,7 @@ def lib():
         L.fl_scene_download_bvh.argtypes = [vp, vp, C.c_size_t, vp, C.c_size_t]
         L.fl_intersect.argtypes = [vp, vp, vp, C.c_uint32, C.c_uint32, C.POINTER(C.c_uint64)]
         L.fl_intersect_device.argtypes = [vp, vp, vp, C.c_uint32, C.c_uint32]
+        L.fl_intersect_device_noalloc.argtypes = [vp, vp, vp, C.c_uint32, C.c_uint32, vp]
         L.fl_render.argtypes = [vp, C.POINTER(Camera), C.POINTER(RenderParams), C.POINTER(Film), C.POINTER(Stats)]
         L.fl_render_batch.argtypes = [vp, C.POINTER(BatchItem), C.c_uint32, C.POINTER(RenderParams),
                                       C.POINTER(Stats)]

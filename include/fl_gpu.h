@@ -310,6 +310,19 @@ int fl_intersect(fl_scene_t scene, const fl_ray *rays, fl_hit *hits, uint32_t n,
 /* Same on device-resident arrays, ordered on the context stream. */
 int fl_intersect_device(fl_scene_t scene, const fl_ray *d_rays, fl_hit *d_hits,
                         uint32_t n, uint32_t flags);
+/* fl_intersect_device without any allocation. No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under
+ * HIP graph capture, serving loops, harnesses that forbid hipMalloc). The
+ * traversal's refill cursor and stack spill area live in a fixed 16 MiB
+ * device array of the library (one per GPU, held until the process ends)
+ * instead of a buffer the context grows; a deep BVH runs fewer persistent
+ * blocks so that its spill area fits (FL_E_OVERFLOW if not even one block
+ * fits). Results are those of fl_intersect_device bit for bit. stream: a
+ * hipStream_t to order the work on (NULL: the context's stream). Every call
+ * on one GPU shares that array: calls that overlap on one GPU race, so order
+ * them on one stream. FL_COUNTERS is rejected. */
+int fl_intersect_device_noalloc(fl_scene_t scene, const fl_ray *d_rays, fl_hit *d_hits,
+                                uint32_t n, uint32_t flags, void *stream);
 
 /* Renders `params->spp` samples per pixel (PathIntegrator::Li for every
  * camera sample).  film may be NULL (result stays on the device). */
