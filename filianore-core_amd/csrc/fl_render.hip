@@ -1013,7 +1013,13 @@ __global__ void __launch_bounds__(kBlock) k_resolve(RenderArgs a, const float *c
     if (x >= it.cam.W || y >= it.cam.H) return;
     const int S = a.grid.stride;
     const float *cx = cmf, *cy = cmf + S, *cz = cmf + 2 * S;
-    float sx = 0.f, sy = 0.f, sz = 0.f, qx = 0.f, qy = 0.f, qz = 0.f;
+    // samples are added to the film one at a time in sample order, starting
+    // from the sums of the earlier passes: the film's bits then do not depend
+    // on how the samples were split into passes (a batch item and the same
+    // render alone split them differently)
+    const size_t pix = (size_t)it.pix_base + y * it.cam.W + x;
+    float sx = a.film_sum[3 * pix + 0], sy = a.film_sum[3 * pix + 1], sz = a.film_sum[3 * pix + 2];
+    float qx = a.film_sq[3 * pix + 0], qy = a.film_sq[3 * pix + 1], qz = a.film_sq[3 * pix + 2];
     uint32_t nan_count = 0;
     for (uint32_t s = 0; s < it.spp_pass; ++s) {
         const uint32_t sid = it.path_base + s * it.npix + pl;
@@ -1033,13 +1039,12 @@ __global__ void __launch_bounds__(kBlock) k_resolve(RenderArgs a, const float *c
         sx += X; sy += Y; sz += Z;
         qx += X * X; qy += Y * Y; qz += Z * Z;
     }
-    const size_t pix = (size_t)it.pix_base + y * it.cam.W + x;
-    a.film_sum[3 * pix + 0] += sx;
-    a.film_sum[3 * pix + 1] += sy;
-    a.film_sum[3 * pix + 2] += sz;
-    a.film_sq[3 * pix + 0] += qx;
-    a.film_sq[3 * pix + 1] += qy;
-    a.film_sq[3 * pix + 2] += qz;
+    a.film_sum[3 * pix + 0] = sx;
+    a.film_sum[3 * pix + 1] = sy;
+    a.film_sum[3 * pix + 2] = sz;
+    a.film_sq[3 * pix + 0] = qx;
+    a.film_sq[3 * pix + 1] = qy;
+    a.film_sq[3 * pix + 2] = qz;
     if (nan_count) atomicAdd(&a.stats[2], (unsigned long long)nan_count);
 }
 

@@ -411,9 +411,12 @@ void batch_scene(Builder &B, const fl_gen_options *o, fl_gen_scene *out) {
     double tgt = std::exp(std::log(1e3) + rng.uniform() * (std::log(2e6) - std::log(1e3)));
     int best_n = 1, best_l = 0;
     double best_err = 1e30;
+    // the closest count of n spheres at subdivision l (+ ground and light
+    // quads) that stays inside [1e3, 2e6]
     for (int l = 0; l <= 7; ++l)
         for (int n = 1; n <= 8; ++n) {
-            double tris = (double)n * 20.0 * std::pow(4.0, l) + 2.0;
+            double tris = (double)n * 20.0 * std::pow(4.0, l) + 4.0;
+            if (tris < 1e3 || tris > 2e6) continue;
             double err = std::fabs(std::log(tris / tgt));
             if (err < best_err) { best_err = err; best_n = n; best_l = l; }
         }
