@@ -45,13 +45,46 @@ def traffic_from_profile():
         return json.load(f).get("hbm_bytes_per_launch")
 
 
-def host_threads():
-    n = os.cpu_count() or 1
+# The GPU box allots this many host CPUs to a process on its one GPU (worker
+# pools are sized to it whatever the machine's total core count is).
+HOST_CPU_SHARE = 16
+
+
+def host_info():
+    """Host CPUs: online count, the process's affinity mask, the CPU model,
+    and the threads the CPU baseline uses (the affinity mask, at most the
+    box's per-GPU share)."""
+    online = os.cpu_count() or 1
     try:
-        n = len(os.sched_getaffinity(0))
+        affinity = len(os.sched_getaffinity(0))
     except AttributeError:
+        affinity = online
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
         pass
-    return max(1, min(16, n))
+    return {"nproc": online, "affinity": affinity, "model": model,
+            "threads": max(1, min(HOST_CPU_SHARE, affinity))}
+
+
+def host_threads():
+    return host_info()["threads"]
+
+
+def issue_from_profile():
+    """What bounds k_trace besides bytes: VALU issue and lane utilisation,
+    L2 hit rate, from rocprofv3 SQ/TCC counters recorded under profiles/;
+    None until measured."""
+    path = os.path.join(ROOT, "profiles", "issue_k_trace_spheres.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        return json.load(f)
 
 
 def cpu_baseline(gen, spp, depth, budget_s, threads):
@@ -81,9 +114,66 @@ GUIDE_FWD_FLOP = 2 * 18240          # algorithmic FLOP per network query (53*64 
 GUIDE_TRAIN_FLOP = 3 * GUIDE_FWD_FLOP
 
 
-def rel_mse(img, ref):
+# CIE XYZ -> linear sRGB (D65), the matrix of xyz_to_rgb
+# (color/spectrumoperations.cpp:58-66)
+XYZ_TO_SRGB = ((3.240479, -1.537150, -0.498535),
+               (-0.969256, 1.875991, 0.041556),
+               (0.055648, -0.204043, 1.057311))
+
+
+def to_srgb(xyz):
     import numpy as np
-    return float(np.mean((img - ref) ** 2 / (ref ** 2 + 0.01)))
+    return xyz @ np.asarray(XYZ_TO_SRGB, np.float64).T
+
+
+def intersection_section(flgpu, ctx, gen, sc, cpu_seconds, threads):
+    """The 4M-ray intersection workload on the spheres scene: 2,097,152
+    coherent primary rays (2048 x 1024 pixel centres) and 2,097,152
+    cosine-diffuse secondaries from their hits. GPU Mrays/s with the rays
+    resident (fl_intersect_device), the CPU BVH's Mrays/s on a bounded slice
+    (host threads), hit IDs compared bit for bit on that slice, and the
+    near-ties (best two distinct triangles within 1e-6 relative t) counted
+    on it."""
+    import numpy as np
+    from raysets import primary_rays, secondary_rays
+    prim = primary_rays(gen.camera, 2048, 1024)
+    pt, ptri, _ = sc.intersect(prim)
+    sec = secondary_rays(gen.triangles(), prim, pt, ptri, prim.shape[0])
+    rays = np.concatenate([prim, sec])
+    out = {"rays": int(rays.shape[0])}
+    for name, sub_rays in (("all", rays), ("primary", prim), ("secondary", sec)):
+        t, tri, uv, ms = sc.intersect_resident(sub_rays, reps=5)
+        out["gpu_mrays_s_" + name] = round(sub_rays.shape[0] / ms / 1e3, 1)
+        if name == "all":
+            g_t, g_tri, g_uv = t, tri, uv
+    if cpu_seconds > 0:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle
+        ors = oracle.Scene(gen)
+        # an interleaved slice of both halves, grown until the time budget
+        n, secs, done, mism, ties, checked = 1 << 14, 0.0, 0, 0, 0, 0
+        rng = np.random.default_rng(3)
+        while secs < cpu_seconds and done < rays.shape[0]:
+            idx = np.sort(rng.choice(rays.shape[0], min(n, rays.shape[0] - done), replace=False))
+            ts = time.perf_counter()
+            ot, otri, ouv = ors.intersect(rays[idx], threads=threads)
+            secs += time.perf_counter() - ts
+            done += idx.size
+            mism += int(np.sum(otri != g_tri[idx]))
+            both = (otri == g_tri[idx]) & (otri != flgpu.FL_NO_HIT)
+            mism += int(np.sum(ot[both].view(np.uint32) != g_t[idx][both].view(np.uint32)))
+            t2, _ = ors.intersect2(rays[idx], threads=threads)
+            ok = np.isfinite(t2[:, 1]) & (t2[:, 0] > 0)
+            gap = np.abs(np.where(ok, t2[:, 1], 0.0) - np.where(ok, t2[:, 0], 0.0))
+            ties += int(np.sum(ok & (gap <= 1e-6 * t2[:, 0])))
+            checked += idx.size
+            n *= 2
+        ors.close()
+        out.update({"cpu_mrays_s": round(done / secs / 1e6, 2), "cpu_threads": threads,
+                    "cpu_rays_timed": done, "checked_rays": checked, "hit_mismatches": mism,
+                    "near_ties": ties,
+                    "gpu_over_cpu": round(out["gpu_mrays_s_all"] / (done / secs / 1e6), 1)})
+    return out
 
 
 def batch_section(flgpu, ctx, n_scenes, tune):
@@ -152,21 +242,27 @@ def guided_section(flgpu, ctx, config, res, ref_spp, spp=64):
            "resolution": res, "triangles": gen.n_triangles, "depth": gen.depth, "reference_spp": ref_spp}
     with ctx.scene(gen) as sc:
         ts = time.perf_counter()
-        rs = rq = None
-        parts = max(1, ref_spp // 4096)
+        halves = [None, None]
+        parts = max(2, ref_spp // 4096)
         for k in range(parts):
             p = flgpu.default_params(spp=ref_spp // parts, depth=gen.depth, seed=1000 + k, flags=base)
-            s_, q_, _ = sc.render(p)
-            rs = s_.astype(np.float64) if rs is None else rs + s_
-            rq = q_.astype(np.float64) if rq is None else rq + q_
+            s_, _, _ = sc.render(p)
+            h = k * 2 // parts
+            halves[h] = s_.astype(np.float64) if halves[h] is None else halves[h] + s_
             log("  reference part %d/%d (%.0f s)" % (k + 1, parts, time.perf_counter() - ts))
-        ref = rs / ref_spp
-        ref_var = np.maximum(rq / ref_spp - ref * ref, 0.0) / ref_spp
+        # relMSE in linear sRGB against the mean of two independent halves;
+        # the reference's own noise, the variance of that mean, is estimated
+        # per pixel and channel without bias by (A - B)^2 / 4 and subtracted
+        A = to_srgb(halves[0] / (ref_spp // 2))
+        B = to_srgb(halves[1] / (ref_spp // 2))
+        ref = 0.5 * (A + B)
+        ref_var = 0.25 * (A - B) ** 2
         den = ref * ref + 0.01
         out["reference_s"] = round(time.perf_counter() - ts, 1)
         out["reference_noise_relmse"] = round(float(np.mean(ref_var / den)), 5)
 
-        def relmse(img):
+        def relmse(img_xyz):
+            img = to_srgb(img_xyz.astype(np.float64))
             return float(np.mean(((img - ref) ** 2 - ref_var) / den))
 
         def timed(p):
@@ -283,8 +379,9 @@ def main():
     ap.add_argument("--batch-scenes", type=int, default=64)
     ap.add_argument("--guided-res", type=int, default=1024)
     ap.add_argument("--guided-ref-spp", type=int, default=16384)
-    ap.add_argument("--guided-skylight", action="store_true",
-                    help="also run the guided comparison on the skylight reading of the room")
+    ap.add_argument("--no-guided-skylight", action="store_true",
+                    help="skip the guided comparison on the skylight reading of the room")
+    ap.add_argument("--no-intersect", action="store_true", help="skip the 4M-ray intersection section")
     ap.add_argument("--terrain", action="store_true", help="also render the full terrain config (8 x 4K x 256 spp)")
     ap.add_argument("--terrain-spp", type=int, default=256)
     ap.add_argument("--terrain-res-div", type=int, default=1)
@@ -351,6 +448,14 @@ def main():
                 "ms_trace": round(tst.ms_extend, 2), "ms_shade": round(tst.ms_shade, 2),
                 "launches": launches,
             }
+            issue = issue_from_profile()
+            if issue:
+                roofline["issue"] = issue
+            if not args.no_intersect:
+                log("intersection section: 4M rays")
+                out["intersection"] = intersection_section(flgpu, ctx, gen, sc, 0.0 if args.no_cpu else
+                                                           args.cpu_seconds / 2, host_threads())
+                log("intersection: %s" % out["intersection"])
             out["mrays_per_s"] = round(rays * args.steps / wall / 1e6, 1)
             out["rays_per_sample"] = round(rays / samples, 3)
             out["bvh_build_ms"] = round(info.build_ms, 2)
@@ -364,7 +469,7 @@ def main():
             log("guided section: %d^2, reference %d spp" % (args.guided_res, args.guided_ref_spp))
             out["guided"] = guided_section(flgpu, ctx, flgpu.FL_GEN_GUIDED, args.guided_res, args.guided_ref_spp)
             log("guided: %s" % out["guided"])
-            if args.guided_skylight:
+            if not args.no_guided_skylight:
                 out["guided_skylight"] = guided_section(flgpu, ctx, flgpu.FL_GEN_GUIDED_SKYLIGHT, args.guided_res,
                                                         args.guided_ref_spp)
                 log("guided skylight: %s" % out["guided_skylight"])
@@ -395,17 +500,26 @@ def main():
                 c_samples = cgen.camera.width * cgen.camera.height * cgen.spp
                 cornell = {"gpu_msamples_s": round(c_samples * len(ct) / sum(ct) / 1e6, 2)}
 
-    threads = host_threads()
+    host = host_info()
+    threads = host["threads"]
     cpu = None
     if not args.no_cpu:
         v, ns, secs = cpu_baseline(gen, 1, gen.depth, args.cpu_seconds, threads)
         cpu = {"value": round(v, 4), "unit": "Msamples/s", "cores": threads, "kind": "port",
+               "host_nproc": host["nproc"], "host_affinity": host["affinity"], "host_model": host["model"],
+               "per_thread": round(v / threads, 4),
                "sample": "%d samples (1 spp, rows of the same %dx%d image, depth %d), %.1f s" % (ns, W, H, gen.depth,
                                                                                              secs)}
         if cornell is not None:
             cv, cns, csecs = cpu_baseline(cgen, cgen.spp, cgen.depth, args.cpu_seconds / 3, threads)
+            # the ratio against every online host core, assuming the CPU
+            # rate scaled linearly from the threads used (a bound in the
+            # CPU's favour)
             cornell.update({"cpu_msamples_s": round(cv, 4), "cpu_cores": threads,
                             "speedup": round(cornell["gpu_msamples_s"] / cv, 1),
+                            "cpu_per_thread": round(cv / threads, 4),
+                            "speedup_vs_all_host_cores_linear": round(
+                                cornell["gpu_msamples_s"] / (cv / threads * host["nproc"]), 1),
                             "cpu_sample": "%d samples of the 256x256x16 image" % cns})
         out["speedup_vs_cpu"] = round(value / v, 1)
 

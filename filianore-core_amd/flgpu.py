@@ -268,6 +268,23 @@ def lib():
     return _lib
 
 
+_hip_rt = None
+
+
+def _hip():
+    """The HIP runtime libfl_gpu.so links (already loaded with it): device
+    buffers for the *_device entry points."""
+    global _hip_rt
+    if _hip_rt is None:
+        lib()
+        h = C.CDLL("libamdhip64.so.7")
+        h.hipMalloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t]
+        h.hipFree.argtypes = [C.c_void_p]
+        h.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+        _hip_rt = h
+    return _hip_rt
+
+
 class FlError(RuntimeError):
     pass
 
@@ -636,6 +653,36 @@ class Scene:
         if counters:
             return t, tri, uv, (int(c[0]), int(c[1]))
         return t, tri, uv
+
+    def intersect_resident(self, rays, flags=FL_CLOSEST_HIT, reps=5):
+        """fl_intersect_device on rays copied once to device memory: returns
+        (t, tri, uv) and the wall time per call in ms over `reps` calls
+        queued back to back (PCIe excluded)."""
+        import time
+        rays = np.ascontiguousarray(rays, np.float32)
+        n = rays.shape[0]
+        hip = _hip()
+        d_r, d_h = C.c_void_p(), C.c_void_p()
+        if hip.hipMalloc(C.byref(d_r), rays.nbytes) or hip.hipMalloc(C.byref(d_h), n * 16):
+            raise FlError("hipMalloc failed")
+        try:
+            if hip.hipMemcpy(d_r, rays.ctypes.data, rays.nbytes, 1) or hip.hipDeviceSynchronize():
+                raise FlError("upload failed")
+            _check(self.ctx.L.fl_intersect_device(self.h, d_r, d_h, n, flags), self.ctx.h, "fl_intersect_device")
+            hip.hipDeviceSynchronize()
+            t0 = time.perf_counter()
+            for _ in range(reps):
+                _check(self.ctx.L.fl_intersect_device(self.h, d_r, d_h, n, flags), self.ctx.h, "fl_intersect_device")
+            if hip.hipDeviceSynchronize():
+                raise FlError("fl_intersect_device failed")
+            ms = (time.perf_counter() - t0) / reps * 1e3
+            hits = np.zeros((n, 4), np.float32)
+            if hip.hipMemcpy(hits.ctypes.data, d_h, hits.nbytes, 2):
+                raise FlError("download failed")
+        finally:
+            hip.hipFree(d_r)
+            hip.hipFree(d_h)
+        return hits[:, 0].copy(), hits[:, 1].view(np.uint32).copy(), hits[:, 2:4].copy(), ms
 
     def render(self, params, camera=None, want_film=True):
         cam = camera if camera is not None else self.gen.camera

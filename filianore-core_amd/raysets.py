@@ -1,4 +1,4 @@
-"""Ray sets for intersection parity: coherent primary rays and cosine-diffuse
+"""Ray sets of the 4M-ray intersection workload: coherent primary rays and cosine-diffuse
 secondary rays spawned from primary hits (spawn rule of
 core/interaction.h:11-31 / :60-63: offset along the geometric normal by
 |n|.pError, pError = |p|*gamma(5), tMin = 1e-4)."""

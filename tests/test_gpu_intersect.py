@@ -92,6 +92,7 @@ def test_config1_4m_rays_bit_exact(ctx):
     # near ties: best two distinct triangles within 1e-6 relative t (reported)
     t2, _ = ors.intersect2(rays[::8], threads=16)
     ok = np.isfinite(t2[:, 1]) & (t2[:, 0] > 0)
-    ties = int(np.sum(ok & (np.abs(t2[:, 1] - t2[:, 0]) <= 1e-6 * t2[:, 0])))
+    gap = np.abs(np.where(ok, t2[:, 1], 0.0) - np.where(ok, t2[:, 0], 0.0))
+    ties = int(np.sum(ok & (gap <= 1e-6 * t2[:, 0])))
     print("4M rays: id mismatches %d, near-ties (1/8 sample) %d" % (mism, ties))
     assert mism == 0 and tb and uvb
