@@ -38,51 +38,68 @@ __device__ __forceinline__ uint32_t hash_u32(uint32_t x) {
     return x;
 }
 
-// API records: already encoded
+// Sources of training samples for k_guide_grad: valid(i), the encoded
+// inputs [16 j, 16 j + 16) of sample i, and its direction and weight.
+// API records: already encoded.
 struct EncSrc {
     const float *rec;
     uint32_t n;
-    __device__ bool get(uint32_t i, float *x, v3 &dir, float &c) const {
-        if (i >= n) return false;
+    __device__ bool valid(uint32_t i) const { return i < n; }
+    __device__ void quarter(uint32_t i, int j, float *x16) const {
+        const float *r = rec + (size_t)i * kRecEnc + 16 * j;
+        for (int k = 0; k < 16; ++k) x16[k] = r[k];
+    }
+    __device__ void target(uint32_t i, v3 &dir, float &c) const {
         const float *r = rec + (size_t)i * kRecEnc;
-        for (int k = 0; k < 64; ++k) x[k] = r[k];
         dir = v3(r[64], r[65], r[66]);
         c = r[67];
-        return true;
     }
 };
 
-// Render records, drawn with probability proportional to their weight (the
-// inclusive prefix sum `cdf` of the weights, in record order) by a counter
-// hash, encoded here. Each drawn sample then carries unit weight: the batch
-// gradient has the expectation of the weighted gradient over all records (up
-// to the constant total weight), and no batch entry is spent on the many
-// records whose path gathered nothing beyond the vertex.
+// Render records, drawn with probability proportional to their weight
+// (k_guide_draw) and encoded here. Each drawn sample carries unit weight: the
+// batch gradient has the expectation of the weighted gradient over all
+// records (up to the constant total weight), and no batch entry is spent on
+// the many records whose path gathered nothing beyond the vertex.
 struct RawSrc {
     const float *rec;
-    const double *cdf;
-    uint32_t n_recs, n_batch, step;
-    uint32_t seed;
-    double total;
-    __device__ bool get(uint32_t i, float *x, v3 &dir, float &c) const {
-        if (i >= n_batch || n_recs == 0 || !(total > 0.0)) return false;
-        const uint32_t h = hash_u32(hash_u32(seed ^ (step * 0x9e3779b9u)) ^ (i * 0x85ebca6bu));
-        const double u = ((double)h + 0.5) * (1.0 / 4294967296.0) * total;
-        // first record whose prefix sum exceeds u (zero-weight records never are)
-        uint32_t lo = 0, hi = n_recs - 1;
-        while (lo < hi) {
-            const uint32_t mid = (lo + hi) >> 1;
-            if (cdf[mid] > u) hi = mid;
-            else lo = mid + 1;
-        }
-        const float *R = rec + (size_t)lo * kRecRaw;
+    const uint32_t *draw;   // this step's record indices, n_batch of them
+    uint32_t n_batch;
+    __device__ bool valid(uint32_t i) const { return i < n_batch; }
+    __device__ void quarter(uint32_t i, int j, float *x16) const {
+        const float *R = rec + (size_t)draw[i] * kRecRaw;
         const float pos[3] = {R[0], R[1], R[2]};
-        guide_encode(pos, v3(R[3], R[4], R[5]), R[6], R[7], x);
+        guide_encode_quarter(j, pos, v3(R[3], R[4], R[5]), R[6], R[7], x16);
+    }
+    __device__ void target(uint32_t i, v3 &dir, float &c) const {
+        const float *R = rec + (size_t)draw[i] * kRecRaw;
         dir = v3(R[8], R[9], R[10]);
         c = 1.f;
-        return true;
     }
 };
+
+// The record indices of `steps` Adam steps of `batch` draws each, all at
+// once (the searches of a step then no longer sit on its critical path):
+// draw i of step k takes the first record whose weight prefix sum exceeds a
+// hashed uniform in [0, total). Step k hashes with seed0 ^ (step0 + k), the
+// guide's step counter at that step.
+__global__ void k_guide_draw(const double *cdf, uint32_t n_recs, double total, uint32_t seed0, uint32_t step0,
+                             uint32_t steps, uint32_t batch, uint32_t *out) {
+    const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (size_t)steps * batch) return;
+    const uint32_t k = (uint32_t)(idx / batch), i = (uint32_t)(idx - (size_t)k * batch);
+    const uint32_t seed = seed0 ^ (step0 + k);
+    const uint32_t h = hash_u32(hash_u32(seed ^ (k * 0x9e3779b9u)) ^ (i * 0x85ebca6bu));
+    const double u = ((double)h + 0.5) * (1.0 / 4294967296.0) * total;
+    // first record whose prefix sum exceeds u (zero-weight records never are)
+    uint32_t lo = 0, hi = n_recs - 1;
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (cdf[mid] > u) hi = mid;
+        else lo = mid + 1;
+    }
+    out[idx] = lo;
+}
 
 // Clean weight of render record i (0 unless positive and finite), as double
 // for the prefix sum.
@@ -94,91 +111,142 @@ struct RecWeight {
     }
 };
 
-// Forward + backward of 64 samples per block (one wave). partial receives
-// this block's gradient sums, loss_part its loss sum.
+// Forward + backward of 64 samples per block of 4 waves, 16 samples per
+// wave. The layers' activations of all 64 samples stay in LDS, by sample and
+// by unit (the weight gradient's operands are sample-contiguous); each wave
+// runs the forward and the data gradient of its own 16 samples (16 x 64
+// tiles), and the 64 x 64 weight gradient over the block's samples is split
+// by rows, 16 per wave. partial receives this block's gradient sums,
+// loss_part its loss sum. Summation orders are fixed: a step is
+// deterministic.
+constexpr int kGradBlock = 256;
 template <class Src>
-__global__ void __launch_bounds__(64) k_guide_grad(const __bf16 *W, const __bf16 *WT, const float *P, Src src,
-                                                   float *partial, float *loss_part) {
-    __shared__ __bf16 sA[kGMats][64 * kGRow];    // X, H1..H4 as [sample][unit]
-    __shared__ __bf16 sAT[kGMats][64 * kGRow];   // the same, [unit][sample]
-    __shared__ __bf16 sD[64 * kGRow], sDT[64 * kGRow];
-    __shared__ float sR[64 * 65];
-    const int q = threadIdx.x;
+__global__ void __launch_bounds__(kGradBlock) k_guide_grad(const __bf16 *W, const __bf16 *WT, const float *P, Src src,
+                                                           float *partial, float *loss_part) {
+    __shared__ __attribute__((aligned(16))) __bf16 sA[kGMats][64 * kGRow];    // X, H1..H4 as [sample][unit]
+    __shared__ __attribute__((aligned(16))) __bf16 sAT[kGMats][64 * kGRow];   // the same, [unit][sample]
+    __shared__ __attribute__((aligned(16))) __bf16 sD[64 * kGRow], sDT[64 * kGRow];
+    __shared__ float sR[64 * 65];                                             // head outputs, fp32 deltas
+    __shared__ float sRed[4][64], sLoss[4];
+    const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+    const int q0 = 16 * w;                       // the wave's first sample
     const uint32_t base = blockIdx.x * 64u;
-    float x[64];
-    v3 dir(0.f, 0.f, 1.f);
-    float c = 0.f;
-    const bool ok = src.get(base + q, x, dir, c);
-    if (!ok)
-        for (int i = 0; i < 64; ++i) x[i] = 0.f;
-    for (int i = 0; i < 64; ++i) {
-        const __bf16 h = to_bf16(x[i]);
-        sA[0][q * kGRow + i] = h;
-        sAT[0][i * kGRow + q] = h;
+    // inputs: lane (sample q0 + l % 16, quarter l / 16 of its 64 inputs)
+    {
+        const int s = q0 + (l & 15), j = l >> 4;
+        float x16[16];
+        if (src.valid(base + s)) src.quarter(base + s, j, x16);
+        else
+            for (int e = 0; e < 16; ++e) x16[e] = 0.f;
+        bf16x8 v0, v1;
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+            v0[e] = to_bf16(x16[e]);
+            v1[e] = to_bf16(x16[8 + e]);
+        }
+        *reinterpret_cast<bf16x8 *>(sA[0] + s * kGRow + 16 * j) = v0;
+        *reinterpret_cast<bf16x8 *>(sA[0] + s * kGRow + 16 * j + 8) = v1;
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+            sAT[0][(16 * j + e) * kGRow + s] = v0[e];
+            sAT[0][(16 * j + 8 + e) * kGRow + s] = v1[e];
+        }
     }
     __syncthreads();
+    // forward of the wave's samples: H[q][o] = relu(sum_i W[o][i] H[q][i] + b[o])
+    const int qn = q0 + (l & 15);                // this lane's sample in the accumulators
     for (int k = 0; k < 4; ++k) {
-        layer_fwd(W + k * 4096, P + k * kGLayerParams + 4096, sA[k], sA[k + 1], sAT[k + 1], true);
+        f32x4 acc[4][1];
+        acc_zero_t(acc);
+        mm_tiles<4, 1>(W + k * 4096, kGH, sA[k] + q0 * kGRow, kGRow, acc);
+        const float *b = P + k * kGLayerParams + 4096;
+#pragma unroll
+        for (int mt = 0; mt < 4; ++mt)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int o = acc_m(mt, r);
+                const __bf16 h = to_bf16(fmaxf(acc[mt][0][r] + b[o], 0.f));
+                sA[k + 1][qn * kGRow + o] = h;
+                sAT[k + 1][o * kGRow + qn] = h;
+            }
         __syncthreads();
     }
-    head_fwd(W + 4 * 4096, P + 4 * kGLayerParams + 4096, sA[4], sR, 65);
+    {   // head: 40 outputs (three 16-row tiles)
+        f32x4 acc[3][1];
+        acc_zero_t(acc);
+        mm_tiles<3, 1>(W + 4 * 4096, kGH, sA[4] + q0 * kGRow, kGRow, acc);
+        const float *b = P + 4 * kGLayerParams + 4096;
+#pragma unroll
+        for (int mt = 0; mt < 3; ++mt)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int o = acc_m(mt, r);
+                if (o < kGOut) sR[qn * 65 + o] = acc[mt][0][r] + b[o];
+            }
+    }
     __syncthreads();
-    float g[kGOut];
+    // head gradient: lane l < 16 owns sample q0 + l
     float loss = 0.f;
-    if (ok && c > 0.f) loss = guide_head_grad(sR + q * 65, dir, c, g);
-    else
-        for (int o = 0; o < kGOut; ++o) g[o] = 0.f;
-    __syncthreads();
-    for (int o = 0; o < 64; ++o) {
-        const float v = o < kGOut ? g[o] : 0.f;
-        sR[q * 65 + o] = v;
-        const __bf16 h = to_bf16(v);
-        sD[q * kGRow + o] = h;
-        sDT[o * kGRow + q] = h;
+    if (l < 16) {
+        const int q = q0 + l;
+        float g[kGOut];
+        v3 dir(0.f, 0.f, 1.f);
+        float c = 0.f;
+        const bool ok = src.valid(base + q);
+        if (ok) src.target(base + q, dir, c);
+        if (ok && c > 0.f) loss = guide_head_grad(sR + q * 65, dir, c, g);
+        else
+            for (int o = 0; o < kGOut; ++o) g[o] = 0.f;
+        for (int o = 0; o < 64; ++o) {
+            const float v = o < kGOut ? g[o] : 0.f;
+            sR[q * 65 + o] = v;
+            const __bf16 h = to_bf16(v);
+            sD[q * kGRow + o] = h;
+            sDT[o * kGRow + q] = h;
+        }
     }
     __syncthreads();
     float *pb = partial + (size_t)blockIdx.x * kGParams;
     for (int k = kGMats - 1; k >= 0; --k) {
-        // bias: sum over samples of the fp32 delta; lane = unit
+        // bias: column l of the fp32 deltas, the wave's 16 samples
         float sb = 0.f;
-        for (int s = 0; s < 64; ++s) sb += sR[s * 65 + q];
-        pb[k * kGLayerParams + 4096 + q] = sb;
-        // weight: dW[o][i] = sum_q D[q][o] A[q][i]  (P = D^T [o][q], Q = A^T [i][q])
-        f32x4 acc[4][4];
-        acc_zero(acc);
-        mm64(sDT, kGRow, sAT[k], kGRow, acc);
-#pragma unroll
-        for (int mt = 0; mt < 4; ++mt)
+        for (int s = 0; s < 16; ++s) sb += sR[(q0 + s) * 65 + l];
+        sRed[w][l] = sb;
+        {   // weight rows [q0, q0 + 16): dW[o][i] = sum_q D[q][o] A[q][i] over the block's samples
+            f32x4 acc[1][4];
+            acc_zero_t(acc);
+            mm_tiles<1, 4>(sDT + q0 * kGRow, kGRow, sAT[k], kGRow, acc);
 #pragma unroll
             for (int nt = 0; nt < 4; ++nt)
 #pragma unroll
-                for (int r = 0; r < 4; ++r) pb[k * kGLayerParams + acc_m(mt, r) * 64 + acc_n(nt)] = acc[mt][nt][r];
+                for (int r = 0; r < 4; ++r) pb[k * kGLayerParams + (q0 + acc_m(0, r)) * 64 + acc_n(nt)] = acc[0][nt][r];
+        }
+        __syncthreads();   // every wave's weight rows read D^T; the bias parts are in
+        if (w == 0) pb[k * kGLayerParams + 4096 + l] = (sRed[0][l] + sRed[1][l]) + (sRed[2][l] + sRed[3][l]);
         if (k == 0) break;
-        // data: dA^T[i][q] = sum_o WT[i][o] D[q][o], then the ReLU mask of A = relu(Z)
-        acc_zero(acc);
-        mm64(WT + k * 4096, kGH, sD, kGRow, acc);
-        __syncthreads();
+        // data gradient of the wave's samples: dA[q][i] = sum_o WT[i][o] D[q][o],
+        // masked by the ReLU of A = relu(Z)
+        f32x4 acc[4][1];
+        acc_zero_t(acc);
+        mm_tiles<4, 1>(WT + k * 4096, kGH, sD + q0 * kGRow, kGRow, acc);
 #pragma unroll
         for (int mt = 0; mt < 4; ++mt)
 #pragma unroll
-            for (int nt = 0; nt < 4; ++nt) {
-                const int s = acc_n(nt);
-#pragma unroll
-                for (int r = 0; r < 4; ++r) {
-                    const int i = acc_m(mt, r);
-                    const float a = (float)sA[k][s * kGRow + i];
-                    const float d = a > 0.f ? acc[mt][nt][r] : 0.f;
-                    sR[s * 65 + i] = d;
-                    const __bf16 h = to_bf16(d);
-                    sD[s * kGRow + i] = h;
-                    sDT[i * kGRow + s] = h;
-                }
+            for (int r = 0; r < 4; ++r) {
+                const int i = acc_m(mt, r);
+                const float a = (float)sA[k][qn * kGRow + i];
+                const float d = a > 0.f ? acc[mt][0][r] : 0.f;
+                sR[qn * 65 + i] = d;
+                const __bf16 h = to_bf16(d);
+                sD[qn * kGRow + i] = h;
+                sDT[i * kGRow + qn] = h;
             }
         __syncthreads();
     }
-    // loss: wave sum
     for (int off = 32; off > 0; off >>= 1) loss += __shfl_xor(loss, off);
-    if (q == 0) loss_part[blockIdx.x] = loss;
+    if (l == 0) sLoss[w] = loss;
+    __syncthreads();
+    if (threadIdx.x == 0) loss_part[blockIdx.x] = (sLoss[0] + sLoss[1]) + (sLoss[2] + sLoss[3]);
 }
 
 // grad[p] = sum over blocks of partial[b][p] in a fixed order. A block owns
@@ -394,7 +462,7 @@ int run_grad(fl_guide_s *g, const Src &src, uint32_t n, hipStream_t st) {
     const uint32_t blocks = (n + 63) / 64;
     int rc = ensure_partial(g, blocks);
     if (rc != FL_OK) return rc;
-    hipLaunchKernelGGL(k_guide_grad<Src>, dim3(blocks), dim3(64), 0, st, g->d_w, g->d_wt, g->d_params, src,
+    hipLaunchKernelGGL(k_guide_grad<Src>, dim3(blocks), dim3(kGradBlock), 0, st, g->d_w, g->d_wt, g->d_params, src,
                        g->d_partial, g->d_loss);
     hipLaunchKernelGGL(k_grad_reduce, dim3(kRedBlocks), dim3(256), 0, st, g->d_partial, blocks, g->d_grad, g->d_sq);
     return FL_OK;
@@ -442,9 +510,14 @@ int guide_train_on_records(fl_guide_s *g, const float *d_recs, uint32_t n_recs, 
         return set_error(g->ctx, FL_E_HIP, "guide record weights");
     if (!(total > 0.0) || !std::isfinite(total)) return FL_OK;   // nothing to learn from yet
     const uint32_t blocks = (batch + 63) / 64;
+    // every step's draws at once
+    const size_t n_draw = (size_t)steps * batch;
+    rc = ensure_bytes(g->ctx, g->d_draw, g->draw_bytes, n_draw * sizeof(uint32_t));
+    if (rc != FL_OK) return rc;
+    hipLaunchKernelGGL(k_guide_draw, dim3((unsigned)((n_draw + 255) / 256)), dim3(256), 0, st, g->d_cdf, n_recs, total,
+                       (uint32_t)(seed * 0x9e3779b97f4a7c15ULL >> 32), g->steps, steps, batch, g->d_draw);
     for (uint32_t k = 0; k < steps; ++k) {
-        RawSrc src{d_recs, g->d_cdf, n_recs, batch, k, (uint32_t)(seed * 0x9e3779b97f4a7c15ULL >> 32) ^ g->steps,
-                   total};
+        RawSrc src{d_recs, g->d_draw + (size_t)k * batch, batch};
         rc = run_grad(g, src, batch, st);
         if (rc != FL_OK) return rc;
         adam(g, blocks, batch, st);
@@ -514,7 +587,8 @@ void fl_guide_destroy(fl_guide_t g) {
     if (g->ctx) (void)hipStreamSynchronize(g->ctx->stream);
     for (void *p : {(void *)g->d_params, (void *)g->d_m, (void *)g->d_v, (void *)g->d_grad, (void *)g->d_sq, (void *)g->d_w,
                     (void *)g->d_wt, (void *)g->d_partial, (void *)g->d_loss, (void *)g->d_stat,
-                    (void *)g->d_scratch_in, (void *)g->d_scratch_out, (void *)g->d_cdf, g->d_scan_tmp})
+                    (void *)g->d_scratch_in, (void *)g->d_scratch_out, (void *)g->d_cdf, g->d_scan_tmp,
+                    (void *)g->d_draw})
         if (p) (void)hipFree(p);
     delete g;
 }

@@ -45,6 +45,64 @@ __device__ __forceinline__ void mm64(const __bf16 *P, int ps, const __bf16 *Q, i
     }
 }
 
+// acc[mt][nt] (+)= sum_k P[16 mt + i][k] * Q[16 nt + j][k] over k = 0..63:
+// MT x NT output tiles of 16 x 16 (rows of P, rows of Q), P and Q already
+// pointing at the first row of the tiles. Element r of acc[mt][nt] is row
+// 16 mt + 4 (lane / 16) + r, column 16 nt + lane % 16.
+template <int MT, int NT>
+__device__ __forceinline__ void mm_tiles(const __bf16 *P, int ps, const __bf16 *Q, int qs, f32x4 (&acc)[MT][NT]) {
+    const int l = threadIdx.x & 63;
+    const int r16 = l & 15, kq = (l >> 4) * 8;
+#pragma unroll
+    for (int ks = 0; ks < 2; ++ks) {
+        bf16x8 a[MT], b[NT];
+#pragma unroll
+        for (int t = 0; t < MT; ++t) a[t] = *reinterpret_cast<const bf16x8 *>(P + (t * 16 + r16) * ps + ks * 32 + kq);
+#pragma unroll
+        for (int t = 0; t < NT; ++t) b[t] = *reinterpret_cast<const bf16x8 *>(Q + (t * 16 + r16) * qs + ks * 32 + kq);
+#pragma unroll
+        for (int mt = 0; mt < MT; ++mt)
+#pragma unroll
+            for (int nt = 0; nt < NT; ++nt)
+                acc[mt][nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[mt], b[nt], acc[mt][nt], 0, 0, 0);
+    }
+}
+
+template <int MT, int NT>
+__device__ __forceinline__ void acc_zero_t(f32x4 (&acc)[MT][NT]) {
+#pragma unroll
+    for (int mt = 0; mt < MT; ++mt)
+#pragma unroll
+        for (int nt = 0; nt < NT; ++nt) acc[mt][nt] = f32x4{0.f, 0.f, 0.f, 0.f};
+}
+
+// the 16 encoded inputs [16 j, 16 j + 16) of a sample (j = 0..3): axis j's 8
+// bands of sin and cos for j < 3 (guide_encode's double-angle recurrence),
+// then normal, wavelength, roughness and zeros for j = 3
+__device__ __forceinline__ void guide_encode_quarter(int j, const float pos01[3], v3 n, float lam01, float rough,
+                                                     float *x16) {
+    if (j < 3) {
+        float sn, cs;
+        sincospif(pos01[j], &sn, &cs);
+#pragma unroll
+        for (int k = 0; k < kGBands; ++k) {
+            x16[2 * k] = sn;
+            x16[2 * k + 1] = cs;
+            const float s2 = 2.f * sn * cs;
+            cs = (cs - sn) * (cs + sn);
+            sn = s2;
+        }
+    } else {
+        x16[0] = n.x;
+        x16[1] = n.y;
+        x16[2] = n.z;
+        x16[3] = lam01;
+        x16[4] = rough;
+#pragma unroll
+        for (int i = 5; i < 16; ++i) x16[i] = 0.f;
+    }
+}
+
 __device__ __forceinline__ void acc_zero(f32x4 acc[4][4]) {
 #pragma unroll
     for (int mt = 0; mt < 4; ++mt)
@@ -160,6 +218,31 @@ __device__ __forceinline__ void layer_fwd(const __bf16 *W, const float *b, const
                 if (HoutT) HoutT[o * kGRow + q] = h;
             }
         }
+}
+
+// layer_fwd with the weights and biases in LDS (row stride kGRow) and the
+// output written four units at a time (each lane's 4 consecutive units of
+// one sample: one 8-byte store). Hout may be Hin: every fragment is loaded
+// before the first store, and a wave's LDS operations complete in order.
+__device__ __forceinline__ void layer_fwd_lds(const __bf16 *W, const float *b, const __bf16 *Hin, __bf16 *Hout) {
+    f32x4 acc[4][4];
+    acc_zero(acc);
+    mm64(W, kGRow, Hin, kGRow, acc);
+#pragma unroll
+    for (int mt = 0; mt < 4; ++mt) {
+        const int o0 = acc_m(mt, 0);
+        const float b0 = b[o0], b1 = b[o0 + 1], b2 = b[o0 + 2], b3 = b[o0 + 3];
+#pragma unroll
+        for (int nt = 0; nt < 4; ++nt) {
+            typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
+            bf16x4 v;
+            v[0] = to_bf16(fmaxf(acc[mt][nt][0] + b0, 0.f));
+            v[1] = to_bf16(fmaxf(acc[mt][nt][1] + b1, 0.f));
+            v[2] = to_bf16(fmaxf(acc[mt][nt][2] + b2, 0.f));
+            v[3] = to_bf16(fmaxf(acc[mt][nt][3] + b3, 0.f));
+            *reinterpret_cast<bf16x4 *>(Hout + acc_n(nt) * kGRow + o0) = v;
+        }
+    }
 }
 
 // Head: raw[q][o] = sum_i W[o][i] H[q][i] + b[o] in fp32 for o < nout, raw

@@ -119,6 +119,8 @@ struct fl_guide_s {
     size_t cdf_bytes = 0;
     void *d_scan_tmp = nullptr;      // hipCUB scan workspace
     size_t scan_tmp_bytes = 0;
+    uint32_t *d_draw = nullptr;      // record indices drawn for every step of a training pass
+    size_t draw_bytes = 0;
     uint32_t steps = 0;
     float last_loss = 0.f;
     int diverged = 0;

@@ -231,9 +231,56 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WAV
     }
 }
 
+// One ray per thread, no refill: a grid of blocks over every slot the queues
+// can hold (blocks past the queue size exit at once). For passes of few
+// paths, where a persistent launch's time is mostly its lanes' tails.
+template <int STACK>
+__global__ void __launch_bounds__(kBlock) k_trace_flat(RenderArgs a, int b) {
+    __shared__ unsigned long long stk[STACK * kBlock];
+    const int cur = b & 1;
+    const uint32_t ns = a.counts[C_SH + (cur ^ 1)];
+    const uint32_t ne = a.counts[C_EXT + cur];
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        a.counts[C_EXT + (cur ^ 1)] = 0u;
+        a.counts[C_SH + cur] = 0u;
+        a.counts[C_FETCH + (cur ^ 1)] = 0u;
+        atomicAdd(&a.stats[0], (unsigned long long)ne);
+        atomicAdd(&a.stats[1], (unsigned long long)ns);
+    }
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= ns + ne) return;
+    LaneStack ls;
+    ls.lds = (lds_u64 *)(stk + threadIdx.x);
+    ls.lds_stride = kBlock;
+    ls.cap = STACK;
+    ls.spill = a.spill + i;
+    ls.spill_stride = a.spill_stride;
+    TravCounters c;
+    TraceSrc src{a.sq_o,    a.sq_d, a.sq_tgt, a.pend,    a.rq_o[cur], a.rq_d[cur], a.rq_sid[cur],
+                 a.hits,    ns,     a.cap,    a.items,   a.st_item,   a.n_items,   a.sc, a.sq_occ};
+    v3 o, d;
+    float tmin, tmax;
+    bool any;
+    TraceScene sc;
+    src.load(i, o, d, tmin, tmax, any, sc);
+    TravState s;
+    trav_init(s, o, d, tmin, tmax);
+    while (!trav_step<false>(sc, s, ls, 1 << 30, c, any, any)) {
+    }
+    src.store(i, s);
+}
+
+// traversal modes of a pass (RenderArgs::trav_mode)
+constexpr int kTravBudget = 0;     // persistent, refill after trav_budget iterations, static chunks
+constexpr int kTravPrefetch = 1;   // persistent, each lane's next ray loaded ahead
+constexpr int kTravFlat = 2;       // one ray per thread (counter passes use kTravPrefetch)
+
 template <int S, int WV>
 void launch_trace_t(bool count, dim3 g, hipStream_t st, const RenderArgs &a, int j) {
-    if (a.trav_prefetch) {
+    if (a.trav_mode == kTravFlat && !count) {
+        hipLaunchKernelGGL((k_trace_flat<S>), dim3((unsigned)((3ull * a.cap + kBlock - 1) / kBlock)), dim3(kBlock), 0, st,
+                           a, j);
+    } else if (a.trav_mode != kTravBudget) {
         if (count) hipLaunchKernelGGL((k_trace<true, S, WV, true>), g, dim3(kBlock), 0, st, a, j);
         else hipLaunchKernelGGL((k_trace<false, S, WV, true>), g, dim3(kBlock), 0, st, a, j);
     } else {
@@ -255,7 +302,7 @@ struct TraceVariant {
     const void *(*fn)(bool);
 };
 const TraceVariant kTraceVariants[] = {
-    {12, 1, launch_trace_t<12, 1>, trace_fn<12, 1>}, {8, 1, launch_trace_t<8, 1>, trace_fn<8, 1>},
+    {8, 1, launch_trace_t<8, 1>, trace_fn<8, 1>},    {12, 1, launch_trace_t<12, 1>, trace_fn<12, 1>},
     {8, 6, launch_trace_t<8, 6>, trace_fn<8, 6>},    {8, 8, launch_trace_t<8, 8>, trace_fn<8, 8>},
     {6, 8, launch_trace_t<6, 8>, trace_fn<6, 8>},    {16, 1, launch_trace_t<16, 1>, trace_fn<16, 1>},
 };
@@ -321,22 +368,42 @@ __device__ __forceinline__ void guide_features(const RenderArgs &a, v3 p, v3 ng,
     rough = m.kind == MAT_CONDUCTOR ? m.ax : 1.f;
 }
 
-// Network evaluation for every guidable hit of extension queue `b & 1`: one
-// wave per block, 64 slots per chunk, lobes written per queue slot. LDS holds
-// one tile, updated in place: a layer's fragments are all loaded (LDS returns
-// a wave's operations in order) before its epilogue overwrites the tile, and
-// the block is that one wave. The head's fp32 outputs (40 per sample, odd row
-// stride against bank conflicts) take the same bytes. 10.5 KB per wave.
+// Network evaluation for every guidable hit of extension queue `b & 1`,
+// lobes written per queue slot. A persistent block of kInferWaves waves per
+// CU holds the five bf16 weight matrices and the biases in LDS (loaded once);
+// each wave takes 64-slot chunks in turn and runs them through its own tile,
+// updated in place (a layer's fragments are all loaded before its epilogue
+// overwrites the tile, and a wave's LDS operations complete in order). The
+// head's fp32 outputs (40 per slot, odd row stride against bank conflicts)
+// take the same bytes. Waves run independently: the only block barrier is
+// the one after the weights are loaded.
 constexpr int kInferRaw = 41;
-__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3, 8))) k_guide_infer(RenderArgs a, int b) {
-    __shared__ __attribute__((aligned(16))) float sB[64 * kInferRaw];
-    static_assert(sizeof(float) * 64 * kInferRaw >= sizeof(__bf16) * 64 * kGRow, "sB holds a hidden tile");
+constexpr int kInferWaves = 8;
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+__global__ void __launch_bounds__(64 * kInferWaves) k_guide_infer(RenderArgs a, int b) {
+    __shared__ __attribute__((aligned(16))) __bf16 sW[kGMats][64 * kGRow];
+    __shared__ float sBias[kGMats][64];
+    __shared__ __attribute__((aligned(16))) float sTile[kInferWaves][64 * kInferRaw];
+    static_assert(sizeof(float) * 64 * kInferRaw >= sizeof(__bf16) * 64 * kGRow, "a tile holds a hidden layer");
+    for (int e = threadIdx.x; e < kGMats * 64 * 8; e += blockDim.x) {   // 16-byte pieces of W[k][o][0..63]
+        const int k = e / 512, o = (e / 8) % 64, c = e % 8;
+        *reinterpret_cast<bf16x8 *>(&sW[k][o * kGRow + 8 * c]) =
+            *reinterpret_cast<const bf16x8 *>(a.gW + k * 4096 + o * 64 + 8 * c);
+    }
+    for (int e = threadIdx.x; e < kGMats * 64; e += blockDim.x)
+        sBias[e / 64][e % 64] = a.gP[(e / 64) * kGLayerParams + 4096 + e % 64];
+    __syncthreads();
+    const int w = threadIdx.x >> 6, q = threadIdx.x & 63;
+    float *sB = sTile[w];
     __bf16 *sT = reinterpret_cast<__bf16 *>(sB);
-    const int q = threadIdx.x;
     const uint32_t n = a.counts[C_EXT + (b & 1)];
     const uint32_t nchunks = (n + 63) / 64;
-    unsigned long long n_queries = 0;   // this block's count, one atomic at the end
-    for (uint32_t ch = blockIdx.x; ch < nchunks; ch += gridDim.x) {
+    unsigned long long n_queries = 0;   // this wave's count, one atomic at the end
+    for (uint32_t ch = blockIdx.x * kInferWaves + w; ch < nchunks; ch += gridDim.x * kInferWaves) {
         const uint32_t i = ch * 64u + q;
         float pos01[3] = {0.f, 0.f, 0.f}, lam01 = 0.f, rough = 0.f;
         v3 n_in(0.f, 0.f, 0.f);
@@ -364,15 +431,31 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3, 8)))
                 }
             }
         }
-        n_queries += (unsigned long long)__popcll(__ballot(want));
+        const unsigned long long wm = __ballot(want);
+        n_queries += (unsigned long long)__popcll(wm);
+        if (wm == 0ull) continue;   // no query in this chunk (wave-uniform)
         encode_row_bf16(sT, q, want, pos01, n_in, lam01, rough);
-        __syncthreads();
+        wave_sync();
         for (int k = 0; k < 4; ++k) {
-            layer_fwd(a.gW + k * 4096, a.gP + k * kGLayerParams + 4096, sT, sT, nullptr, true);
-            __syncthreads();
+            layer_fwd_lds(sW[k], sBias[k], sT, sT);
+            wave_sync();
         }
-        head_fwd(a.gW + 4 * 4096, a.gP + 4 * kGLayerParams + 4096, sT, sB, kInferRaw, kGOut);
-        __syncthreads();
+        {   // head, into the same bytes as fp32 (row stride kInferRaw)
+            f32x4 acc[4][4];
+            acc_zero(acc);
+            mm64(sW[4], kGRow, sT, kGRow, acc);
+            wave_sync();
+#pragma unroll
+            for (int mt = 0; mt < 3; ++mt)
+#pragma unroll
+                for (int nt = 0; nt < 4; ++nt)
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) {
+                        const int o = acc_m(mt, r);
+                        if (o < kGOut) sB[acc_n(nt) * kInferRaw + o] = acc[mt][nt][r] + sBias[4][o];
+                    }
+        }
+        wave_sync();
         if (want) {
             VmfLobe L[kGLobes];
             guide_decode(sB + q * kInferRaw, L);
@@ -387,7 +470,7 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3, 8)))
             }
             a.lobes[i] = pk;
         }
-        __syncthreads();
+        wave_sync();   // the next chunk's encode overwrites the rows just read
     }
     if (q == 0 && n_queries) atomicAdd(&a.stats[6], n_queries);
 }
@@ -1285,7 +1368,7 @@ int render_items(fl_ctx_t ctx, const HostItem *hi, uint32_t n_items, const fl_re
         if ((uint64_t)hi[k].cam.width * hi[k].cam.height > (1u << 26))
             return set_error(ctx, FL_E_INVALID, "fl_render: image larger than 2^26 pixels");
     }
-    if (p->trav_prefetch > 2) return set_error(ctx, FL_E_INVALID, "fl_render: trav_prefetch must be 0, 1 or 2");
+    if (p->trav_prefetch > 3) return set_error(ctx, FL_E_INVALID, "fl_render: trav_prefetch must be 0, 1, 2 or 3");
     fl_guide_s *guide = (p->flags & FL_RENDER_GUIDED) ? p->guide : nullptr;
     if ((p->flags & FL_RENDER_GUIDED) && !guide) return set_error(ctx, FL_E_INVALID, "FL_RENDER_GUIDED needs params.guide");
     if (guide && guide->ctx != ctx) return set_error(ctx, FL_E_INVALID, "guide belongs to another context");
@@ -1386,22 +1469,44 @@ int render_items(fl_ctx_t ctx, const HostItem *hi, uint32_t n_items, const fl_re
     for (const TraceVariant &v : kTraceVariants)
         if (v.stack == want_stack && v.waves == want_waves) tv = &v;
     if (!tv) return set_error(ctx, FL_E_INVALID, "fl_render: no traversal variant with this stack_depth/trav_waves");
-    if (p->trav_prefetch == 1 || p->trav_prefetch == 2) {
-        a.trav_prefetch = p->trav_prefetch == 1 ? 1 : 0;
-    } else {   // auto: prefetch when the largest scene's rays are long (its BVH is deep)
-        uint32_t max_tris = 0;
-        for (uint32_t k = 0; k < n_items; ++k) max_tris = std::max(max_tris, hi[k].scene->n_tris);
-        a.trav_prefetch = max_tris >= kPrefetchTris ? 1 : 0;
+    // Traversal mode, per pass (fl_render_params.trav_prefetch 0 = auto):
+    // - scenes of short rays (all below kPrefetchTris triangles): one ray per
+    //   thread up to 16 M paths (Cornell, 16 spp: 1.06 against 1.42 and
+    //   1.79 ms of traversal), next rays loaded ahead beyond;
+    // - scenes of long rays: next rays loaded ahead in passes from
+    //   kPrefetchPaths paths; a smaller pass is mostly its launches' tails and
+    //   cache refills, which the budgeted refill with static chunks shortens
+    //   (1 M paths on the guided room: 11.9 against 13.3 ms).
+    uint32_t max_tris = 0;
+    for (uint32_t k = 0; k < n_items; ++k) max_tris = std::max(max_tris, hi[k].scene->n_tris);
+    auto pass_mode = [&](uint32_t pass_paths) -> int {
+        if (p->trav_prefetch == 1) return kTravPrefetch;
+        if (p->trav_prefetch == 2) return kTravBudget;
+        if (p->trav_prefetch == 3) return kTravFlat;
+        if (max_tris < kPrefetchTris) return pass_paths <= kFlatPaths ? kTravFlat : kTravPrefetch;
+        return pass_paths >= kPrefetchPaths ? kTravPrefetch : kTravBudget;
+    };
+    a.trav_mode = pass_mode(a.cap);
+    // persistent grids of both instantiations: as many resident blocks as
+    // their occupancy allows on every CU
+    uint32_t grid_mode[2];
+    for (int m = 0; m < 2; ++m) {
+        int occ = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, tv->fn(m == kTravPrefetch), kBlock, 0) != hipSuccess ||
+            occ <= 0) {
+            (void)hipGetLastError();
+            occ = 4;
+        }
+        grid_mode[m] = (uint32_t)(ctx->n_cu * occ);
     }
-    int occ = 0;   // of the instantiation the launches use
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, tv->fn(a.trav_prefetch != 0), kBlock, 0) != hipSuccess ||
-        occ <= 0)
-        occ = 4;
-    const uint32_t grid_pers = (uint32_t)(ctx->n_cu * occ);
+    const uint32_t grid_pers = std::max(grid_mode[0], grid_mode[1]);   // sizes the spill area
     a.trav_budget = p->trav_budget ? (int)p->trav_budget : 16;
     a.refill_lanes = p->refill_lanes ? (int)std::min(64u, p->refill_lanes) : 16;
     size_t spill_per = (size_t)(max_depth > (uint32_t)tv->stack ? max_depth - tv->stack + 1 : 1);
     a.spill_stride = grid_pers * kBlock;
+    // one-ray-per-thread passes index the spill area by ray (at most 3 per path)
+    const bool flat_possible = p->trav_prefetch == 3 || (p->trav_prefetch == 0 && max_tris < kPrefetchTris);
+    if (flat_possible) a.spill_stride = (uint32_t)std::max<size_t>(a.spill_stride, 3ull * cap);
     a.st_beta = grow<float4>(B.p[0], B.cap[0], cap * 16, err);
     a.st_L = grow<float4>(B.p[1], B.cap[1], cap * 16, err);
     a.st_misc = grow<float4>(B.p[2], B.cap[2], cap * 16, err);
@@ -1515,16 +1620,17 @@ int render_items(fl_ctx_t ctx, const HostItem *hi, uint32_t n_items, const fl_re
     };
     const void *shade_fn = shade_kernel(has_std, guide != nullptr, shade_waves);
     const uint32_t shade_grid_max = resident(shade_fn, kBlock, (uint32_t)ctx->n_cu * 8);
-    const uint32_t infer_grid = resident(reinterpret_cast<const void *>(&k_guide_infer), 64, (uint32_t)ctx->n_cu * 16);
+    const uint32_t infer_grid = resident(reinterpret_cast<const void *>(&k_guide_infer), 64 * kInferWaves, (uint32_t)ctx->n_cu);
     auto launch_shade = [&](const RenderArgs &args, dim3 g, int j, bool with_infer = true) {
-        if (with_infer && args.guided) hipLaunchKernelGGL(k_guide_infer, dim3(infer_grid), dim3(64), 0, st, args, j);
+        if (with_infer && args.guided)
+            hipLaunchKernelGGL(k_guide_infer, dim3(infer_grid), dim3(64 * kInferWaves), 0, st, args, j);
         RenderArgs ka = args;
         int kj = j;
         void *kargs[] = {&ka, &kj};
         (void)hipLaunchKernel(shade_fn, g, dim3(kBlock), kargs, 0, st);
     };
     auto launch_bounce = [&](const RenderArgs &args, uint32_t j) {
-        tv->launch(counters, dim3(grid_pers), st, args, (int)j);
+        tv->launch(counters, dim3(grid_mode[args.trav_mode == kTravPrefetch ? 1 : 0]), st, args, (int)j);
         launch_shade(args, dim3(shade_grid_max), (int)j);
     };
     const bool use_graph = (p->flags & FL_RENDER_GRAPH) && !timing;
@@ -1587,6 +1693,7 @@ int render_items(fl_ctx_t ctx, const HostItem *hi, uint32_t n_items, const fl_re
             }
             a.n_paths = path_base;
             a.n_cam_rays = slot_base;
+            a.trav_mode = pass_mode(a.n_cam_rays);
             FL_HIP_CHECK(ctx, hipMemcpyAsync(const_cast<ItemDev *>(a.items), hit, (size_t)n_items * sizeof(ItemDev),
                                              hipMemcpyHostToDevice, st));
             FL_HIP_CHECK(ctx, hipEventRecord(B.item_ev[item_slot], st));
@@ -1629,14 +1736,15 @@ int render_items(fl_ctx_t ctx, const HostItem *hi, uint32_t n_items, const fl_re
         }
         for (; iter < max_iter && (ne + ns) > 0; ++iter) {
             const int cur = iter & 1;
-            uint32_t tg = std::min<uint32_t>(grid_pers, blocks((size_t)ne + ns));
+            uint32_t tg = std::min<uint32_t>(grid_mode[a.trav_mode == kTravPrefetch ? 1 : 0], blocks((size_t)ne + ns));
             if (timing) FL_HIP_CHECK(ctx, hipEventRecord(tev[0], st));
             tv->launch(counters, dim3(tg), st, a, (int)iter);
             ++n_trace;
             if (timing) FL_HIP_CHECK(ctx, hipEventRecord(tev[1], st));
             if (ne > 0) {
                 // guide inference and shading, timed apart
-                if (a.guided) hipLaunchKernelGGL(k_guide_infer, dim3(infer_grid), dim3(64), 0, st, a, (int)iter);
+                if (a.guided)
+                    hipLaunchKernelGGL(k_guide_infer, dim3(infer_grid), dim3(64 * kInferWaves), 0, st, a, (int)iter);
                 if (timing) FL_HIP_CHECK(ctx, hipEventRecord(tev[3], st));
                 launch_shade(a, dim3(std::min<uint32_t>(shade_grid_max, blocks(ne))), (int)iter, false);
                 ++n_shade;
@@ -1831,7 +1939,7 @@ int autotune_items(fl_ctx_t ctx, const HostItem *hi, uint32_t n, const fl_render
         }
     if (rc == FL_OK) rc = try_values([](fl_render_params &q, uint32_t v) { q.trav_budget = v; }, {8u, 32u});
     if (rc == FL_OK) rc = try_values([](fl_render_params &q, uint32_t v) { q.refill_lanes = v; }, {1u, 32u});
-    if (rc == FL_OK) rc = try_values([](fl_render_params &q, uint32_t v) { q.trav_prefetch = v; }, {1u, 2u});
+    if (rc == FL_OK) rc = try_values([](fl_render_params &q, uint32_t v) { q.trav_prefetch = v; }, {1u, 2u, 3u});
     if (rc == FL_OK)
         rc = try_values([](fl_render_params &q, uint32_t v) { q.flags = v ? (q.flags | FL_RENDER_NO_SORT) : q.flags; },
                         {1u});

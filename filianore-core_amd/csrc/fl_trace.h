@@ -321,6 +321,10 @@ struct RayCursor {
 // `any` is per ray: the first hit ends it (emitters skipped when ANY_SKIP).
 // scenes from this many triangles on traverse in prefetch mode by default
 constexpr uint32_t kPrefetchTris = 65536;
+// ... in passes of at least this many paths
+constexpr uint32_t kPrefetchPaths = 4u << 20;
+// scenes below kPrefetchTris trace one ray per thread in passes up to this many paths
+constexpr uint32_t kFlatPaths = 16u << 20;
 constexpr uint32_t kNoRay = ~0u;
 
 template <bool COUNT, bool ANY_SKIP, bool PREFETCH, class Src>

@@ -19,7 +19,7 @@ static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
 namespace {
 
 constexpr int kBlock = 256;
-constexpr int kStack = 12;
+constexpr int kStack = 8;
 constexpr int kBudget = 24;
 
 struct ApiSrc {

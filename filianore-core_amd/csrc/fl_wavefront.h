@@ -122,7 +122,7 @@ struct RenderArgs {
     uint32_t spill_stride;
     int trav_budget;         // traversal iterations between lane refill checks
     int refill_lanes;        // idle lanes that trigger a refill (or all lanes idle)
-    int trav_prefetch;       // 1: next rays loaded ahead, 64-ray pools, yield after refill_lanes finish
+    int trav_mode;           // kTravBudget, kTravPrefetch or kTravFlat (fl_render.hip)
     // learned proposal (guided passes)
     uint32_t guided;         // 1: mix the guide's vMF lobes with BSDF sampling
     float guide_mix;         // probability of sampling the guide

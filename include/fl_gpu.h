@@ -245,20 +245,22 @@ typedef struct {
     uint64_t seed;
     uint32_t sample_offset;      /* progressive rendering: index of first sample */
     uint32_t max_paths;          /* paths in flight per pass (0: auto) */
-    uint32_t stack_depth;        /* LDS short-stack entries per lane (0: auto = 12) */
+    uint32_t stack_depth;        /* LDS short-stack entries per lane (0: auto = 8) */
     uint32_t trav_budget;        /* traversal iterations between lane refills (0: auto) */
     uint32_t shade_waves;        /* shading kernel variant: min waves/SIMD (2, 3 or 4; 0: 3, or 2 with
                                     standard surfaces or textures) */
     uint32_t trav_waves;         /* traversal variant: min waves/SIMD (0: auto); valid
-                                    (stack_depth, trav_waves): (12,0) (8,0) (8,6) (8,8) (6,8) (16,0) */
+                                    (stack_depth, trav_waves): (8,0) (12,0) (8,6) (8,8) (6,8) (16,0) */
     fl_guide_t guide;            /* guiding network when FL_RENDER_GUIDED */
     float guide_mix;             /* proposal probability of the guide (0.5) */
     float guide_train_fraction;  /* with TRAIN_GUIDE: fraction of spp used to train */
     uint32_t refill_lanes;       /* traversal: idle lanes of a wave that trigger a refill (0: auto) */
-    uint32_t trav_prefetch;      /* traversal refill mode: 1 = each lane's next ray loaded ahead, the
-                                    wave yields once refill_lanes lanes are done; 2 = refill after
-                                    trav_budget iterations; 0 = auto (1 for scenes from 65,536
-                                    triangles on, whose rays are long) */
+    uint32_t trav_prefetch;      /* traversal mode: 1 = persistent, each lane's next ray loaded
+                                    ahead, the wave yields once refill_lanes lanes are done;
+                                    2 = persistent, refill after trav_budget iterations; 3 = one ray
+                                    per thread; 0 = auto, per pass: scenes below 65,536 triangles
+                                    3 up to 16 M paths per pass, else 1; larger scenes 1 from 4 M
+                                    paths per pass on, else 2 */
 } fl_render_params;
 
 /* Film: per-pixel sums of linear CIE XYZ (3 floats per pixel) and of its

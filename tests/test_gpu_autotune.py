@@ -71,16 +71,17 @@ def test_bad_node_width_is_rejected(ctx):
 
 
 def test_traversal_refill_modes_give_identical_films(ctx):
-    """trav_prefetch 1 (next rays loaded ahead) and 2 (budgeted refill), at
-    several refill thresholds, only reorder the traversal: on a small scene
-    (auto: budgeted refill) and a larger one (auto: loaded ahead) every film
-    equals the automatic choice's bit for bit."""
+    """trav_prefetch 1 (next rays loaded ahead), 2 (budgeted refill) and 3
+    (one ray per thread), at several refill thresholds, only reorder the
+    traversal: on a small scene (auto: one ray per thread) and a larger one
+    (auto: budgeted refill for this small pass) every film equals the
+    automatic choice's bit for bit."""
     for kind, kw in ((flgpu.FL_GEN_CORNELL, dict(width=96, height=96)),
                      (flgpu.FL_GEN_SPHERES, dict(subdiv=5, width=128, height=96))):
         gen = flgpu.GenSceneData(kind, **kw)
         with ctx.scene(gen) as sc:
             films = []
-            for mode, lanes in ((0, 0), (1, 16), (1, 1), (1, 64), (2, 16), (2, 1)):
+            for mode, lanes in ((0, 0), (1, 16), (1, 1), (1, 64), (2, 16), (2, 1), (3, 0)):
                 p = flgpu.default_params(spp=8, depth=gen.depth, seed=5, flags=flgpu.FL_RENDER_GRAPH)
                 p.trav_prefetch = mode
                 p.refill_lanes = lanes
@@ -90,6 +91,6 @@ def test_traversal_refill_modes_give_identical_films(ctx):
             for k in range(1, len(films)):
                 assert np.array_equal(films[0].view(np.uint32), films[k].view(np.uint32)), (kind, k)
             p = flgpu.default_params(spp=1, depth=gen.depth, seed=5)
-            p.trav_prefetch = 3
+            p.trav_prefetch = 4
             with pytest.raises(flgpu.FlError):
                 sc.render(p)
