@@ -43,7 +43,7 @@ constexpr uint32_t kNoHit = 0xffffffffu;
 
 // Per-ray precomputation of the watertight test.
 struct RayPre {
-    int kx, ky, kz;
+    uint32_t k;              // kx | ky << 2 | kz << 4 (one register per ray)
     float Sx, Sy, Sz;
 };
 
@@ -55,7 +55,7 @@ FL_HD RayPre ray_pre(v3 d) {
     int ky = kx + 1; if (ky == 3) ky = 0;
     float dkz = d[kz];
     if (dkz < 0.f) { int t = kx; kx = ky; ky = t; }
-    r.kx = kx; r.ky = ky; r.kz = kz;
+    r.k = (uint32_t)kx | ((uint32_t)ky << 2) | ((uint32_t)kz << 4);
     r.Sx = d[kx] / dkz;
     r.Sy = d[ky] / dkz;
     r.Sz = 1.0f / dkz;
@@ -69,9 +69,10 @@ FL_HD float comp(v3 a, int k) { return k == 0 ? a.x : (k == 1 ? a.y : a.z); }
 FL_HD bool tri_intersect(const RayPre &rp, v3 o, v3 p0, v3 p1, v3 p2, float tmin, float tmax,
                          float *t_out, float *u_out, float *v_out) {
     v3 A = p0 - o, B = p1 - o, C = p2 - o;
-    float Akx = comp(A, rp.kx), Aky = comp(A, rp.ky), Akz = comp(A, rp.kz);
-    float Bkx = comp(B, rp.kx), Bky = comp(B, rp.ky), Bkz = comp(B, rp.kz);
-    float Ckx = comp(C, rp.kx), Cky = comp(C, rp.ky), Ckz = comp(C, rp.kz);
+    const int kx = (int)(rp.k & 3u), ky = (int)((rp.k >> 2) & 3u), kz = (int)(rp.k >> 4);
+    float Akx = comp(A, kx), Aky = comp(A, ky), Akz = comp(A, kz);
+    float Bkx = comp(B, kx), Bky = comp(B, ky), Bkz = comp(B, kz);
+    float Ckx = comp(C, kx), Cky = comp(C, ky), Ckz = comp(C, kz);
     float Ax = Akx - rp.Sx * Akz, Ay = Aky - rp.Sy * Akz;
     float Bx = Bkx - rp.Sx * Bkz, By = Bky - rp.Sy * Bkz;
     float Cx = Ckx - rp.Sx * Ckz, Cy = Cky - rp.Sy * Ckz;

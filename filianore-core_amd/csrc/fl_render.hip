@@ -180,7 +180,11 @@ struct TraceSrc {
         ro = v3(a.x, a.y, a.z);
         rd = v3(b.x, b.y, b.z);
         tmin = a.w;
-        tmax = b.w;
+        // the queues' segment ends are fixed: a shadow ray stops SHADOW_EPSILON
+        // short of its target (d spans the segment), an extension ray is
+        // unbounded; set here rather than loaded, so that a prefetched ray
+        // holds one register less
+        tmax = i < ns ? 1.f - kShadowEps : kInf;
     }
     __device__ void store(uint32_t i, const TravState &s) const {
         if (i < ns) {
@@ -223,7 +227,11 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WAV
     TravCounters c;
     TraceSrc src{a.sq_o,    a.sq_d, a.sq_tgt, a.pend,    a.rq_o[cur], a.rq_d[cur], a.rq_sid[cur],
                  a.hits,    ns,     a.cap,    a.items,   a.st_item,   a.n_items,   a.sc, a.sq_occ};
-    trace_persistent<COUNT, true, PREFETCH>(src, ns + ne, &a.counts[C_FETCH + cur], ls, a.trav_budget, a.refill_lanes, c);
+    // refill threshold by default: a yield once 24 lanes are done when the
+    // next rays are loaded ahead (spheres, 64 spp: 48.1 against 48.8 ms at
+    // 16 and 52.7 at 8), a refill once 16 lanes are idle otherwise
+    const int refill = a.refill_lanes ? a.refill_lanes : (PREFETCH ? 24 : 16);
+    trace_persistent<COUNT, true, PREFETCH>(src, ns + ne, &a.counts[C_FETCH + cur], ls, a.trav_budget, refill, c);
     if (COUNT) {
         atomicAdd(&a.stats[3], (unsigned long long)c.nodes);
         atomicAdd(&a.stats[4], (unsigned long long)c.tris);
@@ -302,7 +310,7 @@ struct TraceVariant {
     const void *(*fn)(bool);
 };
 const TraceVariant kTraceVariants[] = {
-    {8, 1, launch_trace_t<8, 1>, trace_fn<8, 1>},    {12, 1, launch_trace_t<12, 1>, trace_fn<12, 1>},
+    {8, 5, launch_trace_t<8, 5>, trace_fn<8, 5>},    {12, 1, launch_trace_t<12, 1>, trace_fn<12, 1>},
     {8, 6, launch_trace_t<8, 6>, trace_fn<8, 6>},    {8, 8, launch_trace_t<8, 8>, trace_fn<8, 8>},
     {6, 8, launch_trace_t<6, 8>, trace_fn<6, 8>},    {16, 1, launch_trace_t<16, 1>, trace_fn<16, 1>},
 };
@@ -1464,10 +1472,10 @@ int render_items(fl_ctx_t ctx, const HostItem *hi, uint32_t n_items, const fl_re
     // traversal variant, then the persistent grid: as many resident blocks
     // as its occupancy (LDS stack + VGPRs) allows on every CU
     const int want_stack = p->stack_depth ? (int)p->stack_depth : kTraceVariants[0].stack;
-    const int want_waves = p->trav_waves ? (int)p->trav_waves : 1;
+    // trav_waves 0: the first variant listed with that stack depth
     const TraceVariant *tv = nullptr;
     for (const TraceVariant &v : kTraceVariants)
-        if (v.stack == want_stack && v.waves == want_waves) tv = &v;
+        if (!tv && v.stack == want_stack && (p->trav_waves == 0 || v.waves == (int)p->trav_waves)) tv = &v;
     if (!tv) return set_error(ctx, FL_E_INVALID, "fl_render: no traversal variant with this stack_depth/trav_waves");
     // Traversal mode, per pass (fl_render_params.trav_prefetch 0 = auto):
     // - scenes of short rays (all below kPrefetchTris triangles): one ray per
@@ -1501,7 +1509,7 @@ int render_items(fl_ctx_t ctx, const HostItem *hi, uint32_t n_items, const fl_re
     }
     const uint32_t grid_pers = std::max(grid_mode[0], grid_mode[1]);   // sizes the spill area
     a.trav_budget = p->trav_budget ? (int)p->trav_budget : 16;
-    a.refill_lanes = p->refill_lanes ? (int)std::min(64u, p->refill_lanes) : 16;
+    a.refill_lanes = p->refill_lanes ? (int)std::min(64u, p->refill_lanes) : 0;   // 0: the mode's default
     size_t spill_per = (size_t)(max_depth > (uint32_t)tv->stack ? max_depth - tv->stack + 1 : 1);
     a.spill_stride = grid_pers * kBlock;
     // one-ray-per-thread passes index the spill area by ray (at most 3 per path)
@@ -1938,7 +1946,7 @@ int autotune_items(fl_ctx_t ctx, const HostItem *hi, uint32_t n, const fl_render
             if (rc != FL_OK) break;
         }
     if (rc == FL_OK) rc = try_values([](fl_render_params &q, uint32_t v) { q.trav_budget = v; }, {8u, 32u});
-    if (rc == FL_OK) rc = try_values([](fl_render_params &q, uint32_t v) { q.refill_lanes = v; }, {1u, 32u});
+    if (rc == FL_OK) rc = try_values([](fl_render_params &q, uint32_t v) { q.refill_lanes = v; }, {1u, 16u, 32u});
     if (rc == FL_OK) rc = try_values([](fl_render_params &q, uint32_t v) { q.trav_prefetch = v; }, {1u, 2u, 3u});
     if (rc == FL_OK)
         rc = try_values([](fl_render_params &q, uint32_t v) { q.flags = v ? (q.flags | FL_RENDER_NO_SORT) : q.flags; },
@@ -1950,7 +1958,7 @@ int autotune_items(fl_ctx_t ctx, const HostItem *hi, uint32_t n, const fl_render
     out->stack_depth = cur.stack_depth ? cur.stack_depth : (uint32_t)kTraceVariants[0].stack;
     out->trav_waves = cur.trav_waves;
     out->trav_budget = cur.trav_budget ? cur.trav_budget : 16u;
-    out->refill_lanes = cur.refill_lanes ? cur.refill_lanes : 16u;
+    out->refill_lanes = cur.refill_lanes;   // 0: the mode's default (24 loaded ahead, 16 budgeted)
     out->trav_prefetch = cur.trav_prefetch;
     out->shade_waves = cur.shade_waves;
     out->sort_shading = (cur.flags & FL_RENDER_NO_SORT) ? 0u : 1u;

@@ -5,10 +5,11 @@
 // positions) and a "triangle group" (hit leaf triangles of the last node), in
 // the style of compressed wide BVH traversal (Ylitie, Karras, Laine, HPG
 // 2017), with a short stack of node groups in LDS that spills to a per-lane
-// global area when it is full.  Box tests dequantize planes exactly
-// (org + q*step is exact by construction) and widen the slab interval by a
-// relative 2^-20, so no box containing a hit is ever culled: the closest hit
-// (with the smaller-ID tie rule) is then the same as a brute-force search.
+// global area when it is full.  Box tests take one fma per plane (the ray
+// transformed into the node's grid once per node) and widen each slab
+// interval by a bound on that arithmetic's rounding, so no box containing a
+// hit is ever culled: the closest hit (with the smaller-ID tie rule) is then
+// the same as a brute-force search.
 //
 // The traversal is a resumable state machine (TravState + trav_step with an
 // iteration budget) so that persistent kernels can refill lanes whose ray has
@@ -56,10 +57,16 @@ struct TravState {
     v3 o, d;
     float inv[3];
     float tmin, best_t, bu, bv;
-    uint32_t oct, best_id, best_rec;
+    uint32_t best_id, best_rec;
     RayPre rp;
     uint32_t g_base, g_imask, g_hits, t_base, t_hits;
     int sp;
+    // the direction's octant (bit k: negative along axis k), from the signs of
+    // inv (never zero): recomputed where needed rather than held in a register
+    __device__ __forceinline__ uint32_t oct() const {
+        return (__float_as_uint(inv[0]) >> 31) | ((__float_as_uint(inv[1]) >> 30) & 2u) |
+               ((__float_as_uint(inv[2]) >> 29) & 4u);
+    }
 };
 
 __device__ __forceinline__ void trav_init(TravState &s, v3 o, v3 d, float tmin, float tmax) {
@@ -68,7 +75,6 @@ __device__ __forceinline__ void trav_init(TravState &s, v3 o, v3 d, float tmin, 
     s.inv[0] = safe_inv(d.x);
     s.inv[1] = safe_inv(d.y);
     s.inv[2] = safe_inv(d.z);
-    s.oct = (s.inv[0] < 0.f ? 1u : 0u) | (s.inv[1] < 0.f ? 2u : 0u) | (s.inv[2] < 0.f ? 4u : 0u);
     s.rp = ray_pre(d);
     s.tmin = tmin;
     s.best_t = tmax;
@@ -77,7 +83,7 @@ __device__ __forceinline__ void trav_init(TravState &s, v3 o, v3 d, float tmin, 
     s.bu = s.bv = 0.f;
     s.g_base = 0;
     s.g_imask = 1u;
-    s.g_hits = 1u << s.oct;  // the group holding the root (slot 0)
+    s.g_hits = 1u << s.oct();  // the group holding the root (slot 0)
     s.t_base = 0;
     s.t_hits = 0;
     s.sp = 0;
@@ -101,7 +107,6 @@ __device__ __forceinline__ void trav_init(TravState &s, v3 o, v3 d, float tmin, 
 template <bool COUNT>
 __device__ __forceinline__ bool trav_step(const TraceScene &sc, TravState &s, const LaneStack &ls, int budget,
                                           TravCounters &cnt, bool any, bool skip_lights, int leave = 0) {
-    const float kLo = 1.0f - 0x1p-20f, kHi = 1.0f + 0x1p-20f;
     // (signed: the lane counts are compared with a floor that may be negative)
     const int floor_active = leave > 0 ? (int)__popcll(__ballot(1)) - leave : -1;
     for (int it = 0; it < budget; ++it) {
@@ -131,7 +136,8 @@ __device__ __forceinline__ bool trav_step(const TraceScene &sc, TravState &s, co
         } else if (s.g_hits) {
             uint32_t pos = __ffs(s.g_hits) - 1;
             s.g_hits &= s.g_hits - 1u;
-            uint32_t slot = pos ^ s.oct;
+            const uint32_t oct = s.oct();
+            uint32_t slot = pos ^ oct;
             uint32_t nidx = s.g_base + __popc(s.g_imask & ((1u << slot) - 1u));
             if (s.g_hits) {
                 uint2 e = make_uint2(s.g_base, s.g_imask | (s.g_hits << 8));
@@ -153,11 +159,29 @@ __device__ __forceinline__ bool trav_step(const TraceScene &sc, TravState &s, co
             // hi for a negative one: swap the packed plane words once per node
 #pragma unroll
             for (int k = 0; k < 3; ++k) {
-                if ((s.oct >> k) & 1u) {
+                if ((oct >> k) & 1u) {
                     uint4 t = qn[k];
                     qn[k] = qf[k];
                     qf[k] = t;
                 }
+            }
+            // Along axis k the ray meets grid plane q at
+            //   t(q) = (org + q step - o) inv = q a + b,  a = step inv, b = (org - o) inv,
+            // a exact (a power-of-two scale of inv). One fma per plane against
+            // b -/+ e: the roundings of inv, b, b -/+ e and the fma together
+            // stay below 2^-21 (65536 |a| + |b|) >= 2^-21 (|q a| + |b|), so
+            // e = 2^-19 (65536 |a| + |b|) makes the near distance a lower and
+            // the far distance an upper bound of the exact ones, and no box
+            // holding a hit is culled. (e is per axis: a ray nearly parallel
+            // to an axis has large |b| there, in that axis's units of t only.)
+            float a3[3], bnear[3], bfar[3];
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                a3[k] = step[k] * s.inv[k];
+                const float b = (org[k] - oo[k]) * s.inv[k];
+                const float e = (fabsf(a3[k]) * 65536.0f + fabsf(b)) * 0x1p-19f;
+                bnear[k] = b - e;
+                bfar[k] = b + e;
             }
             uint32_t hits = 0;
 #pragma unroll
@@ -169,17 +193,11 @@ __device__ __forceinline__ bool trav_step(const TraceScene &sc, TravState &s, co
                     uint32_t wf = (q >> 1) == 0 ? qf[k].x : ((q >> 1) == 1 ? qf[k].y : ((q >> 1) == 2 ? qf[k].z : qf[k].w));
                     uint32_t qnear = (q & 1) ? (wn >> 16) : (wn & 0xffffu);
                     uint32_t qfar = (q & 1) ? (wf >> 16) : (wf & 0xffffu);
-                    // org + q * 2^e is exactly representable: the fused form
-                    // returns the same bits as the separate multiply and add
-                    float pn = __builtin_fmaf((float)qnear, step[k], org[k]);
-                    float pf = __builtin_fmaf((float)qfar, step[k], org[k]);
-                    tn3[k] = (pn - oo[k]) * s.inv[k];
-                    tf3[k] = (pf - oo[k]) * s.inv[k];
+                    tn3[k] = __builtin_fmaf((float)qnear, a3[k], bnear[k]);
+                    tf3[k] = __builtin_fmaf((float)qfar, a3[k], bfar[k]);
                 }
-                float n_ = fmaxf(fmaxf(tn3[0], tn3[1]), tn3[2]) * kLo;
-                float f_ = fminf(fminf(tf3[0], tf3[1]), tf3[2]) * kHi;
-                float bn = fmaxf(s.tmin, n_);
-                float bf = fminf(s.best_t, f_);
+                const float bn = fmaxf(fmaxf(tn3[0], tn3[1]), fmaxf(tn3[2], s.tmin));
+                const float bf = fminf(fminf(tf3[0], tf3[1]), fminf(tf3[2], s.best_t));
                 if (bn <= bf) hits |= 1u << q;
             }
             uint32_t inner = hits & imask;
@@ -188,7 +206,7 @@ __device__ __forceinline__ bool trav_step(const TraceScene &sc, TravState &s, co
             while (inner) {
                 uint32_t q = __ffs(inner) - 1;
                 inner &= inner - 1u;
-                ng |= 1u << (q ^ s.oct);
+                ng |= 1u << (q ^ oct);
             }
             uint32_t tm = 0;
             const uint32_t meta_lo = w1.z, meta_hi = w1.w;
@@ -261,7 +279,8 @@ struct RayCursor {
         : n(n_), fetch(fetch_) {
         const uint32_t nthreads = gridDim.x * blockDim.x;
         nwaves = nthreads / 64u;
-        const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) / 64u;
+        // wave-uniform: held in scalar registers
+        const uint32_t wave = __builtin_amdgcn_readfirstlane((blockIdx.x * blockDim.x + threadIdx.x) / 64u);
         const uint32_t rounds = (uint32_t)((uint64_t)n * static_share256 / 256u / nthreads);
         static_chunks = rounds * nwaves;
         dyn_base = rounds ? rounds * nthreads : nthreads;
@@ -286,7 +305,7 @@ struct RayCursor {
             } else {
                 const int leader = __ffsll(m) - 1;
                 if (lane == (uint32_t)leader) nbase = atomicAdd(fetch, kChunk);
-                nbase = dyn_base + __shfl(nbase, leader);
+                nbase = dyn_base + __builtin_amdgcn_readlane(nbase, leader);
                 chunk = kChunk;
             }
         }

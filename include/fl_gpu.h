@@ -254,7 +254,8 @@ typedef struct {
     fl_guide_t guide;            /* guiding network when FL_RENDER_GUIDED */
     float guide_mix;             /* proposal probability of the guide (0.5) */
     float guide_train_fraction;  /* with TRAIN_GUIDE: fraction of spp used to train */
-    uint32_t refill_lanes;       /* traversal: idle lanes of a wave that trigger a refill (0: auto) */
+    uint32_t refill_lanes;       /* traversal: idle lanes of a wave that trigger a refill (0: auto,
+                                    24 with rays loaded ahead, 16 with budgeted refill) */
     uint32_t trav_prefetch;      /* traversal mode: 1 = persistent, each lane's next ray loaded
                                     ahead, the wave yields once refill_lanes lanes are done;
                                     2 = persistent, refill after trav_budget iterations; 3 = one ray
@@ -357,7 +358,7 @@ typedef struct {
     uint32_t sort_shading;        /* 1: per-block material sort on (FL_RENDER_NO_SORT clear) */
     double ms_best, ms_default;   /* device ms of one render with the chosen and the default values */
     uint32_t n_tried;             /* probe configurations measured */
-    uint32_t refill_lanes;        /* traversal refill threshold */
+    uint32_t refill_lanes;        /* traversal refill threshold (0: the mode's default) */
     uint32_t node_width;          /* BVH children per node the scenes now have */
     uint32_t trav_prefetch;       /* traversal refill mode (fl_render_params.trav_prefetch) */
 } fl_tune_result;
