@@ -147,7 +147,14 @@ __global__ void __launch_bounds__(kBlock) k_generate(RenderArgs a) {
 // can pass through an emitter and go on; its shadow rays (target slot tagged
 // kVisClosest) run closest-hit and test the hit's light flag.
 constexpr uint32_t kVisClosest = 0x80000000u;
+// the any-hit word of an extension ray (closest hit), read like a shadow
+// target word so that neither kind of ray tests a loaded value before it starts
+__device__ const uint32_t kClosestWord = kVisClosest;
 
+// MULTI: the pass holds several items (a batch), each ray traverses its
+// path's item's BVH; otherwise every ray traverses sc0, whose pointers stay
+// uniform (scalar registers) in the traversal.
+template <bool MULTI>
 struct TraceSrc {
     const float4 *so, *sd;
     const uint32_t *stgt;
@@ -161,8 +168,9 @@ struct TraceSrc {
     uint32_t n_items;
     TraceScene sc0;
     const float4 *socc;      // NEE term of an occluded shadow ray (shadowColor), or null: 0
+    __device__ TraceScene base_scene() const { return sc0; }
     __device__ TraceScene scene_of_ray(uint32_t i) const {
-        if (n_items <= 1) return sc0;
+        if (!MULTI || n_items <= 1) return sc0;
         uint32_t sid;
         if (i < ns) {
             const uint32_t t = stgt[i] & ~kVisClosest;
@@ -172,10 +180,12 @@ struct TraceSrc {
         }
         return items[st_item[sid]].sc;
     }
-    __device__ void load(uint32_t i, v3 &ro, v3 &rd, float &tmin, float &tmax, bool &any, TraceScene &sc) const {
+    __device__ void load(uint32_t i, v3 &ro, v3 &rd, float &tmin, float &tmax, uint32_t &anyw, TraceScene &sc) const {
         float4 a, b;
-        if (i < ns) { a = so[i]; b = sd[i]; any = (stgt[i] & kVisClosest) == 0u; }
-        else { a = eo[i - ns]; b = ed[i - ns]; any = false; }
+        // a shadow ray is any-hit unless its target word carries kVisClosest
+        anyw = *(i < ns ? stgt + i : &kClosestWord);
+        if (i < ns) { a = so[i]; b = sd[i]; }
+        else { a = eo[i - ns]; b = ed[i - ns]; }
         sc = scene_of_ray(i);
         ro = v3(a.x, a.y, a.z);
         rd = v3(b.x, b.y, b.z);
@@ -205,7 +215,7 @@ struct TraceSrc {
 
 // STACK: LDS short-stack entries per lane (deeper entries spill to global
 // memory); WAVES: minimum waves per SIMD asked of the register allocator.
-template <bool COUNT, int STACK, int WAVES, bool PREFETCH>
+template <bool COUNT, int STACK, int WAVES, bool PREFETCH, bool MULTI>
 __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WAVES))) k_trace(RenderArgs a, int b) {
     __shared__ unsigned long long stk[STACK * kBlock];
     const int cur = b & 1;
@@ -225,8 +235,8 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WAV
     ls.spill = a.spill + blockIdx.x * kBlock + threadIdx.x;
     ls.spill_stride = a.spill_stride;
     TravCounters c;
-    TraceSrc src{a.sq_o,    a.sq_d, a.sq_tgt, a.pend,    a.rq_o[cur], a.rq_d[cur], a.rq_sid[cur],
-                 a.hits,    ns,     a.cap,    a.items,   a.st_item,   a.n_items,   a.sc, a.sq_occ};
+    TraceSrc<MULTI> src{a.sq_o,    a.sq_d, a.sq_tgt, a.pend,    a.rq_o[cur], a.rq_d[cur], a.rq_sid[cur],
+                        a.hits,    ns,     a.cap,    a.items,   a.st_item,   a.n_items,   a.sc, a.sq_occ};
     // refill threshold by default: a yield once 24 lanes are done when the
     // next rays are loaded ahead (spheres, 64 spp: 48.1 against 48.8 ms at
     // 16 and 52.7 at 8), a refill once 16 lanes are idle otherwise
@@ -242,7 +252,7 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WAV
 // One ray per thread, no refill: a grid of blocks over every slot the queues
 // can hold (blocks past the queue size exit at once). For passes of few
 // paths, where a persistent launch's time is mostly its lanes' tails.
-template <int STACK>
+template <int STACK, bool MULTI>
 __global__ void __launch_bounds__(kBlock) k_trace_flat(RenderArgs a, int b) {
     __shared__ unsigned long long stk[STACK * kBlock];
     const int cur = b & 1;
@@ -264,13 +274,14 @@ __global__ void __launch_bounds__(kBlock) k_trace_flat(RenderArgs a, int b) {
     ls.spill = a.spill + i;
     ls.spill_stride = a.spill_stride;
     TravCounters c;
-    TraceSrc src{a.sq_o,    a.sq_d, a.sq_tgt, a.pend,    a.rq_o[cur], a.rq_d[cur], a.rq_sid[cur],
-                 a.hits,    ns,     a.cap,    a.items,   a.st_item,   a.n_items,   a.sc, a.sq_occ};
+    TraceSrc<MULTI> src{a.sq_o,    a.sq_d, a.sq_tgt, a.pend,    a.rq_o[cur], a.rq_d[cur], a.rq_sid[cur],
+                        a.hits,    ns,     a.cap,    a.items,   a.st_item,   a.n_items,   a.sc, a.sq_occ};
     v3 o, d;
     float tmin, tmax;
-    bool any;
+    uint32_t anyw;
     TraceScene sc;
-    src.load(i, o, d, tmin, tmax, any, sc);
+    src.load(i, o, d, tmin, tmax, anyw, sc);
+    const bool any = (int32_t)anyw >= 0;
     TravState s;
     trav_init(s, o, d, tmin, tmax);
     while (!trav_step<false>(sc, s, ls, 1 << 30, c, any, any)) {
@@ -283,23 +294,32 @@ constexpr int kTravBudget = 0;     // persistent, refill after trav_budget itera
 constexpr int kTravPrefetch = 1;   // persistent, each lane's next ray loaded ahead
 constexpr int kTravFlat = 2;       // one ray per thread (counter passes use kTravPrefetch)
 
-template <int S, int WV>
-void launch_trace_t(bool count, dim3 g, hipStream_t st, const RenderArgs &a, int j) {
+template <int S, int WV, bool M>
+void launch_trace_m(bool count, dim3 g, hipStream_t st, const RenderArgs &a, int j) {
     if (a.trav_mode == kTravFlat && !count) {
-        hipLaunchKernelGGL((k_trace_flat<S>), dim3((unsigned)((3ull * a.cap + kBlock - 1) / kBlock)), dim3(kBlock), 0, st,
-                           a, j);
+        hipLaunchKernelGGL((k_trace_flat<S, M>), dim3((unsigned)((3ull * a.cap + kBlock - 1) / kBlock)), dim3(kBlock), 0,
+                           st, a, j);
     } else if (a.trav_mode != kTravBudget) {
-        if (count) hipLaunchKernelGGL((k_trace<true, S, WV, true>), g, dim3(kBlock), 0, st, a, j);
-        else hipLaunchKernelGGL((k_trace<false, S, WV, true>), g, dim3(kBlock), 0, st, a, j);
+        if (count) hipLaunchKernelGGL((k_trace<true, S, WV, true, M>), g, dim3(kBlock), 0, st, a, j);
+        else hipLaunchKernelGGL((k_trace<false, S, WV, true, M>), g, dim3(kBlock), 0, st, a, j);
     } else {
-        if (count) hipLaunchKernelGGL((k_trace<true, S, WV, false>), g, dim3(kBlock), 0, st, a, j);
-        else hipLaunchKernelGGL((k_trace<false, S, WV, false>), g, dim3(kBlock), 0, st, a, j);
+        if (count) hipLaunchKernelGGL((k_trace<true, S, WV, false, M>), g, dim3(kBlock), 0, st, a, j);
+        else hipLaunchKernelGGL((k_trace<false, S, WV, false, M>), g, dim3(kBlock), 0, st, a, j);
     }
 }
+// batches (several items) run the MULTI instantiations
 template <int S, int WV>
-const void *trace_fn(bool prefetch) {
-    return prefetch ? reinterpret_cast<const void *>(k_trace<false, S, WV, true>)
-                    : reinterpret_cast<const void *>(k_trace<false, S, WV, false>);
+void launch_trace_t(bool count, dim3 g, hipStream_t st, const RenderArgs &a, int j) {
+    if (a.n_items > 1) launch_trace_m<S, WV, true>(count, g, st, a, j);
+    else launch_trace_m<S, WV, false>(count, g, st, a, j);
+}
+template <int S, int WV>
+const void *trace_fn(bool prefetch, bool multi) {
+    if (multi)
+        return prefetch ? reinterpret_cast<const void *>(k_trace<false, S, WV, true, true>)
+                        : reinterpret_cast<const void *>(k_trace<false, S, WV, false, true>);
+    return prefetch ? reinterpret_cast<const void *>(k_trace<false, S, WV, true, false>)
+                    : reinterpret_cast<const void *>(k_trace<false, S, WV, false, false>);
 }
 
 // traversal variants (LDS stack entries, minimum waves per SIMD); the first
@@ -307,7 +327,7 @@ const void *trace_fn(bool prefetch) {
 struct TraceVariant {
     int stack, waves;
     void (*launch)(bool, dim3, hipStream_t, const RenderArgs &, int);
-    const void *(*fn)(bool);
+    const void *(*fn)(bool, bool);
 };
 const TraceVariant kTraceVariants[] = {
     {8, 5, launch_trace_t<8, 5>, trace_fn<8, 5>},    {12, 1, launch_trace_t<12, 1>, trace_fn<12, 1>},
@@ -1500,7 +1520,7 @@ int render_items(fl_ctx_t ctx, const HostItem *hi, uint32_t n_items, const fl_re
     uint32_t grid_mode[2];
     for (int m = 0; m < 2; ++m) {
         int occ = 0;
-        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, tv->fn(m == kTravPrefetch), kBlock, 0) != hipSuccess ||
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, tv->fn(m == kTravPrefetch, n_items > 1), kBlock, 0) != hipSuccess ||
             occ <= 0) {
             (void)hipGetLastError();
             occ = 4;

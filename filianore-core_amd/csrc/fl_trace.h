@@ -324,9 +324,11 @@ struct RayCursor {
 // Persistent traversal over n rays with per-lane refill (the two modes are
 // separate instantiations, PREFETCH, so each gets its own register budget).
 // Src provides
-//   void load(uint32_t i, v3 &o, v3 &d, float &tmin, float &tmax, bool &any,
-//             TraceScene &sc) const     (sc: the BVH this ray traverses)
+//   void load(uint32_t i, v3 &o, v3 &d, float &tmin, float &tmax, uint32_t &anyw,
+//             TraceScene &sc) const     (sc: the BVH this ray traverses; the ray
+//             is any-hit when anyw's top bit is clear)
 //   void store(uint32_t i, const TravState &s) const
+//   TraceScene base_scene() const   (the scene of every ray, or of the first item)
 // Rays come from a RayCursor (`fetch` must be 0 at launch). Two refill modes:
 //  - prefetch: each lane holds its next ray already loaded, so a finished
 //    lane starts it at once; the traversal yields to do that once
@@ -349,7 +351,9 @@ constexpr uint32_t kNoRay = ~0u;
 template <bool COUNT, bool ANY_SKIP, bool PREFETCH, class Src>
 __device__ __forceinline__ void trace_persistent(const Src &src, uint32_t n, uint32_t *fetch, const LaneStack &ls,
                                                  int budget, int refill_lanes, TravCounters &cnt) {
-    TraceScene sc{};
+    // a source whose rays all traverse one scene returns it from every load:
+    // starting from it keeps the scene pointers uniform (scalar registers)
+    TraceScene sc = src.base_scene();
     TravState s;
     RayCursor cur(n, fetch, PREFETCH ? 0u : 192u);   // static share: small scenes only
     const uint32_t first = cur.take(true);   // lane g of the grid: ray g
@@ -360,7 +364,9 @@ __device__ __forceinline__ void trace_persistent(const Src &src, uint32_t n, uin
         ray = first;
         v3 o, d;
         float tmin, tmax;
-        src.load(first, o, d, tmin, tmax, any, sc);
+        uint32_t anyw;
+        src.load(first, o, d, tmin, tmax, anyw, sc);
+        any = (int32_t)anyw >= 0;
         trav_init(s, o, d, tmin, tmax);
     } else {
         exhausted = true;
@@ -389,7 +395,9 @@ __device__ __forceinline__ void trace_persistent(const Src &src, uint32_t n, uin
                         ray = idx;
                         v3 o, d;
                         float tmin, tmax;
-                        src.load(idx, o, d, tmin, tmax, any, sc);
+                        uint32_t anyw;
+                        src.load(idx, o, d, tmin, tmax, anyw, sc);
+                        any = (int32_t)anyw >= 0;
                         trav_init(s, o, d, tmin, tmax);
                     }
                 }
@@ -401,8 +409,10 @@ __device__ __forceinline__ void trace_persistent(const Src &src, uint32_t n, uin
     uint32_t nxt = kNoRay;
     v3 no(0.f, 0.f, 0.f), nd(0.f, 0.f, 0.f);
     float ntmin = 0.f, ntmax = 0.f;
-    bool nany = false;
-    TraceScene nsc{};
+    // the next ray's any-hit word as loaded (top bit clear: any-hit), tested
+    // only when the ray starts so that no wait for the load comes earlier
+    uint32_t nanyw = 0x80000000u;
+    TraceScene nsc = sc;
     while (true) {
         const bool need = nxt == kNoRay && !exhausted;
         if (__ballot(need)) {
@@ -412,14 +422,14 @@ __device__ __forceinline__ void trace_persistent(const Src &src, uint32_t n, uin
                     exhausted = true;
                 } else {
                     nxt = idx;
-                    src.load(idx, no, nd, ntmin, ntmax, nany, nsc);
+                    src.load(idx, no, nd, ntmin, ntmax, nanyw, nsc);
                 }
             }
         }
         if (ray == kNoRay && nxt != kNoRay) {
             ray = nxt;
             nxt = kNoRay;
-            any = nany;
+            any = (int32_t)nanyw >= 0;
             sc = nsc;
             trav_init(s, no, nd, ntmin, ntmax);
         }

@@ -27,9 +27,10 @@ struct ApiSrc {
     fl_hit *hits;
     bool any;
     fl::TraceScene sc;
-    __device__ void load(uint32_t i, fl::v3 &o, fl::v3 &d, float &tmin, float &tmax, bool &is_any,
+    __device__ fl::TraceScene base_scene() const { return sc; }
+    __device__ void load(uint32_t i, fl::v3 &o, fl::v3 &d, float &tmin, float &tmax, uint32_t &anyw,
                          fl::TraceScene &ray_sc) const {
-        is_any = any;
+        anyw = any ? 0u : 0x80000000u;
         ray_sc = sc;
         const float4 *rp = reinterpret_cast<const float4 *>(rays + i);
         float4 a = rp[0], b = rp[1];
