@@ -450,6 +450,10 @@ def main():
             }
             issue = issue_from_profile()
             if issue:
+                # the byte roofline is the one the workload is specified
+                # against; what binds the kernel is instruction issue under
+                # divergence, with L2 misses served from the Infinity Cache
+                roofline["binding"] = issue.get("binding")
                 roofline["issue"] = issue
             if not args.no_intersect:
                 log("intersection section: 4M rays")
