@@ -220,14 +220,15 @@ __device__ __forceinline__ void layer_fwd(const __bf16 *W, const float *b, const
         }
 }
 
-// layer_fwd with the weights and biases in LDS (row stride kGRow) and the
+// layer_fwd with the weights at row stride ws (LDS or global) and the
 // output written four units at a time (each lane's 4 consecutive units of
 // one sample: one 8-byte store). Hout may be Hin: every fragment is loaded
 // before the first store, and a wave's LDS operations complete in order.
-__device__ __forceinline__ void layer_fwd_lds(const __bf16 *W, const float *b, const __bf16 *Hin, __bf16 *Hout) {
+__device__ __forceinline__ void layer_fwd_rows(const __bf16 *W, int ws, const float *b, const __bf16 *Hin,
+                                               __bf16 *Hout) {
     f32x4 acc[4][4];
     acc_zero(acc);
-    mm64(W, kGRow, Hin, kGRow, acc);
+    mm64(W, ws, Hin, kGRow, acc);
 #pragma unroll
     for (int mt = 0; mt < 4; ++mt) {
         const int o0 = acc_m(mt, 0);
@@ -241,6 +242,32 @@ __device__ __forceinline__ void layer_fwd_lds(const __bf16 *W, const float *b, c
             v[2] = to_bf16(fmaxf(acc[mt][nt][2] + b2, 0.f));
             v[3] = to_bf16(fmaxf(acc[mt][nt][3] + b3, 0.f));
             *reinterpret_cast<bf16x4 *>(Hout + acc_n(nt) * kGRow + o0) = v;
+        }
+    }
+}
+
+// One hidden layer for samples [32 h, 32 h + 32) of a wave's tile H
+// ([64][kGRow] bf16), in place: H[q][o] = relu(sum_i W[o][i] H[q][i] + b[o]).
+// The half reads and writes only its own 32 rows, so the two halves may run
+// one after the other on the same tile; it holds 32 accumulators instead of
+// the 64 of a whole layer (for kernels with little register room to spare).
+__device__ __forceinline__ void layer_fwd_half(const __bf16 *W, int ws, const float *b, __bf16 *H, int h) {
+    f32x4 acc[4][2];
+    acc_zero_t(acc);
+    mm_tiles<4, 2>(W, ws, H + 32 * h * kGRow, kGRow, acc);
+#pragma unroll
+    for (int mt = 0; mt < 4; ++mt) {
+        const int o0 = acc_m(mt, 0);
+        const float b0 = b[o0], b1 = b[o0 + 1], b2 = b[o0 + 2], b3 = b[o0 + 3];
+#pragma unroll
+        for (int nt = 0; nt < 2; ++nt) {
+            typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
+            bf16x4 v;
+            v[0] = to_bf16(fmaxf(acc[mt][nt][0] + b0, 0.f));
+            v[1] = to_bf16(fmaxf(acc[mt][nt][1] + b1, 0.f));
+            v[2] = to_bf16(fmaxf(acc[mt][nt][2] + b2, 0.f));
+            v[3] = to_bf16(fmaxf(acc[mt][nt][3] + b3, 0.f));
+            *reinterpret_cast<bf16x4 *>(H + (32 * h + acc_n(nt)) * kGRow + o0) = v;
         }
     }
 }

@@ -336,49 +336,35 @@ const TraceVariant kTraceVariants[] = {
 };
 
 // ---------------------------------------------------------------- guide ----
-// Decoded lobes of one queue slot (PackedLobes, fp16). Sampling and density
-// read them the same way: the mean direction is renormalised after the fp16
-// rounding, and the weights are divided by their stored sum, so the density
-// is that of the sampling to fp32 rounding.
-static_assert(sizeof(PackedLobes) == 80, "80 bytes of lobes per slot");
+// The lobes of a guided vertex live in its wave's LDS region for the rest of
+// that vertex's shading (kLobeRows rows of 64 lanes; `lb` points at the
+// lane's column). Row 6 l + j of lobe l: j = 0..2 mean direction, 3 kappa,
+// 4 cumulative weight (of lobes 0..l), 5 weight times the vMF normalisation.
+// Sampling picks a lobe by the cumulative weights and the density sums the
+// scaled lobes, so both use the same mixture to fp32 rounding.
+__device__ __forceinline__ float lobe_val(const float *lb, int l, int j) { return lb[(kLobeVals * l + j) * 64]; }
 
-__device__ __forceinline__ VmfLobe lobe_at(const PackedLobes &P, int i) {
-    const v3 m((float)P.h[5 * i], (float)P.h[5 * i + 1], (float)P.h[5 * i + 2]);
-    VmfLobe L;
-    L.mu = m * (1.f / sqrtf(dot(m, m)));
-    L.kappa = (float)P.h[5 * i + 3];
-    L.w = (float)P.h[5 * i + 4];
-    return L;
-}
-
-__device__ __forceinline__ float lobe_wsum(const PackedLobes &P) {
-    float s = 0.f;
-#pragma unroll
-    for (int i = 0; i < kGLobes; ++i) s += (float)P.h[5 * i + 4];
-    return s;
-}
-
-__device__ __forceinline__ float guide_pdf_at(const PackedLobes *lb, v3 w) {
-    const PackedLobes &P = *lb;
+__device__ __forceinline__ float guide_pdf_at(const float *lb, v3 w) {
     float p = 0.f;
-#pragma unroll
-    for (int i = 0; i < kGLobes; ++i) {
-        const VmfLobe L = lobe_at(P, i);
-        p += L.w * vmf_norm(L.kappa) * expf(L.kappa * (dot(L.mu, w) - 1.f));
+    // one lobe per iteration: 5 values live instead of 40 in code that is
+    // already short of registers
+#pragma unroll 1
+    for (int l = 0; l < kGLobes; ++l) {
+        const v3 mu(lobe_val(lb, l, 0), lobe_val(lb, l, 1), lobe_val(lb, l, 2));
+        p += lobe_val(lb, l, 5) * expf(lobe_val(lb, l, 3) * (dot(mu, w) - 1.f));
     }
-    return p / lobe_wsum(P);
+    return p;
 }
 
-__device__ __forceinline__ v3 guide_sample_at(const PackedLobes *lb, float u_lobe, float u1, float u2) {
-    const PackedLobes &P = *lb;
-    const float target = u_lobe * lobe_wsum(P);
+__device__ __forceinline__ v3 guide_sample_at(const float *lb, float u_lobe, float u1, float u2) {
     int sel = kGLobes - 1;
-    float c = 0.f;
-    for (int i = 0; i < kGLobes - 1; ++i) {
-        c += (float)P.h[5 * i + 4];
-        if (target < c) { sel = i; break; }
-    }
-    return vmf_sample(lobe_at(P, sel), u1, u2);
+    for (int l = 0; l < kGLobes - 1; ++l)
+        if (u_lobe < lobe_val(lb, l, 4)) { sel = l; break; }
+    VmfLobe L;
+    L.mu = v3(lobe_val(lb, sel, 0), lobe_val(lb, sel, 1), lobe_val(lb, sel, 2));
+    L.kappa = lobe_val(lb, sel, 3);
+    L.w = 1.f;
+    return vmf_sample(L, u1, u2);
 }
 
 // Materials the guide serves: the non-specular ones.
@@ -396,117 +382,88 @@ __device__ __forceinline__ void guide_features(const RenderArgs &a, v3 p, v3 ng,
     rough = m.kind == MAT_CONDUCTOR ? m.ax : 1.f;
 }
 
-// Network evaluation for every guidable hit of extension queue `b & 1`,
-// lobes written per queue slot. A persistent block of kInferWaves waves per
-// CU holds the five bf16 weight matrices and the biases in LDS (loaded once);
-// each wave takes 64-slot chunks in turn and runs them through its own tile,
-// updated in place (a layer's fragments are all loaded before its epilogue
-// overwrites the tile, and a wave's LDS operations complete in order). The
-// head's fp32 outputs (40 per slot, odd row stride against bank conflicts)
-// take the same bytes. Waves run independently: the only block barrier is
-// the one after the weights are loaded.
-constexpr int kInferRaw = 41;
-constexpr int kInferWaves = 8;
+// The network for the 64 vertices of a wave, inside the guided shading
+// kernel: the encoded inputs go into the wave's LDS region (bf16 rows), the
+// four hidden layers run on the matrix cores in place with the bf16 weights
+// read through the caches (41 KB, shared by every wave of the device), the
+// head's 40 outputs go to the same region as fp32 rows, and each lane
+// decodes its own row into its lobe column. Lanes without a query feed zeros
+// and get no lobes. All lanes of the wave call this; `reg` is the wave's
+// region (kGuideRegion floats).
+constexpr int kGuideRaw = 41;   // fp32 row stride of the head outputs (odd: no bank conflicts)
+constexpr int kGuideRegion = kLobeRows * 64;   // floats per wave: the largest of the three uses
+static_assert(kGuideRegion * 4 >= 64 * kGRow * 2 && kGuideRegion >= 64 * kGuideRaw, "the region holds every stage");
 __device__ __forceinline__ void wave_sync() {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
-__global__ void __launch_bounds__(64 * kInferWaves) k_guide_infer(RenderArgs a, int b) {
-    __shared__ __attribute__((aligned(16))) __bf16 sW[kGMats][64 * kGRow];
-    __shared__ float sBias[kGMats][64];
-    __shared__ __attribute__((aligned(16))) float sTile[kInferWaves][64 * kInferRaw];
-    static_assert(sizeof(float) * 64 * kInferRaw >= sizeof(__bf16) * 64 * kGRow, "a tile holds a hidden layer");
-    for (int e = threadIdx.x; e < kGMats * 64 * 8; e += blockDim.x) {   // 16-byte pieces of W[k][o][0..63]
-        const int k = e / 512, o = (e / 8) % 64, c = e % 8;
-        *reinterpret_cast<bf16x8 *>(&sW[k][o * kGRow + 8 * c]) =
-            *reinterpret_cast<const bf16x8 *>(a.gW + k * 4096 + o * 64 + 8 * c);
-    }
-    for (int e = threadIdx.x; e < kGMats * 64; e += blockDim.x)
-        sBias[e / 64][e % 64] = a.gP[(e / 64) * kGLayerParams + 4096 + e % 64];
-    __syncthreads();
-    const int w = threadIdx.x >> 6, q = threadIdx.x & 63;
-    float *sB = sTile[w];
-    __bf16 *sT = reinterpret_cast<__bf16 *>(sB);
-    const uint32_t n = a.counts[C_EXT + (b & 1)];
-    const uint32_t nchunks = (n + 63) / 64;
-    unsigned long long n_queries = 0;   // this wave's count, one atomic at the end
-    for (uint32_t ch = blockIdx.x * kInferWaves + w; ch < nchunks; ch += gridDim.x * kInferWaves) {
-        const uint32_t i = ch * 64u + q;
-        float pos01[3] = {0.f, 0.f, 0.f}, lam01 = 0.f, rough = 0.f;
-        v3 n_in(0.f, 0.f, 0.f);
-        bool want = false;
-        if (i < n) {
-            const float4 hh = a.hits[i];
-            const uint32_t rec = __float_as_uint(hh.y);
-            const uint32_t sid = a.rq_sid[b & 1][i];
-            const SceneRef S = scene_of(a, item_of(a, sid));
-            if (rec != kNoHit) {
-                const uint4 *tp = reinterpret_cast<const uint4 *>(S.sc.tris + rec);
-                const uint4 q0 = tp[0], q1 = tp[1], q2 = tp[2];
-                const uint32_t mat = q1.w;
-                if (mat != kNoMaterial && guidable(S.mats[mat].kind)) {
-                    want = true;
-                    const float4 ro = a.rq_o[b & 1][i], rd = a.rq_d[b & 1][i];
-                    const v3 d(rd.x, rd.y, rd.z);
-                    const v3 p = v3(ro.x, ro.y, ro.z) + d * hh.x;
-                    const v3 p0(__uint_as_float(q0.x), __uint_as_float(q0.y), __uint_as_float(q0.z));
-                    const v3 p1(__uint_as_float(q1.x), __uint_as_float(q1.y), __uint_as_float(q1.z));
-                    const v3 p2(__uint_as_float(q2.x), __uint_as_float(q2.y), __uint_as_float(q2.z));
-                    const v3 ng = normalize(cross(p1 - p0, p2 - p0));
-                    const float lam0 = a.st_misc[sid].x;
-                    guide_features(a, p, ng, d, lam0, S.mats[mat], pos01, n_in, lam01, rough);
-                }
-            }
-        }
-        const unsigned long long wm = __ballot(want);
-        n_queries += (unsigned long long)__popcll(wm);
-        if (wm == 0ull) continue;   // no query in this chunk (wave-uniform)
-        encode_row_bf16(sT, q, want, pos01, n_in, lam01, rough);
+__device__ __forceinline__ void guide_wave_mlp(const RenderArgs &a, float *reg, bool want, const float pos01[3], v3 n_in,
+                                               float lam01, float rough) {
+    const int q = threadIdx.x & 63;
+    __bf16 *sT = reinterpret_cast<__bf16 *>(reg);
+    // The weights and biases are the same for every vertex: without this
+    // empty asm, which the compiler must assume changes the two pointers, it
+    // hoists their loads (some 150 values per lane) out of the shading loop
+    // and keeps them live across all of it.
+    const __bf16 *gW = a.gW;
+    const float *gP = a.gP;
+    asm volatile("" : "+s"(gW), "+s"(gP));
+    encode_row_bf16(sT, q, want, pos01, n_in, lam01, rough);
+    wave_sync();
+#pragma unroll 1
+    for (int k = 0; k < 4; ++k) {
+        // in two halves of 32 samples: half the accumulators live at a time
+        layer_fwd_half(gW + k * 4096, kGH, gP + k * kGLayerParams + 4096, sT, 0);
+        layer_fwd_half(gW + k * 4096, kGH, gP + k * kGLayerParams + 4096, sT, 1);
         wave_sync();
-        for (int k = 0; k < 4; ++k) {
-            layer_fwd_lds(sW[k], sBias[k], sT, sT);
-            wave_sync();
-        }
-        {   // head, into the same bytes as fp32 (row stride kInferRaw)
-            f32x4 acc[4][4];
-            acc_zero(acc);
-            mm64(sW[4], kGRow, sT, kGRow, acc);
-            wave_sync();
+    }
+    {   // head: rows 0..47 of the last matrix (40 used), into fp32 rows that
+        // overlap the tile, so both halves' products come before any store
+        f32x4 acc0[3][2], acc1[3][2];
+        acc_zero_t(acc0);
+        acc_zero_t(acc1);
+        mm_tiles<3, 2>(gW + 4 * 4096, kGH, sT, kGRow, acc0);
+        mm_tiles<3, 2>(gW + 4 * 4096, kGH, sT + 32 * kGRow, kGRow, acc1);
+        const float *bh = gP + 4 * kGLayerParams + 4096;
+        wave_sync();   // every fragment of the tile is read before the rows are overwritten
 #pragma unroll
-            for (int mt = 0; mt < 3; ++mt)
+        for (int mt = 0; mt < 3; ++mt)
 #pragma unroll
-                for (int nt = 0; nt < 4; ++nt)
+            for (int nt = 0; nt < 2; ++nt)
 #pragma unroll
-                    for (int r = 0; r < 4; ++r) {
-                        const int o = acc_m(mt, r);
-                        if (o < kGOut) sB[acc_n(nt) * kInferRaw + o] = acc[mt][nt][r] + sBias[4][o];
+                for (int r = 0; r < 4; ++r) {
+                    const int o = acc_m(mt, r);
+                    if (o < kGOut) {
+                        reg[acc_n(nt) * kGuideRaw + o] = acc0[mt][nt][r] + bh[o];
+                        reg[(32 + acc_n(nt)) * kGuideRaw + o] = acc1[mt][nt][r] + bh[o];
                     }
-        }
-        wave_sync();
-        if (want) {
-            VmfLobe L[kGLobes];
-            guide_decode(sB + q * kInferRaw, L);
-            PackedLobes pk;
-#pragma unroll
-            for (int l = 0; l < kGLobes; ++l) {
-                pk.h[5 * l + 0] = (_Float16)L[l].mu.x;
-                pk.h[5 * l + 1] = (_Float16)L[l].mu.y;
-                pk.h[5 * l + 2] = (_Float16)L[l].mu.z;
-                pk.h[5 * l + 3] = (_Float16)L[l].kappa;   // exp of [-3, 7]: within fp16
-                pk.h[5 * l + 4] = (_Float16)L[l].w;
-            }
-            a.lobes[i] = pk;
-        }
-        wave_sync();   // the next chunk's encode overwrites the rows just read
+                }
     }
-    if (q == 0 && n_queries) atomicAdd(&a.stats[6], n_queries);
+    wave_sync();
+    VmfLobe L[kGLobes];
+    if (want) guide_decode(reg + q * kGuideRaw, L);
+    wave_sync();   // every row is read before the lobe columns overwrite them
+    if (want) {
+        float cum = 0.f;
+#pragma unroll
+        for (int l = 0; l < kGLobes; ++l) {
+            cum += L[l].w;
+            float *c = reg + kLobeVals * l * 64 + q;
+            c[0] = L[l].mu.x;
+            c[64] = L[l].mu.y;
+            c[128] = L[l].mu.z;
+            c[192] = L[l].kappa;
+            c[256] = cum;
+            c[320] = L[l].w * vmf_norm(L[l].kappa);
+        }
+    }
 }
 
-// Online training runs in kTrainRounds passes of doubling size over the
-// first train_spp samples per pixel; pass r ends after
-// train_spp (2^(r+1) - 1) / (2^R - 1) samples (r = R - 1 ends at train_spp).
-constexpr uint32_t kTrainRounds = 4;
+// Online training runs in R passes of doubling size (R = the guide's
+// train_rounds, 4 by default) over the first train_spp samples per pixel;
+// pass r ends after train_spp (2^(r+1) - 1) / (2^R - 1) samples
+// (r = R - 1 ends at train_spp).
 inline uint32_t train_round_end(uint32_t train_spp, uint32_t rounds, int r) {
     if (r < 0) return 0u;
     const uint64_t den = (1ull << rounds) - 1ull, num = (2ull << r) - 1ull;
@@ -627,7 +584,7 @@ __device__ __forceinline__ bool nee_sample(const RenderArgs &a, const SceneRef &
                                            const TabSpec &tsp, const Lanes &ln, uint32_t li, float sel,
                                            float ux, float uy, v3 p, v3 perr, v3 ng, v3 fs, v3 ft, v3 woW, v3 wo,
                                            const float beta[4], float contrib[4], float occ[4], float4 &sh_o,
-                                           float4 &sh_d, const PackedLobes *lb, float mix) {
+                                           float4 &sh_d, const float *lb, float mix) {
     const DevLight &Lj = S.lights[li];
     const float *te = S.spec + (size_t)Lj.em_slot * a.grid.stride;
     const float *ts = Lj.shadow_slot ? S.spec + (size_t)Lj.shadow_slot * a.grid.stride : nullptr;
@@ -732,6 +689,9 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WAV
     // four wavelengths, one column per thread (20 KB: registers would hold
     // them across the whole shading of every vertex)
     __shared__ float s_tv[STD ? SS_ETA * 4 * kBlock : 1];
+    // guided renders: each wave's region for the network and its lobes (12 KB)
+    __shared__ __attribute__((aligned(16))) float s_guide[GUIDED ? (kBlock / 64) * kGuideRegion : 1];
+    unsigned long long n_queries = 0;   // guided vertices of this wave (counted by lane 0)
     const uint32_t stride = gridDim.x * kBlock;
     const int cur = b & 1, nxt = cur ^ 1;
     const uint32_t n = a.counts[C_EXT + cur];
@@ -755,22 +715,68 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WAV
         }
         const bool active = i < n;
         const SceneRef S = scene_of(a, active ? item_of(a, a.rq_sid[cur][i]) : 0u);
-        // -------- phase 1: hit, emission, pass-through, material setup
-        bool cont = false, mat_hit = false, rec_want = false;
+        // -------- phase 1a: the hit's geometry and the path's wavelength
+        bool cont = false, mat_hit = false, rec_want = false, hit = false;
         uint32_t rec_prev = 0;   // index + 1 of the previous vertex's training record
         float beta0 = 0.f;
-        uint32_t sid = 0, bounce = 0, flags = 0, rb = 0, mkind = 0;
+        uint32_t sid = 0, bounce = 0, flags = 0, rb = 0, mkind = 0, light = 0;
         float beta[4] = {0.f, 0.f, 0.f, 0.f}, L[4] = {0.f, 0.f, 0.f, 0.f};
-        float lam0 = 0.f, eta_scale = 1.f, prev_pdf = 0.f, eta_bsdf = 1.f;
+        float lam0 = 0.f, eta_scale = 1.f, prev_pdf = 0.f, eta_bsdf = 1.f, t = 0.f;
         v3 p, ng, d, no, nd, perr;
         float2 uv = make_float2(0.f, 0.f);   // texture coordinates of the hit (textured scenes)
         Lanes ln;
         if (active) {
-            float4 ro = a.rq_o[cur][i], rd = a.rq_d[cur][i];
+            const float4 ro = a.rq_o[cur][i], rd = a.rq_d[cur][i];
             sid = a.rq_sid[cur][i];
-            float4 hh = a.hits[i];
-            float4 beta4 = a.st_beta[sid], L4 = a.st_L[sid], misc = a.st_misc[sid];
-            float4 pe0 = a.pend[sid], pe1 = a.pend[a.cap + sid];
+            const float4 hh = a.hits[i], misc = a.st_misc[sid];
+            lam0 = misc.x; eta_scale = misc.y; prev_pdf = misc.z;
+            const uint32_t bits = __float_as_uint(misc.w);
+            bounce = bits & 0xffu;
+            flags = bits & ~0xffu;
+            const uint32_t rec = __float_as_uint(hh.y);
+            t = hh.x;
+            d = v3(rd.x, rd.y, rd.z);
+            if (rec != kNoHit) {
+                const uint4 *tp = reinterpret_cast<const uint4 *>(S.sc.tris + rec);
+                uint4 q0 = tp[0], q1 = tp[1], q2 = tp[2];
+                v3 p0(__uint_as_float(q0.x), __uint_as_float(q0.y), __uint_as_float(q0.z));
+                v3 p1(__uint_as_float(q1.x), __uint_as_float(q1.y), __uint_as_float(q1.z));
+                v3 p2(__uint_as_float(q2.x), __uint_as_float(q2.y), __uint_as_float(q2.z));
+                mkind = q1.w;
+                light = q2.w;
+                p = v3(ro.x, ro.y, ro.z) + d * t;                  // ray.point_at_t(t)
+                ng = normalize(cross(p1 - p0, p2 - p0));
+                if (STD && S.tex) uv = hit_uv(*S.tex, q0.w, hh.z, hh.w);
+                hit = true;
+                // a material vertex unless the first hit ends the path
+                // (NormalIntegrator), the MIS ray of the last vertex is being
+                // closed, or the primitive has no BSDF
+                mat_hit = !(a.flags & FL_RENDER_NORMALS) && !(flags & PF_LAST) && mkind != kNoMaterial;
+            }
+        }
+        // -------- the guide's lobes at this vertex (guided passes): the
+        // network runs for the wave's guidable vertices together, before the
+        // path state is loaded, so that little is live across it
+        const float *lb = nullptr;
+        if constexpr (GUIDED) {
+            if (a.guided) {
+                const bool want = mat_hit && bounce < a.guide_bounces && guidable(S.mats[mkind].kind);
+                const unsigned long long wm = __ballot(want);
+                if (wm) {
+                    if (__lane_id() == 0) n_queries += (unsigned long long)__popcll(wm);
+                    float pos01[3] = {0.f, 0.f, 0.f}, lam01 = 0.f, rough = 0.f;
+                    v3 n_in(0.f, 0.f, 0.f);
+                    if (want) guide_features(a, p, ng, d, lam0, S.mats[mkind], pos01, n_in, lam01, rough);
+                    float *reg = s_guide + (threadIdx.x >> 6) * kGuideRegion;
+                    guide_wave_mlp(a, reg, want, pos01, n_in, lam01, rough);
+                    if (want) lb = reg + (threadIdx.x & 63);
+                }
+            }
+        }
+        // -------- phase 1b: path state, emission, pass-through
+        if (active) {
+            const float4 beta4 = a.st_beta[sid], L4 = a.st_L[sid];
+            const float4 pe0 = a.pend[sid], pe1 = a.pend[a.cap + sid];
             beta[0] = beta4.x; beta[1] = beta4.y; beta[2] = beta4.z; beta[3] = beta4.w;
             L[0] = L4.x + pe0.x + pe1.x; L[1] = L4.y + pe0.y + pe1.y;
             L[2] = L4.z + pe0.z + pe1.z; L[3] = L4.w + pe0.w + pe1.w;
@@ -783,23 +789,7 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WAV
                     a.st_rec[sid] = 0u;
                 }
             }
-            lam0 = misc.x; eta_scale = misc.y; prev_pdf = misc.z;
-            uint32_t bits = __float_as_uint(misc.w);
-            bounce = bits & 0xffu;
-            flags = bits & ~0xffu;
-            const uint32_t rec = __float_as_uint(hh.y);
-            const float t = hh.x;
-            d = v3(rd.x, rd.y, rd.z);
-            if (rec != kNoHit) {
-                const uint4 *tp = reinterpret_cast<const uint4 *>(S.sc.tris + rec);
-                uint4 q0 = tp[0], q1 = tp[1], q2 = tp[2];
-                v3 p0(__uint_as_float(q0.x), __uint_as_float(q0.y), __uint_as_float(q0.z));
-                v3 p1(__uint_as_float(q1.x), __uint_as_float(q1.y), __uint_as_float(q1.z));
-                v3 p2(__uint_as_float(q2.x), __uint_as_float(q2.y), __uint_as_float(q2.z));
-                const uint32_t mat = q1.w, light = q2.w;
-                p = v3(ro.x, ro.y, ro.z) + d * t;                  // ray.point_at_t(t)
-                ng = normalize(cross(p1 - p0, p2 - p0));
-                if (STD && S.tex) uv = hit_uv(*S.tex, q0.w, hh.z, hh.w);
+            if (hit) {
                 ln = make_lanes(a.grid, lam0);
                 // emission: L += beta * le(-dir) at bounce 0 / after specular; MIS otherwise
                 if (a.flags & FL_RENDER_NORMALS) {
@@ -834,7 +824,7 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WAV
                 perr = vabs(p) * gamma_n(5);
                 if (flags & PF_LAST) {
                     // the MIS ray of the last vertex: emission counted, path ends
-                } else if (mat == kNoMaterial) {
+                } else if (mkind == kNoMaterial) {
                     // no BSDF: isect.kindle_ray(ray.dir), bounces-- (pathintegrator.cpp:34-38)
                     no = offset_point_origin(p, perr, ng, d);
                     nd = d;
@@ -842,8 +832,6 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WAV
                     cont = true;
                 } else {
                     flags &= ~PF_PASSED;
-                    mat_hit = true;
-                    mkind = mat;
                 }
             }
         }
@@ -881,7 +869,6 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WAV
             wo = v3(dot(woW, fs), dot(woW, ft), dot(woW, ng));
             rb = rng_base(S.seed, a.st_key[sid].x, a.st_key[sid].y, bounce);
         }
-        const PackedLobes *lb = (GUIDED && a.guided && mat_hit && guidable(bs_.kind)) ? a.lobes + i : nullptr;
         const TabSpec tsp{S.spec, sd ? sd->slot : nullptr, GS, ln, a.grid.linear == 0,
                           (STD && sd && S.tex) ? sd->tex : nullptr, s_tv + (STD ? threadIdx.x : 0u)};
         const bool do_nee = mat_hit && S.n_lights > 0 &&
@@ -1093,7 +1080,9 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WAV
             a.rq_d[nxt][slot] = make_float4(nd.x, nd.y, nd.z, kInf);
             a.rq_sid[nxt][slot] = sid;
         }
+        if constexpr (GUIDED) wave_sync();   // the next vertex's network overwrites the lobes just read
     }
+    if (GUIDED && __lane_id() == 0 && n_queries) atomicAdd(&a.stats[6], n_queries);
 }
 
 // The shading kernel for a render: standard surfaces / textures (STD), a
@@ -1462,8 +1451,7 @@ int render_items(fl_ctx_t ctx, const HostItem *hi, uint32_t n_items, const fl_re
     // samples per pixel per pass.
     uint32_t max_paths = p->max_paths;
     if (!max_paths) {
-        // guided passes also hold 80 B of lobes per path (2^26 paths: 5.4 GB)
-        const size_t per_path = kBytesPerPath + (guide ? sizeof(PackedLobes) : 0u);
+        const size_t per_path = kBytesPerPath;
         size_t free_b = 0, total_b = 0;
         max_paths = 1u << 26;
         if (hipMemGetInfo(&free_b, &total_b) == hipSuccess) {
@@ -1580,6 +1568,7 @@ int render_items(fl_ctx_t ctx, const HostItem *hi, uint32_t n_items, const fl_re
     uint32_t *st_rec = nullptr, rec_cap = 0;
     if (guide) {
         a.guide_mix = p->guide_mix > 0.f && p->guide_mix < 1.f ? p->guide_mix : 0.5f;
+        a.guide_bounces = p->guide_bounces ? p->guide_bounces : ~0u;
         a.gW = guide->d_w;
         a.gP = guide->d_params;
         for (int k = 0; k < 3; ++k) {
@@ -1587,12 +1576,11 @@ int render_items(fl_ctx_t ctx, const HostItem *hi, uint32_t n_items, const fl_re
             a.box_lo[k] = s->bmin[k] - pad;
             a.box_inv[k] = 1.f / (s->bmax[k] - s->bmin[k] + 2.f * pad);
         }
-        a.lobes = grow<PackedLobes>(B.p[20], B.cap[20], cap * sizeof(PackedLobes), err);
         if (train) {
             const float fr = p->guide_train_fraction > 0.f && p->guide_train_fraction <= 1.f ? p->guide_train_fraction
                                                                                               : 0.25f;
             train_spp = std::min(max_spp, std::max(1u, (uint32_t)std::lround(fr * (float)max_spp)));
-            train_rounds = std::min(kTrainRounds, train_spp);
+            train_rounds = std::min(guide->cfg.train_rounds, train_spp);
             const uint32_t round_spp = train_round_end(train_spp, train_rounds, train_rounds - 1) -
                                        train_round_end(train_spp, train_rounds, train_rounds - 2);
             rec_cap = (uint32_t)std::min<size_t>((size_t)a.total_pix * std::min(round_spp, spp_pass) * 6, 1u << 26);
@@ -1615,7 +1603,7 @@ int render_items(fl_ctx_t ctx, const HostItem *hi, uint32_t n_items, const fl_re
     FL_HIP_CHECK(ctx, hipEventRecord(e0, st));
 
     uint32_t done_spp = 0, passes = 0, max_iter_seen = 0, n_trace = 0, n_shade = 0;
-    double ms_trace = 0.0, ms_shade = 0.0, ms_infer = 0.0;
+    double ms_trace = 0.0, ms_shade = 0.0;
     const uint32_t max_iter = (uint32_t)std::max({p->diffuse_depth, p->reflection_depth, p->transmission_depth}) * 2 + 8;
     uint32_t host_counts[16];
 
@@ -1648,10 +1636,8 @@ int render_items(fl_ctx_t ctx, const HostItem *hi, uint32_t n_items, const fl_re
     };
     const void *shade_fn = shade_kernel(has_std, guide != nullptr, shade_waves);
     const uint32_t shade_grid_max = resident(shade_fn, kBlock, (uint32_t)ctx->n_cu * 8);
-    const uint32_t infer_grid = resident(reinterpret_cast<const void *>(&k_guide_infer), 64 * kInferWaves, (uint32_t)ctx->n_cu);
-    auto launch_shade = [&](const RenderArgs &args, dim3 g, int j, bool with_infer = true) {
-        if (with_infer && args.guided)
-            hipLaunchKernelGGL(k_guide_infer, dim3(infer_grid), dim3(64 * kInferWaves), 0, st, args, j);
+    // guided renders evaluate the network inside the shading kernel
+    auto launch_shade = [&](const RenderArgs &args, dim3 g, int j) {
         RenderArgs ka = args;
         int kj = j;
         void *kargs[] = {&ka, &kj};
@@ -1770,25 +1756,17 @@ int render_items(fl_ctx_t ctx, const HostItem *hi, uint32_t n_items, const fl_re
             ++n_trace;
             if (timing) FL_HIP_CHECK(ctx, hipEventRecord(tev[1], st));
             if (ne > 0) {
-                // guide inference and shading, timed apart
-                if (a.guided)
-                    hipLaunchKernelGGL(k_guide_infer, dim3(infer_grid), dim3(64 * kInferWaves), 0, st, a, (int)iter);
-                if (timing) FL_HIP_CHECK(ctx, hipEventRecord(tev[3], st));
-                launch_shade(a, dim3(std::min<uint32_t>(shade_grid_max, blocks(ne))), (int)iter, false);
+                launch_shade(a, dim3(std::min<uint32_t>(shade_grid_max, blocks(ne))), (int)iter);
                 ++n_shade;
-            } else if (timing) {
-                FL_HIP_CHECK(ctx, hipEventRecord(tev[3], st));
             }
             if (timing) FL_HIP_CHECK(ctx, hipEventRecord(tev[2], st));
             FL_HIP_CHECK(ctx, hipMemcpyAsync(host_counts, a.counts, 64, hipMemcpyDeviceToHost, st));
             FL_HIP_CHECK(ctx, hipStreamSynchronize(st));
             if (timing) {
-                float m0 = 0.f, m1 = 0.f, m2 = 0.f;
+                float m0 = 0.f, m1 = 0.f;
                 (void)hipEventElapsedTime(&m0, tev[0], tev[1]);
-                (void)hipEventElapsedTime(&m2, tev[1], tev[3]);
-                (void)hipEventElapsedTime(&m1, tev[3], tev[2]);
+                (void)hipEventElapsedTime(&m1, tev[1], tev[2]);
                 ms_trace += m0;
-                ms_infer += m2;
                 ms_shade += m1;
             }
             if (ne > 0) {
@@ -1856,7 +1834,7 @@ int render_items(fl_ctx_t ctx, const HostItem *hi, uint32_t n_items, const fl_re
         stats->ms_extend = ms_trace;   // combined traversal kernel (extension + shadow rays)
         stats->ms_shade = ms_shade;
         stats->ms_guide = ms_guide;
-        stats->ms_infer = ms_infer;
+        stats->ms_infer = 0.0;   // the network runs inside the shading kernel (ms_shade)
         stats->guide_queries = h[6];
         stats->launches_extend = n_trace;
         stats->launches_shade = n_shade;

@@ -6,12 +6,13 @@
 
 namespace fl {
 
-// The guide's decoded lobes of one extension-queue slot in fp16: 8 x (mu
-// xyz, kappa, weight), 80 bytes, written by the inference kernel and read by
-// shading (fl_render.hip: lobe_at, guide_pdf_at, guide_sample_at).
-struct alignas(16) PackedLobes {
-    _Float16 h[kGOut];
-};
+// The guide's decoded lobes of a vertex, as the guided shading kernel keeps
+// them in LDS for the rest of that vertex's shading: per lobe the mean
+// direction (3), the concentration, the cumulative mixture weight (sampling)
+// and the weight times the vMF normalisation (density), one value per lane
+// and row (lane-contiguous rows: no bank conflicts).
+constexpr int kLobeVals = 6;
+constexpr int kLobeRows = kGLobes * kLobeVals;   // 48 rows of 64 lanes
 
 // path flags (high bits of the packed misc word; bounce in the low 8 bits)
 enum : uint32_t {
@@ -126,9 +127,9 @@ struct RenderArgs {
     // learned proposal (guided passes)
     uint32_t guided;         // 1: mix the guide's vMF lobes with BSDF sampling
     float guide_mix;         // probability of sampling the guide
+    uint32_t guide_bounces;  // vertices of bounce < guide_bounces are guided (all: ~0u)
     const __bf16 *gW;        // bf16 weights [5][64][64]
     const float *gP;         // fp32 parameters (biases)
-    PackedLobes *lobes;      // [cap]: decoded lobes per extension-queue slot
     float box_lo[3], box_inv[3];   // scene box -> unit cube
     // training records (training passes): 16 floats each
     float *recs;

@@ -12,7 +12,7 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libfl_gpu.so")
 # the struct layouts below are those of this version of include/fl_gpu.h
-ABI_VERSION = 5
+ABI_VERSION = 6
 
 FL_OK = 0
 FL_E_INVALID, FL_E_HIP, FL_E_OOM, FL_E_UNSUPPORTED, FL_E_OVERFLOW, FL_E_NUMERIC = -1, -2, -3, -4, -5, -6
@@ -106,7 +106,8 @@ class RenderParams(C.Structure):
                 ("trav_budget", C.c_uint32), ("shade_waves", C.c_uint32), ("trav_waves", C.c_uint32),
                 ("guide", C.c_void_p),
                 ("guide_mix", C.c_float),
-                ("guide_train_fraction", C.c_float), ("refill_lanes", C.c_uint32), ("trav_prefetch", C.c_uint32)]
+                ("guide_train_fraction", C.c_float), ("refill_lanes", C.c_uint32), ("trav_prefetch", C.c_uint32),
+                ("guide_bounces", C.c_uint32)]
 
 
 class Film(C.Structure):
@@ -160,7 +161,7 @@ class GuideConfig(C.Structure):
     _fields_ = [("hidden", C.c_uint32), ("layers", C.c_uint32), ("lobes", C.c_uint32), ("bands", C.c_uint32),
                 ("lr", C.c_float), ("beta1", C.c_float), ("beta2", C.c_float), ("eps", C.c_float),
                 ("batch", C.c_uint32), ("seed", C.c_uint64), ("scene_min", f3), ("scene_max", f3),
-                ("steps_per_pass", C.c_uint32)]
+                ("steps_per_pass", C.c_uint32), ("train_rounds", C.c_uint32)]
 
 
 class DevelopParams(C.Structure):
@@ -476,9 +477,10 @@ class Context:
         _check(self.L.fl_render_batch(self.h, items, n, C.byref(params), C.byref(st)), self.h, "fl_render_batch")
         return [(s, q) for s, q, _ in keep], st
 
-    def guide(self, seed=1, lr=0.0, batch=0, scene_min=(0, 0, 0), scene_max=(1, 1, 1), steps_per_pass=0):
+    def guide(self, seed=1, lr=0.0, batch=0, scene_min=(0, 0, 0), scene_max=(1, 1, 1), steps_per_pass=0,
+              train_rounds=0):
         return Guide(self, seed=seed, lr=lr, batch=batch, scene_min=scene_min, scene_max=scene_max,
-                     steps_per_pass=steps_per_pass)
+                     steps_per_pass=steps_per_pass, train_rounds=train_rounds)
 
     def probe_copy(self, nbytes=1 << 30, iters=5):
         """Device-to-device streaming copy; returns (read + write) GB/s."""
@@ -708,10 +710,12 @@ ITEM_SEED_STRIDE = 0x9E3779B97F4A7C15   # item k of a batch renders with seed + 
 class Guide:
     """The learned vMF-mixture proposal (fl_guide_*)."""
 
-    def __init__(self, ctx, seed=1, lr=0.0, batch=0, scene_min=(0, 0, 0), scene_max=(1, 1, 1), steps_per_pass=0):
+    def __init__(self, ctx, seed=1, lr=0.0, batch=0, scene_min=(0, 0, 0), scene_max=(1, 1, 1), steps_per_pass=0,
+                 train_rounds=0):
         self.ctx = ctx
         cfg = GuideConfig()
         cfg.seed, cfg.lr, cfg.batch, cfg.steps_per_pass = seed, lr, batch, steps_per_pass
+        cfg.train_rounds = train_rounds
         cfg.scene_min = f3(*scene_min)
         cfg.scene_max = f3(*scene_max)
         h = C.c_void_p()

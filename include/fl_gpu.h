@@ -36,7 +36,7 @@
 extern "C" {
 #endif
 
-#define FL_ABI_VERSION 5
+#define FL_ABI_VERSION 6
 
 /* ---- status codes ------------------------------------------------------ */
 enum {
@@ -262,6 +262,8 @@ typedef struct {
                                     per thread; 0 = auto, per pass: scenes below 65,536 triangles
                                     3 up to 16 M paths per pass, else 1; larger scenes 1 from 4 M
                                     paths per pass on, else 2 */
+    uint32_t guide_bounces;      /* guided renders: the guide serves vertices of bounce < guide_bounces
+                                    (deeper vertices sample the BSDF alone); 0 = every vertex */
 } fl_render_params;
 
 /* Film: per-pixel sums of linear CIE XYZ (3 floats per pixel) and of its
@@ -281,7 +283,8 @@ typedef struct {
     uint64_t nan_samples, stack_spills;
     uint32_t passes, max_bounce;
     uint32_t paths_per_pass, stack_depth;
-    double ms_infer;                /* with FL_RENDER_TIMING: guide inference kernels */
+    double ms_infer;                /* 0: guided renders evaluate the network inside the shading
+                                       kernel, whose time is ms_shade */
     uint64_t guide_queries;         /* network evaluations (guided hits) */
 } fl_stats;
 
@@ -379,7 +382,8 @@ typedef struct {
     uint32_t batch;         /* training batch (samples per step) */
     uint64_t seed;
     float scene_min[3], scene_max[3]; /* position normalisation box (fl_render uses the scene's bounds) */
-    uint32_t steps_per_pass;  /* Adam steps after each of the 4 training passes of a guided render (128) */
+    uint32_t steps_per_pass;  /* Adam steps after each training pass of a guided render (128) */
+    uint32_t train_rounds;    /* training passes of doubling size over the training samples (0: 4) */
 } fl_guide_config;
 
 int fl_guide_create(fl_ctx_t ctx, const fl_guide_config *cfg, fl_guide_t *out);

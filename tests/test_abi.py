@@ -39,7 +39,7 @@ def test_embree_subset_exported():
 
 def test_abi_version_and_no_device_behaviour():
     L = flgpu.lib()
-    assert L.fl_abi_version() == 5 == flgpu.ABI_VERSION
+    assert L.fl_abi_version() == 6 == flgpu.ABI_VERSION
     if L.fl_device_count() == 0:
         h = flgpu.C.c_void_p()
         assert L.fl_ctx_create(0, flgpu.C.byref(h)) == flgpu.FL_E_UNSUPPORTED
@@ -55,7 +55,7 @@ def test_struct_sizes_match_header():
     assert C.sizeof(flgpu.BsdfResult) == 56
     assert C.sizeof(flgpu.Camera) == 48
     assert C.sizeof(flgpu.Stats) == 176
-    assert C.sizeof(flgpu.RenderParams) == 88
+    assert C.sizeof(flgpu.RenderParams) == 96
 
 
 @pytest.mark.skipif(shutil.which("gcc") is None, reason="needs a C compiler")
