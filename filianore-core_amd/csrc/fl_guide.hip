@@ -548,7 +548,7 @@ int fl_guide_create(fl_ctx_t ctx, const fl_guide_config *cfg, fl_guide_t *out) {
     if (!(c.eps > 0.f)) c.eps = 1e-8f;
     if (!c.batch) c.batch = 16384;
     if (!c.steps_per_pass) c.steps_per_pass = 128;
-    if (!c.train_rounds) c.train_rounds = 4;
+    if (!c.train_rounds) c.train_rounds = 3;
     if (c.train_rounds > 16) return set_error(ctx, FL_E_INVALID, "fl_guide_create: train_rounds above 16");
     fl_guide_s *g = new (std::nothrow) fl_guide_s();
     if (!g) return set_error(ctx, FL_E_OOM, "fl_guide_create");

@@ -272,6 +272,55 @@ __device__ __forceinline__ void layer_fwd_half(const __bf16 *W, int ws, const fl
     }
 }
 
+// One hidden layer for a wave's 64 samples, in place in the tile H
+// ([64][kGRow] bf16): H[q][o] = relu(sum_i W[o][i] H[q][i] + b[o]), with the
+// weights read once. All of H's fragments are first loaded into registers
+// (32 values per lane), so the tile may be overwritten; the 64 output units
+// are then computed in two halves of 32 (32 accumulators live), each reading
+// only its own weight rows.
+__device__ __forceinline__ void layer_fwd_inreg(const __bf16 *W, int ws, const float *b, __bf16 *H) {
+    const int l = threadIdx.x & 63;
+    const int r16 = l & 15, kq = (l >> 4) * 8;
+    bf16x8 hq[2][4];
+#pragma unroll
+    for (int ks = 0; ks < 2; ++ks)
+#pragma unroll
+        for (int nt = 0; nt < 4; ++nt)
+            hq[ks][nt] = *reinterpret_cast<const bf16x8 *>(H + (nt * 16 + r16) * kGRow + ks * 32 + kq);
+#pragma unroll
+    for (int mh = 0; mh < 2; ++mh) {
+        f32x4 acc[2][4];
+        acc_zero_t(acc);
+#pragma unroll
+        for (int ks = 0; ks < 2; ++ks) {
+            bf16x8 wa[2];
+#pragma unroll
+            for (int t = 0; t < 2; ++t)
+                wa[t] = *reinterpret_cast<const bf16x8 *>(W + ((2 * mh + t) * 16 + r16) * ws + ks * 32 + kq);
+#pragma unroll
+            for (int t = 0; t < 2; ++t)
+#pragma unroll
+                for (int nt = 0; nt < 4; ++nt)
+                    acc[t][nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wa[t], hq[ks][nt], acc[t][nt], 0, 0, 0);
+        }
+#pragma unroll
+        for (int t = 0; t < 2; ++t) {
+            const int o0 = acc_m(2 * mh + t, 0);
+            const float b0 = b[o0], b1 = b[o0 + 1], b2 = b[o0 + 2], b3 = b[o0 + 3];
+#pragma unroll
+            for (int nt = 0; nt < 4; ++nt) {
+                typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
+                bf16x4 v;
+                v[0] = to_bf16(fmaxf(acc[t][nt][0] + b0, 0.f));
+                v[1] = to_bf16(fmaxf(acc[t][nt][1] + b1, 0.f));
+                v[2] = to_bf16(fmaxf(acc[t][nt][2] + b2, 0.f));
+                v[3] = to_bf16(fmaxf(acc[t][nt][3] + b3, 0.f));
+                *reinterpret_cast<bf16x4 *>(H + acc_n(nt) * kGRow + o0) = v;
+            }
+        }
+    }
+}
+
 // Head: raw[q][o] = sum_i W[o][i] H[q][i] + b[o] in fp32 for o < nout, raw
 // row stride rs.
 __device__ __forceinline__ void head_fwd(const __bf16 *W, const float *b, const __bf16 *H, float *raw, int rs,

@@ -44,8 +44,9 @@ constexpr int C_FETCH = 4;  // [4],[5]: trace refill cursors by bounce parity
 // atomic per block and returns the calling wave's first slot. Every thread of
 // the block must call it; `s` (kBlock/64 + 1 words of LDS) may be reused by
 // the next call only after another barrier of the block.
+template <int BS = kBlock>
 __device__ __forceinline__ uint32_t block_reserve(uint32_t wave_count, uint32_t *counter, uint32_t *s) {
-    constexpr uint32_t kWaves = kBlock / 64;
+    constexpr uint32_t kWaves = BS / 64;
     const uint32_t w = threadIdx.x >> 6;
     if (__lane_id() == 0) s[w] = wave_count;
     __syncthreads();
@@ -336,6 +337,13 @@ const TraceVariant kTraceVariants[] = {
 };
 
 // ---------------------------------------------------------------- guide ----
+// Hardware exp, reciprocal and reciprocal square root (about 1 ulp) in the
+// lobes' decode and density: the estimator needs the density to be that of
+// the sampling, which holds to fp32 rounding whichever rounding the
+// transcendental functions have.
+__device__ __forceinline__ float hw_rcp(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ __forceinline__ float hw_rsq(float x) { return __builtin_amdgcn_rsqf(x); }
+
 // The lobes of a guided vertex live in its wave's LDS region for the rest of
 // that vertex's shading (kLobeRows rows of 64 lanes; `lb` points at the
 // lane's column). Row 6 l + j of lobe l: j = 0..2 mean direction, 3 kappa,
@@ -351,7 +359,7 @@ __device__ __forceinline__ float guide_pdf_at(const float *lb, v3 w) {
 #pragma unroll 1
     for (int l = 0; l < kGLobes; ++l) {
         const v3 mu(lobe_val(lb, l, 0), lobe_val(lb, l, 1), lobe_val(lb, l, 2));
-        p += lobe_val(lb, l, 5) * expf(lobe_val(lb, l, 3) * (dot(mu, w) - 1.f));
+        p += lobe_val(lb, l, 5) * __expf(lobe_val(lb, l, 3) * (dot(mu, w) - 1.f));
     }
     return p;
 }
@@ -398,70 +406,105 @@ __device__ __forceinline__ void wave_sync() {
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
-__device__ __forceinline__ void guide_wave_mlp(const RenderArgs &a, float *reg, bool want, const float pos01[3], v3 n_in,
-                                               float lam01, float rough) {
+// W: the five bf16 weight matrices at row stride ws (LDS or global), matrix
+// k at W + k wm; B: the five bias vectors, k at B + k bm.
+__device__ __forceinline__ void guide_wave_mlp(float *reg, bool want, const float pos01[3], v3 n_in, float lam01,
+                                               float rough, const __bf16 *W, int ws, int wm, const float *B, int bm) {
     const int q = threadIdx.x & 63;
     __bf16 *sT = reinterpret_cast<__bf16 *>(reg);
     // The weights and biases are the same for every vertex: without this
-    // empty asm, which the compiler must assume changes the two pointers, it
-    // hoists their loads (some 150 values per lane) out of the shading loop
-    // and keeps them live across all of it.
-    const __bf16 *gW = a.gW;
-    const float *gP = a.gP;
-    asm volatile("" : "+s"(gW), "+s"(gP));
+    // empty asm, which the compiler must assume changes the offset, it hoists
+    // their loads (some 150 values per lane) out of the shading loop and keeps
+    // them live across all of it.
+    int off = 0;
+    asm volatile("" : "+s"(off));
+    const __bf16 *gW = W + off;
+    const float *gP = B + off;
     encode_row_bf16(sT, q, want, pos01, n_in, lam01, rough);
     wave_sync();
 #pragma unroll 1
     for (int k = 0; k < 4; ++k) {
-        // in two halves of 32 samples: half the accumulators live at a time
-        layer_fwd_half(gW + k * 4096, kGH, gP + k * kGLayerParams + 4096, sT, 0);
-        layer_fwd_half(gW + k * 4096, kGH, gP + k * kGLayerParams + 4096, sT, 1);
+        layer_fwd_inreg(gW + k * wm, ws, gP + k * bm, sT);
         wave_sync();
     }
     {   // head: rows 0..47 of the last matrix (40 used), into fp32 rows that
-        // overlap the tile, so both halves' products come before any store
-        f32x4 acc0[3][2], acc1[3][2];
-        acc_zero_t(acc0);
-        acc_zero_t(acc1);
-        mm_tiles<3, 2>(gW + 4 * 4096, kGH, sT, kGRow, acc0);
-        mm_tiles<3, 2>(gW + 4 * 4096, kGH, sT + 32 * kGRow, kGRow, acc1);
-        const float *bh = gP + 4 * kGLayerParams + 4096;
-        wave_sync();   // every fragment of the tile is read before the rows are overwritten
+        // overlap the tile: the tile's fragments are all in registers first,
+        // then each 16-row tile of outputs is computed and stored
+        const int l = threadIdx.x & 63, r16 = l & 15, kq = (l >> 4) * 8;
+        bf16x8 hq[2][4];
 #pragma unroll
-        for (int mt = 0; mt < 3; ++mt)
+        for (int ks = 0; ks < 2; ++ks)
 #pragma unroll
-            for (int nt = 0; nt < 2; ++nt)
+            for (int nt = 0; nt < 4; ++nt)
+                hq[ks][nt] = *reinterpret_cast<const bf16x8 *>(sT + (nt * 16 + r16) * kGRow + ks * 32 + kq);
+        const __bf16 *W4 = gW + 4 * wm;
+        const float *bh = gP + 4 * bm;
+#pragma unroll
+        for (int mt = 0; mt < 3; ++mt) {
+            f32x4 acc[4];
+#pragma unroll
+            for (int nt = 0; nt < 4; ++nt) acc[nt] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+            for (int ks = 0; ks < 2; ++ks) {
+                const bf16x8 wa = *reinterpret_cast<const bf16x8 *>(W4 + (mt * 16 + r16) * ws + ks * 32 + kq);
+#pragma unroll
+                for (int nt = 0; nt < 4; ++nt)
+                    acc[nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wa, hq[ks][nt], acc[nt], 0, 0, 0);
+            }
+#pragma unroll
+            for (int nt = 0; nt < 4; ++nt)
 #pragma unroll
                 for (int r = 0; r < 4; ++r) {
                     const int o = acc_m(mt, r);
-                    if (o < kGOut) {
-                        reg[acc_n(nt) * kGuideRaw + o] = acc0[mt][nt][r] + bh[o];
-                        reg[(32 + acc_n(nt)) * kGuideRaw + o] = acc1[mt][nt][r] + bh[o];
-                    }
+                    if (o < kGOut) reg[acc_n(nt) * kGuideRaw + o] = acc[nt][r] + bh[o];
                 }
+        }
     }
     wave_sync();
-    VmfLobe L[kGLobes];
-    if (want) guide_decode(reg + q * kGuideRaw, L);
-    wave_sync();   // every row is read before the lobe columns overwrite them
+    // decode (guide_decode's mapping): mu = m / |m|, kappa = exp(clamp(raw)),
+    // weights the softmax of the logits; all of the lane's row is read before
+    // any lane writes its lobe column over the rows
+    float rv[kGOut];
+    if (want)
+#pragma unroll
+        for (int o = 0; o < kGOut; ++o) rv[o] = reg[q * kGuideRaw + o];
+    wave_sync();
     if (want) {
+        float mx = -3.0e38f;
+#pragma unroll
+        for (int l = 0; l < kGLobes; ++l) mx = fmaxf(mx, rv[5 * l + 4]);
+        float sum = 0.f;
+#pragma unroll
+        for (int l = 0; l < kGLobes; ++l) {
+            rv[5 * l + 4] = __expf(rv[5 * l + 4] - mx);
+            sum += rv[5 * l + 4];
+        }
+        const float inv_sum = hw_rcp(sum);
         float cum = 0.f;
 #pragma unroll
         for (int l = 0; l < kGLobes; ++l) {
-            cum += L[l].w;
+            const float *r = rv + 5 * l;
+            const float len2 = r[0] * r[0] + r[1] * r[1] + r[2] * r[2];
+            const bool ok = len2 > 1e-12f;
+            const float il = ok ? hw_rsq(len2) : 0.f;
+            const float kappa = __expf(fminf(fmaxf(r[3], kGKappaLo), kGKappaHi));
+            const float w = r[4] * inv_sum;
+            cum += w;
+            // vMF normalisation kappa / (2 pi (1 - exp(-2 kappa))), kappa >= exp(-3)
+            const float norm = kappa * hw_rcp(2.f * kPi * (1.f - __expf(-2.f * kappa)));
             float *c = reg + kLobeVals * l * 64 + q;
-            c[0] = L[l].mu.x;
-            c[64] = L[l].mu.y;
-            c[128] = L[l].mu.z;
-            c[192] = L[l].kappa;
+            c[0] = ok ? r[0] * il : 0.f;
+            c[64] = ok ? r[1] * il : 0.f;
+            c[128] = ok ? r[2] * il : 1.f;
+            c[192] = kappa;
             c[256] = cum;
-            c[320] = L[l].w * vmf_norm(L[l].kappa);
+            c[320] = w * norm;
         }
     }
 }
 
 // Online training runs in R passes of doubling size (R = the guide's
-// train_rounds, 4 by default) over the first train_spp samples per pixel;
+// train_rounds, 3 by default) over the first train_spp samples per pixel;
 // pass r ends after train_spp (2^(r+1) - 1) / (2^R - 1) samples
 // (r = R - 1 ends at train_spp).
 inline uint32_t train_round_end(uint32_t train_spp, uint32_t rounds, int r) {
@@ -678,27 +721,49 @@ __device__ __forceinline__ uint32_t shade_key(const RenderArgs &a, int cur, uint
 // STD: the render has standard-surface materials or image textures. Their
 // code is only compiled into that variant, so renders without them keep the
 // smaller kernel (registers and instruction cache).
+// Threads per block of a shading instantiation: guided renders without
+// standard surfaces run 8-wave blocks, one per CU, so that one LDS copy of the
+// network's weights serves 8 waves (145 KB of LDS with their regions).
+constexpr int kGuideBlock = 512;
+constexpr int shade_block(bool std_, bool guided) { return guided && !std_ ? kGuideBlock : kBlock; }
+
 template <int WAVES, bool STD, bool GUIDED>
-__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WAVES, 8))) k_shade(RenderArgs a, int b) {
+__global__ void __launch_bounds__(shade_block(STD, GUIDED)) __attribute__((amdgpu_waves_per_eu(WAVES, 8)))
+k_shade(RenderArgs a, int b) {
+    constexpr int BS = shade_block(STD, GUIDED);
+    constexpr bool LDS_W = GUIDED && !STD;   // the network's weights in LDS
     __shared__ uint32_t s_cnt[kSortKeys];
-    __shared__ uint16_t s_perm[kBlock];
+    __shared__ uint16_t s_perm[BS];
     // per-block queue reservations: [0] shadow rays, [1] continuations,
     // [2] training records
-    __shared__ uint32_t s_app[3][kBlock / 64 + 1];
+    __shared__ uint32_t s_app[3][BS / 64 + 1];
     // a textured standard material's kd, ks, kt, sheen and coat at the vertex's
     // four wavelengths, one column per thread (20 KB: registers would hold
     // them across the whole shading of every vertex)
-    __shared__ float s_tv[STD ? SS_ETA * 4 * kBlock : 1];
+    __shared__ float s_tv[STD ? SS_ETA * 4 * BS : 1];
     // guided renders: each wave's region for the network and its lobes (12 KB)
-    __shared__ __attribute__((aligned(16))) float s_guide[GUIDED ? (kBlock / 64) * kGuideRegion : 1];
+    __shared__ __attribute__((aligned(16))) float s_guide[GUIDED ? (BS / 64) * kGuideRegion : 1];
+    // and, without standard surfaces, the bf16 weights (rows padded to kGRow
+    // against bank conflicts) and the biases, loaded once per block
+    __shared__ __attribute__((aligned(16))) __bf16 s_gw[LDS_W ? kGMats * 64 * kGRow : 8];
+    __shared__ float s_gb[LDS_W ? kGMats * 64 : 1];
     unsigned long long n_queries = 0;   // guided vertices of this wave (counted by lane 0)
-    const uint32_t stride = gridDim.x * kBlock;
+    if (LDS_W && a.guided) {
+        for (int e = threadIdx.x; e < kGMats * 64 * 8; e += BS) {   // 16-byte pieces of W[k][o][0..63]
+            const int k = e / 512, o = (e / 8) % 64, c = e % 8;
+            *reinterpret_cast<bf16x8 *>(&s_gw[(k * 64 + o) * kGRow + 8 * c]) =
+                *reinterpret_cast<const bf16x8 *>(a.gW + k * 4096 + o * 64 + 8 * c);
+        }
+        for (int e = threadIdx.x; e < kGMats * 64; e += BS) s_gb[e] = a.gP[(e / 64) * kGLayerParams + 4096 + e % 64];
+        __syncthreads();
+    }
+    const uint32_t stride = gridDim.x * BS;
     const int cur = b & 1, nxt = cur ^ 1;
     const uint32_t n = a.counts[C_EXT + cur];
-    const uint32_t nround = (n + kBlock - 1) / kBlock * kBlock;
+    const uint32_t nround = (n + BS - 1) / BS * BS;
     const int GS = a.grid.stride;
     const bool sort = !(a.flags & FL_RENDER_NO_SORT);
-    for (uint32_t base = blockIdx.x * kBlock; base < nround; base += stride) {
+    for (uint32_t base = blockIdx.x * BS; base < nround; base += stride) {
         uint32_t i = base + threadIdx.x;
         if (sort) {
             const uint32_t key = shade_key(a, cur, i, n);
@@ -768,7 +833,11 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WAV
                     v3 n_in(0.f, 0.f, 0.f);
                     if (want) guide_features(a, p, ng, d, lam0, S.mats[mkind], pos01, n_in, lam01, rough);
                     float *reg = s_guide + (threadIdx.x >> 6) * kGuideRegion;
-                    guide_wave_mlp(a, reg, want, pos01, n_in, lam01, rough);
+                    if constexpr (LDS_W)
+                        guide_wave_mlp(reg, want, pos01, n_in, lam01, rough, s_gw, kGRow, 64 * kGRow, s_gb, 64);
+                    else
+                        guide_wave_mlp(reg, want, pos01, n_in, lam01, rough, a.gW, kGH, 4096, a.gP + 4096,
+                                       kGLayerParams);
                     if (want) lb = reg + (threadIdx.x & 63);
                 }
             }
@@ -858,7 +927,7 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WAV
                             float rgb[3];
                             texel_rgb(*S.tex, sd->tex[w] - 1u, uv, rgb);
                             for (int k = 0; k < 4; ++k)
-                                s_tv[(w * 4 + k) * kBlock + threadIdx.x] =
+                                s_tv[(w * 4 + k) * BS + threadIdx.x] =
                                     rgb_spectrum_lane(basis, GS, rgb[0], rgb[1], rgb[2], ln, k);
                         }
             }
@@ -907,7 +976,7 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WAV
             const unsigned long long m0 = __ballot(okv[0]), m1 = __ballot(okv[1]);
             const unsigned long long lt = (1ull << __lane_id()) - 1ull;
             const uint32_t c0 = (uint32_t)__popcll(m0);
-            const uint32_t base = block_reserve(c0 + (uint32_t)__popcll(m1), &a.counts[C_SH + cur], s_app[0]);
+            const uint32_t base = block_reserve<BS>(c0 + (uint32_t)__popcll(m1), &a.counts[C_SH + cur], s_app[0]);
 #pragma unroll
             for (int j = 0; j < 2; ++j) {
                 if (okv[j]) {
@@ -1053,7 +1122,7 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WAV
         // -------- training record of this vertex (training passes only)
         if (GUIDED && a.recs) {
             const unsigned long long mr = __ballot(rec_want);
-            const uint32_t ridx = block_reserve((uint32_t)__popcll(mr), a.rec_count, s_app[2]) +
+            const uint32_t ridx = block_reserve<BS>((uint32_t)__popcll(mr), a.rec_count, s_app[2]) +
                                   (uint32_t)__popcll(mr & ((1ull << __lane_id()) - 1ull));
             if (rec_want && ridx < a.rec_cap) {
                 float *R = a.recs + (size_t)ridx * 16;
@@ -1073,7 +1142,7 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WAV
         }
         // -------- compaction of the continuation: one atomic per block
         const unsigned long long mc = __ballot(cont);
-        const uint32_t slot = block_reserve((uint32_t)__popcll(mc), &a.counts[C_EXT + nxt], s_app[1]) +
+        const uint32_t slot = block_reserve<BS>((uint32_t)__popcll(mc), &a.counts[C_EXT + nxt], s_app[1]) +
                               (uint32_t)__popcll(mc & ((1ull << __lane_id()) - 1ull));
         if (cont) {
             a.rq_o[nxt][slot] = make_float4(no.x, no.y, no.z, kShadowEps);
@@ -1635,13 +1704,14 @@ int render_items(fl_ctx_t ctx, const HostItem *hi, uint32_t n_items, const fl_re
         return (uint32_t)ctx->n_cu * (uint32_t)occ;
     };
     const void *shade_fn = shade_kernel(has_std, guide != nullptr, shade_waves);
-    const uint32_t shade_grid_max = resident(shade_fn, kBlock, (uint32_t)ctx->n_cu * 8);
+    const int shade_bs = shade_block(has_std, guide != nullptr);
+    const uint32_t shade_grid_max = resident(shade_fn, shade_bs, (uint32_t)ctx->n_cu * (uint32_t)(8 * kBlock / shade_bs));
     // guided renders evaluate the network inside the shading kernel
     auto launch_shade = [&](const RenderArgs &args, dim3 g, int j) {
         RenderArgs ka = args;
         int kj = j;
         void *kargs[] = {&ka, &kj};
-        (void)hipLaunchKernel(shade_fn, g, dim3(kBlock), kargs, 0, st);
+        (void)hipLaunchKernel(shade_fn, g, dim3(shade_bs), kargs, 0, st);
     };
     auto launch_bounce = [&](const RenderArgs &args, uint32_t j) {
         tv->launch(counters, dim3(grid_mode[args.trav_mode == kTravPrefetch ? 1 : 0]), st, args, (int)j);
@@ -1756,7 +1826,7 @@ int render_items(fl_ctx_t ctx, const HostItem *hi, uint32_t n_items, const fl_re
             ++n_trace;
             if (timing) FL_HIP_CHECK(ctx, hipEventRecord(tev[1], st));
             if (ne > 0) {
-                launch_shade(a, dim3(std::min<uint32_t>(shade_grid_max, blocks(ne))), (int)iter);
+                launch_shade(a, dim3(std::min<uint32_t>(shade_grid_max, (ne + shade_bs - 1) / shade_bs)), (int)iter);
                 ++n_shade;
             }
             if (timing) FL_HIP_CHECK(ctx, hipEventRecord(tev[2], st));

@@ -383,7 +383,7 @@ typedef struct {
     uint64_t seed;
     float scene_min[3], scene_max[3]; /* position normalisation box (fl_render uses the scene's bounds) */
     uint32_t steps_per_pass;  /* Adam steps after each training pass of a guided render (128) */
-    uint32_t train_rounds;    /* training passes of doubling size over the training samples (0: 4) */
+    uint32_t train_rounds;    /* training passes of doubling size over the training samples (0: 3) */
 } fl_guide_config;
 
 int fl_guide_create(fl_ctx_t ctx, const fl_guide_config *cfg, fl_guide_t *out);
