@@ -367,6 +367,32 @@ def terrain_section(flgpu, ctx, spp, res_div):
     return out
 
 
+def profile_guided(flgpu, spp, config):
+    """A short program for the profiler: on the guided room at 1024^2, one
+    plain render (no roulette) and two guided renders (fresh network, trained
+    online) of spp samples per pixel, with their times on stderr. No
+    reference image and no JSON line."""
+    gen = flgpu.GenSceneData(config, width=1024, height=1024)
+    base = flgpu.FL_RENDER_VNDF_PDF | flgpu.FL_RENDER_GRAPH
+    with flgpu.Context(0) as ctx, ctx.scene(gen) as sc:
+        p = flgpu.default_params(spp=spp, depth=gen.depth, seed=7, flags=base)
+        p.rr_start = NO_RR
+        sc.render(p, want_film=False)
+        ts = time.perf_counter()
+        sc.render(p, want_film=False)
+        log("plain %d spp: %.1f ms" % (spp, (time.perf_counter() - ts) * 1e3))
+        for rep in range(2):
+            with ctx.guide(seed=1) as g:
+                q = flgpu.default_params(spp=spp, depth=gen.depth, seed=7,
+                                         flags=base | flgpu.FL_RENDER_GUIDED | flgpu.FL_RENDER_TRAIN_GUIDE)
+                q.rr_start = NO_RR
+                q.guide = g.h
+                ts = time.perf_counter()
+                _, _, st = sc.render(q, want_film=False)
+                log("guided %d spp: %.1f ms (training %.1f ms, %d passes)" % (
+                    spp, (time.perf_counter() - ts) * 1e3, st.ms_guide, st.passes))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -388,6 +414,8 @@ def main():
     ap.add_argument("--terrain", action="store_true", help="also render the full terrain config (8 x 4K x 256 spp)")
     ap.add_argument("--terrain-spp", type=int, default=256)
     ap.add_argument("--terrain-res-div", type=int, default=1)
+    ap.add_argument("--profile-guided", type=int, default=0, metavar="SPP",
+                    help="only render the guided room plain and guided at SPP (a short program to profile)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run a plain 1 GiB -> 1 GiB copy kernel (HBM counter calibration)")
     args = ap.parse_args()
@@ -398,6 +426,9 @@ def main():
 
     import flgpu
 
+    if args.profile_guided:
+        profile_guided(flgpu, args.profile_guided, flgpu.FL_GEN_GUIDED)
+        return
     cfg = {"spheres": flgpu.FL_GEN_SPHERES, "cornell": flgpu.FL_GEN_CORNELL, "guided": flgpu.FL_GEN_GUIDED}[args.workload]
     t0 = time.time()
     gen = flgpu.GenSceneData(cfg)
