@@ -479,6 +479,14 @@ def main():
                 "avg_launch_ms": round(tst.ms_extend / launches, 4),
                 "nodes_per_ray": round(cst.nodes_visited / max(1, n_rays), 2),
                 "tris_per_ray": round(cst.tris_tested / max(1, n_rays), 2),
+                # by ray type: any-hit shadow rays and closest-hit extension rays
+                "shadow_rays": int(cst.rays_shadow), "extension_rays": int(cst.rays_extension),
+                "shadow_nodes_per_ray": round(cst.shadow_nodes_visited / max(1, cst.rays_shadow), 2),
+                "shadow_tris_per_ray": round(cst.shadow_tris_tested / max(1, cst.rays_shadow), 2),
+                "extension_nodes_per_ray": round((cst.nodes_visited - cst.shadow_nodes_visited) /
+                                                 max(1, cst.rays_extension), 2),
+                "extension_tris_per_ray": round((cst.tris_tested - cst.shadow_tris_tested) /
+                                                max(1, cst.rays_extension), 2),
                 "ms_trace": round(tst.ms_extend, 2), "ms_shade": round(tst.ms_shade, 2),
                 "launches": launches,
             }

@@ -247,6 +247,8 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WAV
         atomicAdd(&a.stats[3], (unsigned long long)c.nodes);
         atomicAdd(&a.stats[4], (unsigned long long)c.tris);
         atomicAdd(&a.stats[5], (unsigned long long)c.spills);
+        atomicAdd(&a.stats[7], (unsigned long long)c.any_nodes);
+        atomicAdd(&a.stats[8], (unsigned long long)c.any_tris);
     }
 }
 
@@ -1900,6 +1902,8 @@ int render_items(fl_ctx_t ctx, const HostItem *hi, uint32_t n_items, const fl_re
         stats->nan_samples = h[2];
         stats->nodes_visited = h[3];
         stats->tris_tested = h[4];
+        stats->shadow_nodes_visited = h[7];
+        stats->shadow_tris_tested = h[8];
         stats->stack_spills = h[5];
         stats->ms_extend = ms_trace;   // combined traversal kernel (extension + shadow rays)
         stats->ms_shade = ms_shade;

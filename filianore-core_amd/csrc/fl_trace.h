@@ -45,6 +45,7 @@ struct LaneStack {
 
 struct TravCounters {
     uint32_t nodes = 0, tris = 0, spills = 0;
+    uint32_t any_nodes = 0, any_tris = 0;   // the share of any-hit rays (shadow rays to area lights)
 };
 
 __device__ __forceinline__ float safe_inv(float d) {
@@ -117,7 +118,10 @@ __device__ __forceinline__ bool trav_step(const TraceScene &sc, TravState &s, co
             uint32_t ri = s.t_base + b;
             const uint4 *tp = reinterpret_cast<const uint4 *>(sc.tris + ri);
             uint4 a = tp[0], bq = tp[1], c = tp[2];
-            if (COUNT) ++cnt.tris;
+            if (COUNT) {
+                ++cnt.tris;
+                if (any) ++cnt.any_tris;
+            }
             if (!(skip_lights && c.w != 0u)) {
                 v3 p0(__uint_as_float(a.x), __uint_as_float(a.y), __uint_as_float(a.z));
                 v3 p1(__uint_as_float(bq.x), __uint_as_float(bq.y), __uint_as_float(bq.z));
@@ -149,7 +153,10 @@ __device__ __forceinline__ bool trav_step(const TraceScene &sc, TravState &s, co
             uint4 w0 = np[0], w1 = np[1];
             uint4 qn[3] = {np[2], np[3], np[4]};
             uint4 qf[3] = {np[5], np[6], np[7]};
-            if (COUNT) ++cnt.nodes;
+            if (COUNT) {
+                ++cnt.nodes;
+                if (any) ++cnt.any_nodes;
+            }
             float org[3] = {__uint_as_float(w0.x), __uint_as_float(w0.y), __uint_as_float(w0.z)};
             float step[3] = {__uint_as_float((w0.w & 0xffu) << 23), __uint_as_float(((w0.w >> 8) & 0xffu) << 23),
                              __uint_as_float(((w0.w >> 16) & 0xffu) << 23)};

@@ -277,8 +277,8 @@ typedef struct {
 typedef struct {
     double ms_total, ms_generate, ms_extend, ms_shade, ms_shadow, ms_resolve, ms_guide;
     uint64_t samples, rays_extension, rays_shadow;
-    uint64_t nodes_visited, tris_tested;   /* with FL_RENDER_COUNTERS: extension rays */
-    uint64_t shadow_nodes_visited, shadow_tris_tested;
+    uint64_t nodes_visited, tris_tested;   /* with FL_RENDER_COUNTERS: all rays traced */
+    uint64_t shadow_nodes_visited, shadow_tris_tested;   /* ... of which any-hit shadow rays */
     uint32_t launches_extend, launches_shade, launches_shadow, pad0;
     uint64_t nan_samples, stack_spills;
     uint32_t passes, max_bounce;
