@@ -337,6 +337,41 @@ def guided_section(flgpu, ctx, config, res, ref_spp, spp=64):
     return out
 
 
+def nee_one_section(flgpu, sc, gen, spp, ref_spp=2048):
+    """Direct lighting as the reference does it, one sample per light
+    (uniform_sample_all_lights, integrators/integrator.cpp:10-23), against one
+    uniformly chosen light per vertex (FL_RENDER_NEE_ONE), on the same scene
+    at equal spp: render time, relMSE (linear sRGB, against an all-lights
+    reference of ref_spp with its own noise subtracted) and relMSE x time.
+    The headline figure always uses the reference's estimator."""
+    import numpy as np
+    base = flgpu.FL_RENDER_VNDF_PDF | flgpu.FL_RENDER_GRAPH
+    halves = []
+    for k in range(2):
+        p = flgpu.default_params(spp=ref_spp // 2, depth=gen.depth, mode=flgpu.FL_SPECTRAL_HERO, seed=2000 + k,
+                                 flags=base)
+        s_, _, _ = sc.render(p)
+        halves.append(to_srgb(s_.astype(np.float64) / (ref_spp // 2)))
+    ref = 0.5 * (halves[0] + halves[1])
+    ref_var = 0.25 * (halves[0] - halves[1]) ** 2
+    den = ref * ref + 0.01
+    out = {"reference_spp": ref_spp, "spp": spp}
+    for name, flags in (("all_lights", base), ("one_light", base | flgpu.FL_RENDER_NEE_ONE)):
+        p = flgpu.default_params(spp=spp, depth=gen.depth, mode=flgpu.FL_SPECTRAL_HERO, seed=7, flags=flags)
+        img, _, st = sc.render(p)
+        e = float(np.mean(((to_srgb(img.astype(np.float64) / spp) - ref) ** 2 - ref_var) / den))
+        ts = []
+        for _ in range(3):
+            t0 = time.perf_counter()
+            sc.render(p, want_film=False)
+            ts.append(time.perf_counter() - t0)
+        t = sum(ts) / len(ts)
+        out[name] = {"s": round(t, 4), "msamples_s": round(img.shape[0] * img.shape[1] * spp / t / 1e6, 1),
+                     "shadow_rays": int(st.rays_shadow), "relmse": round(e, 6), "relmse_x_s": round(e * t, 7)}
+    out["relmse_x_s_ratio"] = round(out["one_light"]["relmse_x_s"] / out["all_lights"]["relmse_x_s"], 3)
+    return out
+
+
 def terrain_section(flgpu, ctx, spp, res_div):
     """The full terrain config: 4096 x 2048 fBm heightfield (16.8 M
     triangles) + 200 Cauchy prisms + 24 GGX metals, 8 cameras at
@@ -411,6 +446,8 @@ def main():
     ap.add_argument("--no-guided-skylight", action="store_true",
                     help="skip the guided comparison on the skylight reading of the room")
     ap.add_argument("--no-intersect", action="store_true", help="skip the 4M-ray intersection section")
+    ap.add_argument("--no-nee-one", action="store_true",
+                    help="skip the one-light-per-vertex direct lighting comparison")
     ap.add_argument("--terrain", action="store_true", help="also render the full terrain config (8 x 4K x 256 spp)")
     ap.add_argument("--terrain-spp", type=int, default=256)
     ap.add_argument("--terrain-res-div", type=int, default=1)
@@ -502,6 +539,10 @@ def main():
                 out["intersection"] = intersection_section(flgpu, ctx, gen, sc, 0.0 if args.no_cpu else
                                                            args.cpu_seconds / 2, host_threads())
                 log("intersection: %s" % out["intersection"])
+            if not args.no_nee_one:
+                log("direct lighting section: all lights vs one light per vertex")
+                out["nee_one"] = nee_one_section(flgpu, sc, gen, spp)
+                log("nee_one: %s" % out["nee_one"])
             out["mrays_per_s"] = round(rays * args.steps / wall / 1e6, 1)
             out["rays_per_sample"] = round(rays / samples, 3)
             out["bvh_build_ms"] = round(info.build_ms, 2)

@@ -1475,7 +1475,8 @@ int render_items(fl_ctx_t ctx, const HostItem *hi, uint32_t n_items, const fl_re
     a.mats = s->d_mats;
     a.lights = s->d_lights;
     a.n_lights = s->n_lights;
-    a.nee_all = s->n_lights <= 2 ? 1u : 0u;
+    const bool nee_one = (p->flags & FL_RENDER_NEE_ONE) != 0;
+    a.nee_all = s->n_lights <= 2 && !nee_one ? 1u : 0u;
     a.rgb_basis = s->rgb_basis;
     a.stds = s->d_std;
     a.tex = s->d_texscene;
@@ -1495,7 +1496,7 @@ int render_items(fl_ctx_t ctx, const HostItem *hi, uint32_t n_items, const fl_re
         it.lights = sk->d_lights;
         it.spec = sk->spec[mode].d_tab;
         it.n_lights = sk->n_lights;
-        it.nee_all = sk->n_lights <= 2 ? 1u : 0u;
+        it.nee_all = sk->n_lights <= 2 && !nee_one ? 1u : 0u;
         it.rgb_basis = sk->rgb_basis;
         it.stds = sk->d_std;
         it.tex = sk->d_texscene;

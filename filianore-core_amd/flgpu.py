@@ -22,6 +22,7 @@ FL_CLOSEST_HIT, FL_ANY_HIT, FL_SKIP_LIGHTS, FL_COUNTERS = 0, 1, 2, 4
 FL_SPECTRAL_HERO, FL_SPECTRAL_PARITY = 0, 1
 FL_RENDER_GUIDED, FL_RENDER_COUNTERS, FL_RENDER_VNDF_PDF, FL_RENDER_GRAPH = 1, 2, 4, 8
 FL_RENDER_TRAIN_GUIDE, FL_RENDER_NO_SORT, FL_RENDER_TIMING, FL_RENDER_NORMALS = 16, 32, 64, 128
+FL_RENDER_NEE_ONE = 256
 FL_LIGHT_POINT, FL_LIGHT_SPOT, FL_LIGHT_DIRECTIONAL = 1, 2, 3
 FL_COLOR_SRGB, FL_COLOR_ACES2065_1 = 0, 1
 FL_TONEMAP_NONE, FL_TONEMAP_ACES_BASIC, FL_TONEMAP_REINHARD, FL_TONEMAP_FILMIC = 1, 2, 3, 4

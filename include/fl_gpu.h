@@ -230,9 +230,16 @@ enum {
     FL_RENDER_TRAIN_GUIDE = 16,  /* train the guide online on this render's samples */
     FL_RENDER_NO_SORT = 32,      /* disable shading sort */
     FL_RENDER_TIMING = 64,       /* per-kernel HIP-event timing (ms_* fields of fl_stats) */
-    FL_RENDER_NORMALS = 128      /* NormalIntegrator (integrators/normalintegrator.cpp:11-26):
+    FL_RENDER_NORMALS = 128,     /* NormalIntegrator (integrators/normalintegrator.cpp:11-26):
                                     the first hit's normal through from_reflectance_rgb, black
                                     on a miss; no light transport */
+    FL_RENDER_NEE_ONE = 256      /* direct lighting from one uniformly chosen light per vertex
+                                    (its estimate divided by the selection probability) also in
+                                    scenes of 1 or 2 lights, which otherwise take one sample
+                                    per light as uniform_sample_all_lights does
+                                    (integrators/integrator.cpp:10-23); scenes of more lights
+                                    always sample one. Same expectation, half the shadow rays
+                                    of a two-triangle emitter */
 };
 
 typedef struct {

@@ -64,6 +64,31 @@ def test_spheres_parity_consistent(ctx):
     assert bad == 0 and frac <= allowed and abs(mean_z) < 3
 
 
+@pytest.mark.parametrize("config", [flgpu.FL_GEN_CORNELL, flgpu.FL_GEN_SPHERES])
+def test_one_light_nee_consistent_with_oracle(ctx, config):
+    """FL_RENDER_NEE_ONE: one uniformly chosen light per vertex instead of
+    uniform_sample_all_lights (integrators/integrator.cpp:10-23). The
+    expectation is the same, so the film passes the z-test against the
+    oracle, which samples every light; the two-triangle emitter of both
+    scenes gets half the shadow rays."""
+    kw = {"subdiv": 3, "width": 32, "height": 32} if config == flgpu.FL_GEN_SPHERES else {"width": 46, "height": 44}
+    gen = flgpu.GenSceneData(config, **kw)
+    spp = 1024
+    with ctx.scene(gen) as sc:
+        p = flgpu.default_params(spp=spp, depth=gen.depth, mode=flgpu.FL_SPECTRAL_PARITY, seed=31,
+                                 flags=flgpu.FL_RENDER_NEE_ONE)
+        gs, gq, st = sc.render(p)
+        _, _, st_all = sc.render(flgpu.default_params(spp=spp, depth=gen.depth, mode=flgpu.FL_SPECTRAL_PARITY,
+                                                      seed=31), want_film=False)
+    assert st.nan_samples == 0
+    print("shadow rays: one light %d, all lights %d" % (st.rays_shadow, st_all.rays_shadow))
+    assert st.rays_shadow < 0.6 * st_all.rays_shadow
+    os_, oq, _ = oracle.Scene(gen).render(spp, gen.depth, seed=32, threads=16)
+    frac, allowed, mean_z, bad = z_stats(gs, gq, os_, oq, spp)
+    print("one light: frac %.4f allowed %.4f mean z %.2f" % (frac, allowed, mean_z))
+    assert bad == 0 and frac <= allowed and abs(mean_z) < 3
+
+
 def test_render_deterministic(ctx):
     gen = flgpu.GenSceneData(flgpu.FL_GEN_CORNELL, width=64, height=64)
     with ctx.scene(gen) as sc:
