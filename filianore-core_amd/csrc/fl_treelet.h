@@ -15,9 +15,14 @@
 // area of the subtree's internal nodes (the SAH cost up to constant factors
 // and the constant primitive term).
 //
-// Memory access goes through a policy M (plain loads and stores on the host;
-// agent-scope relaxed atomics on the device, where other threads wrote the
-// subtree) so that the same code runs in a CPU check.
+// The work is split in three steps so that the lanes of a wave can share one
+// treelet's dynamic program: treelet_form (one thread: grow the treelet),
+// treelet_dp_size (the subsets of one size, spread over `nlanes` lanes; the
+// sizes in increasing order, with a barrier between them) and treelet_rebuild
+// (one thread: write the optimal topology). treelet_restructure runs the three
+// in one thread. Memory access goes through a policy M (plain loads and
+// stores on the host; agent-scope relaxed atomics on the device, where other
+// threads wrote the subtree), so that the same code runs in a CPU check.
 #pragma once
 #include <stdint.h>
 
@@ -46,101 +51,161 @@ FL_HD float treelet_cost(const float b[6], uint32_t c, float split) {
     return box_half_area(b) + split;
 }
 
-// Restructures the treelet rooted at internal node x. cost: 2^K floats;
-// part: 2^K bytes (the caller's scratch), entry S at [S * ST] (a stride lets
-// the threads of a block interleave their scratch in LDS).
-template <int K, int ST, class M>
-FL_HD void treelet_restructure(M &m, uint32_t x, float *cost, uint8_t *part) {
-    uint32_t L[K];       // treelet leaves: internal node ids or primitive ids | leaf bit
-    uint32_t I[K - 1];   // the treelet's internal nodes, I[0] = x
-    int nl = 2, ni = 1;
+// A treelet: its leaves (internal node ids, or primitive ids | leaf bit) with
+// their boxes, subtree costs and primitive counts, and its internal nodes.
+template <int K>
+struct TreeletLeaves {
+    uint32_t L[K];
+    uint32_t I[K - 1];   // I[0] = the treelet's root
+    float lb[K][6];
+    float lc[K];
+    uint32_t lcount[K];
+    int nl;
+};
+
+// The dynamic program's table: per leaf subset S, the least cost of a binary
+// tree over S and the subset P of S that forms its first child.
+template <int K>
+struct TreeletTable {
+    float cost[1u << K];
+    uint8_t part[1u << K];
+};
+
+// a[i] and a[i] = v for a run-time i, as selects over compile-time indices:
+// on the device the arrays then stay in registers (an indexed access would
+// move the whole array to scratch memory).
+template <int N, class T>
+FL_HD T treelet_get(const T (&a)[N], int i) {
+    T v = a[0];
+#pragma unroll
+    for (int k = 1; k < N; ++k)
+        if (k == i) v = a[k];
+    return v;
+}
+template <int N, class T>
+FL_HD void treelet_put(T (&a)[N], int i, T v) {
+#pragma unroll
+    for (int k = 0; k < N; ++k)
+        if (k == i) a[k] = v;
+}
+
+// Grows the treelet rooted at internal node x: starting from x's two
+// children, open the internal leaf of largest surface area until there are K
+// leaves or only primitives are left.
+template <int K, class M>
+FL_HD void treelet_form(M &m, uint32_t x, TreeletLeaves<K> &t) {
+    uint32_t L[K];
+    float la[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        L[k] = kTreeletLeafBit;
+        la[k] = -1.f;
+    }
     L[0] = m.left(x);
     L[1] = m.right(x);
-    I[0] = x;
-    float la[K];         // leaf areas
-    for (int k = 0; k < 2; ++k) la[k] = m.area_of(L[k]);
-    while (nl < K) {
+    t.I[0] = x;
+    la[0] = m.area_of(L[0]);
+    la[1] = m.area_of(L[1]);
+    int nl = 2;
+#pragma unroll
+    for (int step = 1; step < K - 1; ++step) {   // each opening adds one leaf and one internal node
         int best = -1;
         float best_a = -1.f;
-        for (int k = 0; k < nl; ++k)
-            if (!(L[k] & kTreeletLeafBit) && la[k] > best_a) {
+#pragma unroll
+        for (int k = 0; k < K; ++k)
+            if (k < nl && !(L[k] & kTreeletLeafBit) && la[k] > best_a) {
                 best_a = la[k];
                 best = k;
             }
         if (best < 0) break;   // every leaf is a primitive
-        const uint32_t c = L[best];
-        I[ni++] = c;
-        L[best] = m.left(c);
-        la[best] = m.area_of(L[best]);
-        L[nl] = m.right(c);
-        la[nl] = m.area_of(L[nl]);
+        const uint32_t c = treelet_get(L, best);
+        t.I[step] = c;
+        const uint32_t a = m.left(c), b = m.right(c);
+        treelet_put(L, best, a);
+        treelet_put(la, best, m.area_of(a));
+        treelet_put(L, nl, b);
+        treelet_put(la, nl, m.area_of(b));
         ++nl;
     }
-    // leaf boxes and subtree costs
-    float lb[K][6];
-    float lc[K];
-    uint32_t lcount[K];
-    for (int k = 0; k < nl; ++k) {
-        m.box_of(L[k], lb[k]);
-        lcount[k] = (L[k] & kTreeletLeafBit) ? 1u : m.count(L[k]);
-        lc[k] = (L[k] & kTreeletLeafBit) ? 0.f : m.cost(L[k]);
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        t.L[k] = L[k];
+        if (k < nl) {
+            m.box_of(L[k], t.lb[k]);
+            const bool prim = (L[k] & kTreeletLeafBit) != 0;
+            t.lcount[k] = prim ? 1u : m.count(L[k]);
+            t.lc[k] = prim ? 0.f : m.cost(L[k]);
+        }
     }
+    t.nl = nl;
+}
+
+// The optimal split of every subset of sz leaves that this lane owns
+// (subsets lane + 1, lane + 1 + nlanes, ...). Every subset of fewer leaves
+// must be done (by all lanes) before.
+template <int K, int KT>
+FL_HD void treelet_dp_size(const TreeletLeaves<K> &t, TreeletTable<KT> &tab, int sz, uint32_t lane, uint32_t nlanes) {
+    const int nl = t.nl;   // at most KT
     const uint32_t full = (1u << nl) - 1u;
-    // the optimal split of every subset of two or more leaves (subsets in
-    // increasing order: each proper subset comes first)
-    for (uint32_t S = 1; S <= full; ++S) {
-        if ((S & (S - 1u)) == 0u) {   // one leaf
-            int k = 0;
-            while (!(S & (1u << k))) ++k;
-            cost[S * ST] = lc[k];
-            part[S * ST] = 0;
+    for (uint32_t S = lane + 1u; S <= full; S += nlanes) {
+        if (__builtin_popcount(S) != sz) continue;
+        if (sz == 1) {
+            tab.cost[S] = treelet_get(t.lc, __builtin_ctz(S));
+            tab.part[S] = 0;
             continue;
         }
         float b[6] = {kInf, kInf, kInf, -kInf, -kInf, -kInf};
         uint32_t cnt = 0;
-        for (int k = 0; k < nl; ++k)
+#pragma unroll
+        for (int k = 0; k < KT; ++k)
             if (S & (1u << k)) {
                 for (int e = 0; e < 3; ++e) {
-                    b[e] = fminf(b[e], lb[k][e]);
-                    b[3 + e] = fmaxf(b[3 + e], lb[k][3 + e]);
+                    b[e] = fminf(b[e], t.lb[k][e]);
+                    b[3 + e] = fmaxf(b[3 + e], t.lb[k][3 + e]);
                 }
-                cnt += lcount[k];
+                cnt += t.lcount[k];
             }
-        const uint32_t low = S & (0u - S);
+        // the proper subsets P of S that hold S's lowest leaf (each split
+        // once), in decreasing order: P = low | Q, Q a proper subset of R
+        const uint32_t low = S & (0u - S), R = S ^ low;
         float best = kInf;
         uint32_t bp = low;
-        // proper non-empty subsets P of S holding S's lowest leaf (each split once)
-        for (uint32_t P = (S - 1u) & S; P != 0u; P = (P - 1u) & S) {
-            if (!(P & low)) continue;
-            const float c = cost[P * ST] + cost[(S ^ P) * ST];
+        for (uint32_t Q = (R - 1u) & R;; Q = (Q - 1u) & R) {
+            const uint32_t P = low | Q;
+            const float c = tab.cost[P] + tab.cost[S ^ P];
             if (c < best) {
                 best = c;
                 bp = P;
             }
+            if (Q == 0u) break;
         }
-        cost[S * ST] = treelet_cost(b, cnt, best);
-        part[S * ST] = (uint8_t)bp;
+        tab.cost[S] = treelet_cost(b, cnt, best);
+        tab.part[S] = (uint8_t)bp;
     }
-    // rebuild top-down: subset S of the internal node `node`
+}
+
+// Writes the optimal topology of the treelet rooted at x, top-down: subset S
+// of internal node `node`, reusing the treelet's internal node indices.
+template <int K, int KT, class M>
+FL_HD void treelet_rebuild(M &m, uint32_t x, const TreeletLeaves<K> &t, const TreeletTable<KT> &tab) {
+    const int nl = t.nl;
     uint32_t stk_set[K], stk_node[K];
     int sp = 0, next = 1;
-    stk_set[sp] = full;
+    stk_set[sp] = (1u << nl) - 1u;
     stk_node[sp] = x;
     ++sp;
     while (sp > 0) {
         --sp;
         const uint32_t S = stk_set[sp], node = stk_node[sp];
-        const uint32_t P = part[S * ST], Q = S ^ P;
+        const uint32_t P = tab.part[S], Q = S ^ P;
         uint32_t ch[2];
         const uint32_t sets[2] = {P, Q};
         for (int c = 0; c < 2; ++c) {
             const uint32_t T = sets[c];
             if ((T & (T - 1u)) == 0u) {   // one leaf: the treelet leaf itself
-                int k = 0;
-                while (!(T & (1u << k))) ++k;
-                ch[c] = L[k];
+                ch[c] = treelet_get(t.L, __builtin_ctz(T));
             } else {
-                ch[c] = I[next++];
+                ch[c] = treelet_get(t.I, next++);
                 stk_set[sp] = T;
                 stk_node[sp] = ch[c];
                 ++sp;
@@ -151,16 +216,46 @@ FL_HD void treelet_restructure(M &m, uint32_t x, float *cost, uint8_t *part) {
         m.set_children(node, ch[0], ch[1]);
         float b[6] = {kInf, kInf, kInf, -kInf, -kInf, -kInf};
         uint32_t cnt = 0;
-        for (int k = 0; k < nl; ++k)
+#pragma unroll
+        for (int k = 0; k < KT; ++k)
             if (S & (1u << k)) {
                 for (int e = 0; e < 3; ++e) {
-                    b[e] = fminf(b[e], lb[k][e]);
-                    b[3 + e] = fmaxf(b[3 + e], lb[k][3 + e]);
+                    b[e] = fminf(b[e], t.lb[k][e]);
+                    b[3 + e] = fmaxf(b[3 + e], t.lb[k][3 + e]);
                 }
-                cnt += lcount[k];
+                cnt += t.lcount[k];
             }
-        m.set_node(node, b, cnt, cost[S * ST]);
+        m.set_node(node, b, cnt, tab.cost[S]);
     }
+}
+
+// The three steps in one thread; `nlanes` only sets how the subsets are
+// dealt out (the result does not depend on it).
+template <int K, class M>
+FL_HD void treelet_restructure(M &m, uint32_t x, TreeletTable<K> &tab, uint32_t nlanes = 1) {
+    TreeletLeaves<K> t;
+    treelet_form<K>(m, x, t);
+    for (int sz = 1; sz <= t.nl; ++sz)
+        for (uint32_t lane = 0; lane < nlanes; ++lane) treelet_dp_size<K, K>(t, tab, sz, lane, nlanes);
+    treelet_rebuild<K, K>(m, x, t, tab);
+}
+
+// A treelet of at most kTreeletSmall leaves is a node of at most that many
+// primitives (the formation opens every internal leaf below 7 leaves). Such a
+// subtree ends up as the leaf slots of one 8-wide node, whatever its binary
+// topology, so it keeps its topology; only its cost is recorded. (Optimising
+// those too changed neither the nodes visited nor the traversal time on the
+// spheres and room scenes.)
+constexpr int kTreeletSmall = 4;
+template <class M>
+FL_HD void treelet_keep(M &m, uint32_t x) {
+    float b[6];
+    m.box_of(x, b);
+    const uint32_t l = m.left(x), r = m.right(x);
+    float c = box_half_area(b);
+    if (!(l & kTreeletLeafBit)) c += m.cost(l);
+    if (!(r & kTreeletLeafBit)) c += m.cost(r);
+    m.set_cost(x, c);
 }
 
 } // namespace fl

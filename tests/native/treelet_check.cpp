@@ -40,6 +40,7 @@ struct HostM {
         for (int e = 0; e < 6; ++e) t->nbox[6 * x + e] = b[e];
         t->cnt[x] = c; t->ncost[x] = cost;
     }
+    void set_cost(uint32_t x, float cost) { t->ncost[x] = cost; }
 };
 static int delta(const std::vector<uint64_t> &k, int n, int i, int j) {
     if (j < 0 || j >= n) return -1;
@@ -100,12 +101,20 @@ int main(int argc, char **argv) {
     // refit (post-order by the same bottom-up arrival rule)
     auto sweep = [&](bool restructure) {
         std::vector<int> flags(ni, 0);
-        std::vector<float> cost(1 << kTreeletMax); std::vector<uint8_t> part(1 << kTreeletMax);
+        TreeletTable<kTreeletMax> tab;
         for (int i = 0; i < n; ++i) {
             int node = T.pleaf[i];
             while (node >= 0) {
                 if (flags[node]++ == 0) break;
-                if (restructure) treelet_restructure<kTreeletMax, 1>(m, (uint32_t)node, cost.data(), part.data());
+                // as the device kernel does: treelets of at most
+                // kTreeletSmall leaves keep their topology, larger ones are
+                // optimised with the subsets dealt out over 64 lanes
+                if (restructure) {
+                    TreeletLeaves<kTreeletMax> t;
+                    treelet_form<kTreeletMax>(m, (uint32_t)node, t);
+                    if (t.nl <= kTreeletSmall) treelet_keep(m, (uint32_t)node);
+                    else treelet_restructure<kTreeletMax>(m, (uint32_t)node, tab, 64u);
+                }
                 else {
                     float b[6] = {1e30f, 1e30f, 1e30f, -1e30f, -1e30f, -1e30f}, cb[6]; uint32_t c = 0; float cc = 0;
                     for (uint32_t ch : {T.l[node], T.r[node]}) { m.box_of(ch, cb); for (int e = 0; e < 3; ++e) { b[e] = std::min(b[e], cb[e]); b[3 + e] = std::max(b[3 + e], cb[3 + e]); } c += (ch & LB) ? 1 : T.cnt[ch]; cc += (ch & LB) ? 0.f : T.ncost[ch]; }
