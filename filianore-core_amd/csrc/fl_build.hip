@@ -280,12 +280,37 @@ __global__ void __launch_bounds__(kTreeletBlock) k_treelet_level(const uint32_t 
         __syncthreads();
         const int nl = s_leaves.nl;
         for (int sz = 1; sz <= nl; ++sz) {
-            fl::treelet_dp_size<K, K>(s_leaves, s_tab, sz, lane, kTreeletBlock);
+            fl::treelet_dp_size<K>(s_leaves, s_tab, sz, lane, kTreeletBlock);
             __syncthreads();
         }
-        if (lane == j) fl::treelet_rebuild<K, K>(m, node, s_leaves, s_tab);
+        if (lane == j) fl::treelet_rebuild<K>(m, node, s_leaves, s_tab);
         __syncthreads();   // the next treelet overwrites the table
     }
+}
+
+// Levels of more than kTreeletWaveLevel nodes: every lane optimises its own
+// node's treelet, with its table in LDS (the 64 lanes' tables interleaved:
+// 40 KB per wave). Sharing each treelet among the wave's lanes pays only
+// near the root, where a level has few nodes; on a full wave the lanes'
+// treelets one after another took 20-40 us each.
+constexpr uint32_t kTreeletWaveLevel = 8;
+__global__ void __launch_bounds__(kTreeletBlock) k_treelet_level_lanes(const uint32_t *__restrict__ level,
+                                                                       uint32_t count, TreeletMem m) {
+    constexpr int K = fl::kTreeletMax;
+    __shared__ float s_cost[kTreeletBlock << K];
+    __shared__ uint8_t s_part[kTreeletBlock << K];
+    const uint32_t lane = threadIdx.x;
+    const uint32_t i = blockIdx.x * kTreeletBlock + lane;
+    if (i >= count) return;
+    const uint32_t node = level[i];
+    fl::TreeletLeaves<K> mine;
+    fl::treelet_form<K>(m, node, mine);
+    if (mine.nl <= fl::kTreeletSmall) {
+        fl::treelet_keep(m, node);
+        return;
+    }
+    fl::TreeletTableStrided<kTreeletBlock> tab{s_cost + lane, s_part + lane};
+    fl::treelet_optimise<K>(m, node, mine, tab);
 }
 
 struct BuildView {
@@ -641,8 +666,9 @@ int build_bvh(fl_ctx_t ctx, fl_scene_s *s, const float *d_verts, const uint32_t 
             if (r > 0 && (rc = list_levels()) != FL_OK) return rc;   // the depths changed
             for (size_t d = lstart.size() - 1; d-- > 0;) {
                 const uint32_t b = lstart[d], c = lstart[d + 1] - lstart[d];
-                hipLaunchKernelGGL(k_treelet_level, dim3((c + kTreeletBlock - 1) / kTreeletBlock), dim3(kTreeletBlock),
-                                   0, st, order + b, c, tm);
+                hipLaunchKernelGGL(c > kTreeletWaveLevel ? k_treelet_level_lanes : k_treelet_level,
+                                   dim3((c + kTreeletBlock - 1) / kTreeletBlock), dim3(kTreeletBlock), 0, st, order + b,
+                                   c, tm);
             }
         }
     }

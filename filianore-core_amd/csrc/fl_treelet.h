@@ -19,8 +19,8 @@
 // treelet's dynamic program: treelet_form (one thread: grow the treelet),
 // treelet_dp_size (the subsets of one size, spread over `nlanes` lanes; the
 // sizes in increasing order, with a barrier between them) and treelet_rebuild
-// (one thread: write the optimal topology). treelet_restructure runs the three
-// in one thread. Memory access goes through a policy M (plain loads and
+// (one thread: write the optimal topology); treelet_optimise runs the last
+// two in one thread. Memory access goes through a policy M (plain loads and
 // stores on the host; agent-scope relaxed atomics on the device, where other
 // threads wrote the subtree), so that the same code runs in a CPU check.
 #pragma once
@@ -69,6 +69,25 @@ template <int K>
 struct TreeletTable {
     float cost[1u << K];
     uint8_t part[1u << K];
+    FL_HD float c(uint32_t S) const { return cost[S]; }
+    FL_HD uint32_t p(uint32_t S) const { return part[S]; }
+    FL_HD void set(uint32_t S, float cs, uint32_t ps) {
+        cost[S] = cs;
+        part[S] = (uint8_t)ps;
+    }
+};
+// The same table with entry S at [S * ST] (the tables of a wave's lanes
+// interleaved in LDS, so that lanes reading the same S hit different banks).
+template <int ST>
+struct TreeletTableStrided {
+    float *cost;
+    uint8_t *part;
+    FL_HD float c(uint32_t S) const { return cost[S * ST]; }
+    FL_HD uint32_t p(uint32_t S) const { return part[S * ST]; }
+    FL_HD void set(uint32_t S, float cs, uint32_t ps) const {
+        cost[S * ST] = cs;
+        part[S * ST] = (uint8_t)ps;
+    }
 };
 
 // a[i] and a[i] = v for a run-time i, as selects over compile-time indices:
@@ -143,21 +162,20 @@ FL_HD void treelet_form(M &m, uint32_t x, TreeletLeaves<K> &t) {
 // The optimal split of every subset of sz leaves that this lane owns
 // (subsets lane + 1, lane + 1 + nlanes, ...). Every subset of fewer leaves
 // must be done (by all lanes) before.
-template <int K, int KT>
-FL_HD void treelet_dp_size(const TreeletLeaves<K> &t, TreeletTable<KT> &tab, int sz, uint32_t lane, uint32_t nlanes) {
-    const int nl = t.nl;   // at most KT
+template <int K, class T>
+FL_HD void treelet_dp_size(const TreeletLeaves<K> &t, T &tab, int sz, uint32_t lane, uint32_t nlanes) {
+    const int nl = t.nl;
     const uint32_t full = (1u << nl) - 1u;
     for (uint32_t S = lane + 1u; S <= full; S += nlanes) {
         if (__builtin_popcount(S) != sz) continue;
         if (sz == 1) {
-            tab.cost[S] = treelet_get(t.lc, __builtin_ctz(S));
-            tab.part[S] = 0;
+            tab.set(S, treelet_get(t.lc, __builtin_ctz(S)), 0u);
             continue;
         }
         float b[6] = {kInf, kInf, kInf, -kInf, -kInf, -kInf};
         uint32_t cnt = 0;
 #pragma unroll
-        for (int k = 0; k < KT; ++k)
+        for (int k = 0; k < K; ++k)
             if (S & (1u << k)) {
                 for (int e = 0; e < 3; ++e) {
                     b[e] = fminf(b[e], t.lb[k][e]);
@@ -172,22 +190,21 @@ FL_HD void treelet_dp_size(const TreeletLeaves<K> &t, TreeletTable<KT> &tab, int
         uint32_t bp = low;
         for (uint32_t Q = (R - 1u) & R;; Q = (Q - 1u) & R) {
             const uint32_t P = low | Q;
-            const float c = tab.cost[P] + tab.cost[S ^ P];
+            const float c = tab.c(P) + tab.c(S ^ P);
             if (c < best) {
                 best = c;
                 bp = P;
             }
             if (Q == 0u) break;
         }
-        tab.cost[S] = treelet_cost(b, cnt, best);
-        tab.part[S] = (uint8_t)bp;
+        tab.set(S, treelet_cost(b, cnt, best), bp);
     }
 }
 
 // Writes the optimal topology of the treelet rooted at x, top-down: subset S
 // of internal node `node`, reusing the treelet's internal node indices.
-template <int K, int KT, class M>
-FL_HD void treelet_rebuild(M &m, uint32_t x, const TreeletLeaves<K> &t, const TreeletTable<KT> &tab) {
+template <int K, class T, class M>
+FL_HD void treelet_rebuild(M &m, uint32_t x, const TreeletLeaves<K> &t, const T &tab) {
     const int nl = t.nl;
     uint32_t stk_set[K], stk_node[K];
     int sp = 0, next = 1;
@@ -197,16 +214,16 @@ FL_HD void treelet_rebuild(M &m, uint32_t x, const TreeletLeaves<K> &t, const Tr
     while (sp > 0) {
         --sp;
         const uint32_t S = stk_set[sp], node = stk_node[sp];
-        const uint32_t P = tab.part[S], Q = S ^ P;
+        const uint32_t P = tab.p(S), Q = S ^ P;
         uint32_t ch[2];
         const uint32_t sets[2] = {P, Q};
         for (int c = 0; c < 2; ++c) {
-            const uint32_t T = sets[c];
-            if ((T & (T - 1u)) == 0u) {   // one leaf: the treelet leaf itself
-                ch[c] = treelet_get(t.L, __builtin_ctz(T));
+            const uint32_t U = sets[c];
+            if ((U & (U - 1u)) == 0u) {   // one leaf: the treelet leaf itself
+                ch[c] = treelet_get(t.L, __builtin_ctz(U));
             } else {
                 ch[c] = treelet_get(t.I, next++);
-                stk_set[sp] = T;
+                stk_set[sp] = U;
                 stk_node[sp] = ch[c];
                 ++sp;
             }
@@ -217,7 +234,7 @@ FL_HD void treelet_rebuild(M &m, uint32_t x, const TreeletLeaves<K> &t, const Tr
         float b[6] = {kInf, kInf, kInf, -kInf, -kInf, -kInf};
         uint32_t cnt = 0;
 #pragma unroll
-        for (int k = 0; k < KT; ++k)
+        for (int k = 0; k < K; ++k)
             if (S & (1u << k)) {
                 for (int e = 0; e < 3; ++e) {
                     b[e] = fminf(b[e], t.lb[k][e]);
@@ -225,19 +242,18 @@ FL_HD void treelet_rebuild(M &m, uint32_t x, const TreeletLeaves<K> &t, const Tr
                 }
                 cnt += t.lcount[k];
             }
-        m.set_node(node, b, cnt, tab.cost[S]);
+        m.set_node(node, b, cnt, tab.c(S));
     }
 }
 
-// The three steps in one thread; `nlanes` only sets how the subsets are
-// dealt out (the result does not depend on it).
-template <int K, class M>
-FL_HD void treelet_restructure(M &m, uint32_t x, TreeletTable<K> &tab, uint32_t nlanes = 1) {
-    TreeletLeaves<K> t;
-    treelet_form<K>(m, x, t);
+// The dynamic program and the rebuild of a formed treelet in one thread;
+// `nlanes` only sets how the subsets are dealt out (the result does not
+// depend on it).
+template <int K, class T, class M>
+FL_HD void treelet_optimise(M &m, uint32_t x, const TreeletLeaves<K> &t, T &tab, uint32_t nlanes = 1) {
     for (int sz = 1; sz <= t.nl; ++sz)
-        for (uint32_t lane = 0; lane < nlanes; ++lane) treelet_dp_size<K, K>(t, tab, sz, lane, nlanes);
-    treelet_rebuild<K, K>(m, x, t, tab);
+        for (uint32_t lane = 0; lane < nlanes; ++lane) treelet_dp_size<K>(t, tab, sz, lane, nlanes);
+    treelet_rebuild<K>(m, x, t, tab);
 }
 
 // A treelet of at most kTreeletSmall leaves is a node of at most that many

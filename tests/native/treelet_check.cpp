@@ -1,6 +1,6 @@
 // Host check of the BVH builder's treelet restructuring (fl_treelet.h): a
 // binary radix tree (Karras 2012) over the Morton order of synthetic
-// primitive boxes, bottom-up sweeps of treelet_restructure in the order the
+// primitive boxes, bottom-up sweeps of treelet restructuring in the order the
 // device kernel's second-arrival rule gives, then every invariant of the tree:
 // each primitive reached exactly once, each internal node once, parent links
 // matching child links, node boxes equal to the union of their children,
@@ -102,18 +102,23 @@ int main(int argc, char **argv) {
     auto sweep = [&](bool restructure) {
         std::vector<int> flags(ni, 0);
         TreeletTable<kTreeletMax> tab;
+        std::vector<float> scost(3u << kTreeletMax);
+        std::vector<uint8_t> spart(3u << kTreeletMax);
+        TreeletTableStrided<3> strided{scost.data() + 1, spart.data() + 1};
         for (int i = 0; i < n; ++i) {
             int node = T.pleaf[i];
             while (node >= 0) {
                 if (flags[node]++ == 0) break;
-                // as the device kernel does: treelets of at most
-                // kTreeletSmall leaves keep their topology, larger ones are
-                // optimised with the subsets dealt out over 64 lanes
+                // as the device kernels do: treelets of at most
+                // kTreeletSmall leaves keep their topology; larger ones are
+                // optimised in one thread (a strided table, as in LDS) or with
+                // the subsets dealt out over 64 lanes, alternately
                 if (restructure) {
                     TreeletLeaves<kTreeletMax> t;
                     treelet_form<kTreeletMax>(m, (uint32_t)node, t);
                     if (t.nl <= kTreeletSmall) treelet_keep(m, (uint32_t)node);
-                    else treelet_restructure<kTreeletMax>(m, (uint32_t)node, tab, 64u);
+                    else if (node & 1) treelet_optimise<kTreeletMax>(m, (uint32_t)node, t, tab, 64u);
+                    else treelet_optimise<kTreeletMax>(m, (uint32_t)node, t, strided, 1u);
                 }
                 else {
                     float b[6] = {1e30f, 1e30f, 1e30f, -1e30f, -1e30f, -1e30f}, cb[6]; uint32_t c = 0; float cc = 0;

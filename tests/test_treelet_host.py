@@ -1,6 +1,8 @@
 """The BVH builder's treelet restructuring (filianore-core_amd/csrc/fl_treelet.h,
-run on the device by k_treelet in fl_build.hip) checked on the host: the same
-routine, compiled for the CPU with a plain-memory policy, restructures binary
+run on the device level by level by k_treelet_level_lanes and k_treelet_level in
+fl_build.hip) checked on the host: the same routines, compiled for the CPU with
+a plain-memory policy (tables contiguous and lane-interleaved, subsets dealt
+out over 64 lanes as the wave does), restructure binary
 radix trees over synthetic primitive boxes, and every tree invariant is
 checked afterwards (tests/native/treelet_check.cpp). The role is that of the
 Embree build behind BVH::initialize_scene_geometry/finalise (bvh/bvh.cpp:7-47):
