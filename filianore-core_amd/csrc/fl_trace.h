@@ -151,8 +151,19 @@ __device__ __forceinline__ bool trav_step(const TraceScene &sc, TravState &s, co
             }
             const uint4 *np = reinterpret_cast<const uint4 *>(sc.nodes + nidx);
             uint4 w0 = np[0], w1 = np[1];
-            uint4 qn[3] = {np[2], np[3], np[4]};
-            uint4 qf[3] = {np[5], np[6], np[7]};
+            // the ray's near plane per axis is lo (words 2..4) for a positive
+            // direction and hi (words 5..7) for a negative one: each lane
+            // loads the plane words in its own order (a per-lane address,
+            // free since each lane reads its own node; swapping the twelve
+            // words after the load took 24 selects per node, 2.7% of the
+            // traversal time on the spheres scene)
+            uint4 qn[3], qf[3];
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                const uint32_t neg = (oct >> k) & 1u;
+                qn[k] = np[2 + k + 3 * neg];
+                qf[k] = np[5 + k - 3 * neg];
+            }
             if (COUNT) {
                 ++cnt.nodes;
                 if (any) ++cnt.any_nodes;
@@ -162,16 +173,6 @@ __device__ __forceinline__ bool trav_step(const TraceScene &sc, TravState &s, co
                              __uint_as_float(((w0.w >> 16) & 0xffu) << 23)};
             uint32_t imask = w0.w >> 24;
             float oo[3] = {s.o.x, s.o.y, s.o.z};
-            // the ray's near plane per axis is lo for a positive direction and
-            // hi for a negative one: swap the packed plane words once per node
-#pragma unroll
-            for (int k = 0; k < 3; ++k) {
-                if ((oct >> k) & 1u) {
-                    uint4 t = qn[k];
-                    qn[k] = qf[k];
-                    qf[k] = t;
-                }
-            }
             // Along axis k the ray meets grid plane q at
             //   t(q) = (org + q step - o) inv = q a + b,  a = step inv, b = (org - o) inv,
             // a exact (a power-of-two scale of inv). One fma per plane against
@@ -209,12 +210,14 @@ __device__ __forceinline__ bool trav_step(const TraceScene &sc, TravState &s, co
             }
             uint32_t inner = hits & imask;
             uint32_t leafs = hits & ~imask;
-            uint32_t ng = 0;
-            while (inner) {
-                uint32_t q = __ffs(inner) - 1;
-                inner &= inner - 1u;
-                ng |= 1u << (q ^ oct);
-            }
+            // the internal children in visiting order, bit q moved to q ^ oct
+            // by three conditional swaps (bits, pairs, nibbles): no loop over
+            // the hit children, which ran as often as the lane with the most
+            // hits needed (1.5% of the traversal time on the spheres scene)
+            uint32_t ng = inner;
+            ng = (oct & 1u) ? (((ng & 0x55u) << 1) | ((ng >> 1) & 0x55u)) : ng;
+            ng = (oct & 2u) ? (((ng & 0x33u) << 2) | ((ng >> 2) & 0x33u)) : ng;
+            ng = (oct & 4u) ? (((ng & 0x0fu) << 4) | ((ng >> 4) & 0x0fu)) : ng;
             uint32_t tm = 0;
             const uint32_t meta_lo = w1.z, meta_hi = w1.w;
             while (leafs) {
