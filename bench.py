@@ -230,7 +230,7 @@ def guided_section(flgpu, ctx, config, res, ref_spp, spp=64):
     ceiling slot. Each method runs with and without Russian roulette (the
     reference rule after bounce 3); the better setting of each (lower relMSE x
     time) is compared. Guided renders train the proposal online on their
-    first 25% of samples, inside the timed render. relMSE is against a plain
+    first 15% of samples, inside the timed render. relMSE is against a plain
     reference of ref_spp samples per pixel, rendered in 4096-spp parts, with
     the reference's own noise (its per-pixel variance of the mean, from the
     film's sums of squares) subtracted so that the figure estimates the error

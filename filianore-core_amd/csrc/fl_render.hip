@@ -18,6 +18,16 @@
 // needs no host round trip and can be captured in a graph).
 // The MIS BSDF ray of estimate_direct is the next extension ray (same
 // expectation, one traversal fewer per light per vertex).
+//
+// The file is compiled twice. The main unit holds the traversal, generation
+// and resolve kernels and the host driver, with correctly rounded fp32
+// division and square root (the hits must equal the CPU's bit for bit). The
+// shading unit (FL_SHADE_TU) holds the shading kernels and the BSDF query
+// kernels, with the hardware's division and square root (within 2 ulp): a
+// third of the shading kernel's instructions were the correctly rounded
+// division sequences, and shading the spheres workload took 9% longer with
+// them. Shading results are compared with the oracle statistically and with
+// FP64 BSDFs to 1e-5, far above that rounding.
 #include <algorithm>
 #include <chrono>
 #include <cmath>
@@ -297,6 +307,7 @@ constexpr int kTravBudget = 0;     // persistent, refill after trav_budget itera
 constexpr int kTravPrefetch = 1;   // persistent, each lane's next ray loaded ahead
 constexpr int kTravFlat = 2;       // one ray per thread (counter passes use kTravPrefetch)
 
+#ifndef FL_SHADE_TU
 template <int S, int WV, bool M>
 void launch_trace_m(bool count, dim3 g, hipStream_t st, const RenderArgs &a, int j) {
     if (a.trav_mode == kTravFlat && !count) {
@@ -337,6 +348,7 @@ const TraceVariant kTraceVariants[] = {
     {8, 6, launch_trace_t<8, 6>, trace_fn<8, 6>},    {8, 8, launch_trace_t<8, 8>, trace_fn<8, 8>},
     {6, 8, launch_trace_t<6, 8>, trace_fn<6, 8>},    {16, 1, launch_trace_t<16, 1>, trace_fn<16, 1>},
 };
+#endif  // !FL_SHADE_TU
 
 // ---------------------------------------------------------------- guide ----
 // Hardware exp, reciprocal and reciprocal square root (about 1 ulp) in the
@@ -702,10 +714,15 @@ __device__ __forceinline__ void normal_spectrum(const float *basis, int GS, v3 n
 
 // WAVES: minimum waves per SIMD requested from the register allocator (2 lets
 // it use all the registers it wants; 4 caps at 128 VGPRs with a few spills).
-// Shading sort: each block orders its 256 queue entries by material kind
-// (counting sort in LDS) before shading them, so a wave runs one BSDF's code
-// instead of every kind present in its 64 entries. Per-path results do not
-// depend on the order; only the order of the next queue changes.
+// Shading sort (FL_RENDER_SORT_SHADING): each block orders its queue entries
+// by material kind (counting sort in LDS) before shading them, so a wave runs
+// one BSDF's code instead of every kind present in its 64 entries. Per-path
+// results do not depend on the order; only the order of the next queue
+// changes. It is off by default: its key costs three dependent loads (hit,
+// triangle, material) per entry and three barriers per block step, and the
+// shading was faster without it on every scene measured (spheres 20.06 ->
+// 19.60 ms, the room 50.1 -> 44.1 ms, Cornell, the standard-surface box, the
+// terrain and guided shading 1-5%).
 constexpr uint32_t kSortKeys = 6;   // miss / no material, Lambert, conductor, dielectric, standard, past the end
 
 __device__ __forceinline__ uint32_t shade_key(const RenderArgs &a, int cur, uint32_t i, uint32_t n) {
@@ -729,6 +746,7 @@ __device__ __forceinline__ uint32_t shade_key(const RenderArgs &a, int cur, uint
 constexpr int kGuideBlock = 512;
 constexpr int shade_block(bool std_, bool guided) { return guided && !std_ ? kGuideBlock : kBlock; }
 
+#ifdef FL_SHADE_TU
 template <int WAVES, bool STD, bool GUIDED>
 __global__ void __launch_bounds__(shade_block(STD, GUIDED)) __attribute__((amdgpu_waves_per_eu(WAVES, 8)))
 k_shade(RenderArgs a, int b) {
@@ -764,7 +782,7 @@ k_shade(RenderArgs a, int b) {
     const uint32_t n = a.counts[C_EXT + cur];
     const uint32_t nround = (n + BS - 1) / BS * BS;
     const int GS = a.grid.stride;
-    const bool sort = !(a.flags & FL_RENDER_NO_SORT);
+    const bool sort = (a.flags & FL_RENDER_SORT_SHADING) != 0;
     for (uint32_t base = blockIdx.x * BS; base < nround; base += stride) {
         uint32_t i = base + threadIdx.x;
         if (sort) {
@@ -1164,10 +1182,7 @@ const void *shade_kernel_t(uint32_t waves) {
     if (waves == 3) return reinterpret_cast<const void *>(&k_shade<3, STD, GUIDED>);
     return reinterpret_cast<const void *>(&k_shade<2, STD, GUIDED>);
 }
-const void *shade_kernel(bool std_, bool guided, uint32_t waves) {
-    if (std_) return guided ? shade_kernel_t<true, true>(waves) : shade_kernel_t<true, false>(waves);
-    return guided ? shade_kernel_t<false, true>(waves) : shade_kernel_t<false, false>(waves);
-}
+#endif  // FL_SHADE_TU
 
 // -------------------------------------------------------------- resolve ----
 // One thread per padded pixel of all items; sums its samples of this pass in
@@ -1219,7 +1234,9 @@ __global__ void __launch_bounds__(kBlock) k_resolve(RenderArgs a, const float *c
     if (nan_count) atomicAdd(&a.stats[2], (unsigned long long)nan_count);
 }
 
+#ifdef FL_SHADE_TU
 // ----------------------------------------------------------- BSDF queries --
+// (compiled with the shading kernels, so that they run the same BSDF code)
 __global__ void k_bsdf_query(const fl_bsdf_query *__restrict__ q, fl_bsdf_result *__restrict__ r, uint32_t n) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
@@ -1312,6 +1329,8 @@ __global__ void k_bsdf_query_standard(StdParams m, const fl_standard_query *__re
     r[i] = R;
 }
 
+#endif  // FL_SHADE_TU
+
 // The sampler of k_generate / k_shade for (pixel, sample, bounce) keys.
 __global__ void k_sampler_draw(uint64_t seed, const uint32_t *__restrict__ keys, uint32_t n, uint32_t n_dims,
                                float *__restrict__ out) {
@@ -1335,6 +1354,38 @@ T *grow(void *&p, size_t &cap, size_t bytes, bool &err) {
 }
 
 } // namespace
+
+namespace flx {
+// The shading kernel for a render: standard surfaces / textures (STD), a
+// guide (GUIDED), and the waves per SIMD (2, 3 or 4). Defined in the shading
+// unit (this file compiled with FL_SHADE_TU).
+const void *shade_kernel(bool std_, bool guided, uint32_t waves);
+}
+
+#ifdef FL_SHADE_TU
+namespace flx {
+const void *shade_kernel(bool std_, bool guided, uint32_t waves) {
+    if (std_) return guided ? shade_kernel_t<true, true>(waves) : shade_kernel_t<true, false>(waves);
+    return guided ? shade_kernel_t<false, true>(waves) : shade_kernel_t<false, false>(waves);
+}
+
+int bsdf_queries_standard(fl_ctx_t ctx, const fl_standard_surface *m, const fl_standard_query *d_q, fl_bsdf_result *d_r,
+                          uint32_t n) {
+    if (n == 0) return FL_OK;
+    const StdParams P = standard_params(*m);
+    hipLaunchKernelGGL(k_bsdf_query_standard, dim3((n + 255) / 256), dim3(256), 0, ctx->stream, P, d_q, d_r, n);
+    FL_HIP_CHECK(ctx, hipGetLastError());
+    return FL_OK;
+}
+
+int bsdf_queries(fl_ctx_t ctx, const fl_bsdf_query *d_q, fl_bsdf_result *d_r, uint32_t n) {
+    if (n == 0) return FL_OK;
+    hipLaunchKernelGGL(k_bsdf_query, dim3((n + 255) / 256), dim3(256), 0, ctx->stream, d_q, d_r, n);
+    FL_HIP_CHECK(ctx, hipGetLastError());
+    return FL_OK;
+}
+} // namespace flx
+#else  // the main unit: the host driver
 
 // ----------------------------------------------------------- host driver ---
 struct RenderState {
@@ -1649,8 +1700,11 @@ int render_items(fl_ctx_t ctx, const HostItem *hi, uint32_t n_items, const fl_re
             a.box_inv[k] = 1.f / (s->bmax[k] - s->bmin[k] + 2.f * pad);
         }
         if (train) {
+            // (0.15 by default: on the attic room 0.25 reached plain's error in
+            // 7% more time, the shorter training passes paying for the
+            // network's slightly later start)
             const float fr = p->guide_train_fraction > 0.f && p->guide_train_fraction <= 1.f ? p->guide_train_fraction
-                                                                                              : 0.25f;
+                                                                                              : 0.15f;
             train_spp = std::min(max_spp, std::max(1u, (uint32_t)std::lround(fr * (float)max_spp)));
             train_rounds = std::min(guide->cfg.train_rounds, train_spp);
             const uint32_t round_spp = train_round_end(train_spp, train_rounds, train_rounds - 1) -
@@ -2022,7 +2076,7 @@ int autotune_items(fl_ctx_t ctx, const HostItem *hi, uint32_t n, const fl_render
     if (rc == FL_OK) rc = try_values([](fl_render_params &q, uint32_t v) { q.refill_lanes = v; }, {1u, 16u, 32u});
     if (rc == FL_OK) rc = try_values([](fl_render_params &q, uint32_t v) { q.trav_prefetch = v; }, {1u, 2u, 3u});
     if (rc == FL_OK)
-        rc = try_values([](fl_render_params &q, uint32_t v) { q.flags = v ? (q.flags | FL_RENDER_NO_SORT) : q.flags; },
+        rc = try_values([](fl_render_params &q, uint32_t v) { q.flags = v ? (q.flags | FL_RENDER_SORT_SHADING) : q.flags; },
                         {1u});
     if (rc == FL_OK) rc = try_values([](fl_render_params &q, uint32_t v) { q.shade_waves = v; }, {2u, 3u, 4u});
     if (rc != FL_OK) return rc;
@@ -2034,20 +2088,11 @@ int autotune_items(fl_ctx_t ctx, const HostItem *hi, uint32_t n, const fl_render
     out->refill_lanes = cur.refill_lanes;   // 0: the mode's default (24 loaded ahead, 16 budgeted)
     out->trav_prefetch = cur.trav_prefetch;
     out->shade_waves = cur.shade_waves;
-    out->sort_shading = (cur.flags & FL_RENDER_NO_SORT) ? 0u : 1u;
+    out->sort_shading = (cur.flags & FL_RENDER_SORT_SHADING) ? 1u : 0u;
     out->ms_best = ms_cur;
     out->ms_default = ms_default;
     out->n_tried = tried;
     out->node_width = width_best;
-    return FL_OK;
-}
-
-int bsdf_queries_standard(fl_ctx_t ctx, const fl_standard_surface *m, const fl_standard_query *d_q, fl_bsdf_result *d_r,
-                          uint32_t n) {
-    if (n == 0) return FL_OK;
-    const StdParams P = standard_params(*m);
-    hipLaunchKernelGGL(k_bsdf_query_standard, dim3((n + 255) / 256), dim3(256), 0, ctx->stream, P, d_q, d_r, n);
-    FL_HIP_CHECK(ctx, hipGetLastError());
     return FL_OK;
 }
 
@@ -2059,11 +2104,5 @@ int sampler_draw(fl_ctx_t ctx, uint64_t seed, const uint32_t *d_keys, uint32_t n
     return FL_OK;
 }
 
-int bsdf_queries(fl_ctx_t ctx, const fl_bsdf_query *d_q, fl_bsdf_result *d_r, uint32_t n) {
-    if (n == 0) return FL_OK;
-    hipLaunchKernelGGL(k_bsdf_query, dim3((n + 255) / 256), dim3(256), 0, ctx->stream, d_q, d_r, n);
-    FL_HIP_CHECK(ctx, hipGetLastError());
-    return FL_OK;
-}
-
 } // namespace flx
+#endif  // FL_SHADE_TU

@@ -12,7 +12,7 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libfl_gpu.so")
 # the struct layouts below are those of this version of include/fl_gpu.h
-ABI_VERSION = 6
+ABI_VERSION = 7
 
 FL_OK = 0
 FL_E_INVALID, FL_E_HIP, FL_E_OOM, FL_E_UNSUPPORTED, FL_E_OVERFLOW, FL_E_NUMERIC = -1, -2, -3, -4, -5, -6
@@ -21,7 +21,7 @@ FL_FRESNEL_DIELECTRIC, FL_FRESNEL_METALLIC, FL_FRESNEL_WAVELENGTH_METALLIC = 0, 
 FL_CLOSEST_HIT, FL_ANY_HIT, FL_SKIP_LIGHTS, FL_COUNTERS = 0, 1, 2, 4
 FL_SPECTRAL_HERO, FL_SPECTRAL_PARITY = 0, 1
 FL_RENDER_GUIDED, FL_RENDER_COUNTERS, FL_RENDER_VNDF_PDF, FL_RENDER_GRAPH = 1, 2, 4, 8
-FL_RENDER_TRAIN_GUIDE, FL_RENDER_NO_SORT, FL_RENDER_TIMING, FL_RENDER_NORMALS = 16, 32, 64, 128
+FL_RENDER_TRAIN_GUIDE, FL_RENDER_SORT_SHADING, FL_RENDER_TIMING, FL_RENDER_NORMALS = 16, 32, 64, 128
 FL_RENDER_NEE_ONE = 256
 FL_LIGHT_POINT, FL_LIGHT_SPOT, FL_LIGHT_DIRECTIONAL = 1, 2, 3
 FL_COLOR_SRGB, FL_COLOR_ACES2065_1 = 0, 1
@@ -152,9 +152,9 @@ class TuneResult(C.Structure):
         params.refill_lanes = self.refill_lanes
         params.trav_prefetch = self.trav_prefetch
         if self.sort_shading:
-            params.flags &= ~FL_RENDER_NO_SORT
+            params.flags |= FL_RENDER_SORT_SHADING
         else:
-            params.flags |= FL_RENDER_NO_SORT
+            params.flags &= ~FL_RENDER_SORT_SHADING
         return params
 
 
@@ -420,7 +420,7 @@ def default_params(spp=16, depth=8, mode=FL_SPECTRAL_HERO, seed=1, flags=0):
     p.flags = flags
     p.seed = seed
     p.guide_mix = 0.5
-    p.guide_train_fraction = 0.25
+    p.guide_train_fraction = 0.15
     return p
 
 

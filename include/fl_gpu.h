@@ -36,7 +36,7 @@
 extern "C" {
 #endif
 
-#define FL_ABI_VERSION 6
+#define FL_ABI_VERSION 7
 
 /* ---- status codes ------------------------------------------------------ */
 enum {
@@ -228,7 +228,9 @@ enum {
     FL_RENDER_VNDF_PDF = 4,      /* GGX: matched VNDF pdf instead of the reference's D*cos */
     FL_RENDER_GRAPH = 8,         /* capture the bounce loop in a hipGraph */
     FL_RENDER_TRAIN_GUIDE = 16,  /* train the guide online on this render's samples */
-    FL_RENDER_NO_SORT = 32,      /* disable shading sort */
+    FL_RENDER_SORT_SHADING = 32, /* order each shading block's vertices by material kind first
+                                    (an autotuner knob; off by default since ABI 7: slower on
+                                    every scene measured) */
     FL_RENDER_TIMING = 64,       /* per-kernel HIP-event timing (ms_* fields of fl_stats) */
     FL_RENDER_NORMALS = 128,     /* NormalIntegrator (integrators/normalintegrator.cpp:11-26):
                                     the first hit's normal through from_reflectance_rgb, black
@@ -260,7 +262,7 @@ typedef struct {
                                     (stack_depth, trav_waves): (8,0) (12,0) (8,6) (8,8) (6,8) (16,0) */
     fl_guide_t guide;            /* guiding network when FL_RENDER_GUIDED */
     float guide_mix;             /* proposal probability of the guide (0.5) */
-    float guide_train_fraction;  /* with TRAIN_GUIDE: fraction of spp used to train */
+    float guide_train_fraction;  /* with TRAIN_GUIDE: fraction of spp used to train (0: 0.15) */
     uint32_t refill_lanes;       /* traversal: idle lanes of a wave that trigger a refill (0: auto,
                                     24 with rays loaded ahead, 16 with budgeted refill) */
     uint32_t trav_prefetch;      /* traversal mode: 1 = persistent, each lane's next ray loaded
@@ -365,7 +367,7 @@ int fl_render_batch(fl_ctx_t ctx, const fl_batch_item *items, uint32_t n,
  * on/off, shading-kernel occupancy. No counterpart in the reference. */
 typedef struct {
     uint32_t max_paths, stack_depth, trav_waves, trav_budget, shade_waves;
-    uint32_t sort_shading;        /* 1: per-block material sort on (FL_RENDER_NO_SORT clear) */
+    uint32_t sort_shading;        /* 1: per-block material sort on (FL_RENDER_SORT_SHADING) */
     double ms_best, ms_default;   /* device ms of one render with the chosen and the default values */
     uint32_t n_tried;             /* probe configurations measured */
     uint32_t refill_lanes;        /* traversal refill threshold (0: the mode's default) */

@@ -39,7 +39,7 @@ def test_embree_subset_exported():
 
 def test_abi_version_and_no_device_behaviour():
     L = flgpu.lib()
-    assert L.fl_abi_version() == 6 == flgpu.ABI_VERSION
+    assert L.fl_abi_version() == 7 == flgpu.ABI_VERSION
     if L.fl_device_count() == 0:
         h = flgpu.C.c_void_p()
         assert L.fl_ctx_create(0, flgpu.C.byref(h)) == flgpu.FL_E_UNSUPPORTED

@@ -15,7 +15,41 @@ pytestmark = pytest.mark.gpu
 GUIDED_FLAGS = flgpu.FL_RENDER_VNDF_PDF | flgpu.FL_RENDER_GRAPH | flgpu.FL_RENDER_GUIDED
 
 
+def block_z_stats(gs, gq, ps, pq, n, b=2):
+    """The consistency rule of z_stats on b x b pixel blocks: per block and
+    channel z = (m_g - m_p) / sqrt(v_g + v_p), the block means' variances from
+    the pixels' sample variances; the fraction of |z| > 3 must stay within
+    0.27% + 3 binomial sigmas."""
+    H, W = gs.shape[0] // b * b, gs.shape[1] // b * b
+
+    def blocks(s, q):
+        m = s[:H, :W] / n
+        v = np.maximum(q[:H, :W] / n - m * m, 0) * n / (n - 1) / n   # variance of each pixel's mean
+        bm = m.reshape(H // b, b, W // b, b, 3).mean((1, 3))
+        bv = v.reshape(H // b, b, W // b, b, 3).sum((1, 3)) / (b * b) ** 2
+        return bm, bv
+
+    gm, gv = blocks(gs.astype(np.float64), gq.astype(np.float64))
+    pm, pv = blocks(ps.astype(np.float64), pq.astype(np.float64))
+    den = np.sqrt(gv + pv)
+    valid = den > 0
+    z = (gm[valid] - pm[valid]) / den[valid]
+    nz = int(valid.sum())
+    return float(np.mean(np.abs(z) > 3)), 0.0027 + 3 * np.sqrt(0.0027 * 0.9973 / max(nz, 1))
+
+
 def test_guided_render_consistent_with_plain(ctx):
+    """Guided and plain renders of the Cornell box agree within their Monte
+    Carlo error: the image-mean z of Y per pixel (as z_stats) and the |z| > 3
+    rate over 2 x 2 pixel blocks. Per single pixel the guided estimator's
+    heavier tail put the |z| > 3 rate at 0.23-0.60% over 21 seeds and guide
+    trainings (mean 0.32%, and 0.36% at 1,024 spp: not growing with spp, as a
+    bias would; two plain renders give 0.24%), against the 0.49% a normal z
+    allows for 4,800 values, so a few runs in twenty failed. On 2 x 2 blocks
+    over ten seeds the rate was 0-0.58% (mean 0.33%) against the 0.72% allowed
+    for 1,200 values, and the image-mean z stayed within 1. Guided renders are
+    not bit-reproducible (the training records' order follows the queue
+    appends), so each run draws a new value of the statistic."""
     gen = flgpu.GenSceneData(flgpu.FL_GEN_CORNELL, width=40, height=40)
     spp = 512
     with ctx.scene(gen) as sc, ctx.guide(seed=2) as g:
@@ -26,8 +60,10 @@ def test_guided_render_consistent_with_plain(ctx):
         gs, gq, st = sc.render(pg)
         steps, loss, div = g.stats()
     assert st.nan_samples == 0 and div == 0 and steps > 0
-    frac, allowed, mean_z, bad = z_stats(gs, gq, ps, pq, spp)
-    print("guided vs plain: frac |z|>3 %.4f (allowed %.4f), mean z %.2f, %d steps" % (frac, allowed, mean_z, steps))
+    _, _, mean_z, bad = z_stats(gs, gq, ps, pq, spp)
+    frac, allowed = block_z_stats(gs, gq, ps, pq, spp)
+    print("guided vs plain: 2x2-block frac |z|>3 %.4f (allowed %.4f), mean z %.2f, %d steps" % (
+        frac, allowed, mean_z, steps))
     assert bad == 0 and frac <= allowed and abs(mean_z) < 3
 
 

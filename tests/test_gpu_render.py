@@ -110,7 +110,7 @@ def test_bounce_loop_variants_identical(ctx):
         for name, flags in [("eager", flgpu.FL_RENDER_TIMING), ("pipelined", 0), ("graph", flgpu.FL_RENDER_GRAPH),
                             ("graph again", flgpu.FL_RENDER_GRAPH),
                             ("graph counters", flgpu.FL_RENDER_GRAPH | flgpu.FL_RENDER_COUNTERS),
-                            ("unsorted shading", flgpu.FL_RENDER_GRAPH | flgpu.FL_RENDER_NO_SORT),
+                            ("sorted shading", flgpu.FL_RENDER_GRAPH | flgpu.FL_RENDER_SORT_SHADING),
                             ("eager again", flgpu.FL_RENDER_TIMING)]:
             p = flgpu.default_params(spp=8, depth=8, seed=9, flags=flags)
             p.max_paths = 2 * npix   # 2 samples per pixel per pass: 4 passes
