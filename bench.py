@@ -457,7 +457,7 @@ def main():
                     help="also run a plain 1 GiB -> 1 GiB copy kernel (HBM counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py: this build runs on exactly one GPU (renders shard by pixels/samples; see DESIGN.md)",
+        print("bench.py: measures one GPU only (a render would shard across GPUs by pixels and samples)",
               file=sys.stderr)
         sys.exit(2)
 
