@@ -125,17 +125,18 @@ __global__ void __launch_bounds__(kBlock) k_generate(RenderArgs a) {
             o = ld3(it.cam.o);
             v3 dir = llc + hor * su + ver * tv - o - v3(0.f, 0.f, 0.f);
             d = normalize(dir);
-            a.st_beta[sid] = make_float4(1.f, 1.f, 1.f, 1.f);
-            a.st_L[sid] = make_float4(0.f, 0.f, 0.f, 0.f);
-            a.st_misc[sid] = make_float4(lam0, 1.f, 0.f, __uint_as_float(0u));
+            // throughput, radiance and the NEE slots start at 1, 0, 0: the
+            // path's first shade takes them from PF_FRESH instead of memory
+            // (64 B per path neither written here nor read there)
+            a.st_misc[sid] = make_float4(lam0, 1.f, 0.f, __uint_as_float(PF_FRESH));
             a.st_key[sid] = make_uint2(pixel, sample);
         } else {
             a.st_L[sid] = make_float4(0.f, 0.f, 0.f, 0.f);
             a.st_misc[sid] = make_float4(0.f, 1.f, 0.f, __uint_as_float(PF_DONE));
+            a.pend[sid] = make_float4(0.f, 0.f, 0.f, 0.f);
+            a.pend[a.cap + sid] = make_float4(0.f, 0.f, 0.f, 0.f);
         }
         if (a.n_items > 1) a.st_item[sid] = item;
-        a.pend[sid] = make_float4(0.f, 0.f, 0.f, 0.f);
-        a.pend[a.cap + sid] = make_float4(0.f, 0.f, 0.f, 0.f);
         if (a.st_rec) a.st_rec[sid] = 0u;
         if (active) {
             a.rq_o[0][slot] = make_float4(o.x, o.y, o.z, 0.f);       // tMin = time = 0 (physical.cpp:21)
@@ -801,7 +802,7 @@ k_shade(RenderArgs a, int b) {
         const bool active = i < n;
         const SceneRef S = scene_of(a, active ? item_of(a, a.rq_sid[cur][i]) : 0u);
         // -------- phase 1a: the hit's geometry and the path's wavelength
-        bool cont = false, mat_hit = false, rec_want = false, hit = false;
+        bool cont = false, mat_hit = false, rec_want = false, hit = false, fresh = false;
         uint32_t rec_prev = 0;   // index + 1 of the previous vertex's training record
         float beta0 = 0.f;
         uint32_t sid = 0, bounce = 0, flags = 0, rb = 0, mkind = 0, light = 0;
@@ -818,6 +819,8 @@ k_shade(RenderArgs a, int b) {
             const uint32_t bits = __float_as_uint(misc.w);
             bounce = bits & 0xffu;
             flags = bits & ~0xffu;
+            fresh = (flags & PF_FRESH) != 0u;
+            flags &= ~PF_FRESH;
             const uint32_t rec = __float_as_uint(hh.y);
             t = hh.x;
             d = v3(rd.x, rd.y, rd.z);
@@ -863,12 +866,17 @@ k_shade(RenderArgs a, int b) {
             }
         }
         // -------- phase 1b: path state, emission, pass-through
-        if (active) {
+        if (active && fresh) {
+            beta[0] = beta[1] = beta[2] = beta[3] = 1.f;
+            L[0] = L[1] = L[2] = L[3] = 0.f;
+        } else if (active) {
             const float4 beta4 = a.st_beta[sid], L4 = a.st_L[sid];
             const float4 pe0 = a.pend[sid], pe1 = a.pend[a.cap + sid];
             beta[0] = beta4.x; beta[1] = beta4.y; beta[2] = beta4.z; beta[3] = beta4.w;
             L[0] = L4.x + pe0.x + pe1.x; L[1] = L4.y + pe0.y + pe1.y;
             L[2] = L4.z + pe0.z + pe1.z; L[3] = L4.w + pe0.w + pe1.w;
+        }
+        if (active) {
             if (a.st_rec) {
                 // the previous vertex's training record: radiance gathered up to
                 // and including that vertex's own light samples

@@ -21,6 +21,8 @@ enum : uint32_t {
     PF_LAST = 1u << 10,      // depth limit reached: count MIS emission, then stop
     PF_SECTERM = 1u << 11,   // secondary wavelengths terminated (dispersion)
     PF_DONE = 1u << 12,
+    PF_FRESH = 1u << 13,     // a new camera path: throughput 1, radiance and NEE slots 0,
+                             // not yet written to memory (its first shade uses the values)
 };
 
 // PCG output-permutation hash (Jarzynski & Olano, "Hash Functions for GPU
