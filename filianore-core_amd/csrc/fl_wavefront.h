@@ -33,8 +33,12 @@ FL_HD uint32_t hash32(uint32_t x) {
     uint32_t word = ((state >> ((state >> 28u) + 4u)) ^ state) * 277803737u;
     return (word >> 22u) ^ word;
 }
+// The seed and the pixel are hashed apart before they meet: with seed ^ pixel
+// as the first key, seed s + 1 at pixel p ^ s ^ (s + 1) drew exactly what seed
+// s drew at pixel p, so renders of nearby seeds (replicates, the two renders
+// of a comparison) were the same samples shuffled among neighbouring pixels.
 FL_HD uint32_t rng_base(uint64_t seed, uint32_t pixel, uint32_t sample, uint32_t bounce) {
-    uint32_t h = hash32((uint32_t)seed ^ pixel);
+    uint32_t h = hash32(hash32((uint32_t)seed) ^ hash32(pixel ^ 0xA511E9B3u));
     h = hash32(h + sample);
     return hash32(h ^ (bounce * 0x9E3779B9u + (uint32_t)(seed >> 32)));
 }
