@@ -279,11 +279,17 @@ __device__ __forceinline__ bool traverse(const TraceScene &sc, v3 o, v3 d, float
 // window of the queue, which large scenes need for their BVH to stay in the
 // caches; static chunks let waves drift apart but cost no atomics, which
 // serves small, cache-resident scenes.
+// A draw takes kChunk rays while the queue holds many per wave, and fewer as
+// it drains: the rays left at the counter (as of the wave's last draw) over
+// twice the grid's waves, in multiples of 64 down to 64. With a fixed 256 a
+// launch of few rays per wave (2 M rays over 5,120 waves) ended on a few waves
+// that had each drawn 256 rays just before the counter ran out.
 constexpr uint32_t kChunk = 256;
 struct RayCursor {
     uint32_t n, nwaves, static_chunks, dyn_base;
     uint32_t next_static;           // the wave's next static chunk
     uint32_t pool_next, pool_end;   // wave-uniform
+    uint32_t dyn_seen;              // the counter's value at the wave's last draw
     uint32_t *fetch;
     __device__ __forceinline__ RayCursor(uint32_t n_, uint32_t *fetch_, uint32_t static_share256)
         : n(n_), fetch(fetch_) {
@@ -297,6 +303,13 @@ struct RayCursor {
         pool_next = wave * 64u;             // chunk w first, in every case
         pool_end = pool_next + 64u;
         next_static = wave + nwaves;
+        dyn_seen = 0u;
+    }
+    // rays of the next dynamic draw (wave-uniform, a multiple of 64 in [64, kChunk])
+    __device__ __forceinline__ uint32_t dyn_chunk() const {
+        const uint32_t left = n > dyn_base + dyn_seen ? n - dyn_base - dyn_seen : 0u;
+        const uint32_t c = (left / (2u * nwaves)) & ~63u;
+        return c < 64u ? 64u : (c > kChunk ? kChunk : c);
     }
     // the next ray index of each lane with `need` (>= n: none left); call
     // with the whole wave
@@ -314,9 +327,10 @@ struct RayCursor {
                 next_static += nwaves;
             } else {
                 const int leader = __ffsll(m) - 1;
-                if (lane == (uint32_t)leader) nbase = atomicAdd(fetch, kChunk);
-                nbase = dyn_base + __builtin_amdgcn_readlane(nbase, leader);
-                chunk = kChunk;
+                chunk = dyn_chunk();
+                if (lane == (uint32_t)leader) nbase = atomicAdd(fetch, chunk);
+                dyn_seen = __builtin_amdgcn_readlane(nbase, leader) + chunk;
+                nbase = dyn_base + dyn_seen - chunk;
             }
         }
         const uint32_t rank = (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
