@@ -168,6 +168,7 @@ __device__ const uint32_t kClosestWord = kVisClosest;
 // uniform (scalar registers) in the traversal.
 template <bool MULTI>
 struct TraceSrc {
+    static constexpr bool kMulti = MULTI;   // rays of one wave may traverse different BVHs
     const float4 *so, *sd;
     const uint32_t *stgt;
     float4 *pend;

@@ -345,6 +345,189 @@ struct RayCursor {
     }
 };
 
+constexpr uint32_t kNoRay = ~0u;   // a lane without a ray
+
+// Lane `src`'s value of v (every lane of the wave must take part).
+__device__ __forceinline__ uint32_t lane_u32(uint32_t v, int src) {
+    return (uint32_t)__builtin_amdgcn_ds_bpermute(src << 2, (int)v);
+}
+__device__ __forceinline__ float lane_f32(float v, int src) { return __uint_as_float(lane_u32(__float_as_uint(v), src)); }
+
+// The end of a persistent traversal, once the wave's rays are all handed out:
+// a free lane starts its own next ray, or takes over another lane's
+// (prefetch mode: that lane is still busy with its current ray), and
+// otherwise helps a lane still traversing. A lane with work to spare (two or
+// more hit children in its node group, a stack entry, or one child and
+// triangles) gives part of it to a free lane: the far half of the group's
+// children, its top stack entry, or its one child. The helper loads the same
+// ray from the queue and runs that part from the giver's current closest
+// distance, with no hit of its own, and may give parts away in turn. A
+// finished part goes back to the lane that gave it, which keeps the closer
+// hit by the tie rule of `closer` (an any-hit ray: any hit). Every triangle
+// lies in exactly one part and no part culls a box holding a hit at or below
+// its bound, so the ray's result is the one a single lane finds. The owner of
+// a ray stores it once its own part and all parts it gave away are done.
+// Without this, a wave's last rays ran on a few lanes while the rest idled:
+// on the spheres workload 5.7 of the 42 ms of traversal passed between the
+// queue running out and the last wave's exit.
+//
+// Hand-overs move ray indices only (the receiver reloads the ray), so that
+// little is live beside the traversal state. A finished part is handed back
+// in the helper's own first three LDS stack entries (its stack is empty
+// then, or abandoned after an any-hit).
+template <bool COUNT, bool ANY_SKIP, class Src>
+__device__ __forceinline__ void trace_drain(const Src &src, TraceScene sc, TravState &s, uint32_t ray, bool any,
+                                            uint32_t nxt, const LaneStack &ls, TravCounters &cnt) {
+    constexpr int kDrainBudget = 8;
+    const int lane = (int)__lane_id();
+    const lds_u64 *wave_lds = ls.lds - lane;   // lane h's entry j: wave_lds[h + j * lds_stride]
+    // `ray`: the queue index of the ray this lane works on (its own, or the one
+    // it helps with)
+    bool running = ray != kNoRay;   // the lane traverses a part of `ray`
+    bool mine = running;            // the lane owns `ray` and stores it
+    int parent = -1;                // helper: the lane that gave this part
+    unsigned long long helpers = 0ull;   // lanes running parts this lane gave
+    while (__ballot(running || mine || parent >= 0 || helpers != 0ull || nxt != kNoRay)) {
+        // ---- loaded next rays: a free lane starts its own, or another lane's
+        bool free = !running && !mine && parent < 0 && helpers == 0ull;
+        unsigned long long I = __ballot(free), P = __ballot(nxt != kNoRay);
+        if (P) {
+            uint32_t idx = kNoRay;
+            if (free && nxt != kNoRay) {   // its own
+                idx = nxt;
+                nxt = kNoRay;
+            }
+            I = __ballot(free && idx == kNoRay);
+            P = __ballot(nxt != kNoRay);
+            int partner = -1;
+            while (I && P) {   // wave-uniform
+                const int h = __ffsll(I) - 1, g = __ffsll(P) - 1;
+                I &= I - 1ull;
+                P &= P - 1ull;
+                if (lane == h || lane == g) partner = lane == h ? g : h;
+            }
+            const uint32_t other = lane_u32(nxt, partner >= 0 ? partner : lane);
+            if (nxt != kNoRay && partner >= 0) nxt = kNoRay;   // given away
+            if (free && idx == kNoRay && partner >= 0) idx = other;
+            if (idx != kNoRay) {
+                v3 o, d;
+                float t0, t1;
+                uint32_t anyw;
+                src.load(idx, o, d, t0, t1, anyw, sc);
+                ray = idx;
+                any = (int32_t)anyw >= 0;
+                trav_init(s, o, d, t0, t1);
+                mine = running = true;
+                free = false;
+            }
+        }
+        // ---- parts of rays: the k-th free lane helps the k-th lane with work to spare
+        const bool spare = running && (__popc(s.g_hits) >= 2 || s.sp > 0 || (s.g_hits != 0u && s.t_hits != 0u));
+        I = __ballot(free);
+        unsigned long long D = __ballot(spare);
+        if (I && D) {
+            int partner = -1;
+            while (I && D) {   // wave-uniform
+                const int h = __ffsll(I) - 1, g = __ffsll(D) - 1;
+                I &= I - 1ull;
+                D &= D - 1ull;
+                if (lane == h || lane == g) partner = lane == h ? g : h;
+            }
+            uint32_t gb = 0u, gi = 0u, give = 0u;
+            if (spare && partner >= 0) {   // the giver
+                if (__popc(s.g_hits) >= 2) {
+                    // keep the near half (the low bits), give the rest
+                    uint32_t far = s.g_hits;
+                    for (int k = (__popc(s.g_hits) + 1) / 2; k > 0; --k) far &= far - 1u;
+                    gb = s.g_base;
+                    gi = s.g_imask;
+                    give = far;
+                    s.g_hits ^= far;
+                } else if (s.sp > 0) {
+                    --s.sp;
+                    uint2 e;
+                    if (s.sp < ls.cap) {
+                        const unsigned long long v = ls.lds[s.sp * ls.lds_stride];
+                        e = make_uint2((uint32_t)v, (uint32_t)(v >> 32));
+                    } else {
+                        e = ls.spill[(size_t)(s.sp - ls.cap) * ls.spill_stride];
+                    }
+                    gb = e.x;
+                    gi = e.y & 0xffu;
+                    give = e.y >> 8;
+                } else {
+                    gb = s.g_base;
+                    gi = s.g_imask;
+                    give = s.g_hits;
+                    s.g_hits = 0u;
+                }
+                helpers |= 1ull << partner;
+            }
+            const int from = (free && partner >= 0) ? partner : lane;
+            const uint32_t hray = lane_u32(ray, from), hgb = lane_u32(gb, from), hgi = lane_u32(gi, from);
+            const uint32_t hgive = lane_u32(give, from);
+            const float tbest = lane_f32(s.best_t, from);
+            if (free && partner >= 0) {
+                v3 o, d;
+                float t0, t1;
+                uint32_t anyw;
+                src.load(hray, o, d, t0, t1, anyw, sc);
+                ray = hray;
+                any = (int32_t)anyw >= 0;
+                trav_init(s, o, d, t0, tbest);
+                s.g_base = hgb;
+                s.g_imask = hgi;
+                s.g_hits = hgive;
+                parent = partner;
+                running = true;
+            }
+        }
+        // ---- traverse
+        if (running) {
+            if (trav_step<COUNT>(sc, s, ls, kDrainBudget, cnt, any, any && ANY_SKIP)) running = false;
+        }
+        // ---- finished parts go back to the lanes that gave them
+        const bool deliver = !running && helpers == 0ull && parent >= 0;
+        if (deliver) {
+            lds_u64 *r = ls.lds;
+            r[0] = ((unsigned long long)s.best_id << 32) | __float_as_uint(s.best_t);
+            r[ls.lds_stride] = ((unsigned long long)s.best_rec << 32) | __float_as_uint(s.bu);
+            r[2 * ls.lds_stride] = __float_as_uint(s.bv);
+        }
+        const unsigned long long dep = __ballot(deliver);
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        unsigned long long got = helpers & dep;
+        helpers &= ~dep;
+        while (got) {
+            const int h = __ffsll(got) - 1;
+            got &= got - 1ull;
+            const unsigned long long v0 = wave_lds[h];
+            const uint32_t id = (uint32_t)(v0 >> 32);
+            const float t = __uint_as_float((uint32_t)v0);
+            if (id != kNoHit && (any ? s.best_id == kNoHit : closer(t, id, s.best_t, s.best_id))) {
+                const unsigned long long v1 = wave_lds[h + ls.lds_stride], v2 = wave_lds[h + 2 * ls.lds_stride];
+                s.best_t = t;
+                s.best_id = id;
+                s.best_rec = (uint32_t)(v1 >> 32);
+                s.bu = __uint_as_float((uint32_t)v1);
+                s.bv = __uint_as_float((uint32_t)v2);
+            }
+        }
+        if (any && s.best_id != kNoHit) running = false;   // any-hit: one part's hit decides
+        if (deliver) parent = -1;
+        if (mine && !running && helpers == 0ull) {
+            src.store(ray, s);
+            mine = false;
+        }
+        // the next round's LDS writes come after every read of this round
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    }
+}
+
 // Persistent traversal over n rays with per-lane refill (the two modes are
 // separate instantiations, PREFETCH, so each gets its own register budget).
 // Src provides
@@ -370,7 +553,6 @@ constexpr uint32_t kPrefetchTris = 65536;
 constexpr uint32_t kPrefetchPaths = 4u << 20;
 // scenes below kPrefetchTris trace one ray per thread in passes up to this many paths
 constexpr uint32_t kFlatPaths = 16u << 20;
-constexpr uint32_t kNoRay = ~0u;
 
 template <bool COUNT, bool ANY_SKIP, bool PREFETCH, class Src>
 __device__ __forceinline__ void trace_persistent(const Src &src, uint32_t n, uint32_t *fetch, const LaneStack &ls,
@@ -426,6 +608,10 @@ __device__ __forceinline__ void trace_persistent(const Src &src, uint32_t n, uin
                     }
                 }
             }
+            if (__ballot(exhausted) != 0ull) {   // no ray left to hand out
+                trace_drain<COUNT, ANY_SKIP>(src, sc, s, ray, any, kNoRay, ls, cnt);
+                return;
+            }
         }
         return;
     }
@@ -464,6 +650,10 @@ __device__ __forceinline__ void trace_persistent(const Src &src, uint32_t n, uin
                 src.store(ray, s);
                 ray = kNoRay;
             }
+        }
+        if (__ballot(exhausted) != 0ull) {   // no ray left to hand out
+            trace_drain<COUNT, ANY_SKIP>(src, sc, s, ray, any, nxt, ls, cnt);
+            return;
         }
     }
 }

@@ -23,6 +23,7 @@ constexpr int kStack = 8;
 constexpr int kBudget = 24;
 
 struct ApiSrc {
+    static constexpr bool kMulti = false;
     const fl_ray *rays;
     fl_hit *hits;
     bool any;
