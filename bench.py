@@ -526,7 +526,16 @@ def main():
                                                 max(1, cst.rays_extension), 2),
                 "ms_trace": round(tst.ms_extend, 2), "ms_shade": round(tst.ms_shade, 2),
                 "launches": launches,
+                "achieved_definition": "algorithmic bytes (32 B ray + 16 B hit + 128 B per node + 48 B per "
+                                       "triangle visited) per launch / average launch time; they include the "
+                                       "reads L2 and the Infinity Cache serve, so frac can pass 1",
             }
+            if roofline["traffic"]:
+                # the measured HBM traffic per launch (rocprof FETCH_SIZE + WRITE_SIZE,
+                # calibrated) over this run's average launch time
+                hbm = roofline["traffic"] / (tst.ms_extend / launches * 1e-3) / 1e9
+                roofline["hbm_achieved"] = round(hbm, 1)
+                roofline["hbm_frac"] = round(hbm / HBM_PEAK_GBS, 4)
             issue = issue_from_profile()
             if issue:
                 # the byte roofline is the one the workload is specified
