@@ -77,7 +77,10 @@ def test_traversal_refill_modes_give_identical_films(ctx):
     (one ray per thread), at several refill thresholds, only reorder the
     traversal: on a small scene (auto: one ray per thread) and a larger one
     (auto: budgeted refill for this small pass) every film equals the
-    automatic choice's bit for bit."""
+    automatic choice's bit for bit. The persistent modes end each launch by
+    splitting the last rays' traversals across a wave's free lanes
+    (fl_trace.h trace_drain); one ray per thread never does, so the
+    comparison also checks that the split parts merge to the same hits."""
     for kind, kw in ((flgpu.FL_GEN_CORNELL, dict(width=96, height=96)),
                      (flgpu.FL_GEN_SPHERES, dict(subdiv=5, width=128, height=96))):
         gen = flgpu.GenSceneData(kind, **kw)
