@@ -392,7 +392,7 @@ __device__ __forceinline__ void trace_drain(const Src &src, TraceScene sc, TravS
     while (__ballot(running || mine || parent >= 0 || helpers != 0ull || nxt != kNoRay)) {
         // ---- loaded next rays: a free lane starts its own, or another lane's
         bool free = !running && !mine && parent < 0 && helpers == 0ull;
-        unsigned long long I = __ballot(free), P = __ballot(nxt != kNoRay);
+        unsigned long long I, P = __ballot(nxt != kNoRay);
         if (P) {
             uint32_t idx = kNoRay;
             if (free && nxt != kNoRay) {   // its own
@@ -563,7 +563,7 @@ __device__ __forceinline__ void trace_persistent(const Src &src, uint32_t n, uin
     // starting from it keeps the scene pointers uniform (scalar registers)
     TraceScene sc = src.base_scene();
     TravState s;
-    RayCursor cur(n, fetch, PREFETCH ? 0u : 192u);   // static share: small scenes only
+    RayCursor cur(n, fetch, PREFETCH ? 0u : 192u);   // static share: budgeted refill only
     const uint32_t first = cur.take(true);   // lane g of the grid: ray g
     uint32_t ray = kNoRay;
     bool any = false;
