@@ -379,8 +379,11 @@ template <bool COUNT, bool ANY_SKIP, class Src>
 __device__ __forceinline__ void trace_drain(const Src &src, TraceScene sc, TravState &s, uint32_t ray, bool any,
                                             uint32_t nxt, const LaneStack &ls, TravCounters &cnt) {
     // traversal iterations between hand-outs (spheres: 2, 4 and 8 within 0.5%;
-    // a 1-spp pass of the guided room: 10.3, 10.5 and 10.8 ms)
-    constexpr int kDrainBudget = 2;
+    // a 1-spp pass of the guided room: 10.3, 10.5 and 10.8 ms). In batches a
+    // helper reloads its ray through the item table, and hand-outs every 8
+    // iterations win (the 8-camera terrain at 64 spp: 688 Msamples/s, 639
+    // at every 2, 664 without the drain).
+    constexpr int kDrainBudget = Src::kMulti ? 8 : 2;
     const int lane = (int)__lane_id();
     const lds_u64 *wave_lds = ls.lds - lane;   // lane h's entry j: wave_lds[h + j * lds_stride]
     // `ray`: the queue index of the ray this lane works on (its own, or the one
