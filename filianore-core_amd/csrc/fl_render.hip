@@ -42,7 +42,8 @@ using namespace fl;
 
 constexpr int kBlock = 256;
 // device bytes per path slot: state 56, pending NEE 32, two extension queues
-// 72, hit 16, shadow queue (two entries) 72
+// 72, hit 8 (+ 8 of barycentrics in textured renders), shadow queue (two
+// entries) 72
 constexpr size_t kBytesPerPath = 248;
 
 // device counter slots
@@ -174,13 +175,14 @@ struct TraceSrc {
     float4 *pend;
     const float4 *eo, *ed;
     const uint32_t *esid;
-    float4 *hits;
+    float2 *hits;            // per extension ray: t and triangle record
     uint32_t ns, cap;
     const ItemDev *items;
     const uint32_t *st_item;
     uint32_t n_items;
     TraceScene sc0;
     const float4 *socc;      // NEE term of an occluded shadow ray (shadowColor), or null: 0
+    float2 *huv;             // per extension ray: barycentrics (textured renders; else null)
     __device__ TraceScene base_scene() const { return sc0; }
     __device__ TraceScene scene_of_ray(uint32_t i) const {
         if (!MULTI || n_items <= 1) return sc0;
@@ -221,7 +223,8 @@ struct TraceSrc {
             if (occluded)   // Li = shadowColor (integrator.cpp:47-50)
                 pend[t & ~kVisClosest] = socc ? socc[i] : make_float4(0.f, 0.f, 0.f, 0.f);
         } else {
-            hits[i - ns] = make_float4(s.best_t, __uint_as_float(s.best_id != kNoHit ? s.best_rec : kNoHit), s.bu, s.bv);
+            hits[i - ns] = make_float2(s.best_t, __uint_as_float(s.best_id != kNoHit ? s.best_rec : kNoHit));
+            if (huv) huv[i - ns] = make_float2(s.bu, s.bv);
         }
     }
 };
@@ -249,7 +252,7 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WAV
     ls.spill_stride = a.spill_stride;
     TravCounters c;
     TraceSrc<MULTI> src{a.sq_o,    a.sq_d, a.sq_tgt, a.pend,    a.rq_o[cur], a.rq_d[cur], a.rq_sid[cur],
-                        a.hits,    ns,     a.cap,    a.items,   a.st_item,   a.n_items,   a.sc, a.sq_occ};
+                        a.hits,    ns,     a.cap,    a.items,   a.st_item,   a.n_items,   a.sc, a.sq_occ, a.hit_uv};
     // refill threshold by default: a yield once 24 lanes are done when the
     // next rays are loaded ahead (spheres, 64 spp: 48.1 against 48.8 ms at
     // 16 and 52.7 at 8), a refill once 16 lanes are idle otherwise
@@ -290,7 +293,7 @@ __global__ void __launch_bounds__(kBlock) k_trace_flat(RenderArgs a, int b) {
     ls.spill_stride = a.spill_stride;
     TravCounters c;
     TraceSrc<MULTI> src{a.sq_o,    a.sq_d, a.sq_tgt, a.pend,    a.rq_o[cur], a.rq_d[cur], a.rq_sid[cur],
-                        a.hits,    ns,     a.cap,    a.items,   a.st_item,   a.n_items,   a.sc, a.sq_occ};
+                        a.hits,    ns,     a.cap,    a.items,   a.st_item,   a.n_items,   a.sc, a.sq_occ, a.hit_uv};
     v3 o, d;
     float tmin, tmax;
     uint32_t anyw;
@@ -815,7 +818,8 @@ k_shade(RenderArgs a, int b) {
         if (active) {
             const float4 ro = a.rq_o[cur][i], rd = a.rq_d[cur][i];
             sid = a.rq_sid[cur][i];
-            const float4 hh = a.hits[i], misc = a.st_misc[sid];
+            const float2 hh = a.hits[i];
+            const float4 misc = a.st_misc[sid];
             lam0 = misc.x; eta_scale = misc.y; prev_pdf = misc.z;
             const uint32_t bits = __float_as_uint(misc.w);
             bounce = bits & 0xffu;
@@ -835,7 +839,10 @@ k_shade(RenderArgs a, int b) {
                 light = q2.w;
                 p = v3(ro.x, ro.y, ro.z) + d * t;                  // ray.point_at_t(t)
                 ng = normalize(cross(p1 - p0, p2 - p0));
-                if (STD && S.tex) uv = hit_uv(*S.tex, q0.w, hh.z, hh.w);
+                if (STD && S.tex) {
+                    const float2 bc = a.hit_uv[i];
+                    uv = hit_uv(*S.tex, q0.w, bc.x, bc.y);
+                }
                 hit = true;
                 // a material vertex unless the first hit ends the path
                 // (NormalIntegrator), the MIS ray of the last vertex is being
@@ -1668,7 +1675,11 @@ int render_items(fl_ctx_t ctx, const HostItem *hi, uint32_t n_items, const fl_re
     a.rq_o[1] = grow<float4>(B.p[8], B.cap[8], cap * 16, err);
     a.rq_d[1] = grow<float4>(B.p[9], B.cap[9], cap * 16, err);
     a.rq_sid[1] = grow<uint32_t>(B.p[10], B.cap[10], cap * 4, err);
-    a.hits = grow<float4>(B.p[11], B.cap[11], cap * 16, err);
+    // the hits' barycentrics are read only to look up textures
+    bool has_tex = false;
+    for (uint32_t k = 0; k < n_items; ++k) has_tex = has_tex || hi[k].scene->has_textures;
+    a.hits = grow<float2>(B.p[11], B.cap[11], cap * 8, err);
+    a.hit_uv = has_tex ? grow<float2>(B.p[26], B.cap[26], cap * 8, err) : nullptr;
     a.sq_o = grow<float4>(B.p[12], B.cap[12], 2 * cap * 16, err);
     a.sq_d = grow<float4>(B.p[13], B.cap[13], 2 * cap * 16, err);
     a.sq_tgt = grow<uint32_t>(B.p[14], B.cap[14], 2 * cap * 4, err);

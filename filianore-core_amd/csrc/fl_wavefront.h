@@ -117,7 +117,8 @@ struct RenderArgs {
     // queues
     float4 *rq_o[2], *rq_d[2];
     uint32_t *rq_sid[2];
-    float4 *hits;
+    float2 *hits;            // per extension ray: t, triangle record (kNoHit: miss)
+    float2 *hit_uv;          // per extension ray: barycentrics (renders with textures; else null)
     float4 *sq_o, *sq_d;
     uint32_t *sq_tgt;
     float4 *sq_occ;          // per shadow ray: its NEE term when occluded (shadowColor); null if all black
