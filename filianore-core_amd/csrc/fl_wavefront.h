@@ -47,6 +47,12 @@ FL_HD float rng_u(uint32_t base, uint32_t dim) {
     return (float)(h >> 8) * 0x1p-24f;
 }
 
+// A queued ray's direction: 12 bytes (a ray's t range is set by its kind, so
+// no fourth word is stored).
+struct Dir3 {
+    float x, y, z;
+};
+
 struct CameraDev {
     float o[3], llc[3], hor[3], ver[3];
     uint32_t W, H;
@@ -115,11 +121,13 @@ struct RenderArgs {
     uint2 *st_key;
     float4 *pend;            // [2][cap]
     // queues
-    float4 *rq_o[2], *rq_d[2];
+    float4 *rq_o[2];         // origin, t_min
+    Dir3 *rq_d[2];
     uint32_t *rq_sid[2];
     float2 *hits;            // per extension ray: t, triangle record (kNoHit: miss)
     float2 *hit_uv;          // per extension ray: barycentrics (renders with textures; else null)
-    float4 *sq_o, *sq_d;
+    float4 *sq_o;
+    Dir3 *sq_d;
     uint32_t *sq_tgt;
     float4 *sq_occ;          // per shadow ray: its NEE term when occluded (shadowColor); null if all black
     uint32_t *counts;        // [0],[1] ray queues, [2] shadow queue
