@@ -43,9 +43,9 @@ using namespace fl;
 constexpr int kBlock = 256;
 // device bytes per path slot: state 56, pending NEE 32, two extension queues
 // 64, hit 8 (+ 8 of barycentrics in textured renders), shadow queue (two
-// entries) 64, and their NEE terms for occluded rays (scenes with shadow
+// entries) 56, and their NEE terms for occluded rays (scenes with shadow
 // colors) 32
-constexpr size_t kBytesPerPath = 264;
+constexpr size_t kBytesPerPath = 256;
 
 // device counter slots
 constexpr int C_EXT = 0;    // [0],[1]: extension queue sizes by bounce parity
@@ -142,7 +142,7 @@ __global__ void __launch_bounds__(kBlock) k_generate(RenderArgs a) {
         if (a.st_rec) a.st_rec[sid] = 0u;
         if (active) {
             a.rq_o[0][slot] = make_float4(o.x, o.y, o.z, 0.f);       // tMin = time = 0 (physical.cpp:21)
-            a.rq_d[0][slot] = Dir3{d.x, d.y, d.z};
+            a.rq_d[0][slot] = Packed3{d.x, d.y, d.z};
             a.rq_sid[0][slot] = sid;
         }
     }
@@ -171,12 +171,11 @@ __device__ const uint32_t kClosestWord = kVisClosest;
 template <bool MULTI>
 struct TraceSrc {
     static constexpr bool kMulti = MULTI;   // rays of one wave may traverse different BVHs
-    const float4 *so;
-    const Dir3 *sd;
+    const Packed3 *so, *sd;
     const uint32_t *stgt;
     float4 *pend;
     const float4 *eo;
-    const Dir3 *ed;
+    const Packed3 *ed;
     const uint32_t *esid;
     float2 *hits;            // per extension ray: t and triangle record
     uint32_t ns, cap;
@@ -200,11 +199,17 @@ struct TraceSrc {
     }
     __device__ void load(uint32_t i, v3 &ro, v3 &rd, float &tmin, float &tmax, uint32_t &anyw, TraceScene &sc) const {
         float4 a;
-        Dir3 b;
+        Packed3 b;
         // a shadow ray is any-hit unless its target word carries kVisClosest
         anyw = *(i < ns ? stgt + i : &kClosestWord);
-        if (i < ns) { a = so[i]; b = sd[i]; }
-        else { a = eo[i - ns]; b = ed[i - ns]; }
+        if (i < ns) {
+            const Packed3 o3 = so[i];
+            a = make_float4(o3.x, o3.y, o3.z, kShadowEps);   // every shadow ray starts kShadowEps out
+            b = sd[i];
+        } else {
+            a = eo[i - ns];
+            b = ed[i - ns];
+        }
         sc = scene_of_ray(i);
         ro = v3(a.x, a.y, a.z);
         rd = v3(b.x, b.y, b.z);
@@ -821,7 +826,7 @@ k_shade(RenderArgs a, int b) {
         Lanes ln;
         if (active) {
             const float4 ro = a.rq_o[cur][i];
-            const Dir3 rd = a.rq_d[cur][i];
+            const Packed3 rd = a.rq_d[cur][i];
             sid = a.rq_sid[cur][i];
             const float2 hh = a.hits[i];
             const float4 misc = a.st_misc[sid];
@@ -1022,8 +1027,8 @@ k_shade(RenderArgs a, int b) {
             for (int j = 0; j < 2; ++j) {
                 if (okv[j]) {
                     const uint32_t ss = base + (j ? c0 + (uint32_t)__popcll(m1 & lt) : (uint32_t)__popcll(m0 & lt));
-                    a.sq_o[ss] = sh_ov[j];
-                    a.sq_d[ss] = Dir3{sh_dv[j].x, sh_dv[j].y, sh_dv[j].z};
+                    a.sq_o[ss] = Packed3{sh_ov[j].x, sh_ov[j].y, sh_ov[j].z};
+                    a.sq_d[ss] = Packed3{sh_dv[j].x, sh_dv[j].y, sh_dv[j].z};
                     a.sq_tgt[ss] = ((uint32_t)j * a.cap + sid) | (deltav[j] ? kVisClosest : 0u);
                     if (a.sq_occ) a.sq_occ[ss] = occv[j];
                 }
@@ -1187,7 +1192,7 @@ k_shade(RenderArgs a, int b) {
                               (uint32_t)__popcll(mc & ((1ull << __lane_id()) - 1ull));
         if (cont) {
             a.rq_o[nxt][slot] = make_float4(no.x, no.y, no.z, kShadowEps);
-            a.rq_d[nxt][slot] = Dir3{nd.x, nd.y, nd.z};
+            a.rq_d[nxt][slot] = Packed3{nd.x, nd.y, nd.z};
             a.rq_sid[nxt][slot] = sid;
         }
         if constexpr (GUIDED) wave_sync();   // the next vertex's network overwrites the lobes just read
@@ -1675,18 +1680,18 @@ int render_items(fl_ctx_t ctx, const HostItem *hi, uint32_t n_items, const fl_re
     a.st_key = grow<uint2>(B.p[3], B.cap[3], cap * 8, err);
     a.pend = grow<float4>(B.p[4], B.cap[4], 2 * cap * 16, err);
     a.rq_o[0] = grow<float4>(B.p[5], B.cap[5], cap * 16, err);
-    a.rq_d[0] = grow<Dir3>(B.p[6], B.cap[6], cap * 12, err);
+    a.rq_d[0] = grow<Packed3>(B.p[6], B.cap[6], cap * 12, err);
     a.rq_sid[0] = grow<uint32_t>(B.p[7], B.cap[7], cap * 4, err);
     a.rq_o[1] = grow<float4>(B.p[8], B.cap[8], cap * 16, err);
-    a.rq_d[1] = grow<Dir3>(B.p[9], B.cap[9], cap * 12, err);
+    a.rq_d[1] = grow<Packed3>(B.p[9], B.cap[9], cap * 12, err);
     a.rq_sid[1] = grow<uint32_t>(B.p[10], B.cap[10], cap * 4, err);
     // the hits' barycentrics are read only to look up textures
     bool has_tex = false;
     for (uint32_t k = 0; k < n_items; ++k) has_tex = has_tex || hi[k].scene->has_textures;
     a.hits = grow<float2>(B.p[11], B.cap[11], cap * 8, err);
     a.hit_uv = has_tex ? grow<float2>(B.p[26], B.cap[26], cap * 8, err) : nullptr;
-    a.sq_o = grow<float4>(B.p[12], B.cap[12], 2 * cap * 16, err);
-    a.sq_d = grow<Dir3>(B.p[13], B.cap[13], 2 * cap * 12, err);
+    a.sq_o = grow<Packed3>(B.p[12], B.cap[12], 2 * cap * 12, err);
+    a.sq_d = grow<Packed3>(B.p[13], B.cap[13], 2 * cap * 12, err);
     a.sq_tgt = grow<uint32_t>(B.p[14], B.cap[14], 2 * cap * 4, err);
     bool shadow_colors = false;
     for (uint32_t k = 0; k < n_items; ++k) shadow_colors = shadow_colors || hi[k].scene->shadow_colors;

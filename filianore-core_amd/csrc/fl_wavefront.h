@@ -47,9 +47,10 @@ FL_HD float rng_u(uint32_t base, uint32_t dim) {
     return (float)(h >> 8) * 0x1p-24f;
 }
 
-// A queued ray's direction: 12 bytes (a ray's t range is set by its kind, so
-// no fourth word is stored).
-struct Dir3 {
+// Three floats in 12 bytes: queued ray directions and shadow-ray origins (a
+// shadow ray's t range and an extension ray's t_max follow from the ray's
+// kind, so no fourth word is stored).
+struct Packed3 {
     float x, y, z;
 };
 
@@ -122,12 +123,11 @@ struct RenderArgs {
     float4 *pend;            // [2][cap]
     // queues
     float4 *rq_o[2];         // origin, t_min
-    Dir3 *rq_d[2];
+    Packed3 *rq_d[2];
     uint32_t *rq_sid[2];
     float2 *hits;            // per extension ray: t, triangle record (kNoHit: miss)
     float2 *hit_uv;          // per extension ray: barycentrics (renders with textures; else null)
-    float4 *sq_o;
-    Dir3 *sq_d;
+    Packed3 *sq_o, *sq_d;
     uint32_t *sq_tgt;
     float4 *sq_occ;          // per shadow ray: its NEE term when occluded (shadowColor); null if all black
     uint32_t *counts;        // [0],[1] ray queues, [2] shadow queue
