@@ -41,11 +41,11 @@ namespace {
 using namespace fl;
 
 constexpr int kBlock = 256;
-// device bytes per path slot: state 56, pending NEE 32, two extension queues
+// device bytes per path slot: state 48, pending NEE 32, two extension queues
 // 64, hit 8 (+ 8 of barycentrics in textured renders), shadow queue (two
 // entries) 56, and their NEE terms for occluded rays (scenes with shadow
 // colors) 32
-constexpr size_t kBytesPerPath = 256;
+constexpr size_t kBytesPerPath = 248;
 
 // device counter slots
 constexpr int C_EXT = 0;    // [0],[1]: extension queue sizes by bounce parity
@@ -131,7 +131,6 @@ __global__ void __launch_bounds__(kBlock) k_generate(RenderArgs a) {
             // path's first shade takes them from PF_FRESH instead of memory
             // (64 B per path neither written here nor read there)
             a.st_misc[sid] = make_float4(lam0, 1.f, 0.f, __uint_as_float(PF_FRESH));
-            a.st_key[sid] = make_uint2(pixel, sample);
         } else {
             a.st_L[sid] = make_float4(0.f, 0.f, 0.f, 0.f);
             a.st_misc[sid] = make_float4(0.f, 1.f, 0.f, __uint_as_float(PF_DONE));
@@ -749,6 +748,20 @@ __device__ __forceinline__ uint32_t shade_key(const RenderArgs &a, int cur, uint
     return (uint32_t)S.mats[mat].kind;   // MAT_LAMBERT 1, MAT_CONDUCTOR 2, MAT_DIELECTRIC 3
 }
 
+// A path's pixel (y W + x) and sample index, from its id: k_generate lays
+// out an item's paths by sample layer of npix padded pixels in 8x8 tiles.
+__device__ __forceinline__ uint32_t path_pixel(const RenderArgs &a, uint32_t sid) {
+    const ItemDev &it = a.items[item_of(a, sid)];
+    const uint32_t local = sid - it.path_base, pl = local % it.npix;
+    const uint32_t tile = pl >> 6, within = pl & 63u;
+    const uint32_t x = (tile % it.tiles_x) * 8u + (within & 7u), y = (tile / it.tiles_x) * 8u + (within >> 3);
+    return y * it.cam.W + x;
+}
+__device__ __forceinline__ uint32_t path_sample(const RenderArgs &a, uint32_t sid) {
+    const ItemDev &it = a.items[item_of(a, sid)];
+    return it.sample_base + (sid - it.path_base) / it.npix;
+}
+
 // GUIDED: the render has a guide (mixture sampling at guided vertices and
 // training records); other renders run a kernel without that code.
 // STD: the render has standard-surface materials or image textures. Their
@@ -982,7 +995,7 @@ k_shade(RenderArgs a, int b) {
             fs = normalize(fs);
             woW = -d;
             wo = v3(dot(woW, fs), dot(woW, ft), dot(woW, ng));
-            rb = rng_base(S.seed, a.st_key[sid].x, a.st_key[sid].y, bounce);
+            rb = rng_base(S.seed, path_pixel(a, sid), path_sample(a, sid), bounce);
         }
         const TabSpec tsp{S.spec, sd ? sd->slot : nullptr, GS, ln, a.grid.linear == 0,
                           (STD && sd && S.tex) ? sd->tex : nullptr, s_tv + (STD ? threadIdx.x : 0u)};
@@ -1677,7 +1690,6 @@ int render_items(fl_ctx_t ctx, const HostItem *hi, uint32_t n_items, const fl_re
     a.st_beta = grow<float4>(B.p[0], B.cap[0], cap * 16, err);
     a.st_L = grow<float4>(B.p[1], B.cap[1], cap * 16, err);
     a.st_misc = grow<float4>(B.p[2], B.cap[2], cap * 16, err);
-    a.st_key = grow<uint2>(B.p[3], B.cap[3], cap * 8, err);
     a.pend = grow<float4>(B.p[4], B.cap[4], 2 * cap * 16, err);
     a.rq_o[0] = grow<float4>(B.p[5], B.cap[5], cap * 16, err);
     a.rq_d[0] = grow<Packed3>(B.p[6], B.cap[6], cap * 12, err);

@@ -119,7 +119,6 @@ struct RenderArgs {
     float xyz_weight;        // (hi-lo)/(4*CIE_Y_integral)
     // path state, indexed by path id
     float4 *st_beta, *st_L, *st_misc;
-    uint2 *st_key;
     float4 *pend;            // [2][cap]
     // queues
     float4 *rq_o[2];         // origin, t_min
